@@ -1,0 +1,142 @@
+"""Loader for the in-tree native library ``libbeep3.so``.
+
+The library holds the host-side scene code (include/beep_host.h) and the HIP
+device boundary (include/beep_hip.h).  It is built by ``beep3_amd.build`` (or
+``__graft_entry__.build()``); a missing library is an error, never a silent
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbeep3.so")
+
+_lib = None
+
+
+class BeepError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BeepError(
+                f"{LIB_PATH} is missing: build it with `python -m beep3_amd.build` "
+                "(the HIP extension is required; there is no CPU fallback)")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+
+class SceneSizes(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int), ("n_instances", C.c_int), ("n_charges", C.c_int),
+        ("n_qual", C.c_int), ("n_quad", C.c_int), ("n_galerkin", C.c_int),
+        ("n_leaves", C.c_int), ("n_near_cols", C.c_int), ("n_near_pairs", C.c_longlong),
+        ("tree_depth", C.c_int), ("cube_centre", C.c_double * 3), ("cube_edge", C.c_double),
+    ]
+
+
+class HipGeometry(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int),
+        ("centroid", vp), ("normal", vp), ("area", vp), ("eps_ratio", vp), ("gc", vp),
+        ("qual_ptr", vp), ("qual_pts", vp),
+        ("quad_ptr", vp), ("quad_pts", vp),
+        ("gal_ptr", vp), ("gal_pts", vp),
+    ]
+
+
+class HipTree(C.Structure):
+    _fields_ = [
+        ("n_leaves", C.c_int),
+        ("leaf_ptr", vp), ("leaf_idx", vp), ("nb_ptr", vp), ("nb_idx", vp),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int),
+        ("max_refine", C.c_int), ("inner_tol", C.c_double), ("verbose", C.c_int),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("matvecs", C.c_int), ("refinements", C.c_int),
+        ("converged", C.c_int), ("residual", C.c_double), ("rel_residual", C.c_double),
+        ("time_ms", C.c_double), ("matvec_ms", C.c_double),
+    ]
+
+
+class MatvecTiming(C.Structure):
+    _fields_ = [
+        ("near_ms", C.c_double), ("far_ms", C.c_double), ("total_ms", C.c_double),
+        ("near_launches", C.c_int), ("far_launches", C.c_int),
+    ]
+
+
+def _declare(L):
+    L.beep_host_last_error.restype = C.c_char_p
+    L.beep_mesh_icosphere.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
+    L.beep_mesh_star.argtypes = [C.c_int, C.c_double, C.c_int, C.c_double, C.c_uint64, C.POINTER(vp)]
+    L.beep_mesh_set_charges.argtypes = [vp, C.c_int, dp]
+    L.beep_mesh_charges_ball.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
+    L.beep_mesh_charges_walk.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
+    L.beep_mesh_info.argtypes = [vp, ip, ip, ip, dp, dp]
+    L.beep_mesh_vertices.argtypes = [vp, dp, ip]
+    L.beep_mesh_destroy.argtypes = [vp]
+    L.beep_mesh_destroy.restype = None
+    L.beep_scene_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.beep_scene_add_instance.argtypes = [vp, vp, dp, dp, C.c_double]
+    L.beep_scene_build.argtypes = [vp]
+    L.beep_scene_get_sizes.argtypes = [vp, C.POINTER(SceneSizes)]
+    L.beep_scene_patches.argtypes = [vp, dp, dp, dp, dp, dp, ip]
+    L.beep_scene_points.argtypes = [vp, C.c_int, ip, dp]
+    L.beep_scene_tree.argtypes = [vp, ip, ip, ip, ip]
+    L.beep_scene_charges.argtypes = [vp, dp, ip]
+    L.beep_scene_instances.argtypes = [vp, dp, ip, ip]
+    L.beep_scene_destroy.argtypes = [vp]
+    L.beep_scene_destroy.restype = None
+    if hasattr(L, "beep_hip_create"):
+        _declare_hip(L)
+
+
+def _declare_hip(L):
+    L.beep_hip_last_error.restype = C.c_char_p
+    L.beep_hip_last_error.argtypes = [vp]
+    L.beep_hip_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.beep_hip_destroy.argtypes = [vp]
+    L.beep_hip_set_geometry.argtypes = [vp, C.POINTER(HipGeometry)]
+    L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
+    L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
+    L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
+    L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
+    L.beep_hip_matvec_timing.argtypes = [vp, C.POINTER(MatvecTiming)]
+    L.beep_hip_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
+    L.beep_hip_energy_coefficients.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp]
+    L.beep_hip_solve.argtypes = [vp, dp, dp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+    L.beep_hip_device_free.argtypes = [vp, vp]
+    L.beep_hip_memcpy.argtypes = [vp, vp, vp, C.c_size_t, C.c_int]
+    L.beep_hip_synchronize.argtypes = [vp]
+    L.beep_hip_memory_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+
+
+def check_host(rc):
+    if rc != 0:
+        raise BeepError(lib().beep_host_last_error().decode())
+
+
+def check_hip(ctx, rc):
+    if rc != 0:
+        raise BeepError(lib().beep_hip_last_error(ctx).decode())

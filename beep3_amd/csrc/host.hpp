@@ -1,0 +1,169 @@
+// Host-side data model: meshes, node patches, quadrature point sets and the
+// adaptive octree that decides which patch pairs are integrated explicitly.
+//
+// These are the caller-side objects of the BEM mat-vec (the reference keeps
+// them in Mesh / MeshInstance / BasicNodePatch / Octree, src/bem/mesh.h,
+// src/bem/node_patch.h, src/fmm/octree.h).  They produce the flat arrays that
+// cross the device boundary declared in include/beep_hip.h.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace beep {
+
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+    Vec3() = default;
+    Vec3(double a, double b, double c) : x(a), y(b), z(c) {}
+    Vec3 operator+(const Vec3& o) const { return {x + o.x, y + o.y, z + o.z}; }
+    Vec3 operator-(const Vec3& o) const { return {x - o.x, y - o.y, z - o.z}; }
+    Vec3 operator*(double s) const { return {x * s, y * s, z * s}; }
+    Vec3 operator/(double s) const { return {x / s, y / s, z / s}; }
+    Vec3& operator+=(const Vec3& o) { x += o.x; y += o.y; z += o.z; return *this; }
+    double dot(const Vec3& o) const { return x * o.x + y * o.y + z * o.z; }
+    Vec3 cross(const Vec3& o) const {
+        return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x};
+    }
+    double norm() const { return std::sqrt(dot(*this)); }
+    Vec3 unit() const { double n = norm(); return {x / n, y / n, z / n}; }
+};
+
+// One quadrature / qualocation / galerkin point: position, unit normal,
+// weight.  Same content as the reference QuadPointT (src/bem/quad_point.h:82).
+struct QuadPt {
+    Vec3 p, n;
+    double w;
+};
+
+// Triangle quadrature rule in barycentric (u, v) with weights summing to 1.
+struct Rule {
+    int order;
+    std::vector<double> u, v, w;
+};
+// order 0 means "single centroid point" (node_patch.cpp:210-213, 243-246);
+// the others are the tables of src/bem/triangle.h:75-96.
+const Rule& rule_for_order(int order);
+
+struct Quat {  // unit quaternion (w, x, y, z)
+    double w = 1, x = 0, y = 0, z = 0;
+    Vec3 rotate(const Vec3& v) const;
+};
+
+// Reference mesh (the reference's Mesh + BasicNodePatch list).
+struct Mesh {
+    std::vector<Vec3> verts;
+    std::vector<std::array<int, 3>> tris;          // outward CCW
+    std::vector<std::array<double, 4>> charges;    // x, y, z, q
+    std::vector<std::vector<int>> vert_tris;       // triangles around each vertex
+    // node patch per vertex (node_patch.cpp:55-130)
+    std::vector<Vec3> centroid, normal;
+    std::vector<double> area;                      // "bezier_area" of a planar patch
+    Vec3 centre;                                   // centroid of charges (mesh.cpp:1654)
+    double radius = 0;                             // max |patch centroid - centre|
+
+    void finalize();                               // adjacency, patches, centre, radius
+    // constant-basis points of one node patch (triangle.h:388-479): for each
+    // adjacent triangle, the two sub-triangles (centre, vertex, edge midpoint)
+    void patch_points(int vertex, const Rule& rule, std::vector<QuadPt>& out) const;
+};
+
+// Star-shaped surface r(theta, phi) = r0 * (1 + sum_{l=2..lmax} sum_m a_lm Y_lm)
+// with real orthonormal Y_lm and a_lm ~ N(0, sigma0 / l) (standard deviation).
+struct StarShape {
+    double r0 = 1;
+    int lmax = 1;
+    std::vector<double> a;  // indexed l*l + l + m
+    double radius(const Vec3& dir) const;
+};
+StarShape star_shape(double r0, int lmax, double sigma0, uint64_t seed);
+
+Mesh make_icosphere(int subdivisions, double radius);
+Mesh make_star(int subdivisions, const StarShape& shape);
+
+// seeded charge generators
+void charges_uniform_ball(Mesh& m, int n, double rmax, double qmin, double qmax, uint64_t seed);
+void charges_random_walk(Mesh& m, const StarShape& shape, int n, double step, double clearance,
+                         double qsigma, uint64_t seed);
+
+// Deterministic random numbers (splitmix64) shared by the generators.
+struct Rng {
+    uint64_t s;
+    explicit Rng(uint64_t seed) : s(seed * 0x9E3779B97F4A7C15ull + 0x1234567ull) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+    Quat quaternion() {
+        // uniform random rotation (Shoemake)
+        double u1 = uniform(), u2 = uniform(), u3 = uniform();
+        double a = std::sqrt(1 - u1), b = std::sqrt(u1);
+        const double tp = 6.283185307179586;
+        return Quat{a * std::sin(tp * u2), a * std::cos(tp * u2), b * std::sin(tp * u3),
+                    b * std::cos(tp * u3)};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Adaptive octree over patch centroids.  Restates src/fmm/octree.h with the
+// BEM-FMM policy of BEEP::init_fmm_octree (beep.cpp:1585-1641): relaxed
+// insertion, build_neighbourhoods, optimize_neighbourhoods, remove_empty_nodes.
+struct OctNode {
+    int level;
+    uint32_t ix, iy, iz;
+    Vec3 centre;
+    double edge;
+    int parent = -1;
+    int children[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    bool has_children = false, is_leaf = true, is_shadow = false, is_deleted = false;
+    bool removed = false;
+    long cached_nb = -1;
+    std::vector<int> contents;   // patch indices, insertion order
+};
+
+struct Octree {
+    Vec3 centre;
+    double edge = 0;
+    int nbsize = 1500;
+    int max_depth = 20;
+    std::vector<OctNode> nodes;
+    std::vector<std::map<uint64_t, int>> levels;   // per level: key -> node id
+
+    static uint64_t key(uint32_t x, uint32_t y, uint32_t z) {
+        return (uint64_t(x) << 42) | (uint64_t(y) << 21) | uint64_t(z);
+    }
+    int find(int level, long x, long y, long z) const;
+    int neighbour(int node, int nb) const;         // -1 when absent / outside
+    void build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb);
+
+    // flattened result: leaves (non-shadow, non-empty) with their contents and
+    // neighbourhood contents (27 same-level boxes, octree.h:627-648)
+    std::vector<int> leaf_nodes;
+    void leaves_csr(std::vector<int>& leaf_ptr, std::vector<int>& leaf_idx,
+                    std::vector<int>& nb_ptr, std::vector<int>& nb_idx) const;
+    long bottom_level() const { return long(levels.size()) - 1; }
+
+  private:
+    std::vector<Vec3> const* pts_ = nullptr;
+    int new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& c, double e, int parent);
+    void subdivide(int n);
+    void subdivide_neighbourhood(int n);
+    void add(int item, int node);
+    long count_nb(int n) const;
+    long cached_nb(int n);
+    double interacts_per_item(long total_items);
+};
+
+}  // namespace beep

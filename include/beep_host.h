@@ -1,0 +1,95 @@
+/*
+ * beep_host.h -- host-side scene construction (C ABI).
+ *
+ * The caller side of the BEM mat-vec: reference meshes, rigid mesh instances,
+ * node patches with their qualocation / quadrature / galerkin point sets, and
+ * the adaptive octree that fixes the explicit (near-field) pair list.  These
+ * replace, for the hot path, the reference's host objects:
+ *   Mesh / MeshInstance            src/bem/mesh.h, src/bem/mesh_instance.h:23-87
+ *   BasicNodePatch point sets      src/bem/node_patch.cpp:55-273
+ *   BEEP::calculate_bounding_cube  src/beep/beep.cpp:1652-1693
+ *   BEEP::init_fmm_octree          src/beep/beep.cpp:1585-1641 (octree.h)
+ * and emit the flat arrays consumed by the device boundary (beep_hip.h).
+ *
+ * All functions return 0 on success and a negative code on error; the text of
+ * the last error on the calling thread is available from beep_host_last_error().
+ */
+#ifndef BEEP_HOST_H
+#define BEEP_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct beep_mesh beep_mesh;
+typedef struct beep_scene beep_scene;
+
+const char* beep_host_last_error(void);
+
+/* ---- reference meshes (planar node patches, one per vertex) ---- */
+int beep_mesh_icosphere(int subdivisions, double radius, beep_mesh** out);
+/* r(theta,phi) = r0 (1 + sum_{l=2..lmax} a_lm Y_lm), a_lm ~ N(0, sigma0/l) */
+int beep_mesh_star(int subdivisions, double r0, int lmax, double sigma0, uint64_t seed,
+                   beep_mesh** out);
+/* charges: xyzq = n x (x, y, z, q) in the mesh frame */
+int beep_mesh_set_charges(beep_mesh* m, int n, const double* xyzq);
+int beep_mesh_charges_ball(beep_mesh* m, int n, double rmax, double qmin, double qmax,
+                           uint64_t seed);
+/* random-walk chain inside a star mesh (needs the mesh from beep_mesh_star) */
+int beep_mesh_charges_walk(beep_mesh* m, int n, double step, double clearance, double qsigma,
+                           uint64_t seed);
+/* counts and the reference-frame centre (charge centroid) and radius */
+int beep_mesh_info(const beep_mesh* m, int* n_vertices, int* n_triangles, int* n_charges,
+                   double* centre3, double* radius);
+int beep_mesh_vertices(const beep_mesh* m, double* xyz, int* tri3);
+void beep_mesh_destroy(beep_mesh* m);
+
+/* ---- scene: BEEP(Dsolvent, kappa, quad, qual, nbsize) + instances ---- */
+int beep_scene_create(double Dsolvent, double kappa, int quad_rule, int qual_rule, int nbsize,
+                      beep_scene** out);
+/* pose: rotation quaternion (w,x,y,z) about the mesh centre, then the centre
+   is placed at offset3 (MeshInstance, mesh_instance.cpp:23-87) */
+int beep_scene_add_instance(beep_scene* s, const beep_mesh* m, const double* quat4,
+                            const double* offset3, double Dprotein);
+/* patches, bounding cube, octree, point sets */
+int beep_scene_build(beep_scene* s);
+
+typedef struct {
+    int n_patches;
+    int n_instances;
+    int n_charges;
+    int n_qual;        /* total qualocation points */
+    int n_quad;        /* total quadrature points */
+    int n_galerkin;    /* total galerkin (4-point rule) points */
+    int n_leaves;
+    int n_near_cols;   /* sum over leaves of the neighbourhood size */
+    long long n_near_pairs; /* sum over leaves of |leaf| * |neighbourhood| */
+    int tree_depth;    /* bottom level of the octree */
+    double cube_centre[3];
+    double cube_edge;
+} beep_scene_sizes;
+int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* out);
+
+/* exports (caller-allocated, sizes from beep_scene_get_sizes) */
+/* centroid3, normal3: 3N; area, eps_ratio (Dsolvent/Dprotein), gc: N; inst: N */
+int beep_scene_patches(const beep_scene* s, double* centroid3, double* normal3, double* area,
+                       double* eps_ratio, double* gc, int* instance);
+/* kind 0 = qualocation, 1 = quadrature, 2 = galerkin;
+   ptr: N+1, pts: n x 7 (x, y, z, nx, ny, nz, w) */
+int beep_scene_points(const beep_scene* s, int kind, int* ptr, double* pts);
+/* leaf_ptr: L+1, leaf_idx: N, nb_ptr: L+1, nb_idx: n_near_cols */
+int beep_scene_tree(const beep_scene* s, int* leaf_ptr, int* leaf_idx, int* nb_ptr, int* nb_idx);
+/* posed charges: xyzq n x 4, charge_instance n */
+int beep_scene_charges(const beep_scene* s, double* xyzq, int* charge_instance);
+/* per-instance: Dprotein, first patch, first charge (n_instances + 1 entries for
+   the two offsets) */
+int beep_scene_instances(const beep_scene* s, double* Dprotein, int* patch_ptr, int* charge_ptr);
+void beep_scene_destroy(beep_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEEP_HOST_H */
