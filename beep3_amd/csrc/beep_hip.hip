@@ -1,0 +1,677 @@
+// Device context and the C ABI of include/beep_hip.h: geometry and tree
+// upload, precompute, mat-vec (do_bem_fmm_iteration), RHS, energy
+// coefficients and the device GMRES / iterative-refinement driver.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/beep_hip.h"
+#include "device.hpp"
+
+using namespace beepdev;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // grow-only: keeps the buffer when it is already large enough
+    void alloc(size_t count) {
+        if (p && count <= n) return;
+        reset();
+        if (count == 0) return;
+        hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) hip_check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s),
+                             "hipMemcpyAsync H2D");
+    }
+};
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct beep_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // geometry
+    int N = 0;
+    DevBuf<Patch64> patches;
+    DevBuf<double> gc;
+    DevBuf<int> qual_ptr, quad_ptr, gal_ptr;
+    DevBuf<Pt64> qual64, quad64, gal64;
+    DevBuf<Pt32> qual32, quad32;
+    bool have_geom = false;
+
+    // tree
+    int L = 0;
+    std::vector<int> h_leaf_ptr, h_nb_ptr;
+    DevBuf<int> leaf_ptr, leaf_idx, nb_ptr, nb_idx;
+    DevBuf<int64_t> coef_off;
+    DevBuf<int> coef_blk, spmv_blk;
+    int n_coef_blocks = 0, n_spmv_blocks = 0;
+    int64_t nnz = 0;
+    bool have_tree = false;
+
+    // precomputed operator
+    DevBuf<double> diag;
+    DevBuf<float4> coef32;
+    DevBuf<double2> coef64;
+    bool have32 = false, have64 = false, have_diag = false;
+    double kappa = 0;
+
+    // per mat-vec scratch
+    DevBuf<Src64> src;
+    DevBuf<double> xbuf, ybuf;
+
+    // GMRES workspace
+    DevBuf<double> V, hvec, partial, wtmp, bvec, rvec, dvec;
+    int part_blocks = 0;
+
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    beep_matvec_timing last_timing{};
+    bool timing_pending = false;
+};
+
+namespace {
+
+template <typename F>
+int guarded(beep_hip_ctx* ctx, F&& f) {
+    if (!ctx) return -2;
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return -1;
+    } catch (...) {
+        ctx->err = "unknown error";
+        return -1;
+    }
+}
+
+void require(bool cond, const char* msg) {
+    if (!cond) throw std::runtime_error(msg);
+}
+
+void set_device(beep_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
+
+// Convert 7-double points of a CSR point set into local-frame records.
+template <typename T>
+void to_local(int N, const int* ptr, const double* pts, const double* centroid,
+              std::vector<Pt<T>>& out) {
+    const int total = ptr[N];
+    out.resize(total);
+    for (int i = 0; i < N; ++i)
+        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+            const double* d = pts + 7 * size_t(k);
+            Pt<T> p;
+            p.x = T(d[0] - centroid[3 * i]);
+            p.y = T(d[1] - centroid[3 * i + 1]);
+            p.z = T(d[2] - centroid[3 * i + 2]);
+            p.w = T(d[6]);
+            p.nx = T(d[3]);
+            p.ny = T(d[4]);
+            p.nz = T(d[5]);
+            p.pad = T(0);
+            out[k] = p;
+        }
+}
+
+void check_ptr(int N, const int* ptr, const char* what) {
+    require(ptr != nullptr, what);
+    require(ptr[0] == 0, what);
+    for (int i = 0; i < N; ++i) require(ptr[i + 1] >= ptr[i], what);
+}
+
+void timing_collect(beep_hip_ctx* c) {
+    if (!c->timing_pending) return;
+    float a = 0, b = 0;
+    hip_check(hipEventSynchronize(c->ev[2]), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "hipEventElapsedTime");
+    c->last_timing.near_ms = a;
+    c->last_timing.far_ms = b;
+    c->last_timing.total_ms = double(a) + double(b);
+    c->timing_pending = false;
+}
+
+void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
+    require(c->have_geom && c->have_tree, "geometry and tree must be set before a mat-vec");
+    require(c->have_diag, "precompute before the mat-vec");
+    require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
+            "unknown precision");
+    if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
+    else require(c->have32, "FP32 coefficients not precomputed");
+    hipStream_t s = c->stream;
+    hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
+    const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
+    launch_near_spmv(s, mode, c->N, c->n_spmv_blocks, c->L, c->spmv_blk.p, c->leaf_ptr.p,
+                     c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p, c->coef_off.p, c->coef32.p,
+                     c->coef64.p, c->diag.p, dx, dy);
+    hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
+    c->src.alloc(size_t(c->N));
+    launch_far_pack(s, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
+    launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
+                      c->src.p, c->kappa, dy);
+    hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
+    c->last_timing.near_launches = 1;
+    c->last_timing.far_launches = 2;
+    c->timing_pending = true;
+}
+
+double dev_dot(beep_hip_ctx* c, const double* a, const double* b, int n) {
+    c->partial.alloc(size_t(c->part_blocks));
+    c->hvec.alloc(std::max<size_t>(c->hvec.n, 1));
+    launch_multi_dot(c->stream, n, a, 1, b, c->partial.p, c->part_blocks);
+    launch_reduce_partials(c->stream, 1, c->partial.p, c->part_blocks, c->hvec.p);
+    double r = 0;
+    hip_check(hipMemcpyAsync(&r, c->hvec.p, sizeof(double), hipMemcpyDeviceToHost, c->stream),
+              "D2H dot");
+    hip_check(hipStreamSynchronize(c->stream), "sync dot");
+    return r;
+}
+
+// GMRES without restart (BEEP::gmres, beep.cpp:1343-1432) on device vectors:
+// r = K x - b, classical Gram-Schmidt, Givens on the host, stop when
+// |y_j| <= eps |b|, x -= V y.  Returns inner iterations; adds mat-vecs.
+int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double eps, int m,
+          int* matvecs, bool* converged, double* est_resid, bool x_is_zero) {
+    const int n = 2 * c->N;
+    hipStream_t s = c->stream;
+    c->V.alloc(size_t(n) * (m + 1));
+    c->hvec.alloc(size_t(m + 2));
+    c->partial.alloc(size_t(m + 2) * c->part_blocks);
+    c->wtmp.alloc(size_t(m + 1));
+    auto NEAR0 = [](double v) { return v == 0 ? 2.2250738585072014e-308 : v; };
+    const double nrm2b = std::sqrt(dev_dot(c, d_b, d_b, n));
+    double* v0 = c->V.p;
+    if (x_is_zero) {
+        // K 0 - b = -b without a mat-vec
+        launch_scale_copy(s, n, d_b, nullptr, -1.0, v0);
+    } else {
+        matvec_dev(c, d_x, v0, precision);
+        ++*matvecs;
+        launch_axpby(s, n, -1.0, d_b, 1.0, v0);
+    }
+    const double beta = std::sqrt(dev_dot(c, v0, v0, n));
+    launch_scale_copy(s, n, v0, nullptr, 1.0 / NEAR0(beta), v0);
+    std::vector<double> U(size_t(m) * (m + 1) / 2 + 1, 0.0), y(m + 2, 0.0), cs(m + 1), sn(m + 1),
+        hh(m + 2);
+    y[0] = beta;
+    int j = 0, uij = 0;
+    if (beta == 0) {
+        *converged = true;
+        *est_resid = 0;
+        return 0;
+    }
+    do {
+        const int u0j = uij;
+        double* vj = c->V.p + size_t(j) * n;
+        double* vn = c->V.p + size_t(j + 1) * n;
+        matvec_dev(c, vj, vn, precision);
+        ++*matvecs;
+        launch_multi_dot(s, n, c->V.p, j + 1, vn, c->partial.p, c->part_blocks);
+        launch_reduce_partials(s, j + 1, c->partial.p, c->part_blocks, c->hvec.p);
+        launch_gemv_sub(s, n, c->V.p, j + 1, c->hvec.p, vn);
+        launch_multi_dot(s, n, vn, 1, vn, c->partial.p + size_t(j + 1) * c->part_blocks,
+                         c->part_blocks);
+        launch_reduce_partials(s, 1, c->partial.p + size_t(j + 1) * c->part_blocks,
+                               c->part_blocks, c->hvec.p + j + 1);
+        hip_check(hipMemcpyAsync(hh.data(), c->hvec.p, sizeof(double) * (j + 2),
+                                 hipMemcpyDeviceToHost, s),
+                  "D2H arnoldi");
+        hip_check(hipStreamSynchronize(s), "sync arnoldi");
+        const double h = std::sqrt(std::max(0.0, hh[j + 1]));
+        launch_scale_copy(s, n, vn, nullptr, 1.0 / NEAR0(h), vn);
+        for (int q = 0; q <= j; ++q) U[u0j + q] = hh[q];
+        for (int i = 0; i < j; ++i) {
+            const double tmp = cs[i] * U[uij] - sn[i] * U[uij + 1];
+            U[uij + 1] = sn[i] * U[uij] + cs[i] * U[uij + 1];
+            U[uij] = tmp;
+            ++uij;
+        }
+        const double rd = U[uij];
+        const double dd = std::sqrt(rd * rd + h * h);
+        cs[j] = rd / NEAR0(dd);
+        sn[j] = -h / NEAR0(dd);
+        U[uij] = dd;
+        ++uij;
+        y[j + 1] = sn[j] * y[j];
+        y[j] = cs[j] * y[j];
+        ++j;
+    } while (j < m && std::fabs(y[j]) > eps * nrm2b);
+    *est_resid = std::fabs(y[j]);
+    *converged = std::fabs(y[j]) <= eps * nrm2b;
+    double su = 0, sy = 0;
+    for (int k = 0; k < uij; ++k) su += std::fabs(U[k]);
+    for (int k = 0; k < j; ++k) sy += std::fabs(y[k]);
+    if (!(su == 0 && sy == 0)) {
+        // packed column-major upper triangle: (row i, col q) at q(q+1)/2 + i
+        for (int i = j - 1; i >= 0; --i) {
+            double acc = y[i];
+            for (int q = i + 1; q < j; ++q) acc -= U[size_t(q) * (q + 1) / 2 + i] * y[q];
+            y[i] = acc / U[size_t(i) * (i + 1) / 2 + i];
+        }
+    }
+    hip_check(hipMemcpyAsync(c->wtmp.p, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, s),
+              "H2D y");
+    launch_gemv_sub(s, n, c->V.p, j, c->wtmp.p, d_x);
+    hip_check(hipStreamSynchronize(s), "sync gmres");
+    return j;
+}
+
+}  // namespace
+
+extern "C" {
+
+static thread_local std::string g_create_err = "no error";
+
+const char* beep_hip_last_error(const beep_hip_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int beep_hip_create(int device, beep_hip_ctx** out) {
+    if (!out) return -2;
+    *out = nullptr;
+    auto c = std::make_unique<beep_hip_ctx>();
+    c->device = device;
+    int rc = guarded(c.get(), [&] {
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        require(device >= 0 && device < count, "no such HIP device");
+        set_device(c.get());
+        hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
+        for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    });
+    if (rc != 0) {
+        // no context survives: the message is kept for beep_hip_last_error(NULL)
+        g_create_err = c->err;
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        return rc;
+    }
+    *out = c.release();
+    return 0;
+}
+
+int beep_hip_destroy(beep_hip_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* g) {
+    return guarded(ctx, [&] {
+        require(g && g->n_patches > 0, "empty geometry");
+        set_device(ctx);
+        const int N = g->n_patches;
+        require(g->centroid && g->normal && g->area && g->eps_ratio && g->gc, "null patch array");
+        check_ptr(N, g->qual_ptr, "bad qualocation pointer array");
+        check_ptr(N, g->quad_ptr, "bad quadrature pointer array");
+        check_ptr(N, g->gal_ptr, "bad galerkin pointer array");
+        require(g->qual_pts && g->quad_pts && g->gal_pts, "null point array");
+        for (int i = 0; i < N; ++i) {
+            require(g->qual_ptr[i + 1] > g->qual_ptr[i], "patch without qualocation points");
+            require(g->quad_ptr[i + 1] > g->quad_ptr[i], "patch without quadrature points");
+            require(g->gal_ptr[i + 1] > g->gal_ptr[i], "patch without galerkin points");
+            require(g->eps_ratio[i] > 0 && g->area[i] > 0, "non-positive area or eps ratio");
+        }
+        std::vector<Patch64> pr(N);
+        for (int i = 0; i < N; ++i)
+            pr[i] = Patch64{g->centroid[3 * i], g->centroid[3 * i + 1], g->centroid[3 * i + 2],
+                            g->area[i], g->normal[3 * i], g->normal[3 * i + 1],
+                            g->normal[3 * i + 2], 1.0 / g->eps_ratio[i]};
+        std::vector<Pt64> q64, d64, g64;
+        std::vector<Pt32> q32, d32;
+        to_local<double>(N, g->qual_ptr, g->qual_pts, g->centroid, q64);
+        to_local<double>(N, g->quad_ptr, g->quad_pts, g->centroid, d64);
+        to_local<double>(N, g->gal_ptr, g->gal_pts, g->centroid, g64);
+        to_local<float>(N, g->qual_ptr, g->qual_pts, g->centroid, q32);
+        to_local<float>(N, g->quad_ptr, g->quad_pts, g->centroid, d32);
+        hipStream_t s = ctx->stream;
+        ctx->patches.upload(pr.data(), N, s);
+        ctx->gc.upload(g->gc, N, s);
+        ctx->qual_ptr.upload(g->qual_ptr, N + 1, s);
+        ctx->quad_ptr.upload(g->quad_ptr, N + 1, s);
+        ctx->gal_ptr.upload(g->gal_ptr, N + 1, s);
+        ctx->qual64.upload(q64.data(), q64.size(), s);
+        ctx->quad64.upload(d64.data(), d64.size(), s);
+        ctx->gal64.upload(g64.data(), g64.size(), s);
+        ctx->qual32.upload(q32.data(), q32.size(), s);
+        ctx->quad32.upload(d32.data(), d32.size(), s);
+        hip_check(hipStreamSynchronize(s), "sync geometry upload");
+        ctx->N = N;
+        ctx->part_blocks = std::max(1, std::min(256, (2 * N + 1023) / 1024));
+        ctx->have_geom = true;
+        ctx->have_tree = false;
+        ctx->have32 = ctx->have64 = ctx->have_diag = false;
+    });
+}
+
+int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "set the geometry before the tree");
+        require(t && t->n_leaves > 0, "empty tree");
+        set_device(ctx);
+        const int N = ctx->N, L = t->n_leaves;
+        check_ptr(L, t->leaf_ptr, "bad leaf pointer array");
+        check_ptr(L, t->nb_ptr, "bad neighbourhood pointer array");
+        require(t->leaf_ptr[L] == N, "leaves must cover every patch exactly once");
+        std::vector<char> seen(N, 0);
+        for (int k = 0; k < N; ++k) {
+            const int i = t->leaf_idx[k];
+            require(i >= 0 && i < N && !seen[i], "leaf contents are not a permutation of the patches");
+            seen[i] = 1;
+        }
+        const int nnb = t->nb_ptr[L];
+        for (int k = 0; k < nnb; ++k)
+            require(t->nb_idx[k] >= 0 && t->nb_idx[k] < N, "neighbourhood index out of range");
+        std::vector<int64_t> off(L + 1, 0);
+        std::vector<int> cblk(L + 1, 0), sblk(L + 1, 0);
+        for (int l = 0; l < L; ++l) {
+            const int64_t R = t->leaf_ptr[l + 1] - t->leaf_ptr[l];
+            const int64_t C = t->nb_ptr[l + 1] - t->nb_ptr[l];
+            require(R > 0 && C > 0, "empty leaf or neighbourhood");
+            off[l + 1] = off[l] + R * C;
+            const int64_t nb = (R * C + 255) / 256;
+            require(int64_t(cblk[l]) + nb < (int64_t(1) << 31), "too many coefficient blocks");
+            cblk[l + 1] = cblk[l] + int(nb);
+            sblk[l + 1] = sblk[l] + int((R + 255) / 256);
+        }
+        hipStream_t s = ctx->stream;
+        ctx->leaf_ptr.upload(t->leaf_ptr, L + 1, s);
+        ctx->leaf_idx.upload(t->leaf_idx, N, s);
+        ctx->nb_ptr.upload(t->nb_ptr, L + 1, s);
+        ctx->nb_idx.upload(t->nb_idx, nnb, s);
+        ctx->coef_off.upload(off.data(), L + 1, s);
+        ctx->coef_blk.upload(cblk.data(), L + 1, s);
+        ctx->spmv_blk.upload(sblk.data(), L + 1, s);
+        hip_check(hipStreamSynchronize(s), "sync tree upload");
+        ctx->L = L;
+        ctx->nnz = off[L];
+        ctx->n_coef_blocks = cblk[L];
+        ctx->n_spmv_blocks = sblk[L];
+        ctx->h_leaf_ptr.assign(t->leaf_ptr, t->leaf_ptr + L + 1);
+        ctx->h_nb_ptr.assign(t->nb_ptr, t->nb_ptr + L + 1);
+        ctx->have_tree = true;
+        ctx->have32 = ctx->have64 = ctx->have_diag = false;
+        ctx->coef32.reset();
+        ctx->coef64.reset();
+    });
+}
+
+int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom && ctx->have_tree, "geometry and tree must be set");
+        require(kappa >= 0, "kappa must be >= 0");
+        set_device(ctx);
+        if (ctx->have_diag && kappa != ctx->kappa) {
+            ctx->have32 = ctx->have64 = false;
+            ctx->coef32.reset();
+            ctx->coef64.reset();
+        }
+        hipStream_t s = ctx->stream;
+        ctx->kappa = kappa;
+        ctx->diag.alloc(size_t(4) * ctx->N);
+        launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p, kappa,
+                        ctx->diag.p);
+        ctx->have_diag = true;
+        size_t freeb = 0, totalb = 0;
+        hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
+        if (precision == BEEP_FP64) {
+            const size_t need = size_t(ctx->nnz) * 2 * sizeof(double2);
+            require(ctx->coef64.p || need + (size_t(1) << 30) < freeb,
+                    "not enough device memory for FP64 coefficients");
+            ctx->coef64.alloc(size_t(ctx->nnz) * 2);
+            launch_near_coeffs_fp64(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p,
+                                    ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
+                                    ctx->nb_idx.p, ctx->coef_off.p, ctx->patches.p,
+                                    ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
+                                    ctx->quad64.p, kappa, 1, ctx->coef64.p);
+            ctx->have64 = true;
+        } else if (precision == BEEP_FP32 || precision == BEEP_FP32_RESID) {
+            const size_t need = size_t(ctx->nnz) * sizeof(float4);
+            require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
+                    "not enough device memory for FP32 coefficients");
+            ctx->coef32.alloc(size_t(ctx->nnz));
+            launch_near_coeffs_fp32(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p,
+                                    ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
+                                    ctx->nb_idx.p, ctx->coef_off.p, ctx->patches.p,
+                                    ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
+                                    ctx->quad32.p, float(kappa), 1, ctx->coef32.p);
+            ctx->have32 = true;
+        } else {
+            throw std::runtime_error("unknown precision");
+        }
+        hip_check(hipStreamSynchronize(s), "sync precompute");
+    });
+}
+
+int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* dx, double* dy, int precision) {
+    return guarded(ctx, [&] {
+        require(dx && dy && dx != dy, "mat-vec needs distinct device vectors");
+        set_device(ctx);
+        matvec_dev(ctx, dx, dy, precision);
+    });
+}
+
+int beep_hip_matvec(beep_hip_ctx* ctx, const double* x, double* y, int precision) {
+    return guarded(ctx, [&] {
+        require(x && y, "null vector");
+        set_device(ctx);
+        const size_t n = size_t(2) * ctx->N;
+        ctx->xbuf.alloc(n);
+        ctx->ybuf.alloc(n);
+        hip_check(hipMemcpyAsync(ctx->xbuf.p, x, n * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream), "H2D x");
+        matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
+        hip_check(hipMemcpyAsync(y, ctx->ybuf.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H y");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync matvec");
+    });
+}
+
+int beep_hip_matvec_timing(beep_hip_ctx* ctx, beep_matvec_timing* out) {
+    return guarded(ctx, [&] {
+        require(out != nullptr, "null output");
+        set_device(ctx);
+        timing_collect(ctx);
+        *out = ctx->last_timing;
+    });
+}
+
+int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
+                 double* b) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "geometry not set");
+        require(n_charges >= 0 && (n_charges == 0 || xyzq) && b && Dsolvent > 0, "bad rhs args");
+        set_device(ctx);
+        const int N = ctx->N;
+        DevBuf<double> q, out;
+        if (n_charges > 0) q.upload(xyzq, size_t(4) * n_charges, ctx->stream);
+        out.alloc(size_t(2) * N);
+        launch_rhs(ctx->stream, N, ctx->qual_ptr.p, ctx->qual64.p, ctx->patches.p, n_charges,
+                   q.p, FOUR_PI_INV / Dsolvent, out.p);
+        hip_check(hipMemcpyAsync(b, out.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H rhs");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync rhs");
+    });
+}
+
+int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_inst, const int* ipp, const int* icp,
+                                 const double* xyzq, double* ec_f, double* ec_h) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "geometry not set");
+        require(n_inst > 0 && ipp && icp && ec_f && ec_h, "bad energy args");
+        require(ipp[0] == 0 && ipp[n_inst] == ctx->N && icp[0] == 0, "bad instance offsets");
+        set_device(ctx);
+        const int N = ctx->N;
+        std::vector<int> pinst(N);
+        for (int k = 0; k < n_inst; ++k) {
+            require(ipp[k + 1] >= ipp[k] && icp[k + 1] >= icp[k], "bad instance offsets");
+            for (int i = ipp[k]; i < ipp[k + 1]; ++i) pinst[i] = k;
+        }
+        const int nch = icp[n_inst];
+        DevBuf<double> q, ef, eh;
+        DevBuf<int> pi, cp;
+        if (nch > 0) q.upload(xyzq, size_t(4) * nch, ctx->stream);
+        pi.upload(pinst.data(), N, ctx->stream);
+        cp.upload(icp, n_inst + 1, ctx->stream);
+        ef.alloc(N);
+        eh.alloc(N);
+        launch_energy_coeffs(ctx->stream, N, ctx->gal_ptr.p, ctx->gal64.p, ctx->patches.p, pi.p,
+                             cp.p, q.p, ef.p, eh.p);
+        hip_check(hipMemcpyAsync(ec_f, ef.p, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H ec_f");
+        hip_check(hipMemcpyAsync(ec_h, eh.p, sizeof(double) * N, hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H ec_h");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync energy");
+    });
+}
+
+int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_solve_opts* o,
+                   beep_solve_stats* st) {
+    return guarded(ctx, [&] {
+        require(b && x && o, "bad solve args");
+        require(o->max_iter > 0 && o->tol > 0, "bad solve options");
+        set_device(ctx);
+        const int n = 2 * ctx->N;
+        hipStream_t s = ctx->stream;
+        beep_solve_stats stats{};
+        const double t0 = now_ms();
+        ctx->bvec.alloc(n);
+        ctx->dvec.alloc(n);
+        ctx->rvec.alloc(n);
+        ctx->xbuf.alloc(n);
+        hip_check(hipMemcpyAsync(ctx->bvec.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s), "H2D b");
+        hip_check(hipMemcpyAsync(ctx->xbuf.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s), "H2D x");
+        bool x_zero = true;
+        for (int k = 0; k < n && x_zero; ++k) x_zero = (x[k] == 0.0);
+        const double bnorm = std::sqrt(dev_dot(ctx, ctx->bvec.p, ctx->bvec.p, n));
+        int matvecs = 0;
+        if (o->mode == BEEP_SOLVE_FP64 || o->mode == BEEP_SOLVE_FP32) {
+            const int prec = o->mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32;
+            bool conv = false;
+            double est = 0;
+            stats.iterations = gmres(ctx, prec, ctx->bvec.p, ctx->xbuf.p, o->tol, o->max_iter,
+                                     &matvecs, &conv, &est, x_zero);
+            stats.converged = conv;
+            stats.residual = est;
+            stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
+        } else if (o->mode == BEEP_SOLVE_MIXED) {
+            require(o->max_refine > 0 && o->inner_tol > 0, "bad refinement options");
+            bool done = false;
+            double rn = 0;
+            for (int k = 0; k <= o->max_refine; ++k) {
+                // r = b - K x with the FP64 residual operator
+                if (k == 0 && x_zero) {
+                    hip_check(hipMemcpyAsync(ctx->rvec.p, ctx->bvec.p, sizeof(double) * n,
+                                             hipMemcpyDeviceToDevice, s), "D2D r");
+                } else {
+                    matvec_dev(ctx, ctx->xbuf.p, ctx->rvec.p, BEEP_FP32_RESID);
+                    ++matvecs;
+                    launch_axpby(s, n, 1.0, ctx->bvec.p, -1.0, ctx->rvec.p);
+                }
+                rn = std::sqrt(dev_dot(ctx, ctx->rvec.p, ctx->rvec.p, n));
+                if (o->verbose)
+                    fprintf(stderr, "refine %d: |r|/|b| = %.3e\n", k, bnorm > 0 ? rn / bnorm : 0.0);
+                if (rn <= o->tol * bnorm) {
+                    done = true;
+                    break;
+                }
+                if (k == o->max_refine) break;
+                launch_fill(s, n, 0.0, ctx->dvec.p);
+                bool conv = false;
+                double est = 0;
+                stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, o->inner_tol,
+                                          o->max_iter, &matvecs, &conv, &est, true);
+                launch_axpby(s, n, 1.0, ctx->dvec.p, 1.0, ctx->xbuf.p);
+                stats.refinements = k + 1;
+            }
+            stats.converged = done;
+            stats.residual = rn;
+            stats.rel_residual = bnorm > 0 ? rn / bnorm : 0;
+        } else {
+            throw std::runtime_error("unknown solve mode");
+        }
+        hip_check(hipMemcpyAsync(x, ctx->xbuf.p, sizeof(double) * n, hipMemcpyDeviceToHost, s), "D2H x");
+        hip_check(hipStreamSynchronize(s), "sync solve");
+        stats.matvecs = matvecs;
+        stats.time_ms = now_ms() - t0;
+        if (st) *st = stats;
+    });
+}
+
+int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out) {
+    return guarded(ctx, [&] {
+        require(out != nullptr, "null output");
+        set_device(ctx);
+        hip_check(hipMalloc(out, bytes), "hipMalloc");
+    });
+}
+
+int beep_hip_device_free(beep_hip_ctx* ctx, void* p) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        if (p) hip_check(hipFree(p), "hipFree");
+    });
+}
+
+int beep_hip_memcpy(beep_hip_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                          : kind == 2 ? hipMemcpyDeviceToHost
+                          : kind == 3 ? hipMemcpyDeviceToDevice
+                                      : hipMemcpyDefault;
+        require(kind >= 1 && kind <= 3, "memcpy kind must be 1, 2 or 3");
+        hip_check(hipMemcpyAsync(dst, src, bytes, k, ctx->stream), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync memcpy");
+    });
+}
+
+int beep_hip_synchronize(beep_hip_ctx* ctx) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int beep_hip_memory_info(beep_hip_ctx* ctx, size_t* free_bytes, size_t* total_bytes) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        size_t f = 0, t = 0;
+        hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// Internal declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace beepdev {
+
+constexpr double FOUR_PI_INV = 0.07957747154594767;  // 1 / (4 pi)
+
+// One point of a node patch, stored relative to the centroid of the patch it
+// belongs to (the "local frame": distances between nearby points are then
+// formed from small numbers, which keeps FP32 differences accurate).
+template <typename T>
+struct alignas(16) Pt {
+    T x, y, z, w;     // offset from own patch centroid, weight
+    T nx, ny, nz, pad;
+};
+using Pt32 = Pt<float>;
+using Pt64 = Pt<double>;
+
+// Patch record: centroid and area, unit normal and 1/eps.
+struct alignas(16) Patch64 {
+    double cx, cy, cz, area;
+    double nx, ny, nz, inv_eps;
+};
+
+// Far-field source record (FP64): position + 1/eps, dipole vector a f n and
+// monopole a h.
+struct alignas(16) Src64 {
+    double x, y, z, inv_eps;
+    double dx, dy, dz, sh;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- launchers (defined in the .hip files) ----
+void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,
+                     const int* gal_ptr, const Pt64* gal, double kappa, double* diag4);
+
+// near-field coefficients; coef points to float4 (FP32) or double2 pairs (FP64)
+void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
+                             const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
+                             const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
+                             const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
+                             const Pt32* quad, float kappa, int subtract_1pt, float4* coef);
+void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
+                             const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
+                             const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
+                             const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
+                             const Pt64* quad, double kappa, int subtract_1pt, double2* coef);
+
+// y = near(x) + diag(x), overwriting y.  mode: 0 FP64 coef/FP64 sums,
+// 1 FP32 coef/FP32 sums, 2 FP32 coef/FP64 sums
+void launch_near_spmv(hipStream_t s, int mode, int n, int n_blocks, int n_leaves,
+                      const int* blk_ptr, const int* leaf_ptr, const int* leaf_idx,
+                      const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
+                      const float4* coef32, const double2* coef64, const double* diag4,
+                      const double* x, double* y);
+
+// far field: pack sources from x, then y += full 1-point sum over all pairs
+void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,
+                     const double* x, Src64* src);
+void launch_far_direct(hipStream_t s, int fp32, int n, const int* order, const Patch64* patches,
+                       const Src64* src, double kappa, double* y);
+
+// RHS and energy coefficients (FP64 direct sums)
+void launch_rhs(hipStream_t s, int n, const int* qual_ptr, const Pt64* qual,
+                const Patch64* patches, int n_charges, const double* xyzq, double scale,
+                double* b);
+void launch_energy_coeffs(hipStream_t s, int n, const int* gal_ptr, const Pt64* gal,
+                          const Patch64* patches, const int* patch_inst,
+                          const int* inst_charge_ptr, const double* xyzq, double* ec_f,
+                          double* ec_h);
+
+// vector kernels for the GMRES driver
+void launch_multi_dot(hipStream_t s, int n, const double* V, int ncols, const double* w,
+                      double* partial, int n_part_blocks);
+void launch_reduce_partials(hipStream_t s, int ncols, const double* partial, int n_part_blocks,
+                            double* out);
+void launch_gemv_sub(hipStream_t s, int n, const double* V, int ncols, const double* h,
+                     double* w);
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
+void launch_fill(hipStream_t s, int n, double v, double* y);
+void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
+                       double scale, double* y);
+
+}  // namespace beepdev

@@ -1,0 +1,259 @@
+// Right-hand side, energy coefficients and the FP64 vector kernels of the
+// device GMRES driver.
+#include "device.hpp"
+
+namespace beepdev {
+
+// Coulomb potential and gradient (field = +grad phi, src/fmm/eval_pt.h:175-197)
+// of all charges at a point, FP64.  Charges are staged through LDS in tiles.
+constexpr int CH_TILE = 256;
+
+// f_rhs[i] = scale * sum_q w_q phi(x_q);  h_rhs[i] = scale * sum_q w_q grad phi . n_q
+// (BEEP::set_rhs, beep.cpp:1124-1260).  One thread per patch.
+__global__ __launch_bounds__(256) void k_rhs(int n, const int* __restrict__ qual_ptr,
+                                             const Pt64* __restrict__ qual,
+                                             const Patch64* __restrict__ patches, int nch,
+                                             const double* __restrict__ xyzq, double scale,
+                                             double* __restrict__ b) {
+    constexpr int QB = 8;  // qualocation points in flight per thread
+    __shared__ double4 ch[CH_TILE];
+    __shared__ int max_chunks;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    const int q0 = valid ? qual_ptr[i] : 0, q1 = valid ? qual_ptr[i + 1] : 0;
+    if (threadIdx.x == 0) max_chunks = 0;
+    __syncthreads();
+    atomicMax(&max_chunks, (q1 - q0 + QB - 1) / QB);
+    __syncthreads();
+    const int nchunks = max_chunks;  // same trip count for every thread of the block
+    const Patch64 p = valid ? patches[i] : Patch64{0, 0, 0, 0, 0, 0, 0, 0};
+    double fr = 0, hr = 0;
+    for (int chk = 0; chk < nchunks; ++chk) {
+        const int qb = q0 + chk * QB;
+        const int nq = max(0, min(QB, q1 - qb));
+        double px[QB], py[QB], pz[QB], pot[QB], gx[QB], gy[QB], gz[QB];
+        for (int a = 0; a < QB; ++a) {
+            pot[a] = gx[a] = gy[a] = gz[a] = 0;
+            px[a] = py[a] = pz[a] = 0;
+            if (a < nq) {
+                const Pt64 q = qual[qb + a];
+                px[a] = p.cx + q.x; py[a] = p.cy + q.y; pz[a] = p.cz + q.z;
+            }
+        }
+        for (int c0 = 0; c0 < nch; c0 += CH_TILE) {
+            __syncthreads();
+            if (c0 + threadIdx.x < nch) {
+                const double* c = xyzq + 4 * (c0 + threadIdx.x);
+                ch[threadIdx.x] = make_double4(c[0], c[1], c[2], c[3]);
+            }
+            __syncthreads();
+            const int m = min(CH_TILE, nch - c0);
+            for (int a = 0; a < nq; ++a) {
+                double pa = 0, xa = 0, ya = 0, za = 0;
+                for (int k = 0; k < m; ++k) {
+                    const double4 c = ch[k];
+                    const double rx = px[a] - c.x, ry = py[a] - c.y, rz = pz[a] - c.z;
+                    const double d2 = rx * rx + ry * ry + rz * rz;
+                    if (d2 <= 2.220446049250313e-16) continue;
+                    const double id = 1.0 / sqrt(d2);
+                    const double pt = c.w * id;
+                    pa += pt;
+                    const double ft = -pt * id * id;
+                    xa += ft * rx; ya += ft * ry; za += ft * rz;
+                }
+                pot[a] += pa; gx[a] += xa; gy[a] += ya; gz[a] += za;
+            }
+        }
+        for (int a = 0; a < nq; ++a) {
+            const Pt64 q = qual[qb + a];
+            fr += q.w * pot[a];
+            hr += q.w * (gx[a] * q.nx + gy[a] * q.ny + gz[a] * q.nz);
+        }
+    }
+    if (valid) {
+        b[i] = fr * scale;
+        b[n + i] = hr * scale;
+    }
+}
+
+void launch_rhs(hipStream_t s, int n, const int* qual_ptr, const Pt64* qual,
+                const Patch64* patches, int n_charges, const double* xyzq, double scale,
+                double* b) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_rhs, dim3((n + 255) / 256), dim3(256), 0, s, n, qual_ptr, qual, patches,
+                       n_charges, xyzq, scale, b);
+    hip_check(hipGetLastError(), "k_rhs launch");
+}
+
+// Energy coefficients (Mesh::init_energy_precalcs, mesh.cpp:730-887) before
+// the Gauss-law renormalisation: over the galerkin points of each patch,
+//   ec_f = sum w (grad phi . n) / 4pi,  ec_h = sum w phi / 4pi,
+// with phi from the charges of the patch's own instance.  One wave per patch,
+// lane = galerkin point.
+__global__ __launch_bounds__(64) void k_energy_coeffs(int n, const int* __restrict__ gal_ptr,
+                                                      const Pt64* __restrict__ gal,
+                                                      const Patch64* __restrict__ patches,
+                                                      const int* __restrict__ patch_inst,
+                                                      const int* __restrict__ inst_charge_ptr,
+                                                      const double* __restrict__ xyzq,
+                                                      double* __restrict__ ec_f,
+                                                      double* __restrict__ ec_h) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int g0 = gal_ptr[i], ng = gal_ptr[i + 1] - g0;
+    const int inst = patch_inst[i];
+    const int c0 = inst_charge_ptr[inst], c1 = inst_charge_ptr[inst + 1];
+    const Patch64 p = patches[i];
+    double ef = 0, eh = 0;
+    for (int a = threadIdx.x; a < ng; a += 64) {
+        const Pt64 q = gal[g0 + a];
+        const double x = p.cx + q.x, y = p.cy + q.y, z = p.cz + q.z;
+        double pot = 0, fx = 0, fy = 0, fz = 0;
+        for (int c = c0; c < c1; ++c) {
+            const double rx = x - xyzq[4 * c], ry = y - xyzq[4 * c + 1], rz = z - xyzq[4 * c + 2];
+            const double d2 = rx * rx + ry * ry + rz * rz;
+            if (d2 <= 2.220446049250313e-16) continue;
+            const double id = 1.0 / sqrt(d2);
+            const double pt = xyzq[4 * c + 3] * id;
+            pot += pt;
+            const double ft = -pt * id * id;
+            fx += ft * rx; fy += ft * ry; fz += ft * rz;
+        }
+        ef += (fx * q.nx + fy * q.ny + fz * q.nz) * q.w * FOUR_PI_INV;
+        eh += pot * q.w * FOUR_PI_INV;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ef += __shfl_down(ef, off, 64);
+        eh += __shfl_down(eh, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        ec_f[i] = ef;
+        ec_h[i] = eh;
+    }
+}
+
+void launch_energy_coeffs(hipStream_t s, int n, const int* gal_ptr, const Pt64* gal,
+                          const Patch64* patches, const int* patch_inst,
+                          const int* inst_charge_ptr, const double* xyzq, double* ec_f,
+                          double* ec_h) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_energy_coeffs, dim3(n), dim3(64), 0, s, n, gal_ptr, gal, patches,
+                       patch_inst, inst_charge_ptr, xyzq, ec_f, ec_h);
+    hip_check(hipGetLastError(), "k_energy_coeffs launch");
+}
+
+// ---- FP64 vector kernels for GMRES ----
+// partial[q * nb + blk] = sum over the block's rows of V[q*n + k] * w[k]
+__global__ __launch_bounds__(256) void k_multi_dot(int n, const double* __restrict__ V, int ncols,
+                                                   const double* __restrict__ w,
+                                                   double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int q = blockIdx.y;
+    const int nb = gridDim.x;
+    double acc = 0;
+    const double* v = V + size_t(q) * n;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += nb * 256) acc += v[k] * w[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[size_t(q) * nb + blockIdx.x] = red[0];
+}
+
+void launch_multi_dot(hipStream_t s, int n, const double* V, int ncols, const double* w,
+                      double* partial, int n_part_blocks) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_multi_dot, dim3(n_part_blocks, ncols), dim3(256), 0, s, n, V, ncols, w,
+                       partial);
+    hip_check(hipGetLastError(), "k_multi_dot launch");
+}
+
+__global__ __launch_bounds__(256) void k_reduce_partials(int ncols, const double* __restrict__ partial,
+                                                         int nb, double* __restrict__ out) {
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    double acc = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) acc += partial[size_t(q) * nb + k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+}
+
+void launch_reduce_partials(hipStream_t s, int ncols, const double* partial, int n_part_blocks,
+                            double* out) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(ncols), dim3(256), 0, s, ncols, partial,
+                       n_part_blocks, out);
+    hip_check(hipGetLastError(), "k_reduce_partials launch");
+}
+
+// w[k] -= sum_q h[q] V[q*n + k]
+__global__ __launch_bounds__(256) void k_gemv_sub(int n, const double* __restrict__ V, int ncols,
+                                                  const double* __restrict__ h,
+                                                  double* __restrict__ w) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    double acc = 0;
+    for (int q = 0; q < ncols; ++q) acc += h[q] * V[size_t(q) * n + k];
+    w[k] -= acc;
+}
+
+void launch_gemv_sub(hipStream_t s, int n, const double* V, int ncols, const double* h,
+                     double* w) {
+    if (n <= 0 || ncols <= 0) return;
+    hipLaunchKernelGGL(k_gemv_sub, dim3((n + 255) / 256), dim3(256), 0, s, n, V, ncols, h, w);
+    hip_check(hipGetLastError(), "k_gemv_sub launch");
+}
+
+// y = a x + b y
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b,
+                        double* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = a * x[k] + b * y[k];
+}
+
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, a, x, b, y);
+    hip_check(hipGetLastError(), "k_axpby launch");
+}
+
+// y = x * (scale_ptr ? 1 / *scale_ptr : scale); a zero divisor is replaced by
+// the smallest normal double (the NEAR0 guard of beep.cpp:1366)
+__global__ void k_scale_copy(int n, const double* __restrict__ x, const double* __restrict__ sp,
+                             double scale, double* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double sc = scale;
+    if (sp) {
+        const double v = *sp;
+        sc = 1.0 / (v == 0 ? 2.2250738585072014e-308 : v);
+    }
+    if (k < n) y[k] = x[k] * sc;
+}
+
+void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
+                       double scale, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scale_copy, dim3((n + 255) / 256), dim3(256), 0, s, n, x, scale_ptr,
+                       scale, y);
+    hip_check(hipGetLastError(), "k_scale_copy launch");
+}
+
+__global__ void k_fill(int n, double v, double* __restrict__ y) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = v;
+}
+
+void launch_fill(hipStream_t s, int n, double v, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, n, v, y);
+    hip_check(hipGetLastError(), "k_fill launch");
+}
+
+}  // namespace beepdev

@@ -1,0 +1,333 @@
+// Near-field kernels for gfx950: singular self terms, explicit-pair
+// coefficient precompute and the per-leaf block SpMV.
+//
+// Integrand: evaluate_local_BEM_kernels (src/bem/bem_kernels.h:134-251) in the
+// factored form
+//   A += w (G0 - u)
+//   B += -w (n.d) (gr3/eps - g1)
+//   C +=  w (n0.d) (gr3 - g1/eps)
+//   D -= w [ (3 gr5 - p) (n.d)(n0.d) - (gr3 - g1)(n.n0) ]      (kappa != 0)
+// with d = y - x, gr3 = 1/(4 pi r^3), g1 = e^{-kr}(1 + k r)/(4 pi r^3),
+// p = e^{-kr}(k^2 r^2 + 3 k r + 3)/(4 pi r^5), summed over the quadrature points
+// y of the column patch and the qualocation points x of the row patch, D
+// finally scaled by 1/eps_col.  Algebraically identical to the reference's
+// 3x3 loops over gdd - udd.
+#include "device.hpp"
+
+namespace beepdev {
+
+template <typename T>
+struct Math;
+template <>
+struct Math<float> {
+    static __device__ __forceinline__ float rsqrt_(float v) { return __frsqrt_rn(v); }
+    static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
+};
+template <>
+struct Math<double> {
+    static __device__ __forceinline__ double rsqrt_(double v) { return 1.0 / sqrt(v); }
+    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
+};
+
+// Accumulate A..D of one (qual point, quad point) pair into the per-qual-point
+// partial sums.  Returns without effect for coincident points (r^2 < 1e-9,
+// bem_kernels.h:185).
+template <typename T>
+__device__ __forceinline__ void integrand(T kappa, T inv_eps, T dx, T dy, T dz, T n0x, T n0y,
+                                          T n0z, T nx, T ny, T nz, T wt, T& A, T& B, T& C,
+                                          T& D) {
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < T(1e-9)) return;
+    const T ir = Math<T>::rsqrt_(r2);
+    const T r = r2 * ir;
+    const T ir2 = ir * ir;
+    const T ir3 = ir2 * ir;
+    const T K4 = T(FOUR_PI_INV);
+    const T e = Math<T>::exp_(-kappa * r) * K4;
+    const T gr1 = K4 * ir;
+    const T gr3 = K4 * ir3;
+    const T ur1 = e * ir;
+    const T ur3ur2 = e * (ir3 + kappa * ir2);
+    const T nd = nx * dx + ny * dy + nz * dz;
+    const T n0d = n0x * dx + n0y * dy + n0z * dz;
+    A += wt * (gr1 - ur1);
+    B -= wt * nd * (inv_eps * gr3 - ur3ur2);
+    C += wt * n0d * (gr3 - ur3ur2 * inv_eps);
+    if (kappa != T(0)) {
+        const T ir5 = ir3 * ir2;
+        const T gr5 = K4 * ir5;
+        const T p = e * (kappa * (T(3) * ir2 * ir2 + kappa * ir3) + T(3) * ir5);
+        const T nn0 = nx * n0x + ny * n0y + nz * n0z;
+        D -= wt * ((T(3) * gr5 - p) * nd * n0d - (gr3 - ur3ur2) * nn0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Singular self terms (singular_BEM_kernels, bem_kernels.h:18-132) in FP64
+// over the galerkin points of each patch, normalised by the summed weights,
+// plus the geometric correction (bem_kernels.cpp:123-136; beep.cpp:836-841).
+// One wave per patch; lane = source point.
+__global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restrict__ patches,
+                                                 const double* __restrict__ gc,
+                                                 const int* __restrict__ gal_ptr,
+                                                 const Pt64* __restrict__ gal, double kappa,
+                                                 double* __restrict__ diag4) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int p0 = gal_ptr[i], np = gal_ptr[i + 1] - p0;
+    const double inv_eps = patches[i].inv_eps;
+    double A = 0, B = 0, C = 0, D = 0, wsum = 0;
+    for (int s = threadIdx.x; s < np; s += 64) {
+        const Pt64 q = gal[p0 + s];
+        double a = 0, b = 0, c = 0, d = 0;
+        for (int t = 0; t < np; ++t) {
+            const Pt64 p = gal[p0 + t];
+            integrand<double>(kappa, inv_eps, p.x - q.x, p.y - q.y, p.z - q.z, q.nx, q.ny, q.nz,
+                              p.nx, p.ny, p.nz, p.w, a, b, c, d);
+        }
+        A += a * q.w;
+        B += b * q.w;
+        C += c * q.w;
+        D += d * q.w;
+        wsum += q.w;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        A += __shfl_down(A, off, 64);
+        B += __shfl_down(B, off, 64);
+        C += __shfl_down(C, off, 64);
+        D += __shfl_down(D, off, 64);
+        wsum += __shfl_down(wsum, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        const double e = 1.0 / inv_eps, g = gc[i];
+        diag4[4 * i + 0] = A / wsum;
+        diag4[4 * i + 1] = B / wsum + ((1.0 - g) + g / e);
+        diag4[4 * i + 2] = C / wsum - (g + (1.0 - g) / e);
+        diag4[4 * i + 3] = D * inv_eps / wsum;
+    }
+}
+
+void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,
+                     const int* gal_ptr, const Pt64* gal, double kappa, double* diag4) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_singular, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr, gal, kappa,
+                       diag4);
+    hip_check(hipGetLastError(), "k_singular launch");
+}
+
+// ---------------------------------------------------------------------------
+// Explicit-pair coefficients.  Leaf L owns an R x C block (R rows = patches
+// of the leaf, C columns = its neighbourhood) stored column-major at
+// coef_off[L]: entry e = c * R + r.  One thread per entry; a block covers 256
+// consecutive entries of one leaf (blk_ptr maps blocks to leaves).  The
+// thread integrates the qualocation points of row i against the quadrature
+// points of column j; with subtract_1pt it stores the difference to the
+// 1-point centroid interaction, which the far-field kernels add back for
+// every pair.
+__device__ __forceinline__ int find_leaf(const int* __restrict__ blk_ptr, int n_leaves, int b) {
+    int lo = 0, hi = n_leaves;  // blk_ptr[lo] <= b < blk_ptr[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (blk_ptr[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj, int qb, int qn,
+                                            const Pt<T>* __restrict__ qual, int pb, int pn,
+                                            const Pt<T>* __restrict__ quad, T kappa,
+                                            int subtract_1pt, T out[4]) {
+    // centroid difference in FP64, then the working precision
+    const T dcx = T(pj.cx - pi.cx), dcy = T(pj.cy - pi.cy), dcz = T(pj.cz - pi.cz);
+    const T inv_eps = T(pj.inv_eps);
+    T A = 0, B = 0, C = 0, D = 0;
+    for (int a = 0; a < qn; ++a) {
+        const Pt<T> q = qual[qb + a];
+        T a_ = 0, b_ = 0, c_ = 0, d_ = 0;
+        const T ox = dcx - q.x, oy = dcy - q.y, oz = dcz - q.z;
+#pragma unroll 4
+        for (int k = 0; k < pn; ++k) {
+            const Pt<T> p = quad[pb + k];
+            integrand<T>(kappa, inv_eps, ox + p.x, oy + p.y, oz + p.z, q.nx, q.ny, q.nz, p.nx,
+                         p.ny, p.nz, p.w, a_, b_, c_, d_);
+        }
+        A += a_ * q.w;
+        B += b_ * q.w;
+        C += c_ * q.w;
+        D += d_ * q.w;
+    }
+    if (subtract_1pt) {
+        T a_ = 0, b_ = 0, c_ = 0, d_ = 0;
+        integrand<T>(kappa, inv_eps, dcx, dcy, dcz, T(pi.nx), T(pi.ny), T(pi.nz), T(pj.nx),
+                     T(pj.ny), T(pj.nz), T(pj.area), a_, b_, c_, d_);
+        A -= a_;
+        B -= b_;
+        C -= c_;
+        D -= d_;
+    }
+    out[0] = A;
+    out[1] = B;
+    out[2] = C;
+    out[3] = D * inv_eps;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_near_coeffs(
+    int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
+    const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
+    const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,
+    const Patch64* __restrict__ patches, const int* __restrict__ qual_ptr,
+    const Pt<T>* __restrict__ qual, const int* __restrict__ quad_ptr,
+    const Pt<T>* __restrict__ quad, T kappa, int subtract_1pt, void* __restrict__ coef) {
+    const int b = blockIdx.x;
+    const int L = find_leaf(blk_ptr, n_leaves, b);
+    const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
+    const int np = nb_ptr[L], Cn = nb_ptr[L + 1] - np;
+    const int64_t e = int64_t(b - blk_ptr[L]) * 256 + threadIdx.x;
+    if (e >= int64_t(R) * Cn) return;
+    const int c = int(e / R);
+    const int r = int(e - int64_t(c) * R);
+    const int i = leaf_idx[lp + r];
+    const int j = nb_idx[np + c];
+    T out[4] = {0, 0, 0, 0};
+    if (i != j) {
+        const Patch64 pi = patches[i];
+        const Patch64 pj = patches[j];
+        const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+        const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
+        pair_coeffs<T>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, subtract_1pt, out);
+    }
+    const int64_t idx = coef_off[L] + e;
+    if constexpr (sizeof(T) == 4) {
+        reinterpret_cast<float4*>(coef)[idx] = make_float4(out[0], out[1], out[2], out[3]);
+    } else {
+        double2* c2 = reinterpret_cast<double2*>(coef);
+        c2[2 * idx] = make_double2(out[0], out[1]);
+        c2[2 * idx + 1] = make_double2(out[2], out[3]);
+    }
+}
+
+void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
+                             const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
+                             const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
+                             const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
+                             const Pt32* quad, float kappa, int subtract_1pt, float4* coef) {
+    if (n_blocks <= 0) return;
+    hipLaunchKernelGGL(k_near_coeffs<float>, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr,
+                       leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual,
+                       quad_ptr, quad, kappa, subtract_1pt, (void*)coef);
+    hip_check(hipGetLastError(), "k_near_coeffs<float> launch");
+}
+
+void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
+                             const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
+                             const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
+                             const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
+                             const Pt64* quad, double kappa, int subtract_1pt, double2* coef) {
+    if (n_blocks <= 0) return;
+    hipLaunchKernelGGL(k_near_coeffs<double>, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr,
+                       leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual,
+                       quad_ptr, quad, kappa, subtract_1pt, (void*)coef);
+    hip_check(hipGetLastError(), "k_near_coeffs<double> launch");
+}
+
+// ---------------------------------------------------------------------------
+// Block SpMV over the explicit pairs, plus the diagonal:
+//   y_f[i] = sum_j B_ij f_j - A_ij h_j + (B_ii' f_i - A_ii h_i)
+//   y_h[i] = sum_j D_ij f_j - C_ij h_j + (D_ii f_i - C_ii' h_i)
+// (LocalIntegrations::evaluate_local_contributions, local_integrations.h:123-130).
+// A block handles up to 256 rows of one leaf (blk_ptr maps blocks to leaves,
+// ceil(R/256) blocks per leaf).  The leaf's neighbourhood x values are
+// gathered into LDS in chunks; thread (r, s) walks columns s, s+S, ... of row
+// r, so each step of the block reads one contiguous run of coefficients.
+constexpr int SPMV_CHUNK = 1024;
+
+template <typename TC, typename TA>
+__global__ __launch_bounds__(256) void k_near_spmv(
+    int n, int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
+    const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
+    const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,
+    const void* __restrict__ coef, const double* __restrict__ diag4,
+    const double* __restrict__ x, double* __restrict__ y) {
+    __shared__ TA xs[2][SPMV_CHUNK];
+    __shared__ TA red[2][256];
+    const int b = blockIdx.x;
+    const int L = find_leaf(blk_ptr, n_leaves, b);
+    const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
+    const int np = nb_ptr[L], Cn = nb_ptr[L + 1] - np;
+    const int rg = (b - blk_ptr[L]) * 256;          // first row of this block
+    const int Rb = min(256, R - rg);                 // rows handled here
+    const int S = (R <= 256) ? (256 / R) : 1;        // column slices
+    const int t = threadIdx.x;
+    const int r = (R <= 256) ? (t % R) : t;
+    const int s = (R <= 256) ? (t / R) : 0;
+    const bool active = (R <= 256) ? (t < R * S) : (t < Rb);
+    const int64_t base = coef_off[L] + rg + r;
+    TA af = 0, ah = 0;
+    for (int c0 = 0; c0 < Cn; c0 += SPMV_CHUNK) {
+        const int cn = min(SPMV_CHUNK, Cn - c0);
+        __syncthreads();
+        for (int k = t; k < cn; k += 256) {
+            const int j = nb_idx[np + c0 + k];
+            xs[0][k] = TA(x[j]);
+            xs[1][k] = TA(x[n + j]);
+        }
+        __syncthreads();
+        if (active) {
+            for (int c = s; c < cn; c += S) {
+                const int64_t idx = base + int64_t(c0 + c) * R;
+                TA A, B, C, D;
+                if constexpr (sizeof(TC) == 4) {
+                    const float4 v = reinterpret_cast<const float4*>(coef)[idx];
+                    A = TA(v.x); B = TA(v.y); C = TA(v.z); D = TA(v.w);
+                } else {
+                    const double2* c2 = reinterpret_cast<const double2*>(coef);
+                    const double2 v0 = c2[2 * idx], v1 = c2[2 * idx + 1];
+                    A = v0.x; B = v0.y; C = v1.x; D = v1.y;
+                }
+                const TA xf = xs[0][c], xh = xs[1][c];
+                af += B * xf - A * xh;
+                ah += D * xf - C * xh;
+            }
+        }
+    }
+    red[0][t] = af;
+    red[1][t] = ah;
+    __syncthreads();
+    if (active && s == 0) {
+        for (int k = 1; k < S; ++k) {
+            af += red[0][k * R + r];
+            ah += red[1][k * R + r];
+        }
+        const int i = leaf_idx[lp + rg + r];
+        const double* d = diag4 + 4 * i;
+        const double xf = x[i], xh = x[n + i];
+        y[i] = double(af) + (d[1] * xf - d[0] * xh);
+        y[n + i] = double(ah) + (d[3] * xf - d[2] * xh);
+    }
+}
+
+void launch_near_spmv(hipStream_t s, int mode, int n, int n_blocks, int n_leaves,
+                      const int* blk_ptr, const int* leaf_ptr, const int* leaf_idx,
+                      const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
+                      const float4* coef32, const double2* coef64, const double* diag4,
+                      const double* x, double* y) {
+    if (n_blocks <= 0) return;
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_near_spmv<double, double>), dim3(n_blocks), dim3(256), 0, s, n,
+                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
+                           (const void*)coef64, diag4, x, y);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL((k_near_spmv<float, float>), dim3(n_blocks), dim3(256), 0, s, n,
+                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
+                           (const void*)coef32, diag4, x, y);
+    } else {
+        hipLaunchKernelGGL((k_near_spmv<float, double>), dim3(n_blocks), dim3(256), 0, s, n,
+                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
+                           (const void*)coef32, diag4, x, y);
+    }
+    hip_check(hipGetLastError(), "k_near_spmv launch");
+}
+
+}  // namespace beepdev

@@ -1,0 +1,148 @@
+"""Python view of the device boundary (include/beep_hip.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (BeepError, HipGeometry, HipTree, MatvecTiming, SolveOpts, SolveStats, check_hip,
+                   dp, ip, lib, vp)
+
+FP64, FP32, FP32_RESID = 0, 1, 2
+SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32_resid": FP32_RESID}
+SOLVE_MODES = {"fp64": SOLVE_FP64, "mixed": SOLVE_MIXED, "fp32": SOLVE_FP32}
+
+
+def _dp(a):
+    return a.ctypes.data_as(dp)
+
+
+def _ip(a):
+    return a.ctypes.data_as(ip)
+
+
+def _prec(p):
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
+
+
+class Device:
+    """One HIP context on one GPU (beep_hip_create)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        if not hasattr(L, "beep_hip_create"):
+            raise BeepError("libbeep3.so was built without the HIP boundary")
+        h = vp()
+        rc = L.beep_hip_create(device, C.byref(h))
+        if rc != 0:
+            raise BeepError(L.beep_hip_last_error(None).decode())
+        self._h = h
+        self.n = 0
+        self._keep = []
+
+    def _check(self, rc):
+        check_hip(self._h, rc)
+
+    def set_geometry(self, a: dict):
+        self.n = n = int(a["area"].shape[0])
+        keep = {k: np.ascontiguousarray(a[k]) for k in (
+            "centroid", "normal", "area", "eps", "gc", "qual_ptr", "qual", "quad_ptr", "quad",
+            "gal_ptr", "gal")}
+        g = HipGeometry()
+        g.n_patches = n
+        g.centroid, g.normal = keep["centroid"].ctypes.data, keep["normal"].ctypes.data
+        g.area, g.eps_ratio, g.gc = keep["area"].ctypes.data, keep["eps"].ctypes.data, keep["gc"].ctypes.data
+        g.qual_ptr, g.qual_pts = keep["qual_ptr"].ctypes.data, keep["qual"].ctypes.data
+        g.quad_ptr, g.quad_pts = keep["quad_ptr"].ctypes.data, keep["quad"].ctypes.data
+        g.gal_ptr, g.gal_pts = keep["gal_ptr"].ctypes.data, keep["gal"].ctypes.data
+        self._check(lib().beep_hip_set_geometry(self._h, C.byref(g)))
+
+    def set_tree(self, a: dict):
+        keep = {k: np.ascontiguousarray(a[k], dtype=np.int32) for k in (
+            "leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx")}
+        t = HipTree()
+        t.n_leaves = int(keep["leaf_ptr"].shape[0] - 1)
+        t.leaf_ptr, t.leaf_idx = keep["leaf_ptr"].ctypes.data, keep["leaf_idx"].ctypes.data
+        t.nb_ptr, t.nb_idx = keep["nb_ptr"].ctypes.data, keep["nb_idx"].ctypes.data
+        self._check(lib().beep_hip_set_tree(self._h, C.byref(t)))
+
+    def precompute(self, kappa: float, precision="fp32"):
+        self._check(lib().beep_hip_precompute(self._h, float(kappa), _prec(precision)))
+
+    def matvec(self, x, precision="fp64"):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (2 * self.n,):
+            raise ValueError(f"x must have shape ({2 * self.n},)")
+        y = np.empty_like(x)
+        self._check(lib().beep_hip_matvec(self._h, _dp(x), _dp(y), _prec(precision)))
+        return y
+
+    def matvec_device(self, d_x: int, d_y: int, precision="fp64"):
+        self._check(lib().beep_hip_matvec_device(self._h, vp(d_x), vp(d_y), _prec(precision)))
+
+    def timing(self) -> MatvecTiming:
+        t = MatvecTiming()
+        self._check(lib().beep_hip_matvec_timing(self._h, C.byref(t)))
+        return t
+
+    def rhs(self, xyzq, Dsolvent: float):
+        q = np.ascontiguousarray(xyzq, dtype=np.float64).reshape(-1, 4)
+        b = np.zeros(2 * self.n)
+        self._check(lib().beep_hip_rhs(self._h, q.shape[0], _dp(q), float(Dsolvent), _dp(b)))
+        return b
+
+    def energy_coefficients(self, inst_patch_ptr, inst_charge_ptr, xyzq):
+        pp = np.ascontiguousarray(inst_patch_ptr, dtype=np.int32)
+        cp = np.ascontiguousarray(inst_charge_ptr, dtype=np.int32)
+        q = np.ascontiguousarray(xyzq, dtype=np.float64).reshape(-1, 4)
+        ef, eh = np.zeros(self.n), np.zeros(self.n)
+        self._check(lib().beep_hip_energy_coefficients(self._h, len(pp) - 1, _ip(pp), _ip(cp),
+                                                       _dp(q), _dp(ef), _dp(eh)))
+        return ef, eh
+
+    def solve(self, b, x0=None, tol=1e-6, max_iter=100, mode="fp64", max_refine=10,
+              inner_tol=1e-3, verbose=False):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+        o = SolveOpts(tol, max_iter, SOLVE_MODES[mode] if isinstance(mode, str) else int(mode),
+                      max_refine, inner_tol, int(verbose))
+        st = SolveStats()
+        self._check(lib().beep_hip_solve(self._h, _dp(b), _dp(x), C.byref(o), C.byref(st)))
+        return x, {f: getattr(st, f) for f, _ in SolveStats._fields_}
+
+    # device memory helpers (bench / serving keep vectors resident)
+    def alloc(self, nbytes: int) -> int:
+        p = vp()
+        self._check(lib().beep_hip_device_alloc(self._h, nbytes, C.byref(p)))
+        return p.value
+
+    def free(self, p: int):
+        self._check(lib().beep_hip_device_free(self._h, vp(p)))
+
+    def upload(self, d_ptr: int, arr):
+        a = np.ascontiguousarray(arr)
+        self._check(lib().beep_hip_memcpy(self._h, vp(d_ptr), a.ctypes.data_as(vp), a.nbytes, 1))
+
+    def download(self, arr, d_ptr: int):
+        self._check(lib().beep_hip_memcpy(self._h, arr.ctypes.data_as(vp), vp(d_ptr), arr.nbytes, 2))
+        return arr
+
+    def synchronize(self):
+        self._check(lib().beep_hip_synchronize(self._h))
+
+    def memory_info(self):
+        f, t = C.c_size_t(), C.c_size_t()
+        self._check(lib().beep_hip_memory_info(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().beep_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

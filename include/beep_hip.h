@@ -1,0 +1,164 @@
+/*
+ * beep_hip.h -- device boundary of the BEM mat-vec (C ABI, HIP on gfx950).
+ *
+ * Drop-in for the reference's hot path:
+ *
+ *   long BEEP::do_bem_fmm_iteration(unsigned num_patches, double lhs[], double result[])
+ *        src/beep/beep.h:276, src/beep/beep.cpp:1485-1583
+ *     -> beep_hip_matvec(ctx, lhs, result, precision)
+ *        same layout: lhs = [f(0..N-1); h(0..N-1)], result overwritten with
+ *        the same layout (near field + far field + diagonal).
+ *
+ *   BEEP::precalc_neighbour_interactions   src/beep/beep.cpp:768-956
+ *     -> beep_hip_precompute(ctx, kappa, precision)
+ *
+ *   BEEP::gmres(eps, max_iter)             src/beep/beep.cpp:1262-1483
+ *     -> beep_hip_solve(ctx, b, x, opts, stats)
+ *
+ *   BEEP::set_rhs                          src/beep/beep.cpp:1124-1260
+ *     -> beep_hip_rhs(ctx, ...)
+ *
+ *   Mesh::init_energy_precalcs             src/bem/mesh.cpp:730-887
+ *     -> beep_hip_energy_coefficients(ctx, ...)
+ *
+ *   OpenCL_WorkBlob / OpenCL_Handler       src/opencl/opencl_workblob.h:20-52,
+ *                                          src/opencl/opencl_handler.h:49-90
+ *     -> replaced by the context's own HIP stream; calls are stream-ordered
+ *        and return when their results are in caller memory.
+ *
+ * Inputs that the reference reads from its host objects arrive as flat arrays
+ * (beep_hip_geometry, beep_hip_tree; produced by beep_host.h or by the
+ * caller's own Mesh / NodePatch / Octree code).  Host arrays are copied; the
+ * device copies belong to the context.
+ *
+ * Conventions: every function returns 0 on success or a negative code; no
+ * exception crosses the ABI; beep_hip_last_error(ctx) gives the message.  A
+ * context belongs to one host thread at a time; contexts are independent.
+ */
+#ifndef BEEP_HIP_H
+#define BEEP_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision of a mat-vec / precompute */
+#define BEEP_FP64 0       /* FP64 integrals, FP64 coefficients, FP64 vectors and sums        */
+#define BEEP_FP32 1       /* FP32 integrals and coefficients, FP32 vectors and sums          */
+#define BEEP_FP32_RESID 2 /* FP32 coefficients promoted to FP64, FP64 vectors, sums and far
+                             field: the refinement residual of the mixed-precision solve   */
+
+/* solve modes */
+#define BEEP_SOLVE_FP64 0  /* GMRES with BEEP_FP64 mat-vecs (plain FP64 solve)            */
+#define BEEP_SOLVE_MIXED 1 /* FP64 iterative refinement (BEEP_FP32_RESID residuals) around
+                              GMRES with BEEP_FP32 mat-vecs and FP64 Arnoldi/Givens        */
+#define BEEP_SOLVE_FP32 2  /* GMRES with BEEP_FP32 mat-vecs only (no refinement)          */
+
+typedef struct beep_hip_ctx beep_hip_ctx;
+
+/* Node patches and their point sets (BasicNodePatch, node_patch.h:46-224;
+ * QuadPointT, quad_point.h:82-231).  Points are 7 doubles each:
+ * x, y, z, nx, ny, nz, weight.  qual = qualocation points of a row patch
+ * (weights sum to 1), quad = quadrature points of a column patch (area
+ * weights), gal = 4-point galerkin points used for the singular self term. */
+typedef struct {
+    int n_patches;
+    const double* centroid;  /* 3N */
+    const double* normal;    /* 3N, unit */
+    const double* area;      /* N  */
+    const double* eps_ratio; /* N, Dsolvent / Dprotein of the patch's instance */
+    const double* gc;        /* N, geometric correction (0.5) */
+    const int* qual_ptr;     /* N + 1 */
+    const double* qual_pts;
+    const int* quad_ptr;     /* N + 1 */
+    const double* quad_pts;
+    const int* gal_ptr;      /* N + 1 */
+    const double* gal_pts;
+} beep_hip_geometry;
+
+/* Explicit-pair structure from the octree: every leaf's rows interact
+ * explicitly with the columns of its 27-box neighbourhood
+ * (octree.h:627-648, beep.cpp:859-950). */
+typedef struct {
+    int n_leaves;
+    const int* leaf_ptr; /* n_leaves + 1 */
+    const int* leaf_idx; /* N: rows of each leaf */
+    const int* nb_ptr;   /* n_leaves + 1 */
+    const int* nb_idx;   /* columns of each leaf */
+} beep_hip_tree;
+
+typedef struct {
+    double tol;        /* stop when |residual| <= tol * |b| (beep.cpp:1421) */
+    int max_iter;      /* GMRES inner iterations per cycle (m) */
+    int mode;          /* BEEP_SOLVE_* */
+    int max_refine;    /* refinement steps (mixed mode) */
+    double inner_tol;  /* relative tolerance of each inner solve (mixed mode) */
+    int verbose;
+} beep_solve_opts;
+
+typedef struct {
+    int iterations;    /* total inner GMRES iterations */
+    int matvecs;       /* total mat-vecs of any precision */
+    int refinements;   /* refinement steps taken (mixed mode) */
+    int converged;
+    double residual;   /* final |b - K x| (FP64 residual operator of the mode) */
+    double rel_residual;
+    double time_ms;
+    double matvec_ms;
+} beep_solve_stats;
+
+typedef struct {
+    double near_ms;    /* last mat-vec: near-field SpMV + diagonal */
+    double far_ms;     /* last mat-vec: far field */
+    double total_ms;
+    int near_launches;
+    int far_launches;
+} beep_matvec_timing;
+
+const char* beep_hip_last_error(const beep_hip_ctx* ctx);
+
+int beep_hip_create(int device, beep_hip_ctx** out);
+int beep_hip_destroy(beep_hip_ctx* ctx);
+
+int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* geom);
+int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
+
+/* Singular self terms (FP64 always) and near-field coefficients of the given
+ * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set). */
+int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);
+
+/* y = K x on host arrays (do_bem_fmm_iteration); blocks until y is written. */
+int beep_hip_matvec(beep_hip_ctx* ctx, const double* x, double* y, int precision);
+/* y = K x on device arrays; stream-ordered on the context stream. */
+int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* d_x, double* d_y, int precision);
+/* per-phase device time of the last mat-vec (HIP events on the context stream) */
+int beep_hip_matvec_timing(beep_hip_ctx* ctx, beep_matvec_timing* out);
+
+/* b = [f_rhs; h_rhs] from point charges xyzq (n x 4, universe frame) */
+int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
+                 double* b);
+/* raw energy coefficients (before the Gauss-law renormalisation) of every
+ * patch from the charges of its own instance */
+int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_instances, const int* inst_patch_ptr,
+                                 const int* inst_charge_ptr, const double* xyzq, double* ec_f,
+                                 double* ec_h);
+
+/* Solve K x = b; x holds the initial guess on entry (zeros: cold start). */
+int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_solve_opts* opts,
+                   beep_solve_stats* stats);
+
+/* device memory helpers for callers that keep vectors resident */
+int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out);
+int beep_hip_device_free(beep_hip_ctx* ctx, void* p);
+/* kind: 1 = host to device, 2 = device to host, 3 = device to device */
+int beep_hip_memcpy(beep_hip_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int beep_hip_synchronize(beep_hip_ctx* ctx);
+int beep_hip_memory_info(beep_hip_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEEP_HIP_H */
