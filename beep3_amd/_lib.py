@@ -81,7 +81,7 @@ class SolveStats(C.Structure):
 class MatvecTiming(C.Structure):
     _fields_ = [
         ("near_ms", C.c_double), ("far_ms", C.c_double), ("total_ms", C.c_double),
-        ("near_launches", C.c_int), ("far_launches", C.c_int),
+        ("count", C.c_int), ("near_ms_sum", C.c_double), ("far_ms_sum", C.c_double),
     ]
 
 
@@ -122,6 +122,7 @@ def _declare_hip(L):
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
     L.beep_hip_matvec_timing.argtypes = [vp, C.POINTER(MatvecTiming)]
+    L.beep_hip_timing_reset.argtypes = [vp]
     L.beep_hip_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
     L.beep_hip_energy_coefficients.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp]
     L.beep_hip_solve.argtypes = [vp, dp, dp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]

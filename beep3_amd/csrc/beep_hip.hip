@@ -90,9 +90,11 @@ struct beep_hip_ctx {
     DevBuf<double> V, hvec, partial, wtmp, bvec, rvec, dvec;
     int part_blocks = 0;
 
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    beep_matvec_timing last_timing{};
-    bool timing_pending = false;
+    // ring of event triples (start, near done, far done), one per mat-vec
+    static constexpr int RING = 64;
+    hipEvent_t ev[RING][3] = {};
+    int ring_head = 0, ring_pending = 0;  // next slot, slots not yet read
+    beep_matvec_timing timing{};
 };
 
 namespace {
@@ -146,16 +148,25 @@ void check_ptr(int N, const int* ptr, const char* what) {
     for (int i = 0; i < N; ++i) require(ptr[i + 1] >= ptr[i], what);
 }
 
-void timing_collect(beep_hip_ctx* c) {
-    if (!c->timing_pending) return;
+// read the oldest pending event triple into the timing sums
+void timing_pop(beep_hip_ctx* c) {
+    const int slot = (c->ring_head - c->ring_pending + beep_hip_ctx::RING) % beep_hip_ctx::RING;
+    hipEvent_t* e = c->ev[slot];
     float a = 0, b = 0;
-    hip_check(hipEventSynchronize(c->ev[2]), "hipEventSynchronize");
-    hip_check(hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "hipEventElapsedTime");
-    hip_check(hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "hipEventElapsedTime");
-    c->last_timing.near_ms = a;
-    c->last_timing.far_ms = b;
-    c->last_timing.total_ms = double(a) + double(b);
-    c->timing_pending = false;
+    hip_check(hipEventSynchronize(e[2]), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&a, e[0], e[1]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&b, e[1], e[2]), "hipEventElapsedTime");
+    c->timing.near_ms = a;
+    c->timing.far_ms = b;
+    c->timing.total_ms = double(a) + double(b);
+    c->timing.count += 1;
+    c->timing.near_ms_sum += a;
+    c->timing.far_ms_sum += b;
+    c->ring_pending -= 1;
+}
+
+void timing_collect(beep_hip_ctx* c) {
+    while (c->ring_pending > 0) timing_pop(c);
 }
 
 void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
@@ -166,20 +177,21 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
     else require(c->have32, "FP32 coefficients not precomputed");
     hipStream_t s = c->stream;
-    hip_check(hipEventRecord(c->ev[0], s), "hipEventRecord");
+    if (c->ring_pending == beep_hip_ctx::RING) timing_pop(c);
+    hipEvent_t* ev = c->ev[c->ring_head];
+    hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
     launch_near_spmv(s, mode, c->N, c->n_spmv_blocks, c->L, c->spmv_blk.p, c->leaf_ptr.p,
                      c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p, c->coef_off.p, c->coef32.p,
                      c->coef64.p, c->diag.p, dx, dy);
-    hip_check(hipEventRecord(c->ev[1], s), "hipEventRecord");
+    hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     c->src.alloc(size_t(c->N));
     launch_far_pack(s, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
     launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
                       c->src.p, c->kappa, dy);
-    hip_check(hipEventRecord(c->ev[2], s), "hipEventRecord");
-    c->last_timing.near_launches = 1;
-    c->last_timing.far_launches = 2;
-    c->timing_pending = true;
+    hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+    c->ring_head = (c->ring_head + 1) % beep_hip_ctx::RING;
+    c->ring_pending += 1;
 }
 
 double dev_dot(beep_hip_ctx* c, const double* a, const double* b, int n) {
@@ -304,7 +316,8 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
         require(device >= 0 && device < count, "no such HIP device");
         set_device(c.get());
         hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
-        for (auto& e : c->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        for (auto& tri : c->ev)
+            for (auto& e : tri) hip_check(hipEventCreate(&e), "hipEventCreate");
     });
     if (rc != 0) {
         // no context survives: the message is kept for beep_hip_last_error(NULL)
@@ -320,8 +333,9 @@ int beep_hip_destroy(beep_hip_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto& e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& tri : ctx->ev)
+        for (auto& e : tri)
+            if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -503,7 +517,15 @@ int beep_hip_matvec_timing(beep_hip_ctx* ctx, beep_matvec_timing* out) {
         require(out != nullptr, "null output");
         set_device(ctx);
         timing_collect(ctx);
-        *out = ctx->last_timing;
+        *out = ctx->timing;
+    });
+}
+
+int beep_hip_timing_reset(beep_hip_ctx* ctx) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        timing_collect(ctx);
+        ctx->timing = beep_matvec_timing{};
     });
 }
 

@@ -86,6 +86,9 @@ class Device:
         self._check(lib().beep_hip_matvec_timing(self._h, C.byref(t)))
         return t
 
+    def timing_reset(self):
+        self._check(lib().beep_hip_timing_reset(self._h))
+
     def rhs(self, xyzq, Dsolvent: float):
         q = np.ascontiguousarray(xyzq, dtype=np.float64).reshape(-1, 4)
         b = np.zeros(2 * self.n)

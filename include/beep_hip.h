@@ -110,11 +110,12 @@ typedef struct {
 } beep_solve_stats;
 
 typedef struct {
-    double near_ms;    /* last mat-vec: near-field SpMV + diagonal */
-    double far_ms;     /* last mat-vec: far field */
+    double near_ms;     /* last mat-vec: near-field SpMV + diagonal */
+    double far_ms;      /* last mat-vec: far field */
     double total_ms;
-    int near_launches;
-    int far_launches;
+    int count;          /* mat-vecs since the last beep_hip_timing_reset */
+    double near_ms_sum; /* summed over those mat-vecs */
+    double far_ms_sum;
 } beep_matvec_timing;
 
 const char* beep_hip_last_error(const beep_hip_ctx* ctx);
@@ -133,8 +134,10 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);
 int beep_hip_matvec(beep_hip_ctx* ctx, const double* x, double* y, int precision);
 /* y = K x on device arrays; stream-ordered on the context stream. */
 int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* d_x, double* d_y, int precision);
-/* per-phase device time of the last mat-vec (HIP events on the context stream) */
+/* per-phase device time of mat-vecs (HIP events on the context stream):
+ * the last one and the sums since beep_hip_timing_reset */
 int beep_hip_matvec_timing(beep_hip_ctx* ctx, beep_matvec_timing* out);
+int beep_hip_timing_reset(beep_hip_ctx* ctx);
 
 /* b = [f_rhs; h_rhs] from point charges xyzq (n x 4, universe frame) */
 int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
