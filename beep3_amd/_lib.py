@@ -119,6 +119,7 @@ def _declare_hip(L):
     L.beep_hip_set_geometry.argtypes = [vp, C.POINTER(HipGeometry)]
     L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
+    L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
     L.beep_hip_matvec_timing.argtypes = [vp, C.POINTER(MatvecTiming)]

@@ -32,11 +32,12 @@ CONVERT_ENERGY_TO_KJ_PER_MOL = AVOGADRO * ELEMENTARY_CHARGE**2 / (EPSILON0 * ANG
 
 class BEEP:
     def __init__(self, Dsolvent=80.0, kappa=0.0, quad_points=0, qual_points=4, nbsize=1500,
-                 device=0, precision="fp32"):
+                 device=0, precision="fp32", far_field="auto", fmm_order=5, fmm_leaf=48):
         self.Dsolvent, self.kappa = float(Dsolvent), float(kappa)
         self.quad, self.qual, self.nbsize = int(quad_points), int(qual_points), int(nbsize)
         self.precision = precision  # coefficient set precomputed at init
         self.device_id = device
+        self.far_field, self.fmm_order, self.fmm_leaf = far_field, int(fmm_order), int(fmm_leaf)
         self.library: list[Mesh] = []
         self.instances: list[tuple] = []
         self.scene: Scene | None = None
@@ -73,6 +74,7 @@ class BEEP:
         a = sc.arrays()
         if self.dev is None:
             self.dev = Device(self.device_id)
+        self.dev.set_far_field(self.far_field, self.fmm_order, self.fmm_leaf)
         self.dev.set_geometry(a)
         self.dev.set_tree(a)
         t2 = time.perf_counter()

@@ -11,6 +11,7 @@
 
 #include "../../include/beep_hip.h"
 #include "device.hpp"
+#include "fmm.hpp"
 
 using namespace beepdev;
 
@@ -82,6 +83,12 @@ struct beep_hip_ctx {
     bool have32 = false, have64 = false, have_diag = false;
     double kappa = 0;
 
+    // far field
+    int far_mode = BEEP_FAR_AUTO, fmm_order = 4, fmm_leaf = 48;
+    std::vector<double> h_centroid;
+    FmmPlan* fmm = nullptr;
+    bool use_fmm = false;
+
     // per mat-vec scratch
     DevBuf<Src64> src;
     DevBuf<double> xbuf, ybuf;
@@ -119,6 +126,17 @@ void require(bool cond, const char* msg) {
 }
 
 void set_device(beep_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
+
+// (re)build the far-field plan for the current geometry and options
+void far_setup(beep_hip_ctx* c) {
+    if (c->fmm) {
+        fmm_plan_destroy(c->fmm);
+        c->fmm = nullptr;
+    }
+    c->use_fmm = c->far_mode == BEEP_FAR_FMM || (c->far_mode == BEEP_FAR_AUTO && c->N > 20000);
+    if (c->use_fmm)
+        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf, c->stream);
+}
 
 // Convert 7-double points of a CSR point set into local-frame records.
 template <typename T>
@@ -185,10 +203,16 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
                      c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p, c->coef_off.p, c->coef32.p,
                      c->coef64.p, c->diag.p, dx, dy);
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
-    c->src.alloc(size_t(c->N));
-    launch_far_pack(s, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
-    launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
-                      c->src.p, c->kappa, dy);
+    if (c->use_fmm) {
+        // kernel tables are built on first use for the current kappa
+        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, s);
+        fmm_evaluate(c->fmm, s, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa, dy);
+    } else {
+        c->src.alloc(size_t(c->N));
+        launch_far_pack(s, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
+        launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
+                          c->src.p, c->kappa, dy);
+    }
     hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
     c->ring_head = (c->ring_head + 1) % beep_hip_ctx::RING;
     c->ring_pending += 1;
@@ -333,6 +357,7 @@ int beep_hip_destroy(beep_hip_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->fmm) fmm_plan_destroy(ctx->fmm);
     for (auto& tri : ctx->ev)
         for (auto& e : tri)
             if (e) (void)hipEventDestroy(e);
@@ -382,6 +407,8 @@ int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* g) {
         ctx->quad32.upload(d32.data(), d32.size(), s);
         hip_check(hipStreamSynchronize(s), "sync geometry upload");
         ctx->N = N;
+        ctx->h_centroid.assign(g->centroid, g->centroid + 3 * size_t(N));
+        far_setup(ctx);
         ctx->part_blocks = std::max(1, std::min(256, (2 * N + 1023) / 1024));
         ctx->have_geom = true;
         ctx->have_tree = false;
@@ -470,6 +497,7 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
                                     ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
                                     ctx->quad64.p, kappa, 1, ctx->coef64.p);
             ctx->have64 = true;
+            if (ctx->use_fmm) fmm_prepare(ctx->fmm, kappa, true, false, s);
         } else if (precision == BEEP_FP32 || precision == BEEP_FP32_RESID) {
             const size_t need = size_t(ctx->nnz) * sizeof(float4);
             require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
@@ -481,10 +509,27 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
                                     ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
                                     ctx->quad32.p, float(kappa), 1, ctx->coef32.p);
             ctx->have32 = true;
+            // FP32 coefficients serve the FP32 mat-vec (FP32 far field) and the
+            // refinement residual (FP64 far field)
+            if (ctx->use_fmm) fmm_prepare(ctx->fmm, kappa, true, true, s);
         } else {
             throw std::runtime_error("unknown precision");
         }
         hip_check(hipStreamSynchronize(s), "sync precompute");
+    });
+}
+
+int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target) {
+    return guarded(ctx, [&] {
+        require(mode == BEEP_FAR_AUTO || mode == BEEP_FAR_DIRECT || mode == BEEP_FAR_FMM,
+                "unknown far-field mode");
+        require(order >= 3 && order <= 6, "FMM order must be 3..6");
+        require(leaf_target >= 1, "leaf_target must be positive");
+        set_device(ctx);
+        ctx->far_mode = mode;
+        ctx->fmm_order = order;
+        ctx->fmm_leaf = leaf_target;
+        if (ctx->have_geom) far_setup(ctx);  // kernel tables follow on first use
     });
 }
 

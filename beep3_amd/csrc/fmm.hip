@@ -1,0 +1,805 @@
+// Far field by a Chebyshev-interpolation fast multipole method on a uniform
+// sparse octree (gfx950).
+//
+// What it computes is exactly the quantity of the direct far-field kernel
+// (kernels_far.hip): for every target patch i the 1-point centroid
+// interaction with every other patch j, written as three potentials of
+// monopole + dipole sources
+//   Phi_L (Laplace 1/r):      monopole -a h,    dipole  ie a f n
+//   Phi_1 (Yukawa e^-kr / r): monopole  a h,    dipole     -a f n
+//   Phi_2 (Yukawa):           monopole ie a h,  dipole  -ie a f n
+//   f_i += (Phi_L + Phi_1)(x_i) / 4pi,  h_i += n_i . grad(Phi_L + Phi_2)(x_i) / 4pi.
+// This replaces the reference's 12-component Yukawa/Laplace plane-wave FMM
+// (src/fmm/fmm_bem_hybrid.h:184-642): the same interaction, three expansions
+// instead of twelve, every translation a small dense matrix.
+//
+// Expansions hold the values at the P^3 tensor Chebyshev nodes of each box:
+//   P2M  M_m = sum_j [ q_j S_m(y_j) + d_j . grad S_m(y_j) ]      (leaf boxes)
+//   M2M  M_parent = T_c M_child       (separable P x P factors, exact)
+//   M2L  L_t += K(t) M_s              (K = kernel between node sets, one dense
+//                                      NP x NP matrix per relative offset t;
+//                                      Laplace scaled by 1/h, Yukawa per level)
+//   L2L  L_child += T_c^T L_parent
+//   L2P  Phi(x) = sum_n S_n(x) L_n,  grad Phi(x) = sum_n grad S_n(x) L_n
+//   P2P  direct sum over the 27 neighbour leaf boxes (self excluded)
+// with S_n(x) = prod_d S_P(xi_{n_d}, u_d), S_P(xi, u) = 1/P + 2/P sum_{k>=1}
+// T_k(xi) T_k(u), u = box-local coordinate in [-1, 1].
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+#include <vector>
+
+#include "device.hpp"
+#include "fmm.hpp"
+
+namespace beepdev {
+
+namespace {
+
+constexpr double PI = 3.14159265358979323846;
+
+template <typename T>
+struct Fm;
+template <>
+struct Fm<float> {
+    static __device__ __forceinline__ float rsqrt_(float v) { return __frsqrt_rn(v); }
+    static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
+};
+template <>
+struct Fm<double> {
+    static __device__ __forceinline__ double rsqrt_(double v) { return 1.0 / sqrt(v); }
+    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
+};
+
+// 1-D interpolation basis and derivative at u for all P nodes.
+// tn[k * P + j] = T_j(xi_k).
+template <typename T, int P>
+__device__ __forceinline__ void basis1d(T u, const T* __restrict__ tn, T* s, T* ds) {
+    T tj[P], dj[P];
+    tj[0] = T(1);
+    dj[0] = T(0);
+    if (P > 1) {
+        tj[1] = u;
+        dj[1] = T(1);
+    }
+    // U_{j-1} recurrence for derivatives: T_j' = j U_{j-1}
+    T um2 = T(1), um1 = T(2) * u;  // U_0, U_1
+    for (int j = 2; j < P; ++j) {
+        tj[j] = T(2) * u * tj[j - 1] - tj[j - 2];
+        dj[j] = T(j) * um1;
+        const T un = T(2) * u * um1 - um2;
+        um2 = um1;
+        um1 = un;
+    }
+    for (int k = 0; k < P; ++k) {
+        T a = T(0), b = T(0);
+        for (int j = 1; j < P; ++j) {
+            a += tn[k * P + j] * tj[j];
+            b += tn[k * P + j] * dj[j];
+        }
+        s[k] = T(1) / T(P) + T(2) / T(P) * a;
+        ds[k] = T(2) / T(P) * b;
+    }
+}
+
+template <typename T>
+struct alignas(16) SrcR {
+    T x, y, z, ie;
+    T dx, dy, dz, sh;
+};
+
+// ---------------------------------------------------------------------------
+// P2M over leaf boxes.  One block per leaf box; thread n = node.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_p2m(const FmmDev d, const Src64* __restrict__ src,
+                                             T* __restrict__ M) {
+    constexpr int NP = P * P * P;
+    constexpr int CH = 64;
+    __shared__ T sb[CH][3][P], dsb[CH][3][P];
+    __shared__ T q[CH][4];  // sh, ie, (dv . grad handled through dvx..)
+    __shared__ T dvv[CH][3];
+    __shared__ T tn[P * P];
+    const int bl = blockIdx.x;                  // leaf box (local index at leaf level)
+    const int box = d.leaf_first + bl;
+    const int s0 = d.leaf_src[bl], s1 = d.leaf_src[bl + 1];
+    const double cx = d.box_c[box].x, cy = d.box_c[box].y, cz = d.box_c[box].z;
+    const double inv_half = 2.0 / d.box_h[d.D];
+    const int t = threadIdx.x;
+    for (int k = t; k < P * P; k += blockDim.x) tn[k] = T(d.tn[k]);
+    const int a = t / (P * P), b = (t / P) % P, c = t % P;
+    T mL = 0, m1 = 0, m2 = 0;
+    const T g2 = T(inv_half);
+    for (int j0 = s0; j0 < s1; j0 += CH) {
+        __syncthreads();
+        if (t < CH && j0 + t < s1) {
+            const Src64 s = src[j0 + t];
+            T sx[P], dx[P];
+            basis1d<T, P>(T((s.x - cx) * inv_half), tn, sx, dx);
+            for (int k = 0; k < P; ++k) { sb[t][0][k] = sx[k]; dsb[t][0][k] = dx[k]; }
+            basis1d<T, P>(T((s.y - cy) * inv_half), tn, sx, dx);
+            for (int k = 0; k < P; ++k) { sb[t][1][k] = sx[k]; dsb[t][1][k] = dx[k]; }
+            basis1d<T, P>(T((s.z - cz) * inv_half), tn, sx, dx);
+            for (int k = 0; k < P; ++k) { sb[t][2][k] = sx[k]; dsb[t][2][k] = dx[k]; }
+            q[t][0] = T(s.sh);
+            q[t][1] = T(s.inv_eps);
+            dvv[t][0] = T(s.dx);
+            dvv[t][1] = T(s.dy);
+            dvv[t][2] = T(s.dz);
+        }
+        __syncthreads();
+        const int m = min(CH, s1 - j0);
+        if (t < NP) {
+            for (int j = 0; j < m; ++j) {
+                const T sx = sb[j][0][a], sy = sb[j][1][b], sz = sb[j][2][c];
+                const T syz = sy * sz;
+                const T s = sx * syz;
+                const T gx = dsb[j][0][a] * syz;
+                const T gy = sx * dsb[j][1][b] * sz;
+                const T gz = sx * sy * dsb[j][2][c];
+                const T dg = g2 * (dvv[j][0] * gx + dvv[j][1] * gy + dvv[j][2] * gz);
+                const T sh = q[j][0], ie = q[j][1];
+                mL += ie * dg - sh * s;
+                const T c1 = sh * s - dg;
+                m1 += c1;
+                m2 += ie * c1;
+            }
+        }
+    }
+    if (t < NP) {
+        T* out = M + size_t(box) * 3 * NP;
+        out[t] = mL;
+        out[NP + t] = m1;
+        out[2 * NP + t] = m2;
+    }
+}
+
+// M2M: one block per parent box at level l-1 (boxes [first, first+count));
+// children listed in child_ptr/child_box with octant bits.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_m2m(const FmmDev d, int first, T* __restrict__ M) {
+    constexpr int NP = P * P * P;
+    __shared__ T cm[3 * NP];
+    __shared__ T tt[2][P][P];
+    const int box = first + blockIdx.x;
+    const int t = threadIdx.x;
+    for (int k = t; k < 2 * P * P; k += blockDim.x) (&tt[0][0][0])[k] = T(d.t_oct[k]);
+    const int a = t / (P * P), b = (t / P) % P, c = t % P;
+    T acc[3] = {0, 0, 0};
+    for (int k = d.child_ptr[box]; k < d.child_ptr[box + 1]; ++k) {
+        const int ch = d.child_box[k];
+        const int oct = d.child_oct[k];
+        __syncthreads();
+        for (int q = t; q < 3 * NP; q += blockDim.x) cm[q] = M[size_t(ch) * 3 * NP + q];
+        __syncthreads();
+        if (t < NP) {
+            const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+            for (int a2 = 0; a2 < P; ++a2) {
+                const T fx = tt[ox][a][a2];
+                for (int b2 = 0; b2 < P; ++b2) {
+                    const T fxy = fx * tt[oy][b][b2];
+                    for (int c2 = 0; c2 < P; ++c2) {
+                        const T f = fxy * tt[oz][c][c2];
+                        const int m = (a2 * P + b2) * P + c2;
+                        acc[0] += f * cm[m];
+                        acc[1] += f * cm[NP + m];
+                        acc[2] += f * cm[2 * NP + m];
+                    }
+                }
+            }
+        }
+    }
+    if (t < NP) {
+        T* out = M + size_t(box) * 3 * NP;
+        out[t] = acc[0];
+        out[NP + t] = acc[1];
+        out[2 * NP + t] = acc[2];
+    }
+}
+
+// M2L: one block per chunk of G pairs sharing one offset at one level.
+// K rows are staged through LDS in blocks of MB source nodes; every thread
+// (target node n) keeps G x 3 partial sums, then adds them to L atomically.
+template <typename T, int P, int G, int MB>
+__global__ __launch_bounds__(256) void k_m2l(const FmmDev d, int level, const int4* __restrict__ chunks,
+                                             const T* __restrict__ kl, const T* __restrict__ ky,
+                                             const T* __restrict__ M, T* __restrict__ L) {
+    constexpr int NP = P * P * P;
+    __shared__ T sKL[MB][NP];
+    __shared__ T sKY[MB][NP];
+    __shared__ T sM[G][3][MB];
+    const int4 ch = chunks[blockIdx.x];  // offset id, pair begin, pair end, unused
+    const int off = ch.x, p0 = ch.y, np = ch.z - ch.y;
+    const int t = threadIdx.x;
+    const T* KL = kl + size_t(off) * NP * NP;
+    const T* KY = ky + size_t(off) * NP * NP;
+    T acc[G][3];
+#pragma unroll
+    for (int g = 0; g < G; ++g) acc[g][0] = acc[g][1] = acc[g][2] = T(0);
+    for (int m0 = 0; m0 < NP; m0 += MB) {
+        const int mb = min(MB, NP - m0);
+        __syncthreads();
+        for (int q = t; q < mb * NP; q += blockDim.x) {
+            const int mm = q / NP, nn = q - mm * NP;
+            sKL[mm][nn] = KL[size_t(m0 + mm) * NP + nn];
+            sKY[mm][nn] = KY[size_t(m0 + mm) * NP + nn];
+        }
+        for (int q = t; q < G * 3 * mb; q += blockDim.x) {
+            const int g = q / (3 * mb), r = q - g * 3 * mb, e = r / mb, mm = r - e * mb;
+            T v = T(0);
+            if (g < np) v = M[size_t(d.pair_src[p0 + g]) * 3 * NP + e * NP + m0 + mm];
+            sM[g][e][mm] = v;
+        }
+        __syncthreads();
+        if (t < NP) {
+            for (int mm = 0; mm < mb; ++mm) {
+                const T a = sKL[mm][t], b = sKY[mm][t];
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    acc[g][0] += a * sM[g][0][mm];
+                    acc[g][1] += b * sM[g][1][mm];
+                    acc[g][2] += b * sM[g][2][mm];
+                }
+            }
+        }
+    }
+    if (t < NP) {
+        const T sl = T(1.0 / d.box_h[level]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (g >= np) break;
+            T* out = L + size_t(d.pair_tgt[p0 + g]) * 3 * NP;
+            atomicAdd(out + t, acc[g][0] * sl);
+            atomicAdd(out + NP + t, acc[g][1]);
+            atomicAdd(out + 2 * NP + t, acc[g][2]);
+        }
+    }
+}
+
+// L2L: one block per child box at level l+1 (boxes [first, first+count)).
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_l2l(const FmmDev d, int first, T* __restrict__ L) {
+    constexpr int NP = P * P * P;
+    __shared__ T pl[3 * NP];
+    __shared__ T tt[2][P][P];
+    const int box = first + blockIdx.x;
+    const int par = d.box_parent[box];
+    const int oct = d.box_oct[box];
+    const int t = threadIdx.x;
+    for (int k = t; k < 2 * P * P; k += blockDim.x) (&tt[0][0][0])[k] = T(d.t_oct[k]);
+    for (int q = t; q < 3 * NP; q += blockDim.x) pl[q] = L[size_t(par) * 3 * NP + q];
+    __syncthreads();
+    if (t < NP) {
+        const int a2 = t / (P * P), b2 = (t / P) % P, c2 = t % P;
+        const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
+        T acc[3] = {0, 0, 0};
+        for (int a = 0; a < P; ++a) {
+            const T fx = tt[ox][a][a2];
+            for (int b = 0; b < P; ++b) {
+                const T fxy = fx * tt[oy][b][b2];
+                for (int c = 0; c < P; ++c) {
+                    const T f = fxy * tt[oz][c][c2];
+                    const int m = (a * P + b) * P + c;
+                    acc[0] += f * pl[m];
+                    acc[1] += f * pl[NP + m];
+                    acc[2] += f * pl[2 * NP + m];
+                }
+            }
+        }
+        T* out = L + size_t(box) * 3 * NP;
+        out[t] += acc[0];
+        out[NP + t] += acc[1];
+        out[2 * NP + t] += acc[2];
+    }
+}
+
+// Leaf pass: L2P + P2P for the targets of one leaf box, one write per target.
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __restrict__ src,
+                                              const Patch64* __restrict__ patches, T kappa,
+                                              int use_local, const T* __restrict__ L,
+                                              double* __restrict__ y, int n) {
+    constexpr int NP = P * P * P;
+    constexpr int TILE = 256;
+    __shared__ SrcR<T> tile[TILE];
+    __shared__ int tidx[TILE];
+    __shared__ T lb[3 * NP];
+    __shared__ T tn[P * P];
+    const int bl = blockIdx.x;
+    const int box = d.leaf_first + bl;
+    const int s0 = d.leaf_src[bl], s1 = d.leaf_src[bl + 1];
+    const double cx = d.box_c[box].x, cy = d.box_c[box].y, cz = d.box_c[box].z;
+    const double inv_half = 2.0 / d.box_h[d.D];
+    const int t = threadIdx.x;
+    for (int k = t; k < P * P; k += blockDim.x) tn[k] = T(d.tn[k]);
+    if (use_local)
+        for (int q = t; q < 3 * NP; q += blockDim.x) lb[q] = L[size_t(box) * 3 * NP + q];
+    for (int tc = s0; tc < s1; tc += blockDim.x) {
+        const int ti = tc + t;
+        const bool valid = ti < s1;
+        const int i = valid ? d.perm[ti] : -1;
+        T xi = 0, yi = 0, zi = 0, n0x = 0, n0y = 0, n0z = 0;
+        if (valid) {
+            const Patch64 p = patches[i];
+            xi = T(p.cx - cx); yi = T(p.cy - cy); zi = T(p.cz - cz);
+            n0x = T(p.nx); n0y = T(p.ny); n0z = T(p.nz);
+        }
+        T af = 0, ah = 0;
+        // P2P over the neighbour leaf boxes (positions relative to this box)
+        for (int k = d.nb_ptr[bl]; k < d.nb_ptr[bl + 1]; ++k) {
+            const int nbl = d.nb_leaf[k];
+            const int j0 = d.leaf_src[nbl], j1 = d.leaf_src[nbl + 1];
+            for (int jt = j0; jt < j1; jt += TILE) {
+                __syncthreads();
+                if (jt + t < j1) {
+                    const Src64 q = src[jt + t];
+                    tile[t] = SrcR<T>{T(q.x - cx), T(q.y - cy), T(q.z - cz), T(q.inv_eps),
+                                      T(q.dx), T(q.dy), T(q.dz), T(q.sh)};
+                    tidx[t] = d.perm[jt + t];
+                }
+                __syncthreads();
+                const int m = min(TILE, j1 - jt);
+                if (valid) {
+                    for (int jj = 0; jj < m; ++jj) {
+                        if (tidx[jj] == i) continue;
+                        const SrcR<T> q = tile[jj];
+                        const T dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                        const T r2 = dx * dx + dy * dy + dz * dz;
+                        const T ir = Fm<T>::rsqrt_(r2);
+                        const T r = r2 * ir;
+                        const T ir2 = ir * ir;
+                        const T ir3 = ir2 * ir;
+                        const T kr = kappa * r;
+                        const T e = Fm<T>::exp_(-kr);
+                        const T u = e * ir;
+                        const T g1 = e * (T(1) + kr) * ir3;
+                        const T ir5 = ir3 * ir2;
+                        const T gL2 = T(3) * ir5;
+                        const T g2 = e * (kr * kr + T(3) * kr + T(3)) * ir5;
+                        const T dd = q.dx * dx + q.dy * dy + q.dz * dz;
+                        const T nd = n0x * dx + n0y * dy + n0z * dz;
+                        const T ndv = n0x * q.dx + n0y * q.dy + n0z * q.dz;
+                        af += q.sh * (u - ir) + dd * (g1 - q.ie * ir3);
+                        ah += -q.sh * nd * ir3 +
+                              q.ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + q.sh * nd * g1);
+                    }
+                }
+            }
+        }
+        // L2P
+        if (valid && use_local) {
+            T sx[P], dsx[P], sy[P], dsy[P], sz[P], dsz[P];
+            const T ih = T(inv_half);
+            basis1d<T, P>(xi * ih, tn, sx, dsx);
+            basis1d<T, P>(yi * ih, tn, sy, dsy);
+            basis1d<T, P>(zi * ih, tn, sz, dsz);
+            T pL = 0, p1 = 0, gLx = 0, gLy = 0, gLz = 0, g2x = 0, g2y = 0, g2z = 0;
+            for (int a = 0; a < P; ++a)
+                for (int b = 0; b < P; ++b) {
+                    const T sab = sx[a] * sy[b], dab = dsx[a] * sy[b], adb = sx[a] * dsy[b];
+                    for (int c = 0; c < P; ++c) {
+                        const int m = (a * P + b) * P + c;
+                        const T s = sab * sz[c];
+                        const T gx = dab * sz[c], gy = adb * sz[c], gz = sab * dsz[c];
+                        const T l0 = lb[m], l1 = lb[NP + m], l2 = lb[2 * NP + m];
+                        pL += s * l0;
+                        p1 += s * l1;
+                        gLx += gx * l0; gLy += gy * l0; gLz += gz * l0;
+                        g2x += gx * l2; g2y += gy * l2; g2z += gz * l2;
+                    }
+                }
+            af += pL + p1;
+            ah += ih * (n0x * (gLx + g2x) + n0y * (gLy + g2y) + n0z * (gLz + g2z));
+        }
+        if (valid) {
+            y[i] += double(af) * FOUR_PI_INV;
+            y[n + i] += double(ah) * FOUR_PI_INV;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_zero(size_t n, T* __restrict__ p) {
+    const size_t k = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = T(0);
+}
+
+// Kernel tables between node sets: out[off][m][n] = G(|t + (xi_m - xi_n)/2| * h)
+// for offsets t in [-3, 3]^3 (off = (tx+3)*49 + (ty+3)*7 + (tz+3)); laplace:
+// 1/r with h = 1 (scaled by 1/h at use), else e^{-kappa r}/r at box size h.
+template <typename T, int P>
+__global__ void k_ktab(const double* __restrict__ xi, double h, double kappa, int laplace,
+                       T* __restrict__ out) {
+    constexpr int NP = P * P * P;
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= size_t(343) * NP * NP) return;
+    const int off = int(idx / (size_t(NP) * NP));
+    const int rem = int(idx - size_t(off) * NP * NP);
+    const int m = rem / NP, nn = rem - m * NP;
+    const int tx = off / 49 - 3, ty = (off / 7) % 7 - 3, tz = off % 7 - 3;
+    const int ma = m / (P * P), mb = (m / P) % P, mc = m % P;
+    const int na = nn / (P * P), nb = (nn / P) % P, nc = nn % P;
+    const double vx = tx + 0.5 * (xi[ma] - xi[na]);
+    const double vy = ty + 0.5 * (xi[mb] - xi[nb]);
+    const double vz = tz + 0.5 * (xi[mc] - xi[nc]);
+    const double r = sqrt(vx * vx + vy * vy + vz * vz);
+    double v = 0;
+    if (max(abs(tx), max(abs(ty), abs(tz))) > 1) {
+        if (laplace) v = 1.0 / r;
+        else v = exp(-kappa * h * r) / (h * r);
+    }
+    out[idx] = T(v);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side: plan construction and launches.
+
+namespace {
+
+uint64_t bkey(uint32_t x, uint32_t y, uint32_t z) {
+    return (uint64_t(x) << 42) | (uint64_t(y) << 21) | uint64_t(z);
+}
+
+template <typename T>
+void upload(T*& dst, const std::vector<T>& v, hipStream_t s) {
+    if (dst) (void)hipFree(dst);
+    dst = nullptr;
+    if (v.empty()) return;
+    hip_check(hipMalloc(&dst, v.size() * sizeof(T)), "hipMalloc fmm");
+    hip_check(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+              "H2D fmm");
+}
+
+template <typename T>
+void free_dev(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+FmmPlan::~FmmPlan() {
+    free_dev(dv.box_c); free_dev(dv.box_h); free_dev(dv.box_parent); free_dev(dv.box_oct);
+    free_dev(dv.child_ptr); free_dev(dv.child_box); free_dev(dv.child_oct);
+    free_dev(dv.leaf_src); free_dev(dv.perm); free_dev(dv.nb_ptr); free_dev(dv.nb_leaf);
+    free_dev(dv.pair_src); free_dev(dv.pair_tgt); free_dev(dv.tn); free_dev(dv.t_oct);
+    free_dev(d_chunks); free_dev(d_xi);
+    for (auto& p : ktab64) free_dev(p);
+    for (auto& p : ktab32) free_dev(p);
+    free_dev(exp64); free_dev(exp32); free_dev(src);
+}
+
+FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
+                         hipStream_t s) {
+    if (order < 3 || order > 6) throw std::runtime_error("FMM order must be 3..6");
+    auto* pl = new FmmPlan();
+    pl->N = N;
+    pl->P = order;
+    const int NP = order * order * order;
+    // bounding cube
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < N; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], centroid[3 * i + k]);
+            hi[k] = std::max(hi[k], centroid[3 * i + k]);
+        }
+    double edge = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    edge = edge * (1.0 + 1e-9) + 1e-9;
+    const double ctr[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+    double x0[3];
+    for (int k = 0; k < 3; ++k) x0[k] = ctr[k] - edge / 2;
+    // leaf level: first level whose mean occupancy of non-empty boxes <= target
+    auto cell = [&](int i, int lev, int k) {
+        const long cells = 1L << lev;
+        long v = long(std::floor((centroid[3 * i + k] - x0[k]) / edge * cells));
+        return uint32_t(std::min(cells - 1, std::max(0L, v)));
+    };
+    int D = 0;
+    for (; D < 12; ++D) {
+        std::unordered_map<uint64_t, int> occ;
+        occ.reserve(size_t(N));
+        for (int i = 0; i < N; ++i) occ[bkey(cell(i, D, 0), cell(i, D, 1), cell(i, D, 2))]++;
+        if (double(N) / double(occ.size()) <= leaf_target) break;
+    }
+    pl->D = D;
+    // sort patches by leaf key (Morton-interleaved for locality)
+    auto morton = [&](uint32_t x, uint32_t y, uint32_t z) {
+        uint64_t m = 0;
+        for (int b = 0; b < 21; ++b)
+            m |= (uint64_t((x >> b) & 1) << (3 * b + 2)) | (uint64_t((y >> b) & 1) << (3 * b + 1)) |
+                 (uint64_t((z >> b) & 1) << (3 * b));
+        return m;
+    };
+    std::vector<uint64_t> code(N);
+    std::vector<uint32_t> cx(N), cy(N), cz(N);
+    for (int i = 0; i < N; ++i) {
+        cx[i] = cell(i, D, 0); cy[i] = cell(i, D, 1); cz[i] = cell(i, D, 2);
+        code[i] = morton(cx[i], cy[i], cz[i]);
+    }
+    std::vector<int> perm(N);
+    for (int i = 0; i < N; ++i) perm[i] = i;
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+    // boxes per level; leaf level first
+    struct Box { uint32_t x, y, z; };
+    std::vector<std::vector<Box>> lev(D + 1);
+    std::vector<std::unordered_map<uint64_t, int>> idx(D + 1);
+    std::vector<int> leaf_src;
+    for (int k = 0; k < N; ++k) {
+        const int i = perm[k];
+        const uint64_t key = bkey(cx[i], cy[i], cz[i]);
+        if (!idx[D].count(key)) {
+            idx[D][key] = int(lev[D].size());
+            lev[D].push_back({cx[i], cy[i], cz[i]});
+            leaf_src.push_back(k);
+        }
+    }
+    leaf_src.push_back(N);
+    for (int l = D - 1; l >= 0; --l)
+        for (const Box& b : lev[l + 1]) {
+            const Box pb{b.x >> 1, b.y >> 1, b.z >> 1};
+            const uint64_t key = bkey(pb.x, pb.y, pb.z);
+            if (!idx[l].count(key)) {
+                idx[l][key] = int(lev[l].size());
+                lev[l].push_back(pb);
+            }
+        }
+    // global box numbering: level 0 first
+    pl->lvl_first.assign(D + 2, 0);
+    for (int l = 0; l <= D; ++l) pl->lvl_first[l + 1] = pl->lvl_first[l] + int(lev[l].size());
+    const int nbox = pl->lvl_first[D + 1];
+    std::vector<double4> box_c(nbox);
+    std::vector<double> box_h(D + 1);
+    std::vector<int> box_parent(nbox, -1), box_oct(nbox, 0);
+    std::vector<std::vector<std::pair<int, int>>> kids(nbox);
+    for (int l = 0; l <= D; ++l) {
+        const double h = edge / double(1L << l);
+        box_h[l] = h;
+        for (size_t q = 0; q < lev[l].size(); ++q) {
+            const Box& b = lev[l][q];
+            const int g = pl->lvl_first[l] + int(q);
+            box_c[g] = make_double4(x0[0] + (b.x + 0.5) * h, x0[1] + (b.y + 0.5) * h,
+                                    x0[2] + (b.z + 0.5) * h, h);
+            if (l > 0) {
+                const int par = pl->lvl_first[l - 1] + idx[l - 1].at(bkey(b.x >> 1, b.y >> 1, b.z >> 1));
+                const int oct = int(b.x & 1) | (int(b.y & 1) << 1) | (int(b.z & 1) << 2);
+                box_parent[g] = par;
+                box_oct[g] = oct;
+                kids[par].push_back({g, oct});
+            }
+        }
+    }
+    std::vector<int> child_ptr(nbox + 1, 0), child_box, child_oct;
+    for (int g = 0; g < nbox; ++g) {
+        for (auto& kc : kids[g]) { child_box.push_back(kc.first); child_oct.push_back(kc.second); }
+        child_ptr[g + 1] = int(child_box.size());
+    }
+    // leaf neighbour lists (27 boxes incl. self)
+    const int nleaf = int(lev[D].size());
+    std::vector<int> nb_ptr(nleaf + 1, 0), nb_leaf;
+    for (int q = 0; q < nleaf; ++q) {
+        const Box& b = lev[D][q];
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const long x = long(b.x) + dx, y = long(b.y) + dy, z = long(b.z) + dz;
+                    const long cells = 1L << D;
+                    if (x < 0 || y < 0 || z < 0 || x >= cells || y >= cells || z >= cells) continue;
+                    auto it = idx[D].find(bkey(uint32_t(x), uint32_t(y), uint32_t(z)));
+                    if (it != idx[D].end()) nb_leaf.push_back(it->second);
+                }
+        nb_ptr[q + 1] = int(nb_leaf.size());
+    }
+    // interaction lists per level (l >= 2), grouped by offset into chunks
+    constexpr int G = FmmPlan::CHUNK;
+    std::vector<int> pair_src, pair_tgt;
+    std::vector<int4> chunks;
+    pl->lvl_chunk.assign(D + 2, 0);
+    for (int l = 0; l <= D; ++l) {
+        pl->lvl_chunk[l] = int(chunks.size());
+        if (l < 2) continue;
+        std::vector<std::vector<std::pair<int, int>>> byoff(343);
+        for (size_t q = 0; q < lev[l].size(); ++q) {
+            const Box& b = lev[l][q];
+            const long px = b.x >> 1, py = b.y >> 1, pz = b.z >> 1;
+            const long pcells = 1L << (l - 1);
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const long X = px + dx, Y = py + dy, Z = pz + dz;
+                        if (X < 0 || Y < 0 || Z < 0 || X >= pcells || Y >= pcells || Z >= pcells) continue;
+                        auto pit = idx[l - 1].find(bkey(uint32_t(X), uint32_t(Y), uint32_t(Z)));
+                        if (pit == idx[l - 1].end()) continue;
+                        const int pg = pl->lvl_first[l - 1] + pit->second;
+                        for (auto& kc : kids[pg]) {
+                            const int sg = kc.first;
+                            const Box& sb = lev[l][sg - pl->lvl_first[l]];
+                            const int tx = int(long(sb.x) - long(b.x)), ty = int(long(sb.y) - long(b.y)),
+                                      tz = int(long(sb.z) - long(b.z));
+                            if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) <= 1) continue;
+                            const int off = (tx + 3) * 49 + (ty + 3) * 7 + (tz + 3);
+                            byoff[off].push_back({pl->lvl_first[l] + int(q), sg});
+                        }
+                    }
+        }
+        for (int off = 0; off < 343; ++off) {
+            auto& v = byoff[off];
+            for (size_t k = 0; k < v.size(); k += G) {
+                const int b0 = int(pair_src.size());
+                for (size_t j = k; j < std::min(v.size(), k + G); ++j) {
+                    pair_tgt.push_back(v[j].first);
+                    pair_src.push_back(v[j].second);
+                }
+                chunks.push_back(make_int4(off, b0, int(pair_src.size()), l));
+            }
+        }
+    }
+    pl->lvl_chunk[D + 1] = int(chunks.size());
+    pl->n_pairs = int(pair_src.size());
+    // 1-D Chebyshev tables
+    const int P = order;
+    std::vector<double> xi(P), tn(P * P), t_oct(2 * P * P);
+    for (int k = 0; k < P; ++k) xi[k] = std::cos((2.0 * k + 1.0) * PI / (2.0 * P));
+    for (int k = 0; k < P; ++k)
+        for (int j = 0; j < P; ++j) tn[k * P + j] = std::cos(j * (2.0 * k + 1.0) * PI / (2.0 * P));
+    auto S1 = [&](int k, double u) {
+        double a = 0, tm2 = 1, tm1 = u;
+        for (int j = 1; j < P; ++j) {
+            const double tj = (j == 1) ? u : 2 * u * tm1 - tm2;
+            if (j >= 2) { tm2 = tm1; tm1 = tj; }
+            a += tn[k * P + j] * tj;
+        }
+        return 1.0 / P + 2.0 / P * a;
+    };
+    // t_oct[o][a][a2] = S_P(xi_a, (xi_a2 + (o ? 1 : -1)) / 2)
+    for (int o = 0; o < 2; ++o)
+        for (int a = 0; a < P; ++a)
+            for (int a2 = 0; a2 < P; ++a2)
+                t_oct[(o * P + a) * P + a2] = S1(a, (xi[a2] + (o ? 1.0 : -1.0)) / 2.0);
+    pl->box_h = box_h;
+    pl->edge = edge;
+    pl->nbox = nbox;
+    pl->nleaf = nleaf;
+    pl->dv.D = D;
+    pl->dv.leaf_first = pl->lvl_first[D];
+    upload(pl->dv.box_c, box_c, s);
+    upload(pl->dv.box_h, box_h, s);
+    upload(pl->dv.box_parent, box_parent, s);
+    upload(pl->dv.box_oct, box_oct, s);
+    upload(pl->dv.child_ptr, child_ptr, s);
+    upload(pl->dv.child_box, child_box, s);
+    upload(pl->dv.child_oct, child_oct, s);
+    upload(pl->dv.leaf_src, leaf_src, s);
+    upload(pl->dv.perm, perm, s);
+    upload(pl->dv.nb_ptr, nb_ptr, s);
+    upload(pl->dv.nb_leaf, nb_leaf, s);
+    upload(pl->dv.pair_src, pair_src, s);
+    upload(pl->dv.pair_tgt, pair_tgt, s);
+    upload(pl->dv.tn, tn, s);
+    upload(pl->dv.t_oct, t_oct, s);
+    upload(pl->d_chunks, chunks, s);
+    upload(pl->d_xi, xi, s);
+    pl->ktab64.assign(D + 2, nullptr);
+    pl->ktab32.assign(D + 2, nullptr);
+    hip_check(hipMalloc(&pl->src, sizeof(Src64) * std::max(N, 1)), "hipMalloc fmm src");
+    const size_t ne = size_t(nbox) * 3 * NP;
+    pl->n_exp = ne;
+    hip_check(hipStreamSynchronize(s), "sync fmm plan");
+    (void)NP;
+    return pl;
+}
+
+void fmm_plan_destroy(FmmPlan* p) { delete p; }
+
+template <typename T, int P>
+static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, hipStream_t s) {
+    constexpr int NP = P * P * P;
+    const size_t n = size_t(343) * NP * NP;
+    const int blocks = int((n + 255) / 256);
+    for (auto& p : tabs) free_dev(p);
+    tabs.assign(pl->D + 2, nullptr);
+    // slot D+1: Laplace (level independent, h = 1); slot l: Yukawa at level l
+    hip_check(hipMalloc(&tabs[pl->D + 1], n * sizeof(T)), "hipMalloc ktab");
+    hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, 1.0, 0.0, 1,
+                       tabs[pl->D + 1]);
+    for (int l = 2; l <= pl->D; ++l) {
+        hip_check(hipMalloc(&tabs[l], n * sizeof(T)), "hipMalloc ktab");
+        hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, pl->box_h[l],
+                           kappa, 0, tabs[l]);
+    }
+    hip_check(hipGetLastError(), "k_ktab launch");
+}
+
+void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s) {
+    if (pl->tables_kappa == kappa && (!fp64 || pl->have64) && (!fp32 || pl->have32)) return;
+    if (pl->tables_kappa != kappa) {
+        pl->have64 = pl->have32 = false;
+        pl->tables_kappa = kappa;
+    }
+    if (fp64 && !pl->have64) {
+        switch (pl->P) {
+            case 3: make_tables<double, 3>(pl, kappa, pl->ktab64, s); break;
+            case 4: make_tables<double, 4>(pl, kappa, pl->ktab64, s); break;
+            case 5: make_tables<double, 5>(pl, kappa, pl->ktab64, s); break;
+            case 6: make_tables<double, 6>(pl, kappa, pl->ktab64, s); break;
+        }
+        if (!pl->exp64) {
+            hip_check(hipMalloc(&pl->exp64, 2 * pl->n_exp * sizeof(double)), "hipMalloc fmm exp");
+        }
+        pl->have64 = true;
+    }
+    if (fp32 && !pl->have32) {
+        switch (pl->P) {
+            case 3: make_tables<float, 3>(pl, kappa, pl->ktab32, s); break;
+            case 4: make_tables<float, 4>(pl, kappa, pl->ktab32, s); break;
+            case 5: make_tables<float, 5>(pl, kappa, pl->ktab32, s); break;
+            case 6: make_tables<float, 6>(pl, kappa, pl->ktab32, s); break;
+        }
+        if (!pl->exp32) {
+            hip_check(hipMalloc(&pl->exp32, 2 * pl->n_exp * sizeof(float)), "hipMalloc fmm exp");
+        }
+        pl->have32 = true;
+    }
+    hip_check(hipStreamSynchronize(s), "sync fmm tables");
+}
+
+template <typename T, int P>
+static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const double* x,
+                     double kappa, T* M, T* L, const std::vector<T*>& tabs, double* y) {
+    constexpr int NP = P * P * P;
+    constexpr int NT = ((NP + 63) / 64) * 64;
+    constexpr int MB = (NP <= 64) ? NP : 32;
+    const FmmDev& d = pl->dv;
+    launch_far_pack(s, pl->N, patches, d.perm, x, pl->src);
+    {
+        const size_t n = pl->n_exp;
+        hipLaunchKernelGGL((k_zero<T>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, L);
+    }
+    const int D = pl->D;
+    hipLaunchKernelGGL((k_p2m<T, P>), dim3(pl->nleaf), dim3(NT), 0, s, d, pl->src, M);
+    // parents at levels D-1 .. 2 (level-1 multipoles are never translated)
+    for (int l = D; l >= 3; --l) {
+        const int first = pl->lvl_first[l - 1], cnt = pl->lvl_first[l] - first;
+        if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, M);
+    }
+    for (int l = 2; l <= D; ++l) {
+        const int c0 = pl->lvl_chunk[l], c1 = pl->lvl_chunk[l + 1];
+        if (c1 > c0)
+            hipLaunchKernelGGL((k_m2l<T, P, FmmPlan::CHUNK, MB>), dim3(c1 - c0), dim3(NT), 0, s, d, l,
+                               pl->d_chunks + c0, tabs[D + 1], tabs[l], M, L);
+    }
+    for (int l = 2; l < D; ++l) {
+        const int first = pl->lvl_first[l + 1], cnt = pl->lvl_first[l + 2] - first;
+        if (cnt > 0) hipLaunchKernelGGL((k_l2l<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, L);
+    }
+    hipLaunchKernelGGL((k_leaf<T, P>), dim3(pl->nleaf), dim3(256), 0, s, d, pl->src, patches,
+                       T(kappa), D >= 2 ? 1 : 0, L, y, pl->N);
+    hip_check(hipGetLastError(), "fmm launch");
+}
+
+void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, const double* x,
+                  double kappa, double* y) {
+    if (fp32) {
+        if (!pl->have32) throw std::runtime_error("FMM FP32 tables not prepared");
+        float* M = pl->exp32;
+        float* L = pl->exp32 + pl->n_exp;
+        switch (pl->P) {
+            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
+            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
+            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
+            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
+        }
+    } else {
+        if (!pl->have64) throw std::runtime_error("FMM FP64 tables not prepared");
+        double* M = pl->exp64;
+        double* L = pl->exp64 + pl->n_exp;
+        switch (pl->P) {
+            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
+            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
+            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
+            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
+        }
+    }
+}
+
+}  // namespace beepdev

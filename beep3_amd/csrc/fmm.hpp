@@ -1,0 +1,62 @@
+// Plan and launch interface of the Chebyshev-interpolation FMM (fmm.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device.hpp"
+
+namespace beepdev {
+
+// Device view of the plan (passed by value to kernels).
+struct FmmDev {
+    int D = 0;             // leaf level
+    int leaf_first = 0;    // global index of the first leaf box
+    double4* box_c = nullptr;   // centre xyz, edge
+    double* box_h = nullptr;    // edge per level
+    int* box_parent = nullptr;
+    int* box_oct = nullptr;     // octant of a box within its parent
+    int* child_ptr = nullptr;   // per box
+    int* child_box = nullptr;
+    int* child_oct = nullptr;
+    int* leaf_src = nullptr;    // per leaf: first source in FMM order (+ end)
+    int* perm = nullptr;        // FMM order -> patch index
+    int* nb_ptr = nullptr;      // per leaf: neighbour leaf list
+    int* nb_leaf = nullptr;
+    int* pair_src = nullptr;    // M2L pairs grouped by offset
+    int* pair_tgt = nullptr;
+    double* tn = nullptr;       // T_j(xi_k), P x P
+    double* t_oct = nullptr;    // child-node interpolation factors, 2 x P x P
+};
+
+struct FmmPlan {
+    static constexpr int CHUNK = 16;  // M2L pairs per block
+    int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0;
+    double edge = 0;
+    size_t n_exp = 0;                 // values per expansion set (nbox * 3 * P^3)
+    std::vector<int> lvl_first;       // D + 2
+    std::vector<int> lvl_chunk;       // D + 2
+    std::vector<double> box_h;
+    FmmDev dv;
+    int4* d_chunks = nullptr;
+    double* d_xi = nullptr;
+    std::vector<double*> ktab64;      // [l] Yukawa at level l, [D+1] Laplace
+    std::vector<float*> ktab32;
+    double* exp64 = nullptr;          // M then L
+    float* exp32 = nullptr;
+    Src64* src = nullptr;
+    bool have64 = false, have32 = false;
+    double tables_kappa = -1;
+    ~FmmPlan();
+};
+
+FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
+                         hipStream_t s);
+void fmm_plan_destroy(FmmPlan* p);
+void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
+// y += far field (1-point interaction of every pair) of x
+void fmm_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, const double* x,
+                  double kappa, double* y);
+
+}  // namespace beepdev

@@ -30,14 +30,14 @@ FP64_VECTOR_TFLOPS = 78.6
 FLOP_PER_EVAL = 92  # integrand evaluation (qual point x quad point)
 
 
-def build_workload(name):
+def build_workload(name, **kw):
     from beep3_amd import configs
     if name == "globular":
-        return configs.globular(), "globular surface r=25 A, 163,842 patches, 2,000 charges"
+        return configs.globular(**kw), "globular surface r=25 A, 163,842 patches, 2,000 charges"
     if name == "off_centre":
-        return configs.off_centre(), "icosphere r=20 A, 40,962 patches, 200 charges"
+        return configs.off_centre(**kw), "icosphere r=20 A, 40,962 patches, 200 charges"
     if name == "born":
-        return configs.born_ion(), "icosphere r=2 A, 2,562 patches, 1 charge"
+        return configs.born_ion(**kw), "icosphere r=2 A, 2,562 patches, 1 charge"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -105,6 +105,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--far", default="fmm", choices=["fmm", "direct", "auto"])
+    ap.add_argument("--order", type=int, default=5)
+    ap.add_argument("--leaf", type=int, default=160)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
@@ -113,7 +116,8 @@ def main():
         sys.exit(2)
 
     t_setup = time.perf_counter()
-    b, workload = build_workload(args.workload)
+    b, workload = build_workload(args.workload, far_field=args.far, fmm_order=args.order,
+                                 fmm_leaf=args.leaf)
     b.init(precisions=["fp64", "fp32"])
     setup_s = time.perf_counter() - t_setup
     print(f"[bench] setup done in {setup_s:.1f}s: {b.run_info}", file=sys.stderr, flush=True)
@@ -182,9 +186,19 @@ def main():
                             "rel_residual": ri["rel_residual"], "energy_kJ_mol": E}
             print(f"[bench] solve {mode}: {solves[mode]}", file=sys.stderr, flush=True)
         solves["mixed_speedup_vs_fp64"] = solves["fp64"]["solve_ms"] / solves["mixed"]["solve_ms"]
-        solves["energy_rel_diff_mixed_vs_fp64"] = abs(
-            solves["mixed"]["energy_kJ_mol"] - solves["fp64"]["energy_kJ_mol"]) / abs(
-            solves["fp64"]["energy_kJ_mol"])
+        # reference energy: FP64 coefficients, exact direct far field (the
+        # explicit-pair + 1-point operator the oracle defines), FP64 GMRES
+        if args.far != "direct":
+            dev.set_far_field("direct")
+            b.solve(1e-8, 100, mode="fp64")
+            E_ref = b.calculate_energies()
+            dev.set_far_field(args.far, args.order, args.leaf)
+        else:
+            E_ref = solves["fp64"]["energy_kJ_mol"]
+        solves["reference_energy_kJ_mol"] = E_ref
+        for mode in ("fp64", "mixed"):
+            solves[mode]["energy_rel_err_vs_reference"] = abs(
+                solves[mode]["energy_kJ_mol"] - E_ref) / abs(E_ref)
         extra["solve"] = solves
 
     cpu = None
@@ -201,7 +215,8 @@ def main():
         "data": "synthetic (seeded mesh, charges and x)",
         "config": {"workload": workload, "patches": N, "quad_rule": 6, "qual_rule": 0,
                    "nbsize": b.nbsize, "kappa": b.kappa, "eps_in": 2.0, "eps_out": 80.0,
-                   "far_field": "direct 1-point sum"},
+                   "far_field": (f"Chebyshev FMM order {args.order}, leaf target {args.leaf}"
+                                 if args.far == "fmm" else args.far)},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out))

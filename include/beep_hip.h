@@ -126,6 +126,19 @@ int beep_hip_destroy(beep_hip_ctx* ctx);
 int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* geom);
 int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
 
+/* Far-field engine for the 1-point interaction of all pairs (the role of
+ * FMM_BEM_Octree::solve, fmm_bem_hybrid.h:184-279):
+ *   BEEP_FAR_DIRECT  exact O(N^2) sum,
+ *   BEEP_FAR_FMM     Chebyshev-interpolation FMM of the given order (3..6),
+ *                    leaf boxes holding about leaf_target patches,
+ *   BEEP_FAR_AUTO    FMM above 20,000 patches, direct below.
+ * Takes effect immediately (kernel tables are built by the first mat-vec or
+ * precompute that needs them); default BEEP_FAR_AUTO, order 4, leaf_target 48. */
+#define BEEP_FAR_AUTO (-1)
+#define BEEP_FAR_DIRECT 0
+#define BEEP_FAR_FMM 1
+int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target);
+
 /* Singular self terms (FP64 always) and near-field coefficients of the given
  * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set). */
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);

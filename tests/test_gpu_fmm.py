@@ -1,0 +1,67 @@
+"""GPU: the Chebyshev FMM far field against the exact direct far field (both
+through the C ABI), at the 40,962-patch off-centre workload."""
+import numpy as np
+import pytest
+
+from beep3_amd import configs
+from beep3_amd.device import Device
+from beep3_amd.scene import Scene
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def off_centre_arrays():
+    b = configs.off_centre()
+    sc = Scene(b.Dsolvent, b.kappa, b.quad, b.qual, b.nbsize)
+    sc.add_instance(b.library[0], 4.0)
+    sc.build()
+    return sc.arrays(), b.kappa
+
+
+def _device(a, kappa, mode, order=4, leaf=48, precs=("fp64",)):
+    d = Device(0)
+    d.set_far_field(mode, order, leaf)
+    d.set_geometry(a)
+    d.set_tree(a)
+    for p in precs:
+        d.precompute(kappa, p)
+    return d
+
+
+def test_fmm_vs_direct_fp64(off_centre_arrays):
+    a, kappa = off_centre_arrays
+    x = np.random.default_rng(3).normal(size=2 * a["area"].shape[0])
+    dd = _device(a, kappa, "direct")
+    y_dir = dd.matvec(x, "fp64")
+    # far-only reference: remove the near + diagonal part (far disabled by
+    # comparing against the same operator with a zero far field is not
+    # exposed), so report relative to the whole product
+    errs = {}
+    for order in (3, 4, 5, 6):
+        df = _device(a, kappa, "fmm", order)
+        y = df.matvec(x, "fp64")
+        errs[order] = rel(y, y_dir)
+        df.close()
+    dd.close()
+    print("FMM fp64 relative error vs direct, by order:", errs)
+    assert errs[4] < 1e-3
+    assert errs[6] < errs[3]
+
+
+def test_fmm_fp32_and_resid(off_centre_arrays):
+    a, kappa = off_centre_arrays
+    x = np.random.default_rng(4).normal(size=2 * a["area"].shape[0])
+    dd = _device(a, kappa, "direct", precs=("fp64", "fp32"))
+    df = _device(a, kappa, "fmm", 5, precs=("fp64", "fp32"))
+    out = {}
+    for p in ("fp64", "fp32", "fp32_resid"):
+        out[p] = rel(df.matvec(x, p), dd.matvec(x, "fp64"))
+    print("FMM order 5 vs direct fp64:", out)
+    assert out["fp32"] < 1e-3
+    dd.close()
+    df.close()
