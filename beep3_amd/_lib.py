@@ -85,6 +85,13 @@ class MatvecTiming(C.Structure):
     ]
 
 
+class FarInfo(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("order", C.c_int), ("leaf_level", C.c_int), ("n_leaf_boxes", C.c_int),
+        ("n_boxes", C.c_int), ("m2l_pairs", C.c_longlong), ("p2p_pairs", C.c_longlong),
+    ]
+
+
 def _declare(L):
     L.beep_host_last_error.restype = C.c_char_p
     L.beep_mesh_icosphere.argtypes = [C.c_int, C.c_double, C.POINTER(vp)]
@@ -94,6 +101,9 @@ def _declare(L):
     L.beep_mesh_charges_walk.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64]
     L.beep_mesh_info.argtypes = [vp, ip, ip, ip, dp, dp]
     L.beep_mesh_vertices.argtypes = [vp, dp, ip]
+    L.beep_mesh_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.beep_mesh_write_off.argtypes = [vp, C.c_char_p]
+    L.beep_mesh_read_charges.argtypes = [vp, C.c_char_p]
     L.beep_mesh_destroy.argtypes = [vp]
     L.beep_mesh_destroy.restype = None
     L.beep_scene_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
@@ -120,6 +130,7 @@ def _declare_hip(L):
     L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
     L.beep_hip_matvec_timing.argtypes = [vp, C.POINTER(MatvecTiming)]
@@ -127,6 +138,7 @@ def _declare_hip(L):
     L.beep_hip_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
     L.beep_hip_energy_coefficients.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp]
     L.beep_hip_solve.argtypes = [vp, dp, dp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.beep_hip_selftest_mfma.argtypes = [vp, C.c_int, dp, dp, dp]
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.beep_hip_device_free.argtypes = [vp, vp]
     L.beep_hip_memcpy.argtypes = [vp, vp, vp, C.c_size_t, C.c_int]

@@ -71,8 +71,12 @@ struct beep_hip_ctx {
     std::vector<int> h_leaf_ptr, h_nb_ptr;
     DevBuf<int> leaf_ptr, leaf_idx, nb_ptr, nb_idx;
     DevBuf<int64_t> coef_off;
-    DevBuf<int> coef_blk, spmv_blk;
-    int n_coef_blocks = 0, n_spmv_blocks = 0;
+    DevBuf<int> coef_blk;
+    int n_coef_blocks = 0;
+    DevBuf<int4> spmv_work, spmv_groups;
+    DevBuf<int> spmv_group_nblk;
+    DevBuf<double> spmv_part;
+    SpmvPlan spmv;
     int64_t nnz = 0;
     bool have_tree = false;
 
@@ -199,9 +203,9 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     hipEvent_t* ev = c->ev[c->ring_head];
     hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
-    launch_near_spmv(s, mode, c->N, c->n_spmv_blocks, c->L, c->spmv_blk.p, c->leaf_ptr.p,
-                     c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p, c->coef_off.p, c->coef32.p,
-                     c->coef64.p, c->diag.p, dx, dy);
+    launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
+                     c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
+                     c->spmv_part.p, dy);
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->use_fmm) {
         // kernel tables are built on first use for the current kappa
@@ -435,7 +439,9 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         for (int k = 0; k < nnb; ++k)
             require(t->nb_idx[k] >= 0 && t->nb_idx[k] < N, "neighbourhood index out of range");
         std::vector<int64_t> off(L + 1, 0);
-        std::vector<int> cblk(L + 1, 0), sblk(L + 1, 0);
+        std::vector<int> cblk(L + 1, 0);
+        std::vector<int4> work, groups;
+        std::vector<int> gnb;
         for (int l = 0; l < L; ++l) {
             const int64_t R = t->leaf_ptr[l + 1] - t->leaf_ptr[l];
             const int64_t C = t->nb_ptr[l + 1] - t->nb_ptr[l];
@@ -444,7 +450,13 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
             const int64_t nb = (R * C + 255) / 256;
             require(int64_t(cblk[l]) + nb < (int64_t(1) << 31), "too many coefficient blocks");
             cblk[l + 1] = cblk[l] + int(nb);
-            sblk[l + 1] = sblk[l] + int((R + 255) / 256);
+            for (int r0 = 0; r0 < R; r0 += 256) {
+                const int first = int(work.size());
+                for (int c0 = 0; c0 < C; c0 += SPMV_CW)
+                    work.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
+                groups.push_back(make_int4(l, r0, int(std::min<int64_t>(256, R - r0)), first));
+                gnb.push_back(int(work.size()) - first);
+            }
         }
         hipStream_t s = ctx->stream;
         ctx->leaf_ptr.upload(t->leaf_ptr, L + 1, s);
@@ -453,12 +465,19 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         ctx->nb_idx.upload(t->nb_idx, nnb, s);
         ctx->coef_off.upload(off.data(), L + 1, s);
         ctx->coef_blk.upload(cblk.data(), L + 1, s);
-        ctx->spmv_blk.upload(sblk.data(), L + 1, s);
+        ctx->spmv_work.upload(work.data(), work.size(), s);
+        ctx->spmv_groups.upload(groups.data(), groups.size(), s);
+        ctx->spmv_group_nblk.upload(gnb.data(), gnb.size(), s);
+        ctx->spmv_part.alloc(work.size() * 256 * 2);
         hip_check(hipStreamSynchronize(s), "sync tree upload");
         ctx->L = L;
         ctx->nnz = off[L];
         ctx->n_coef_blocks = cblk[L];
-        ctx->n_spmv_blocks = sblk[L];
+        ctx->spmv.work = ctx->spmv_work.p;
+        ctx->spmv.n_work = int(work.size());
+        ctx->spmv.groups = ctx->spmv_groups.p;
+        ctx->spmv.group_nblk = ctx->spmv_group_nblk.p;
+        ctx->spmv.n_groups = int(groups.size());
         ctx->h_leaf_ptr.assign(t->leaf_ptr, t->leaf_ptr + L + 1);
         ctx->h_nb_ptr.assign(t->nb_ptr, t->nb_ptr + L + 1);
         ctx->have_tree = true;
@@ -530,6 +549,22 @@ int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_targ
         ctx->fmm_order = order;
         ctx->fmm_leaf = leaf_target;
         if (ctx->have_geom) far_setup(ctx);  // kernel tables follow on first use
+    });
+}
+
+int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
+    return guarded(ctx, [&] {
+        require(out != nullptr, "null output");
+        *out = beep_far_info{};
+        out->mode = ctx->use_fmm ? BEEP_FAR_FMM : BEEP_FAR_DIRECT;
+        if (ctx->use_fmm && ctx->fmm) {
+            out->order = ctx->fmm->P;
+            out->leaf_level = ctx->fmm->D;
+            out->n_leaf_boxes = ctx->fmm->nleaf;
+            out->n_boxes = ctx->fmm->nbox;
+            out->m2l_pairs = ctx->fmm->n_pairs;
+            out->p2p_pairs = ctx->fmm->n_p2p;
+        }
     });
 }
 
@@ -693,6 +728,22 @@ int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_sol
         stats.matvecs = matvecs;
         stats.time_ms = now_ms() - t0;
         if (st) *st = stats;
+    });
+}
+
+int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
+                           double* d) {
+    return guarded(ctx, [&] {
+        require(a && b && d, "null array");
+        set_device(ctx);
+        DevBuf<double> da, db, dd;
+        da.upload(a, 64, ctx->stream);
+        db.upload(b, 64, ctx->stream);
+        dd.alloc(256);
+        launch_mfma_selftest(ctx->stream, da.p, db.p, fp32, dd.p);
+        hip_check(hipMemcpyAsync(d, dd.p, 256 * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H selftest");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync selftest");
     });
 }
 

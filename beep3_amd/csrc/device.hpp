@@ -35,6 +35,17 @@ struct alignas(16) Src64 {
     double dx, dy, dz, sh;
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  xcd_remap turns block b of n into a logical index such
+// that consecutive logical indices run on the same XCD, so blocks that share
+// data (same M2L offset, neighbouring leaves) share one L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+    constexpr int NX = 8;
+    const int q = n / NX, r = n % NX;
+    const int x = b % NX, k = b / NX;
+    return x * q + min(x, r) + k;
+}
+
 inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -56,13 +67,25 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
                              const Pt64* quad, double kappa, int subtract_1pt, double2* coef);
 
+// Work decomposition of the near-field SpMV (built on the host from the
+// tree): work items (leaf, first row, first column, columns) of <= 256 rows x
+// <= SPMV_CW columns, and row groups (leaf, first row, rows, first work item)
+// with their work-item counts.
+constexpr int SPMV_CW = 512;
+struct SpmvPlan {
+    const int4* work = nullptr;
+    int n_work = 0;
+    const int4* groups = nullptr;
+    const int* group_nblk = nullptr;
+    int n_groups = 0;
+};
 // y = near(x) + diag(x), overwriting y.  mode: 0 FP64 coef/FP64 sums,
-// 1 FP32 coef/FP32 sums, 2 FP32 coef/FP64 sums
-void launch_near_spmv(hipStream_t s, int mode, int n, int n_blocks, int n_leaves,
-                      const int* blk_ptr, const int* leaf_ptr, const int* leaf_idx,
-                      const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
-                      const float4* coef32, const double2* coef64, const double* diag4,
-                      const double* x, double* y);
+// 1 FP32 coef/FP32 sums, 2 FP32 coef/FP64 sums.  part: scratch of
+// n_work * 256 * 2 doubles.
+void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
+                      const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
+                      const int64_t* coef_off, const float4* coef32, const double2* coef64,
+                      const double* diag4, const double* x, void* part, double* y);
 
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,
@@ -88,6 +111,7 @@ void launch_gemv_sub(hipStream_t s, int n, const double* V, int ncols, const dou
                      double* w);
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
 void launch_fill(hipStream_t s, int n, double v, double* y);
+void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out);
 void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
                        double scale, double* y);
 

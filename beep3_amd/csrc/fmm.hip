@@ -196,62 +196,124 @@ __global__ __launch_bounds__(256) void k_m2m(const FmmDev d, int first, T* __res
     }
 }
 
-// M2L: one block per chunk of G pairs sharing one offset at one level.
-// K rows are staged through LDS in blocks of MB source nodes; every thread
-// (target node n) keeps G x 3 partial sums, then adds them to L atomically.
-template <typename T, int P, int G, int MB>
-__global__ __launch_bounds__(256) void k_m2l(const FmmDev d, int level, const int4* __restrict__ chunks,
-                                             const T* __restrict__ kl, const T* __restrict__ ky,
-                                             const T* __restrict__ M, T* __restrict__ L) {
+// M2L on the matrix cores: one block per chunk of up to 16 pairs that share
+// one offset at one level.  Per chunk, C_e (P^3 x 16) = K_e^T (P^3 x P^3) .
+// M_e (P^3 x 16) for e = Laplace (K_L) and the two Yukawa sets (K_Y), as
+// 16x16x4 MFMA tiles: A = kernel table rows (read from the L2-resident table),
+// B = source expansions staged in LDS as [e][m][g].  Results are added to the
+// target locals atomically (several offsets reach one target).
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    using acc_t = mf64x4;
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds row l/16 + 4r (column l % 16)
+    static __device__ __forceinline__ int row(int l, int r) { return l / 16 + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    using acc_t = mf32x4;
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator element r of lane l holds row 4 (l/16) + r (column l % 16)
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l / 16) + r; }
+};
+
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
+                                             const T* __restrict__ kl,
+                                             const T* const* __restrict__ ytabs,
+                                             const T* __restrict__ M, T* __restrict__ Cout) {
     constexpr int NP = P * P * P;
-    __shared__ T sKL[MB][NP];
-    __shared__ T sKY[MB][NP];
-    __shared__ T sM[G][3][MB];
-    const int4 ch = chunks[blockIdx.x];  // offset id, pair begin, pair end, unused
-    const int off = ch.x, p0 = ch.y, np = ch.z - ch.y;
+    constexpr int NPP = (NP + 15) / 16 * 16;
+    constexpr int NTILE = NPP / 16;
+    constexpr int NTW = (NTILE + 3) / 4;  // n tiles per wave
+    constexpr int G = 16;
+    using acc_t = typename Mfma<T>::acc_t;
+    __shared__ T sM[3][NPP][G];  // B operands, then the result tile [3][G][NPP]
+    const int4 ch = chunks[xcd_remap(blockIdx.x, gridDim.x)];  // offset, pair begin/end, level
+    const int off = ch.x, p0 = ch.y, np = ch.z - ch.y, level = ch.w;
     const int t = threadIdx.x;
+    for (int q = t; q < 3 * NPP * G; q += 256) {
+        const int e = q / (NPP * G), rem = q - e * NPP * G, m = rem / G, g = rem - m * G;
+        T v = T(0);
+        if (g < np && m < NP) v = M[size_t(d.pair_src[p0 + g]) * 3 * NP + e * NP + m];
+        sM[e][m][g] = v;
+    }
+    __syncthreads();
+    const int w = t >> 6, l = t & 63;
     const T* KL = kl + size_t(off) * NP * NP;
-    const T* KY = ky + size_t(off) * NP * NP;
-    T acc[G][3];
+    const T* KY = ytabs[level] + size_t(off) * NP * NP;
+    acc_t aL[NTW], a1[NTW], a2[NTW];
 #pragma unroll
-    for (int g = 0; g < G; ++g) acc[g][0] = acc[g][1] = acc[g][2] = T(0);
-    for (int m0 = 0; m0 < NP; m0 += MB) {
-        const int mb = min(MB, NP - m0);
-        __syncthreads();
-        for (int q = t; q < mb * NP; q += blockDim.x) {
-            const int mm = q / NP, nn = q - mm * NP;
-            sKL[mm][nn] = KL[size_t(m0 + mm) * NP + nn];
-            sKY[mm][nn] = KY[size_t(m0 + mm) * NP + nn];
-        }
-        for (int q = t; q < G * 3 * mb; q += blockDim.x) {
-            const int g = q / (3 * mb), r = q - g * 3 * mb, e = r / mb, mm = r - e * mb;
-            T v = T(0);
-            if (g < np) v = M[size_t(d.pair_src[p0 + g]) * 3 * NP + e * NP + m0 + mm];
-            sM[g][e][mm] = v;
-        }
-        __syncthreads();
-        if (t < NP) {
-            for (int mm = 0; mm < mb; ++mm) {
-                const T a = sKL[mm][t], b = sKY[mm][t];
+    for (int k = 0; k < NTW; ++k) {
+        aL[k] = acc_t{0, 0, 0, 0};
+        a1[k] = acc_t{0, 0, 0, 0};
+        a2[k] = acc_t{0, 0, 0, 0};
+    }
+    const int kl16 = l / 16, il = l % 16;
+    for (int m0 = 0; m0 < NPP; m0 += 4) {
+        const int m = m0 + kl16;
+        const T bL = sM[0][m][il], b1 = sM[1][m][il], b2 = sM[2][m][il];
 #pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    acc[g][0] += a * sM[g][0][mm];
-                    acc[g][1] += b * sM[g][1][mm];
-                    acc[g][2] += b * sM[g][2][mm];
+        for (int k = 0; k < NTW; ++k) {
+            const int nt = w + 4 * k;
+            if (nt < NTILE) {
+                const int n = nt * 16 + il;
+                T xa = T(0), ya = T(0);
+                if (m < NP && n < NP) {
+                    xa = KL[size_t(m) * NP + n];
+                    ya = KY[size_t(m) * NP + n];
                 }
+                aL[k] = Mfma<T>::op(xa, bL, aL[k]);
+                a1[k] = Mfma<T>::op(ya, b1, a1[k]);
+                a2[k] = Mfma<T>::op(ya, b2, a2[k]);
             }
         }
     }
-    if (t < NP) {
-        const T sl = T(1.0 / d.box_h[level]);
+    __syncthreads();  // sM is reused for the result tile
+    T* sC = &sM[0][0][0];  // [3][G][NPP]
+    const T sl = T(1.0 / d.box_h[level]);
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (g >= np) break;
-            T* out = L + size_t(d.pair_tgt[p0 + g]) * 3 * NP;
-            atomicAdd(out + t, acc[g][0] * sl);
-            atomicAdd(out + NP + t, acc[g][1]);
-            atomicAdd(out + 2 * NP + t, acc[g][2]);
+    for (int k = 0; k < NTW; ++k) {
+        const int nt = w + 4 * k;
+        if (nt >= NTILE) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int n = nt * 16 + Mfma<T>::row(l, r);
+            sC[(0 * G + il) * NPP + n] = aL[k][r] * sl;
+            sC[(1 * G + il) * NPP + n] = a1[k][r];
+            sC[(2 * G + il) * NPP + n] = a2[k][r];
         }
+    }
+    __syncthreads();
+    // per-pair slots C[pair][e][n], written contiguously
+    for (int q = t; q < np * 3 * NP; q += 256) {
+        const int g = q / (3 * NP), rem = q - g * 3 * NP, e = rem / NP, n = rem - e * NP;
+        Cout[size_t(p0 + g) * 3 * NP + rem] = sC[(e * G + g) * NPP + n];
+    }
+}
+
+// Sum the M2L slots of every target box into its local expansion (one block
+// per box at levels >= 2; pairs listed per target in tgt_ptr / tgt_pair).
+template <typename T, int P>
+__global__ __launch_bounds__(256) void k_m2l_sum(const FmmDev d, int first_box,
+                                                 const T* __restrict__ Cin, T* __restrict__ L) {
+    constexpr int NP = P * P * P;
+    const int box = first_box + blockIdx.x;
+    const int k0 = d.tgt_ptr[box], k1 = d.tgt_ptr[box + 1];
+    if (k0 == k1) return;
+    for (int q = threadIdx.x; q < 3 * NP; q += 256) {
+        T acc = T(0);
+        for (int k = k0; k < k1; ++k) acc += Cin[size_t(d.tgt_pair[k]) * 3 * NP + q];
+        L[size_t(box) * 3 * NP + q] += acc;
     }
 }
 
@@ -293,6 +355,9 @@ __global__ __launch_bounds__(256) void k_l2l(const FmmDev d, int first, T* __res
 }
 
 // Leaf pass: L2P + P2P for the targets of one leaf box, one write per target.
+// 256 threads = 64 target slots x 4 sub-lanes; the sub-lanes of a target
+// split the sources of every tile and the x-slices of the L2P sum, and are
+// combined through LDS at the end.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __restrict__ src,
                                               const Patch64* __restrict__ patches, T kappa,
@@ -300,21 +365,24 @@ __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __res
                                               double* __restrict__ y, int n) {
     constexpr int NP = P * P * P;
     constexpr int TILE = 256;
+    constexpr int SUB = 4;
     __shared__ SrcR<T> tile[TILE];
     __shared__ int tidx[TILE];
     __shared__ T lb[3 * NP];
     __shared__ T tn[P * P];
-    const int bl = blockIdx.x;
+    __shared__ T red[2][SUB][64];
+    const int bl = xcd_remap(blockIdx.x, gridDim.x);
     const int box = d.leaf_first + bl;
     const int s0 = d.leaf_src[bl], s1 = d.leaf_src[bl + 1];
     const double cx = d.box_c[box].x, cy = d.box_c[box].y, cz = d.box_c[box].z;
     const double inv_half = 2.0 / d.box_h[d.D];
     const int t = threadIdx.x;
+    const int slot = t & 63, sub = t >> 6;
     for (int k = t; k < P * P; k += blockDim.x) tn[k] = T(d.tn[k]);
     if (use_local)
         for (int q = t; q < 3 * NP; q += blockDim.x) lb[q] = L[size_t(box) * 3 * NP + q];
-    for (int tc = s0; tc < s1; tc += blockDim.x) {
-        const int ti = tc + t;
+    for (int tc = s0; tc < s1; tc += 64) {
+        const int ti = tc + slot;
         const bool valid = ti < s1;
         const int i = valid ? d.perm[ti] : -1;
         T xi = 0, yi = 0, zi = 0, n0x = 0, n0y = 0, n0z = 0;
@@ -339,7 +407,7 @@ __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __res
                 __syncthreads();
                 const int m = min(TILE, j1 - jt);
                 if (valid) {
-                    for (int jj = 0; jj < m; ++jj) {
+                    for (int jj = sub; jj < m; jj += SUB) {
                         if (tidx[jj] == i) continue;
                         const SrcR<T> q = tile[jj];
                         const T dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
@@ -365,7 +433,7 @@ __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __res
                 }
             }
         }
-        // L2P
+        // L2P, x-slices a = sub, sub + SUB, ...
         if (valid && use_local) {
             T sx[P], dsx[P], sy[P], dsy[P], sz[P], dsz[P];
             const T ih = T(inv_half);
@@ -373,7 +441,7 @@ __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __res
             basis1d<T, P>(yi * ih, tn, sy, dsy);
             basis1d<T, P>(zi * ih, tn, sz, dsz);
             T pL = 0, p1 = 0, gLx = 0, gLy = 0, gLz = 0, g2x = 0, g2y = 0, g2z = 0;
-            for (int a = 0; a < P; ++a)
+            for (int a = sub; a < P; a += SUB)
                 for (int b = 0; b < P; ++b) {
                     const T sab = sx[a] * sy[b], dab = dsx[a] * sy[b], adb = sx[a] * dsy[b];
                     for (int c = 0; c < P; ++c) {
@@ -390,7 +458,14 @@ __global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __res
             af += pL + p1;
             ah += ih * (n0x * (gLx + g2x) + n0y * (gLy + g2y) + n0z * (gLz + g2z));
         }
-        if (valid) {
+        red[0][sub][slot] = af;
+        red[1][sub][slot] = ah;
+        __syncthreads();
+        if (valid && sub == 0) {
+            for (int k = 1; k < SUB; ++k) {
+                af += red[0][k][slot];
+                ah += red[1][k][slot];
+            }
             y[i] += double(af) * FOUR_PI_INV;
             y[n + i] += double(ah) * FOUR_PI_INV;
         }
@@ -464,6 +539,8 @@ FmmPlan::~FmmPlan() {
     free_dev(dv.child_ptr); free_dev(dv.child_box); free_dev(dv.child_oct);
     free_dev(dv.leaf_src); free_dev(dv.perm); free_dev(dv.nb_ptr); free_dev(dv.nb_leaf);
     free_dev(dv.pair_src); free_dev(dv.pair_tgt); free_dev(dv.tn); free_dev(dv.t_oct);
+    free_dev(dv.tgt_ptr); free_dev(dv.tgt_pair); free_dev(ytab64); free_dev(ytab32);
+    free_dev(slots64); free_dev(slots32);
     free_dev(d_chunks); free_dev(d_xi);
     for (auto& p : ktab64) free_dev(p);
     for (auto& p : ktab32) free_dev(p);
@@ -590,6 +667,12 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                 }
         nb_ptr[q + 1] = int(nb_leaf.size());
     }
+    for (int q = 0; q < nleaf; ++q) {
+        long long srcs = 0;
+        for (int k = nb_ptr[q]; k < nb_ptr[q + 1]; ++k)
+            srcs += leaf_src[nb_leaf[k] + 1] - leaf_src[nb_leaf[k]];
+        pl->n_p2p += srcs * (leaf_src[q + 1] - leaf_src[q]);
+    }
     // interaction lists per level (l >= 2), grouped by offset into chunks
     constexpr int G = FmmPlan::CHUNK;
     std::vector<int> pair_src, pair_tgt;
@@ -636,6 +719,13 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     }
     pl->lvl_chunk[D + 1] = int(chunks.size());
     pl->n_pairs = int(pair_src.size());
+    std::vector<int> tgt_ptr(nbox + 1, 0), tgt_pair(pair_tgt.size());
+    for (int tg : pair_tgt) tgt_ptr[tg + 1]++;
+    for (int g = 0; g < nbox; ++g) tgt_ptr[g + 1] += tgt_ptr[g];
+    {
+        std::vector<int> fill(tgt_ptr.begin(), tgt_ptr.end() - 1);
+        for (size_t k = 0; k < pair_tgt.size(); ++k) tgt_pair[fill[pair_tgt[k]]++] = int(k);
+    }
     // 1-D Chebyshev tables
     const int P = order;
     std::vector<double> xi(P), tn(P * P), t_oct(2 * P * P);
@@ -675,6 +765,8 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     upload(pl->dv.nb_leaf, nb_leaf, s);
     upload(pl->dv.pair_src, pair_src, s);
     upload(pl->dv.pair_tgt, pair_tgt, s);
+    upload(pl->dv.tgt_ptr, tgt_ptr, s);
+    upload(pl->dv.tgt_pair, tgt_pair, s);
     upload(pl->dv.tn, tn, s);
     upload(pl->dv.t_oct, t_oct, s);
     upload(pl->d_chunks, chunks, s);
@@ -692,7 +784,8 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
 void fmm_plan_destroy(FmmPlan* p) { delete p; }
 
 template <typename T, int P>
-static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, hipStream_t s) {
+static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& ytab,
+                        hipStream_t s) {
     constexpr int NP = P * P * P;
     const size_t n = size_t(343) * NP * NP;
     const int blocks = int((n + 255) / 256);
@@ -708,6 +801,10 @@ static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, hipStr
                            kappa, 0, tabs[l]);
     }
     hip_check(hipGetLastError(), "k_ktab launch");
+    free_dev(ytab);
+    hip_check(hipMalloc(&ytab, tabs.size() * sizeof(T*)), "hipMalloc ytab");
+    hip_check(hipMemcpyAsync(ytab, tabs.data(), tabs.size() * sizeof(T*), hipMemcpyHostToDevice, s),
+              "H2D ytab");
 }
 
 void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s) {
@@ -718,25 +815,29 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
     }
     if (fp64 && !pl->have64) {
         switch (pl->P) {
-            case 3: make_tables<double, 3>(pl, kappa, pl->ktab64, s); break;
-            case 4: make_tables<double, 4>(pl, kappa, pl->ktab64, s); break;
-            case 5: make_tables<double, 5>(pl, kappa, pl->ktab64, s); break;
-            case 6: make_tables<double, 6>(pl, kappa, pl->ktab64, s); break;
+            case 3: make_tables<double, 3>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
+            case 4: make_tables<double, 4>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
+            case 5: make_tables<double, 5>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
+            case 6: make_tables<double, 6>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
         }
         if (!pl->exp64) {
             hip_check(hipMalloc(&pl->exp64, 2 * pl->n_exp * sizeof(double)), "hipMalloc fmm exp");
+            const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+            hip_check(hipMalloc(&pl->slots64, ns * sizeof(double)), "hipMalloc m2l slots");
         }
         pl->have64 = true;
     }
     if (fp32 && !pl->have32) {
         switch (pl->P) {
-            case 3: make_tables<float, 3>(pl, kappa, pl->ktab32, s); break;
-            case 4: make_tables<float, 4>(pl, kappa, pl->ktab32, s); break;
-            case 5: make_tables<float, 5>(pl, kappa, pl->ktab32, s); break;
-            case 6: make_tables<float, 6>(pl, kappa, pl->ktab32, s); break;
+            case 3: make_tables<float, 3>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
+            case 4: make_tables<float, 4>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
+            case 5: make_tables<float, 5>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
+            case 6: make_tables<float, 6>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
         }
         if (!pl->exp32) {
             hip_check(hipMalloc(&pl->exp32, 2 * pl->n_exp * sizeof(float)), "hipMalloc fmm exp");
+            const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+            hip_check(hipMalloc(&pl->slots32, ns * sizeof(float)), "hipMalloc m2l slots");
         }
         pl->have32 = true;
     }
@@ -745,10 +846,10 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
 
 template <typename T, int P>
 static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const double* x,
-                     double kappa, T* M, T* L, const std::vector<T*>& tabs, double* y) {
+                     double kappa, T* M, T* L, const std::vector<T*>& tabs, T* const* ytab,
+                     T* slots, double* y) {
     constexpr int NP = P * P * P;
     constexpr int NT = ((NP + 63) / 64) * 64;
-    constexpr int MB = (NP <= 64) ? NP : 32;
     const FmmDev& d = pl->dv;
     launch_far_pack(s, pl->N, patches, d.perm, x, pl->src);
     {
@@ -762,11 +863,14 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
         const int first = pl->lvl_first[l - 1], cnt = pl->lvl_first[l] - first;
         if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, M);
     }
-    for (int l = 2; l <= D; ++l) {
-        const int c0 = pl->lvl_chunk[l], c1 = pl->lvl_chunk[l + 1];
-        if (c1 > c0)
-            hipLaunchKernelGGL((k_m2l<T, P, FmmPlan::CHUNK, MB>), dim3(c1 - c0), dim3(NT), 0, s, d, l,
-                               pl->d_chunks + c0, tabs[D + 1], tabs[l], M, L);
+    {
+        const int c0 = pl->lvl_chunk[2], c1 = pl->lvl_chunk[D + 1];
+        if (D >= 2 && c1 > c0) {
+            hipLaunchKernelGGL((k_m2l<T, P>), dim3(c1 - c0), dim3(256), 0, s, d, pl->d_chunks + c0,
+                               tabs[D + 1], ytab, M, slots);
+            const int b0 = pl->lvl_first[2], nb = pl->lvl_first[D + 1] - b0;
+            hipLaunchKernelGGL((k_m2l_sum<T, P>), dim3(nb), dim3(256), 0, s, d, b0, slots, L);
+        }
     }
     for (int l = 2; l < D; ++l) {
         const int first = pl->lvl_first[l + 1], cnt = pl->lvl_first[l + 2] - first;
@@ -784,20 +888,20 @@ void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, 
         float* M = pl->exp32;
         float* L = pl->exp32 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
-            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
-            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
-            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, y); break;
+            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
+            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
+            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
+            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
         }
     } else {
         if (!pl->have64) throw std::runtime_error("FMM FP64 tables not prepared");
         double* M = pl->exp64;
         double* L = pl->exp64 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
-            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
-            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
-            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, y); break;
+            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
+            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
+            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
+            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
         }
     }
 }

@@ -26,6 +26,8 @@ struct FmmDev {
     int* nb_leaf = nullptr;
     int* pair_src = nullptr;    // M2L pairs grouped by offset
     int* pair_tgt = nullptr;
+    int* tgt_ptr = nullptr;     // per box: its M2L pairs (indices into pair_*)
+    int* tgt_pair = nullptr;
     double* tn = nullptr;       // T_j(xi_k), P x P
     double* t_oct = nullptr;    // child-node interpolation factors, 2 x P x P
 };
@@ -33,6 +35,7 @@ struct FmmDev {
 struct FmmPlan {
     static constexpr int CHUNK = 16;  // M2L pairs per block
     int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0;
+    long long n_p2p = 0;              // target-source pairs of the leaf pass (self included)
     double edge = 0;
     size_t n_exp = 0;                 // values per expansion set (nbox * 3 * P^3)
     std::vector<int> lvl_first;       // D + 2
@@ -45,6 +48,10 @@ struct FmmPlan {
     std::vector<float*> ktab32;
     double* exp64 = nullptr;          // M then L
     float* exp32 = nullptr;
+    double** ytab64 = nullptr;        // device copies of the table pointers
+    float** ytab32 = nullptr;
+    double* slots64 = nullptr;        // M2L results per pair (n_pairs x 3 x P^3)
+    float* slots32 = nullptr;
     Src64* src = nullptr;
     bool have64 = false, have32 = false;
     double tables_kappa = -1;

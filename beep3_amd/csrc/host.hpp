@@ -167,3 +167,13 @@ struct Octree {
 };
 
 }  // namespace beep
+
+// Opaque handle behind beep_mesh* of include/beep_host.h.
+struct beep_mesh {
+    beep::Mesh mesh;
+    bool has_shape = false;
+    beep::StarShape shape;
+};
+
+// Sets the message returned by beep_host_last_error() on this thread.
+void beep_host_set_error(const char* msg);

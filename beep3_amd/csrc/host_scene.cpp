@@ -16,12 +16,6 @@
 
 using namespace beep;
 
-struct beep_mesh {
-    Mesh mesh;
-    bool has_shape = false;
-    StarShape shape;
-};
-
 struct Instance {
     const beep_mesh* mesh;
     Quat rot;
@@ -52,6 +46,8 @@ struct beep_scene {
 static thread_local std::string g_err;
 
 extern "C" const char* beep_host_last_error(void) { return g_err.c_str(); }
+
+void beep_host_set_error(const char* msg) { g_err = msg; }
 
 template <typename F>
 static int guarded(F&& f) {

@@ -257,3 +257,36 @@ void launch_fill(hipStream_t s, int n, double v, double* y) {
 }
 
 }  // namespace beepdev
+
+namespace beepdev {
+
+// Self-test of the 16x16x4 MFMA operand layout used by the M2L GEMM:
+// A (16x4) lane l holds A[l % 16][l / 16], B (4x16) lane l holds
+// B[l / 16][l % 16]; out[4 l + r] = accumulator element r of lane l for
+// D = A B, A[i][k] = a[i*4+k], B[k][j] = b[k*16+j].
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_mfma_selftest(const double* __restrict__ a,
+                                                      const double* __restrict__ b, int fp32,
+                                                      double* __restrict__ out) {
+    const int l = threadIdx.x;
+    const int i = l % 16, k = l / 16;
+    if (fp32) {
+        f4v acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(float(a[i * 4 + k]), float(b[k * 16 + i]), acc,
+                                                   0, 0, 0);
+        for (int r = 0; r < 4; ++r) out[4 * l + r] = double(acc[r]);
+    } else {
+        d4v acc = {0.0, 0.0, 0.0, 0.0};
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i * 4 + k], b[k * 16 + i], acc, 0, 0, 0);
+        for (int r = 0; r < 4; ++r) out[4 * l + r] = acc[r];
+    }
+}
+
+void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out) {
+    hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, s, a, b, fp32, out);
+    hip_check(hipGetLastError(), "k_mfma_selftest launch");
+}
+
+}  // namespace beepdev

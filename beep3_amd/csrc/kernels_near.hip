@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
     const Patch64* __restrict__ patches, const int* __restrict__ qual_ptr,
     const Pt<T>* __restrict__ qual, const int* __restrict__ quad_ptr,
     const Pt<T>* __restrict__ quad, T kappa, int subtract_1pt, void* __restrict__ coef) {
-    const int b = blockIdx.x;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int L = find_leaf(blk_ptr, n_leaves, b);
     const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
     const int np = nb_ptr[L], Cn = nb_ptr[L + 1] - np;
@@ -237,59 +237,75 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
 //   y_f[i] = sum_j B_ij f_j - A_ij h_j + (B_ii' f_i - A_ii h_i)
 //   y_h[i] = sum_j D_ij f_j - C_ij h_j + (D_ii f_i - C_ii' h_i)
 // (LocalIntegrations::evaluate_local_contributions, local_integrations.h:123-130).
-// A block handles up to 256 rows of one leaf (blk_ptr maps blocks to leaves,
-// ceil(R/256) blocks per leaf).  The leaf's neighbourhood x values are
-// gathered into LDS in chunks; thread (r, s) walks columns s, s+S, ... of row
-// r, so each step of the block reads one contiguous run of coefficients.
-constexpr int SPMV_CHUNK = 1024;
+//
+// Pass 1: a block handles one work item = (leaf, group of <= 256 rows, run of
+// <= SPMV_CW columns), so blocks carry similar byte counts.  The x values of
+// its columns are gathered into LDS; thread (r, s) walks columns s, s+S, ...
+// of row r, so each step of the block reads one contiguous run of
+// coefficients, U steps in flight per thread.  Partial row sums go to a
+// scratch array (no atomics, deterministic).
+// Pass 2: a block per row group sums its partials, adds the diagonal and
+// writes y.
+template <typename TC>
+struct Coef4 {
+    TC a, b, c, d;
+};
 
-template <typename TC, typename TA>
-__global__ __launch_bounds__(256) void k_near_spmv(
-    int n, int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
-    const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
-    const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,
-    const void* __restrict__ coef, const double* __restrict__ diag4,
-    const double* __restrict__ x, double* __restrict__ y) {
-    __shared__ TA xs[2][SPMV_CHUNK];
+template <typename TC>
+__device__ __forceinline__ Coef4<TC> load_coef(const void* __restrict__ coef, int64_t idx) {
+    if constexpr (sizeof(TC) == 4) {
+        const float4 v = reinterpret_cast<const float4*>(coef)[idx];
+        return Coef4<TC>{v.x, v.y, v.z, v.w};
+    } else {
+        const double2* c2 = reinterpret_cast<const double2*>(coef);
+        const double2 v0 = c2[2 * idx], v1 = c2[2 * idx + 1];
+        return Coef4<TC>{v0.x, v0.y, v1.x, v1.y};
+    }
+}
+
+template <typename TC, typename TA, int U>
+__global__ __launch_bounds__(256) void k_near_spmv_part(
+    const int4* __restrict__ work, const int* __restrict__ leaf_ptr,
+    const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx,
+    const int64_t* __restrict__ coef_off, const void* __restrict__ coef,
+    const double* __restrict__ x, int n, TA* __restrict__ part) {
+    __shared__ TA xs[2][SPMV_CW];
     __shared__ TA red[2][256];
-    const int b = blockIdx.x;
-    const int L = find_leaf(blk_ptr, n_leaves, b);
-    const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
-    const int np = nb_ptr[L], Cn = nb_ptr[L + 1] - np;
-    const int rg = (b - blk_ptr[L]) * 256;          // first row of this block
-    const int Rb = min(256, R - rg);                 // rows handled here
-    const int S = (R <= 256) ? (256 / R) : 1;        // column slices
+    const int4 w = work[blockIdx.x];  // leaf, first row, first column, columns
+    const int L = w.x, row0 = w.y, col0 = w.z, nc = w.w;
+    const int R = leaf_ptr[L + 1] - leaf_ptr[L];
+    const int np = nb_ptr[L];
+    const int Rb = min(256, R - row0);
+    const int S = 256 / Rb;
     const int t = threadIdx.x;
-    const int r = (R <= 256) ? (t % R) : t;
-    const int s = (R <= 256) ? (t / R) : 0;
-    const bool active = (R <= 256) ? (t < R * S) : (t < Rb);
-    const int64_t base = coef_off[L] + rg + r;
+    const int r = t % Rb, s = t / Rb;
+    const bool active = t < Rb * S;
+    for (int k = t; k < nc; k += 256) {
+        const int j = nb_idx[np + col0 + k];
+        xs[0][k] = TA(x[j]);
+        xs[1][k] = TA(x[n + j]);
+    }
+    __syncthreads();
     TA af = 0, ah = 0;
-    for (int c0 = 0; c0 < Cn; c0 += SPMV_CHUNK) {
-        const int cn = min(SPMV_CHUNK, Cn - c0);
-        __syncthreads();
-        for (int k = t; k < cn; k += 256) {
-            const int j = nb_idx[np + c0 + k];
-            xs[0][k] = TA(x[j]);
-            xs[1][k] = TA(x[n + j]);
-        }
-        __syncthreads();
-        if (active) {
-            for (int c = s; c < cn; c += S) {
-                const int64_t idx = base + int64_t(c0 + c) * R;
-                TA A, B, C, D;
-                if constexpr (sizeof(TC) == 4) {
-                    const float4 v = reinterpret_cast<const float4*>(coef)[idx];
-                    A = TA(v.x); B = TA(v.y); C = TA(v.z); D = TA(v.w);
-                } else {
-                    const double2* c2 = reinterpret_cast<const double2*>(coef);
-                    const double2 v0 = c2[2 * idx], v1 = c2[2 * idx + 1];
-                    A = v0.x; B = v0.y; C = v1.x; D = v1.y;
-                }
-                const TA xf = xs[0][c], xh = xs[1][c];
-                af += B * xf - A * xh;
-                ah += D * xf - C * xh;
+    if (active) {
+        const int64_t base = coef_off[L] + row0 + r + int64_t(col0) * R;
+        int c = s;
+        for (; c + (U - 1) * S < nc; c += U * S) {
+            Coef4<TC> v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = load_coef<TC>(coef, base + int64_t(c + u * S) * R);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const TA xf = xs[0][c + u * S], xh = xs[1][c + u * S];
+                af += TA(v[u].b) * xf - TA(v[u].a) * xh;
+                ah += TA(v[u].d) * xf - TA(v[u].c) * xh;
             }
+        }
+        for (; c < nc; c += S) {
+            const Coef4<TC> v = load_coef<TC>(coef, base + int64_t(c) * R);
+            const TA xf = xs[0][c], xh = xs[1][c];
+            af += TA(v.b) * xf - TA(v.a) * xh;
+            ah += TA(v.d) * xf - TA(v.c) * xh;
         }
     }
     red[0][t] = af;
@@ -297,35 +313,60 @@ __global__ __launch_bounds__(256) void k_near_spmv(
     __syncthreads();
     if (active && s == 0) {
         for (int k = 1; k < S; ++k) {
-            af += red[0][k * R + r];
-            ah += red[1][k * R + r];
+            af += red[0][k * Rb + r];
+            ah += red[1][k * Rb + r];
         }
-        const int i = leaf_idx[lp + rg + r];
-        const double* d = diag4 + 4 * i;
-        const double xf = x[i], xh = x[n + i];
-        y[i] = double(af) + (d[1] * xf - d[0] * xh);
-        y[n + i] = double(ah) + (d[3] * xf - d[2] * xh);
+        part[2 * (size_t(blockIdx.x) * 256 + r)] = af;
+        part[2 * (size_t(blockIdx.x) * 256 + r) + 1] = ah;
     }
 }
 
-void launch_near_spmv(hipStream_t s, int mode, int n, int n_blocks, int n_leaves,
-                      const int* blk_ptr, const int* leaf_ptr, const int* leaf_idx,
-                      const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
-                      const float4* coef32, const double2* coef64, const double* diag4,
-                      const double* x, double* y) {
-    if (n_blocks <= 0) return;
+template <typename TA>
+__global__ __launch_bounds__(256) void k_near_spmv_sum(
+    const int4* __restrict__ groups, const int* __restrict__ group_nblk,
+    const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_idx,
+    const double* __restrict__ diag4, const double* __restrict__ x, int n,
+    const TA* __restrict__ part, double* __restrict__ y) {
+    const int4 g = groups[blockIdx.x];  // leaf, first row, rows, first work block
+    const int nb = group_nblk[blockIdx.x];
+    const int r = threadIdx.x;
+    if (r >= g.z) return;
+    TA af = 0, ah = 0;
+    for (int k = 0; k < nb; ++k) {
+        const size_t q = 2 * (size_t(g.w + k) * 256 + r);
+        af += part[q];
+        ah += part[q + 1];
+    }
+    const int i = leaf_idx[leaf_ptr[g.x] + g.y + r];
+    const double* d = diag4 + 4 * size_t(i);
+    const double xf = x[i], xh = x[n + i];
+    y[i] = double(af) + (d[1] * xf - d[0] * xh);
+    y[n + i] = double(ah) + (d[3] * xf - d[2] * xh);
+}
+
+void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
+                      const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
+                      const int64_t* coef_off, const float4* coef32, const double2* coef64,
+                      const double* diag4, const double* x, void* part, double* y) {
+    if (p.n_work <= 0) return;
     if (mode == 0) {
-        hipLaunchKernelGGL((k_near_spmv<double, double>), dim3(n_blocks), dim3(256), 0, s, n,
-                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef64, diag4, x, y);
+        hipLaunchKernelGGL((k_near_spmv_part<double, double, 4>), dim3(p.n_work), dim3(256), 0, s,
+                           p.work, leaf_ptr, nb_ptr, nb_idx, coef_off, (const void*)coef64, x, n,
+                           (double*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
+                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_near_spmv<float, float>), dim3(n_blocks), dim3(256), 0, s, n,
-                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef32, diag4, x, y);
+        hipLaunchKernelGGL((k_near_spmv_part<float, float, 8>), dim3(p.n_work), dim3(256), 0, s,
+                           p.work, leaf_ptr, nb_ptr, nb_idx, coef_off, (const void*)coef32, x, n,
+                           (float*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<float>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
+                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const float*)part, y);
     } else {
-        hipLaunchKernelGGL((k_near_spmv<float, double>), dim3(n_blocks), dim3(256), 0, s, n,
-                           n_leaves, blk_ptr, leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef32, diag4, x, y);
+        hipLaunchKernelGGL((k_near_spmv_part<float, double, 8>), dim3(p.n_work), dim3(256), 0, s,
+                           p.work, leaf_ptr, nb_ptr, nb_idx, coef_off, (const void*)coef32, x, n,
+                           (double*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
+                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
     }
     hip_check(hipGetLastError(), "k_near_spmv launch");
 }

@@ -5,8 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BeepError, HipGeometry, HipTree, MatvecTiming, SolveOpts, SolveStats, check_hip,
-                   dp, ip, lib, vp)
+from ._lib import (BeepError, FarInfo, HipGeometry, HipTree, MatvecTiming, SolveOpts, SolveStats,
+                   check_hip, dp, ip, lib, vp)
 
 FP64, FP32, FP32_RESID = 0, 1, 2
 SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
@@ -72,6 +72,11 @@ class Device:
     def set_far_field(self, mode="auto", order=4, leaf_target=48):
         m = self.FAR_MODES[mode] if isinstance(mode, str) else int(mode)
         self._check(lib().beep_hip_set_far_field(self._h, m, int(order), int(leaf_target)))
+
+    def far_info(self) -> dict:
+        f = FarInfo()
+        self._check(lib().beep_hip_far_info(self._h, C.byref(f)))
+        return {k: getattr(f, k) for k, _ in FarInfo._fields_}
 
     def precompute(self, kappa: float, precision="fp32"):
         self._check(lib().beep_hip_precompute(self._h, float(kappa), _prec(precision)))

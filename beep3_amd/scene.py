@@ -40,6 +40,19 @@ class Mesh:
         check_host(lib().beep_mesh_star(subdivisions, r0, lmax, sigma0, seed, C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def read(cls, path: str, charges: str | None = None) -> "Mesh":
+        """Surface from a .off or .gts file, optional 'x y z q [r]' charges."""
+        h = vp()
+        check_host(lib().beep_mesh_read(str(path).encode(), C.byref(h)))
+        m = cls(h)
+        if charges is not None:
+            check_host(lib().beep_mesh_read_charges(m._h, str(charges).encode()))
+        return m
+
+    def write_off(self, path: str):
+        check_host(lib().beep_mesh_write_off(self._h, str(path).encode()))
+
     def set_charges(self, xyzq) -> "Mesh":
         a = np.ascontiguousarray(xyzq, dtype=np.float64).reshape(-1, 4)
         check_host(lib().beep_mesh_set_charges(self._h, a.shape[0], _dp(a)))

@@ -139,6 +139,20 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
 #define BEEP_FAR_FMM 1
 int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target);
 
+/* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
+ * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs
+ * and P2P target-source pairs (zeros for the direct sum). */
+typedef struct {
+    int mode;
+    int order;
+    int leaf_level;
+    int n_leaf_boxes;
+    int n_boxes;
+    long long m2l_pairs;
+    long long p2p_pairs;
+} beep_far_info;
+int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out);
+
 /* Singular self terms (FP64 always) and near-field coefficients of the given
  * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set). */
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);
@@ -164,6 +178,13 @@ int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_instances, const int* 
 /* Solve K x = b; x holds the initial guess on entry (zeros: cold start). */
 int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_solve_opts* opts,
                    beep_solve_stats* stats);
+
+/* Diagnostic: D = A B through one 16x16x4 MFMA (fp32 = 0: FP64 operands,
+ * 1: FP32), A 16x4 and B 4x16 row-major host arrays; d[4 l + r] receives
+ * accumulator element r of lane l (256 values).  Checks the operand and
+ * result layout the M2L kernels rely on. */
+int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
+                           double* d);
 
 /* device memory helpers for callers that keep vectors resident */
 int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out);

@@ -44,6 +44,11 @@ int beep_mesh_charges_walk(beep_mesh* m, int n, double step, double clearance, d
 int beep_mesh_info(const beep_mesh* m, int* n_vertices, int* n_triangles, int* n_charges,
                    double* centre3, double* radius);
 int beep_mesh_vertices(const beep_mesh* m, double* xyz, int* tri3);
+/* Surface files: .off (src/bem/off_utils.h:26-160) or .gts (gts_utils.h:32-140);
+ * charges: lines "x y z q [r]" (the xyzqr member of a .mtz, mesh_tarball.h:52-75) */
+int beep_mesh_read(const char* path, beep_mesh** out);
+int beep_mesh_write_off(const beep_mesh* m, const char* path);
+int beep_mesh_read_charges(beep_mesh* m, const char* path);
 void beep_mesh_destroy(beep_mesh* m);
 
 /* ---- scene: BEEP(Dsolvent, kappa, quad, qual, nbsize) + instances ---- */

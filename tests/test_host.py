@@ -166,3 +166,56 @@ def test_bad_arguments_raise():
         Scene(80.0, 0.1, 5, 0, 1500)
     with pytest.raises(BeepError):
         Scene(80.0, 0.1, 6, 0, 1500).build()
+
+
+def test_off_round_trip(tmp_path):
+    m = Mesh.icosphere(2, 3.0).set_charges([[0.1, 0, 0, 1.0]])
+    p = tmp_path / "ico.off"
+    m.write_off(str(p))
+    q = tmp_path / "ico.xyzqr"
+    q.write_text("# x y z q r\n0.1 0 0 1.0 1.5\n")
+    m2 = Mesh.read(str(p), charges=str(q))
+    a1 = Scene(80.0, 0.1, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
+    a2 = Scene(80.0, 0.1, 6, 0, 1500).add_instance(m2, 2.0).build().arrays()
+    for k in ("centroid", "normal", "area", "quad", "charges"):
+        assert np.allclose(a1[k], a2[k], atol=1e-13), k
+
+
+def test_gts_reader(tmp_path):
+    """GTS stores faces as edge triples; the reader recovers the vertex
+    triples (shared vertex of consecutive edges) and so the same patches."""
+    m = Mesh.icosphere(1, 2.0).set_charges([[0, 0, 0, 1.0]])
+    xyz, tri = m.vertices()
+    edges, eidx, faces = [], {}, []
+    for t in tri:
+        fe = []
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            k = (min(a, b), max(a, b))
+            if k not in eidx:
+                eidx[k] = len(edges)
+                edges.append(k)
+            fe.append(eidx[k] + 1)
+        faces.append(fe)
+    lines = [f"{len(xyz)} {len(edges)} {len(faces)}"]
+    lines += [f"{x:.17g} {y:.17g} {z:.17g}" for x, y, z in xyz]
+    lines += [f"{a + 1} {b + 1}" for a, b in edges]
+    lines += [f"{a} {b} {c}" for a, b, c in faces]
+    p = tmp_path / "ico.gts"
+    p.write_text("# icosphere\n" + "\n".join(lines) + "\n")
+    m2 = Mesh.read(str(p)).set_charges([[0, 0, 0, 1.0]])
+    a1 = Scene(80.0, 0.1, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
+    a2 = Scene(80.0, 0.1, 6, 0, 1500).add_instance(m2, 2.0).build().arrays()
+    assert np.allclose(a1["area"], a2["area"], atol=1e-13)
+    assert np.allclose(a1["normal"], a2["normal"], atol=1e-13)
+
+
+def test_bad_mesh_file(tmp_path):
+    from beep3_amd._lib import BeepError
+    p = tmp_path / "bad.off"
+    p.write_text("OFF\n3 1 0\n0 0 0\n1 0 0\n0 1 0\n4 0 1 2 0\n")
+    with pytest.raises(BeepError, match="triangles"):
+        Mesh.read(str(p))
+    ply = tmp_path / "x.ply"
+    ply.write_text("ply\n")
+    with pytest.raises(BeepError, match="off or .gts"):
+        Mesh.read(str(ply))
