@@ -82,6 +82,7 @@ class MatvecTiming(C.Structure):
     _fields_ = [
         ("near_ms", C.c_double), ("far_ms", C.c_double), ("total_ms", C.c_double),
         ("count", C.c_int), ("near_ms_sum", C.c_double), ("far_ms_sum", C.c_double),
+        ("total_ms_sum", C.c_double),
     ]
 
 
@@ -89,6 +90,26 @@ class FarInfo(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("order", C.c_int), ("leaf_level", C.c_int), ("n_leaf_boxes", C.c_int),
         ("n_boxes", C.c_int), ("m2l_pairs", C.c_longlong), ("p2p_pairs", C.c_longlong),
+    ]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("device", C.c_int), ("slots", C.c_int), ("hbm_budget", C.c_double),
+        ("solve_mode", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
+        ("max_refine", C.c_int), ("inner_tol", C.c_double), ("far_mode", C.c_int),
+        ("fmm_order", C.c_int), ("fmm_leaf", C.c_int), ("keep_fh", C.c_int),
+    ]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_longlong), ("status", C.c_int), ("n_patches", C.c_int),
+        ("energy", C.c_double), ("iterations", C.c_int), ("matvecs", C.c_int),
+        ("refinements", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
+        ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+        ("host_ms", C.c_double), ("device_ms", C.c_double), ("bytes", C.c_double),
+        ("slot", C.c_int), ("message", C.c_char * 128),
     ]
 
 
@@ -104,6 +125,7 @@ def _declare(L):
     L.beep_mesh_read.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.beep_mesh_write_off.argtypes = [vp, C.c_char_p]
     L.beep_mesh_read_charges.argtypes = [vp, C.c_char_p]
+    L.beep_mesh_clone.argtypes = [vp, C.POINTER(vp)]
     L.beep_mesh_destroy.argtypes = [vp]
     L.beep_mesh_destroy.restype = None
     L.beep_scene_create.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
@@ -117,6 +139,18 @@ def _declare(L):
     L.beep_scene_instances.argtypes = [vp, dp, ip, ip]
     L.beep_scene_destroy.argtypes = [vp]
     L.beep_scene_destroy.restype = None
+    L.beep_server_default_opts.argtypes = [C.POINTER(ServerOpts)]
+    L.beep_server_default_opts.restype = None
+    L.beep_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.beep_server_submit.argtypes = [vp, vp, C.POINTER(C.c_longlong)]
+    L.beep_server_poll.argtypes = [vp, C.c_double, C.POINTER(ServerResult)]
+    L.beep_server_take_fh.argtypes = [vp, C.c_longlong, dp, dp]
+    L.beep_server_clock.argtypes = [vp]
+    L.beep_server_clock.restype = C.c_double
+    L.beep_server_pending.argtypes = [vp]
+    L.beep_server_last_error.argtypes = [vp]
+    L.beep_server_last_error.restype = C.c_char_p
+    L.beep_server_destroy.argtypes = [vp]
     if hasattr(L, "beep_hip_create"):
         _declare_hip(L)
 
@@ -130,6 +164,7 @@ def _declare_hip(L):
     L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.beep_hip_set_overlap.argtypes = [vp, C.c_int]
     L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
@@ -142,6 +177,7 @@ def _declare_hip(L):
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.beep_hip_device_free.argtypes = [vp, vp]
     L.beep_hip_memcpy.argtypes = [vp, vp, vp, C.c_size_t, C.c_int]
+    L.beep_hip_memory_held.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.beep_hip_synchronize.argtypes = [vp]
     L.beep_hip_memory_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
 

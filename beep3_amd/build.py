@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libbeep3.so")
 
-HOST_SOURCES = ["host_mesh.cpp", "host_octree.cpp", "host_scene.cpp", "host_io.cpp"]
+HOST_SOURCES = ["host_mesh.cpp", "host_octree.cpp", "host_scene.cpp", "host_io.cpp", "serve.cpp"]
 HIP_SOURCES = ["kernels_near.hip", "kernels_far.hip", "kernels_aux.hip", "fmm.hip", "beep_hip.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -27,7 +27,7 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
-HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-sign-compare"]
 
 
 def _run(cmd):
@@ -54,7 +54,7 @@ def build(verbose: bool = False) -> str:
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, OUT)
     return OUT
 

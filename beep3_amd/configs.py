@@ -66,3 +66,41 @@ def pose_pair(mesh: Mesh, q, offset, nbsize=1500, **kw):
     b.insert_mesh_instance(lib_id, [0.0, 0.0, 0.0], None, 2.0)
     b.insert_mesh_instance(lib_id, offset, q, 2.0)
     return b
+
+
+# ---- serving stream: Poisson arrivals of single-molecule solves ----
+SERVING_LEVELS = (4, 5, 6, 7)            # 2,562 / 10,242 / 40,962 / 163,842 patches
+SERVING_WEIGHTS = (0.4, 0.3, 0.2, 0.1)
+
+
+def serving_schedule(rate=50.0, duration=6.0, seed=11, shapes=2):
+    """Arrival times of a Poisson stream (rate per second over duration
+    seconds) with, per request, the icosphere level (weights 0.4 / 0.3 / 0.2
+    / 0.1 over 2,562 .. 163,842 patches), which of `shapes` star shapes of
+    that level, the charge count (uniform 50..2,000) and the charge seed."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= duration:
+            break
+        lvl = int(rng.choice(SERVING_LEVELS, p=SERVING_WEIGHTS))
+        out.append({"t": t, "level": lvl, "shape": int(rng.integers(shapes)),
+                    "n_charges": int(rng.integers(50, 2001)), "charge_seed": int(rng.integers(1 << 30))})
+    return out
+
+
+def serving_meshes(shapes=2, levels=SERVING_LEVELS):
+    """Star surfaces (r = 25 A, the 163,842-patch globular generator at each
+    level) with shape seeds 100 + k; charges are set per request."""
+    return {(lvl, k): Mesh.star(lvl, 25.0, 8, 0.04, 100 + k) for lvl in levels for k in range(shapes)}
+
+
+def serving_request(meshes, req, nbsize=1500):
+    """Scene of one request: its own copy of the surface with a 3.8 A
+    random-walk chain of n_charges charges, q ~ N(0, 0.4) (eps_in 2, eps_out
+    80, kappa 0.125)."""
+    from .scene import Scene
+    m = meshes[(req["level"], req["shape"])].clone()
+    m.charges_walk(req["n_charges"], 3.8, 2.0, 0.4, req["charge_seed"])
+    return Scene(80.0, 0.125, QUAD, QUAL, nbsize).add_instance(m, 2.0), m

@@ -55,6 +55,11 @@ double now_ms() {
 struct beep_hip_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // far field runs on its own stream, concurrently with the near SpMV, into
+    // yfar; the context stream adds it to y (fork / join through events)
+    hipStream_t far_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool overlap = true;
     std::string err;
 
     // geometry
@@ -90,20 +95,25 @@ struct beep_hip_ctx {
     // far field
     int far_mode = BEEP_FAR_AUTO, fmm_order = 4, fmm_leaf = 48;
     std::vector<double> h_centroid;
+    DevArena fmm_arena;  // device buffers of the FMM plan, reused across plans
     FmmPlan* fmm = nullptr;
     bool use_fmm = false;
 
     // per mat-vec scratch
     DevBuf<Src64> src;
-    DevBuf<double> xbuf, ybuf;
+    DevBuf<double> xbuf, ybuf, yfar;
+    // charges and outputs of the RHS / energy-coefficient calls (kept, so a
+    // solve frees nothing: hipFree waits for the whole device)
+    DevBuf<double> qbuf, rhsbuf, ecf, ech;
+    DevBuf<int> pinst_d, icp_d;
 
     // GMRES workspace
     DevBuf<double> V, hvec, partial, wtmp, bvec, rvec, dvec;
     int part_blocks = 0;
 
-    // ring of event triples (start, near done, far done), one per mat-vec
+    // ring of event sets (start, near done, end, far done), one per mat-vec
     static constexpr int RING = 64;
-    hipEvent_t ev[RING][3] = {};
+    hipEvent_t ev[RING][4] = {};
     int ring_head = 0, ring_pending = 0;  // next slot, slots not yet read
     beep_matvec_timing timing{};
 };
@@ -131,15 +141,23 @@ void require(bool cond, const char* msg) {
 
 void set_device(beep_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
 
+// wait for both streams (before buffers they may use are freed or replaced)
+void sync_streams(beep_hip_ctx* c) {
+    if (c->far_stream) hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
+    if (c->stream) hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
 // (re)build the far-field plan for the current geometry and options
 void far_setup(beep_hip_ctx* c) {
+    sync_streams(c);
     if (c->fmm) {
         fmm_plan_destroy(c->fmm);
         c->fmm = nullptr;
     }
     c->use_fmm = c->far_mode == BEEP_FAR_FMM || (c->far_mode == BEEP_FAR_AUTO && c->N > 20000);
     if (c->use_fmm)
-        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf, c->stream);
+        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
+                                 c->fmm_arena, c->stream);
 }
 
 // Convert 7-double points of a CSR point set into local-frame records.
@@ -174,16 +192,19 @@ void check_ptr(int N, const int* ptr, const char* what) {
 void timing_pop(beep_hip_ctx* c) {
     const int slot = (c->ring_head - c->ring_pending + beep_hip_ctx::RING) % beep_hip_ctx::RING;
     hipEvent_t* e = c->ev[slot];
-    float a = 0, b = 0;
+    float a = 0, b = 0, t = 0;
     hip_check(hipEventSynchronize(e[2]), "hipEventSynchronize");
     hip_check(hipEventElapsedTime(&a, e[0], e[1]), "hipEventElapsedTime");
-    hip_check(hipEventElapsedTime(&b, e[1], e[2]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&b, e[0], e[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&t, e[0], e[2]), "hipEventElapsedTime");
+    if (!c->overlap) b -= a;  // serial: the far part starts when the near part ends
     c->timing.near_ms = a;
     c->timing.far_ms = b;
-    c->timing.total_ms = double(a) + double(b);
+    c->timing.total_ms = t;
     c->timing.count += 1;
     c->timing.near_ms_sum += a;
     c->timing.far_ms_sum += b;
+    c->timing.total_ms_sum += t;
     c->ring_pending -= 1;
 }
 
@@ -199,24 +220,41 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
     else require(c->have32, "FP32 coefficients not precomputed");
     hipStream_t s = c->stream;
+    // kernel tables are built on first use for the current kappa
+    if (c->use_fmm)
+        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, s);
     if (c->ring_pending == beep_hip_ctx::RING) timing_pop(c);
     hipEvent_t* ev = c->ev[c->ring_head];
+    const int n2 = 2 * c->N;
+    c->yfar.alloc(size_t(n2));
     hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+    hipStream_t sf = s;
+    if (c->overlap) {
+        sf = c->far_stream;
+        hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(sf, c->ev_fork, 0), "hipStreamWaitEvent");
+    }
+    // far field (1-point interaction of all pairs) into yfar
+    launch_fill(sf, n2, 0.0, c->yfar.p);
+    if (c->use_fmm) {
+        fmm_evaluate(c->fmm, sf, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa,
+                     c->yfar.p);
+    } else {
+        c->src.alloc(size_t(c->N));
+        launch_far_pack(sf, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
+        launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
+                          c->src.p, c->kappa, c->yfar.p);
+    }
+    hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
+    if (c->overlap) hip_check(hipEventRecord(c->ev_join, sf), "hipEventRecord");
+    // near field (explicit pairs) + diagonal into y
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
     launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
                      c->spmv_part.p, dy);
     hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
-    if (c->use_fmm) {
-        // kernel tables are built on first use for the current kappa
-        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, s);
-        fmm_evaluate(c->fmm, s, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa, dy);
-    } else {
-        c->src.alloc(size_t(c->N));
-        launch_far_pack(s, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
-        launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
-                          c->src.p, c->kappa, dy);
-    }
+    if (c->overlap) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
+    launch_axpby(s, n2, 1.0, c->yfar.p, 1.0, dy);
     hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
     c->ring_head = (c->ring_head + 1) % beep_hip_ctx::RING;
     c->ring_pending += 1;
@@ -344,12 +382,21 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
         require(device >= 0 && device < count, "no such HIP device");
         set_device(c.get());
         hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
-        for (auto& tri : c->ev)
-            for (auto& e : tri) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipStreamCreate(&c->far_stream), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+        for (auto& set : c->ev)
+            for (auto& e : set) hip_check(hipEventCreate(&e), "hipEventCreate");
     });
     if (rc != 0) {
         // no context survives: the message is kept for beep_hip_last_error(NULL)
         g_create_err = c->err;
+        for (auto& set : c->ev)
+            for (auto& e : set)
+                if (e) (void)hipEventDestroy(e);
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+        if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+        if (c->far_stream) (void)hipStreamDestroy(c->far_stream);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         return rc;
     }
@@ -360,11 +407,15 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
 int beep_hip_destroy(beep_hip_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
+    if (ctx->far_stream) (void)hipStreamSynchronize(ctx->far_stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->fmm) fmm_plan_destroy(ctx->fmm);
-    for (auto& tri : ctx->ev)
-        for (auto& e : tri)
+    for (auto& set : ctx->ev)
+        for (auto& e : set)
             if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->far_stream) (void)hipStreamDestroy(ctx->far_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -374,6 +425,7 @@ int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* g) {
     return guarded(ctx, [&] {
         require(g && g->n_patches > 0, "empty geometry");
         set_device(ctx);
+        sync_streams(ctx);
         const int N = g->n_patches;
         require(g->centroid && g->normal && g->area && g->eps_ratio && g->gc, "null patch array");
         check_ptr(N, g->qual_ptr, "bad qualocation pointer array");
@@ -552,6 +604,14 @@ int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_targ
     });
 }
 
+int beep_hip_set_overlap(beep_hip_ctx* ctx, int on) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->overlap = on != 0;
+    });
+}
+
 int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
     return guarded(ctx, [&] {
         require(out != nullptr, "null output");
@@ -616,11 +676,12 @@ int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Ds
         require(n_charges >= 0 && (n_charges == 0 || xyzq) && b && Dsolvent > 0, "bad rhs args");
         set_device(ctx);
         const int N = ctx->N;
-        DevBuf<double> q, out;
+        DevBuf<double>& q = ctx->qbuf;
+        DevBuf<double>& out = ctx->rhsbuf;
         if (n_charges > 0) q.upload(xyzq, size_t(4) * n_charges, ctx->stream);
         out.alloc(size_t(2) * N);
         launch_rhs(ctx->stream, N, ctx->qual_ptr.p, ctx->qual64.p, ctx->patches.p, n_charges,
-                   q.p, FOUR_PI_INV / Dsolvent, out.p);
+                   n_charges > 0 ? q.p : nullptr, FOUR_PI_INV / Dsolvent, out.p);
         hip_check(hipMemcpyAsync(b, out.p, sizeof(double) * 2 * N, hipMemcpyDeviceToHost,
                                  ctx->stream), "D2H rhs");
         hip_check(hipStreamSynchronize(ctx->stream), "sync rhs");
@@ -641,8 +702,11 @@ int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_inst, const int* ipp, 
             for (int i = ipp[k]; i < ipp[k + 1]; ++i) pinst[i] = k;
         }
         const int nch = icp[n_inst];
-        DevBuf<double> q, ef, eh;
-        DevBuf<int> pi, cp;
+        DevBuf<double>& q = ctx->qbuf;
+        DevBuf<double>& ef = ctx->ecf;
+        DevBuf<double>& eh = ctx->ech;
+        DevBuf<int>& pi = ctx->pinst_d;
+        DevBuf<int>& cp = ctx->icp_d;
         if (nch > 0) q.upload(xyzq, size_t(4) * nch, ctx->stream);
         pi.upload(pinst.data(), N, ctx->stream);
         cp.upload(icp, n_inst + 1, ctx->stream);
@@ -779,6 +843,24 @@ int beep_hip_synchronize(beep_hip_ctx* ctx) {
     return guarded(ctx, [&] {
         set_device(ctx);
         hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
+    return guarded(ctx, [&] {
+        require(bytes != nullptr, "null output");
+        size_t b = 0;
+        auto add = [&](const auto& buf) { b += buf.n * sizeof(*buf.p); };
+        add(ctx->patches); add(ctx->gc); add(ctx->qual_ptr); add(ctx->quad_ptr); add(ctx->gal_ptr);
+        add(ctx->qual64); add(ctx->quad64); add(ctx->gal64); add(ctx->qual32); add(ctx->quad32);
+        add(ctx->leaf_ptr); add(ctx->leaf_idx); add(ctx->nb_ptr); add(ctx->nb_idx);
+        add(ctx->coef_off); add(ctx->coef_blk); add(ctx->spmv_work); add(ctx->spmv_groups);
+        add(ctx->spmv_group_nblk); add(ctx->spmv_part); add(ctx->diag); add(ctx->coef32);
+        add(ctx->coef64); add(ctx->src); add(ctx->xbuf); add(ctx->ybuf); add(ctx->yfar);
+        add(ctx->qbuf); add(ctx->rhsbuf); add(ctx->ecf); add(ctx->ech); add(ctx->pinst_d);
+        add(ctx->icp_d); add(ctx->V); add(ctx->hvec); add(ctx->partial); add(ctx->wtmp);
+        add(ctx->bvec); add(ctx->rvec); add(ctx->dvec);
+        *bytes = b + ctx->fmm_arena.held;
     });
 }
 

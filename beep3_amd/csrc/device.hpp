@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 
 namespace beepdev {
 
@@ -50,6 +51,47 @@ inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+// Named grow-only device buffers.  A name keeps its allocation until a larger
+// size is asked for, so a plan rebuilt for a new geometry of no larger size
+// allocates and frees nothing (hipFree waits for the whole device, which
+// would stall other contexts running on it).
+struct DevArena {
+    struct Slot {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::unordered_map<std::string, Slot> slots;
+    size_t held = 0;
+    DevArena() = default;
+    DevArena(const DevArena&) = delete;
+    DevArena& operator=(const DevArena&) = delete;
+    ~DevArena() { release(); }
+    void* get(const std::string& name, size_t bytes) {
+        Slot& sl = slots[name];
+        if (sl.p && bytes <= sl.bytes) return sl.p;
+        if (sl.p) {
+            (void)hipFree(sl.p);
+            held -= sl.bytes;
+            sl = Slot{};
+        }
+        const size_t b = bytes ? bytes : 16;
+        hip_check(hipMalloc(&sl.p, b), "hipMalloc");
+        sl.bytes = b;
+        held += b;
+        return sl.p;
+    }
+    template <typename T>
+    T* get(const std::string& name, size_t count) {
+        return static_cast<T*>(get(name, count * sizeof(T)));
+    }
+    void release() {
+        for (auto& kv : slots)
+            if (kv.second.p) (void)hipFree(kv.second.p);
+        slots.clear();
+        held = 0;
+    }
+};
 
 // ---- launchers (defined in the .hip files) ----
 void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,

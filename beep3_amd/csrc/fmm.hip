@@ -354,118 +354,184 @@ __global__ __launch_bounds__(256) void k_l2l(const FmmDev d, int first, T* __res
     }
 }
 
-// Leaf pass: L2P + P2P for the targets of one leaf box, one write per target.
-// 256 threads = 64 target slots x 4 sub-lanes; the sub-lanes of a target
-// split the sources of every tile and the x-slices of the L2P sum, and are
-// combined through LDS at the end.
+// L2P: potentials and gradients of the local expansions at the targets of
+// one leaf box.  One wave per leaf box; the box's expansions and the
+// Chebyshev table sit in LDS, lane = target.
 template <typename T, int P>
-__global__ __launch_bounds__(256) void k_leaf(const FmmDev d, const Src64* __restrict__ src,
-                                              const Patch64* __restrict__ patches, T kappa,
-                                              int use_local, const T* __restrict__ L,
-                                              double* __restrict__ y, int n) {
+__global__ __launch_bounds__(64) void k_l2p(const FmmDev d, const Patch64* __restrict__ patches,
+                                            const T* __restrict__ L, double* __restrict__ y,
+                                            int n) {
     constexpr int NP = P * P * P;
-    constexpr int TILE = 256;
-    constexpr int SUB = 4;
-    __shared__ SrcR<T> tile[TILE];
-    __shared__ int tidx[TILE];
     __shared__ T lb[3 * NP];
     __shared__ T tn[P * P];
-    __shared__ T red[2][SUB][64];
     const int bl = xcd_remap(blockIdx.x, gridDim.x);
     const int box = d.leaf_first + bl;
     const int s0 = d.leaf_src[bl], s1 = d.leaf_src[bl + 1];
     const double cx = d.box_c[box].x, cy = d.box_c[box].y, cz = d.box_c[box].z;
-    const double inv_half = 2.0 / d.box_h[d.D];
+    const T ih = T(2.0 / d.box_h[d.D]);
     const int t = threadIdx.x;
-    const int slot = t & 63, sub = t >> 6;
-    for (int k = t; k < P * P; k += blockDim.x) tn[k] = T(d.tn[k]);
-    if (use_local)
-        for (int q = t; q < 3 * NP; q += blockDim.x) lb[q] = L[size_t(box) * 3 * NP + q];
-    for (int tc = s0; tc < s1; tc += 64) {
-        const int ti = tc + slot;
-        const bool valid = ti < s1;
-        const int i = valid ? d.perm[ti] : -1;
+    for (int k = t; k < P * P; k += 64) tn[k] = T(d.tn[k]);
+    for (int q = t; q < 3 * NP; q += 64) lb[q] = L[size_t(box) * 3 * NP + q];
+    __syncthreads();
+    for (int ti = s0 + t; ti < s1; ti += 64) {
+        const int i = d.perm[ti];
+        const Patch64 p = patches[i];
+        T sx[P], dsx[P], sy[P], dsy[P], sz[P], dsz[P];
+        basis1d<T, P>(T(p.cx - cx) * ih, tn, sx, dsx);
+        basis1d<T, P>(T(p.cy - cy) * ih, tn, sy, dsy);
+        basis1d<T, P>(T(p.cz - cz) * ih, tn, sz, dsz);
+        T pL = 0, p1 = 0, gLx = 0, gLy = 0, gLz = 0, g2x = 0, g2y = 0, g2z = 0;
+#pragma unroll 1
+        for (int a = 0; a < P; ++a)
+            for (int b = 0; b < P; ++b) {
+                const T sab = sx[a] * sy[b], dab = dsx[a] * sy[b], adb = sx[a] * dsy[b];
+                for (int c = 0; c < P; ++c) {
+                    const int m = (a * P + b) * P + c;
+                    const T s = sab * sz[c];
+                    const T gx = dab * sz[c], gy = adb * sz[c], gz = sab * dsz[c];
+                    const T l0 = lb[m], l1 = lb[NP + m], l2 = lb[2 * NP + m];
+                    pL += s * l0;
+                    p1 += s * l1;
+                    gLx += gx * l0; gLy += gy * l0; gLz += gz * l0;
+                    g2x += gx * l2; g2y += gy * l2; g2z += gz * l2;
+                }
+            }
+        const T nx = T(p.nx), ny = T(p.ny), nz = T(p.nz);
+        y[i] += double(pL + p1) * FOUR_PI_INV;
+        y[n + i] += double(ih * (nx * (gLx + g2x) + ny * (gLy + g2y) + nz * (gLz + g2z))) *
+                    FOUR_PI_INV;
+    }
+}
+
+// 1/sqrt(r2), and 0 for r2 == 0 (the self pair: target and source are the
+// same centroid, formed from the same bits, so every term it feeds vanishes).
+__device__ __forceinline__ float rsq_or_zero(float r2) {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    return r2 > 0.f ? ir : 0.f;
+}
+__device__ __forceinline__ double rsq_or_zero(double r2) {
+    // hardware estimate + two Newton steps y <- y + y (1/2 - (r2/2) y^2)
+    double yv = __builtin_amdgcn_rsq(r2);
+    const double h = 0.5 * r2;
+    double e = fma(-h * yv, yv, 0.5);
+    yv = fma(yv, e, yv);
+    e = fma(-h * yv, yv, 0.5);
+    yv = fma(yv, e, yv);
+    return r2 > 0.0 ? yv : 0.0;
+}
+
+template <typename T>
+struct P2PCap {
+    static constexpr int value = 256;  // 16 KB of FP64 sources
+};
+template <>
+struct P2PCap<float> {
+    static constexpr int value = 512;  // 16 KB of FP32 sources
+};
+
+// P2P: direct sum over the sources of the (at most 27) neighbour leaf boxes,
+// self excluded.  One block per leaf box, 256 threads = 16 groups of 16
+// lanes.  The neighbour sources are staged in LDS in passes of CAP (positions
+// relative to the box centre, formed in FP64); a group takes a chunk of 16
+// targets (lane = target) and every ns-th staged source, so small leaves
+// still keep all lanes busy; the ns partial sums of a chunk are combined in
+// LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_p2p(const FmmDev d, const Src64* __restrict__ src,
+                                             const Patch64* __restrict__ patches, T kappa,
+                                             double* __restrict__ y, int n) {
+    constexpr int CAP = P2PCap<T>::value;
+    __shared__ SrcR<T> tile[CAP];
+    __shared__ int pre[28];
+    __shared__ int beg[27];
+    __shared__ T red[2][16][16];
+    const int bl = xcd_remap(blockIdx.x, gridDim.x);
+    const int box = d.leaf_first + bl;
+    const int s0 = d.leaf_src[bl], s1 = d.leaf_src[bl + 1];
+    const double cx = d.box_c[box].x, cy = d.box_c[box].y, cz = d.box_c[box].z;
+    const int t = threadIdx.x;
+    const int k0 = d.nb_ptr[bl], nk = d.nb_ptr[bl + 1] - k0;  // <= 27 by construction
+    if (t == 0) {
+        int acc = 0;
+        for (int k = 0; k < nk; ++k) {
+            const int nbl = d.nb_leaf[k0 + k];
+            beg[k] = d.leaf_src[nbl];
+            pre[k] = acc;
+            acc += d.leaf_src[nbl + 1] - d.leaf_src[nbl];
+        }
+        pre[nk] = acc;
+    }
+    __syncthreads();
+    const int S = pre[nk];
+    const int g = t >> 4, lane = t & 15;
+    const int nch = (s1 - s0 + 15) >> 4;
+    // nch < 16: group g works on chunk g % nch with source phase g / nch of ns;
+    // nch >= 16: one phase, group g takes chunks g, g + 16, ... (one per round)
+    const int ns = nch < 16 ? 16 / nch : 1;
+    const int rounds = nch < 16 ? 1 : (nch + 15) >> 4;
+    const bool active = g < nch * ns || nch >= 16;
+    const int phase = nch < 16 ? g / nch : 0;
+    for (int r = 0; r < rounds; ++r) {
+        const int c = nch < 16 ? g % nch : g + 16 * r;
+        const int ti = s0 + 16 * c + lane;
+        const bool valid = active && c < nch && ti < s1;
         T xi = 0, yi = 0, zi = 0, n0x = 0, n0y = 0, n0z = 0;
         if (valid) {
-            const Patch64 p = patches[i];
+            const Patch64 p = patches[d.perm[ti]];
             xi = T(p.cx - cx); yi = T(p.cy - cy); zi = T(p.cz - cz);
             n0x = T(p.nx); n0y = T(p.ny); n0z = T(p.nz);
         }
         T af = 0, ah = 0;
-        // P2P over the neighbour leaf boxes (positions relative to this box)
-        for (int k = d.nb_ptr[bl]; k < d.nb_ptr[bl + 1]; ++k) {
-            const int nbl = d.nb_leaf[k];
-            const int j0 = d.leaf_src[nbl], j1 = d.leaf_src[nbl + 1];
-            for (int jt = j0; jt < j1; jt += TILE) {
-                __syncthreads();
-                if (jt + t < j1) {
-                    const Src64 q = src[jt + t];
-                    tile[t] = SrcR<T>{T(q.x - cx), T(q.y - cy), T(q.z - cz), T(q.inv_eps),
-                                      T(q.dx), T(q.dy), T(q.dz), T(q.sh)};
-                    tidx[t] = d.perm[jt + t];
-                }
-                __syncthreads();
-                const int m = min(TILE, j1 - jt);
-                if (valid) {
-                    for (int jj = sub; jj < m; jj += SUB) {
-                        if (tidx[jj] == i) continue;
-                        const SrcR<T> q = tile[jj];
-                        const T dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-                        const T r2 = dx * dx + dy * dy + dz * dz;
-                        const T ir = Fm<T>::rsqrt_(r2);
-                        const T r = r2 * ir;
-                        const T ir2 = ir * ir;
-                        const T ir3 = ir2 * ir;
-                        const T kr = kappa * r;
-                        const T e = Fm<T>::exp_(-kr);
-                        const T u = e * ir;
-                        const T g1 = e * (T(1) + kr) * ir3;
-                        const T ir5 = ir3 * ir2;
-                        const T gL2 = T(3) * ir5;
-                        const T g2 = e * (kr * kr + T(3) * kr + T(3)) * ir5;
-                        const T dd = q.dx * dx + q.dy * dy + q.dz * dz;
-                        const T nd = n0x * dx + n0y * dy + n0z * dz;
-                        const T ndv = n0x * q.dx + n0y * q.dy + n0z * q.dz;
-                        af += q.sh * (u - ir) + dd * (g1 - q.ie * ir3);
-                        ah += -q.sh * nd * ir3 +
-                              q.ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + q.sh * nd * g1);
-                    }
+        for (int base = 0; base < S; base += CAP) {
+            const int m = min(CAP, S - base);
+            __syncthreads();
+            for (int q = t; q < m; q += 256) {
+                const int pidx = base + q;
+                int k = 0;
+                while (pre[k + 1] <= pidx) ++k;
+                const Src64 sv = src[beg[k] + (pidx - pre[k])];
+                tile[q] = SrcR<T>{T(sv.x - cx), T(sv.y - cy), T(sv.z - cz), T(sv.inv_eps),
+                                  T(sv.dx), T(sv.dy), T(sv.dz), T(sv.sh)};
+            }
+            __syncthreads();
+            if (valid) {
+                for (int jj = phase; jj < m; jj += ns) {
+                    const SrcR<T> q = tile[jj];
+                    const T dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                    const T r2 = dx * dx + dy * dy + dz * dz;
+                    const T ir = rsq_or_zero(r2);
+                    const T r = r2 * ir;
+                    const T ir2 = ir * ir;
+                    const T ir3 = ir2 * ir;
+                    const T kr = kappa * r;
+                    const T e = Fm<T>::exp_(-kr);
+                    const T u = e * ir;
+                    const T g1 = e * (T(1) + kr) * ir3;
+                    const T ir5 = ir3 * ir2;
+                    const T gL2 = T(3) * ir5;
+                    const T g2 = e * (kr * kr + T(3) * kr + T(3)) * ir5;
+                    const T dd = q.dx * dx + q.dy * dy + q.dz * dz;
+                    const T nd = n0x * dx + n0y * dy + n0z * dz;
+                    const T ndv = n0x * q.dx + n0y * q.dy + n0z * q.dz;
+                    af += q.sh * (u - ir) + dd * (g1 - q.ie * ir3);
+                    ah += -q.sh * nd * ir3 +
+                          q.ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + q.sh * nd * g1);
                 }
             }
         }
-        // L2P, x-slices a = sub, sub + SUB, ...
-        if (valid && use_local) {
-            T sx[P], dsx[P], sy[P], dsy[P], sz[P], dsz[P];
-            const T ih = T(inv_half);
-            basis1d<T, P>(xi * ih, tn, sx, dsx);
-            basis1d<T, P>(yi * ih, tn, sy, dsy);
-            basis1d<T, P>(zi * ih, tn, sz, dsz);
-            T pL = 0, p1 = 0, gLx = 0, gLy = 0, gLz = 0, g2x = 0, g2y = 0, g2z = 0;
-            for (int a = sub; a < P; a += SUB)
-                for (int b = 0; b < P; ++b) {
-                    const T sab = sx[a] * sy[b], dab = dsx[a] * sy[b], adb = sx[a] * dsy[b];
-                    for (int c = 0; c < P; ++c) {
-                        const int m = (a * P + b) * P + c;
-                        const T s = sab * sz[c];
-                        const T gx = dab * sz[c], gy = adb * sz[c], gz = sab * dsz[c];
-                        const T l0 = lb[m], l1 = lb[NP + m], l2 = lb[2 * NP + m];
-                        pL += s * l0;
-                        p1 += s * l1;
-                        gLx += gx * l0; gLy += gy * l0; gLz += gz * l0;
-                        g2x += gx * l2; g2y += gy * l2; g2z += gz * l2;
-                    }
+        if (ns > 1) {
+            red[0][g][lane] = af;
+            red[1][g][lane] = ah;
+            __syncthreads();
+            if (valid && phase == 0) {
+                for (int k = 1; k < ns; ++k) {
+                    af += red[0][g + k * nch][lane];
+                    ah += red[1][g + k * nch][lane];
                 }
-            af += pL + p1;
-            ah += ih * (n0x * (gLx + g2x) + n0y * (gLy + g2y) + n0z * (gLz + g2z));
-        }
-        red[0][sub][slot] = af;
-        red[1][sub][slot] = ah;
-        __syncthreads();
-        if (valid && sub == 0) {
-            for (int k = 1; k < SUB; ++k) {
-                af += red[0][k][slot];
-                ah += red[1][k][slot];
             }
+        }
+        if (valid && phase == 0) {
+            const int i = d.perm[ti];
             y[i] += double(af) * FOUR_PI_INV;
             y[n + i] += double(ah) * FOUR_PI_INV;
         }
@@ -517,40 +583,20 @@ uint64_t bkey(uint32_t x, uint32_t y, uint32_t z) {
 }
 
 template <typename T>
-void upload(T*& dst, const std::vector<T>& v, hipStream_t s) {
-    if (dst) (void)hipFree(dst);
-    dst = nullptr;
-    if (v.empty()) return;
-    hip_check(hipMalloc(&dst, v.size() * sizeof(T)), "hipMalloc fmm");
-    hip_check(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
-              "H2D fmm");
-}
-
-template <typename T>
-void free_dev(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
+void upload(DevArena& ar, const char* name, T*& dst, const std::vector<T>& v, hipStream_t s) {
+    dst = ar.get<T>(name, std::max<size_t>(v.size(), 1));
+    if (!v.empty())
+        hip_check(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s),
+                  "H2D fmm");
 }
 
 }  // namespace
 
-FmmPlan::~FmmPlan() {
-    free_dev(dv.box_c); free_dev(dv.box_h); free_dev(dv.box_parent); free_dev(dv.box_oct);
-    free_dev(dv.child_ptr); free_dev(dv.child_box); free_dev(dv.child_oct);
-    free_dev(dv.leaf_src); free_dev(dv.perm); free_dev(dv.nb_ptr); free_dev(dv.nb_leaf);
-    free_dev(dv.pair_src); free_dev(dv.pair_tgt); free_dev(dv.tn); free_dev(dv.t_oct);
-    free_dev(dv.tgt_ptr); free_dev(dv.tgt_pair); free_dev(ytab64); free_dev(ytab32);
-    free_dev(slots64); free_dev(slots32);
-    free_dev(d_chunks); free_dev(d_xi);
-    for (auto& p : ktab64) free_dev(p);
-    for (auto& p : ktab32) free_dev(p);
-    free_dev(exp64); free_dev(exp32); free_dev(src);
-}
-
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
-                         hipStream_t s) {
+                         DevArena& arena, hipStream_t s) {
     if (order < 3 || order > 6) throw std::runtime_error("FMM order must be 3..6");
     auto* pl = new FmmPlan();
+    pl->arena = &arena;
     pl->N = N;
     pl->P = order;
     const int NP = order * order * order;
@@ -752,28 +798,28 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     pl->nleaf = nleaf;
     pl->dv.D = D;
     pl->dv.leaf_first = pl->lvl_first[D];
-    upload(pl->dv.box_c, box_c, s);
-    upload(pl->dv.box_h, box_h, s);
-    upload(pl->dv.box_parent, box_parent, s);
-    upload(pl->dv.box_oct, box_oct, s);
-    upload(pl->dv.child_ptr, child_ptr, s);
-    upload(pl->dv.child_box, child_box, s);
-    upload(pl->dv.child_oct, child_oct, s);
-    upload(pl->dv.leaf_src, leaf_src, s);
-    upload(pl->dv.perm, perm, s);
-    upload(pl->dv.nb_ptr, nb_ptr, s);
-    upload(pl->dv.nb_leaf, nb_leaf, s);
-    upload(pl->dv.pair_src, pair_src, s);
-    upload(pl->dv.pair_tgt, pair_tgt, s);
-    upload(pl->dv.tgt_ptr, tgt_ptr, s);
-    upload(pl->dv.tgt_pair, tgt_pair, s);
-    upload(pl->dv.tn, tn, s);
-    upload(pl->dv.t_oct, t_oct, s);
-    upload(pl->d_chunks, chunks, s);
-    upload(pl->d_xi, xi, s);
+    upload(arena, "box_c", pl->dv.box_c, box_c, s);
+    upload(arena, "box_h", pl->dv.box_h, box_h, s);
+    upload(arena, "box_parent", pl->dv.box_parent, box_parent, s);
+    upload(arena, "box_oct", pl->dv.box_oct, box_oct, s);
+    upload(arena, "child_ptr", pl->dv.child_ptr, child_ptr, s);
+    upload(arena, "child_box", pl->dv.child_box, child_box, s);
+    upload(arena, "child_oct", pl->dv.child_oct, child_oct, s);
+    upload(arena, "leaf_src", pl->dv.leaf_src, leaf_src, s);
+    upload(arena, "perm", pl->dv.perm, perm, s);
+    upload(arena, "nb_ptr", pl->dv.nb_ptr, nb_ptr, s);
+    upload(arena, "nb_leaf", pl->dv.nb_leaf, nb_leaf, s);
+    upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
+    upload(arena, "pair_tgt", pl->dv.pair_tgt, pair_tgt, s);
+    upload(arena, "tgt_ptr", pl->dv.tgt_ptr, tgt_ptr, s);
+    upload(arena, "tgt_pair", pl->dv.tgt_pair, tgt_pair, s);
+    upload(arena, "tn", pl->dv.tn, tn, s);
+    upload(arena, "t_oct", pl->dv.t_oct, t_oct, s);
+    upload(arena, "chunks", pl->d_chunks, chunks, s);
+    upload(arena, "xi", pl->d_xi, xi, s);
     pl->ktab64.assign(D + 2, nullptr);
     pl->ktab32.assign(D + 2, nullptr);
-    hip_check(hipMalloc(&pl->src, sizeof(Src64) * std::max(N, 1)), "hipMalloc fmm src");
+    pl->src = arena.get<Src64>("src", size_t(std::max(N, 1)));
     const size_t ne = size_t(nbox) * 3 * NP;
     pl->n_exp = ne;
     hip_check(hipStreamSynchronize(s), "sync fmm plan");
@@ -789,20 +835,19 @@ static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& y
     constexpr int NP = P * P * P;
     const size_t n = size_t(343) * NP * NP;
     const int blocks = int((n + 255) / 256);
-    for (auto& p : tabs) free_dev(p);
+    const std::string pre = sizeof(T) == 8 ? "ktab64_" : "ktab32_";
     tabs.assign(pl->D + 2, nullptr);
     // slot D+1: Laplace (level independent, h = 1); slot l: Yukawa at level l
-    hip_check(hipMalloc(&tabs[pl->D + 1], n * sizeof(T)), "hipMalloc ktab");
+    tabs[pl->D + 1] = pl->arena->get<T>(pre + "laplace", n);
     hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, 1.0, 0.0, 1,
                        tabs[pl->D + 1]);
     for (int l = 2; l <= pl->D; ++l) {
-        hip_check(hipMalloc(&tabs[l], n * sizeof(T)), "hipMalloc ktab");
+        tabs[l] = pl->arena->get<T>(pre + std::to_string(l), n);
         hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, pl->box_h[l],
                            kappa, 0, tabs[l]);
     }
     hip_check(hipGetLastError(), "k_ktab launch");
-    free_dev(ytab);
-    hip_check(hipMalloc(&ytab, tabs.size() * sizeof(T*)), "hipMalloc ytab");
+    ytab = pl->arena->get<T*>(pre + "ptrs", tabs.size());
     hip_check(hipMemcpyAsync(ytab, tabs.data(), tabs.size() * sizeof(T*), hipMemcpyHostToDevice, s),
               "H2D ytab");
 }
@@ -820,11 +865,9 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
             case 5: make_tables<double, 5>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
             case 6: make_tables<double, 6>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
         }
-        if (!pl->exp64) {
-            hip_check(hipMalloc(&pl->exp64, 2 * pl->n_exp * sizeof(double)), "hipMalloc fmm exp");
-            const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
-            hip_check(hipMalloc(&pl->slots64, ns * sizeof(double)), "hipMalloc m2l slots");
-        }
+        const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+        pl->exp64 = pl->arena->get<double>("exp64", 2 * pl->n_exp);
+        pl->slots64 = pl->arena->get<double>("slots64", ns);
         pl->have64 = true;
     }
     if (fp32 && !pl->have32) {
@@ -834,11 +877,9 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
             case 5: make_tables<float, 5>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
             case 6: make_tables<float, 6>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
         }
-        if (!pl->exp32) {
-            hip_check(hipMalloc(&pl->exp32, 2 * pl->n_exp * sizeof(float)), "hipMalloc fmm exp");
-            const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
-            hip_check(hipMalloc(&pl->slots32, ns * sizeof(float)), "hipMalloc m2l slots");
-        }
+        const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+        pl->exp32 = pl->arena->get<float>("exp32", 2 * pl->n_exp);
+        pl->slots32 = pl->arena->get<float>("slots32", ns);
         pl->have32 = true;
     }
     hip_check(hipStreamSynchronize(s), "sync fmm tables");
@@ -876,8 +917,11 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
         const int first = pl->lvl_first[l + 1], cnt = pl->lvl_first[l + 2] - first;
         if (cnt > 0) hipLaunchKernelGGL((k_l2l<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, L);
     }
-    hipLaunchKernelGGL((k_leaf<T, P>), dim3(pl->nleaf), dim3(256), 0, s, d, pl->src, patches,
-                       T(kappa), D >= 2 ? 1 : 0, L, y, pl->N);
+    if (D >= 2)
+        hipLaunchKernelGGL((k_l2p<T, P>), dim3(pl->nleaf), dim3(64), 0, s, d, patches, L, y,
+                           pl->N);
+    hipLaunchKernelGGL((k_p2p<T>), dim3(pl->nleaf), dim3(256), 0, s, d, pl->src, patches,
+                       T(kappa), y, pl->N);
     hip_check(hipGetLastError(), "fmm launch");
 }
 

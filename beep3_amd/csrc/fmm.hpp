@@ -55,11 +55,13 @@ struct FmmPlan {
     Src64* src = nullptr;
     bool have64 = false, have32 = false;
     double tables_kappa = -1;
-    ~FmmPlan();
+    DevArena* arena = nullptr;        // owner of every device buffer above
 };
 
+// The plan's device buffers come from the arena (owned by the caller, reused
+// by the next plan built on it); one live plan per arena.
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
-                         hipStream_t s);
+                         DevArena& arena, hipStream_t s);
 void fmm_plan_destroy(FmmPlan* p);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 // y += far field (1-point interaction of every pair) of x

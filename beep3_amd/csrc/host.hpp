@@ -67,6 +67,7 @@ struct Mesh {
     double radius = 0;                             // max |patch centroid - centre|
 
     void finalize();                               // adjacency, patches, centre, radius
+    void finalize_charges();                       // centre and radius only
     // constant-basis points of one node patch (triangle.h:388-479): for each
     // adjacent triangle, the two sub-triangles (centre, vertex, edge midpoint)
     void patch_points(int vertex, const Rule& rule, std::vector<QuadPt>& out) const;
@@ -177,3 +178,35 @@ struct beep_mesh {
 
 // Sets the message returned by beep_host_last_error() on this thread.
 void beep_host_set_error(const char* msg);
+
+// A rigid placement of a reference mesh (MeshInstance, mesh_instance.h:23-87).
+struct Instance {
+    const beep_mesh* mesh;
+    beep::Quat rot;
+    beep::Vec3 offset;
+    double Dprotein;
+};
+
+// Opaque handle behind beep_scene* of include/beep_host.h: solver parameters,
+// instances and, after beep_scene_build, the flattened patches, point sets
+// (QuadPt = 7 doubles x, y, z, nx, ny, nz, w), charges and octree lists.
+struct beep_scene {
+    double Dsolvent, kappa;
+    int quad, qual, nbsize;
+    std::vector<Instance> inst;
+    bool built = false;
+    std::vector<beep::Vec3> centroid, normal;
+    std::vector<double> area, eps, gc;
+    std::vector<int> patch_inst;
+    std::vector<int> ptr[3];
+    std::vector<beep::QuadPt> pts[3];
+    std::vector<double> charges;  // xyzq
+    std::vector<int> charge_inst;
+    std::vector<int> inst_patch_ptr, inst_charge_ptr;
+    beep::Octree tree;
+    std::vector<int> leaf_ptr, leaf_idx, nb_ptr, nb_idx;
+    beep::Vec3 cube_centre;
+    double cube_edge = 0;
+};
+static_assert(sizeof(beep::Vec3) == 3 * sizeof(double), "Vec3 is 3 packed doubles");
+static_assert(sizeof(beep::QuadPt) == 7 * sizeof(double), "QuadPt is 7 packed doubles");

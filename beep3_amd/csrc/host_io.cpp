@@ -150,7 +150,7 @@ extern "C" int beep_mesh_read_charges(beep_mesh* m, const char* path) {
             if (!(ls >> x >> y >> z >> q)) throw std::runtime_error("charges: need x y z q [r]");
             m->mesh.charges.push_back({x, y, z, q});
         }
-        m->mesh.finalize();
+        m->mesh.finalize_charges();
         return 0;
     } catch (const std::exception& e) {
         beep_host_set_error(e.what());

@@ -145,6 +145,10 @@ void Mesh::finalize() {
         centroid[v] = c / a;
         area[v] = a;
     }
+    finalize_charges();
+}
+
+void Mesh::finalize_charges() {
     Vec3 s;
     for (const auto& q : charges) s += Vec3(q[0], q[1], q[2]);
     centre = charges.empty() ? Vec3() : s / double(charges.size());

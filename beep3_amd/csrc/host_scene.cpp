@@ -5,43 +5,20 @@
 // instances.  build() places every instance's patches and points in universe
 // coordinates (NodePatch::change_coordinate_frame), computes the bounding
 // cube (BEEP::calculate_bounding_cube, beep.cpp:1652-1693) and the octree.
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "../../include/beep_host.h"
 #include "host.hpp"
 
 using namespace beep;
-
-struct Instance {
-    const beep_mesh* mesh;
-    Quat rot;
-    Vec3 offset;
-    double Dprotein;
-};
-
-struct beep_scene {
-    double Dsolvent, kappa;
-    int quad, qual, nbsize;
-    std::vector<Instance> inst;
-    bool built = false;
-    // flattened
-    std::vector<Vec3> centroid, normal;
-    std::vector<double> area, eps, gc;
-    std::vector<int> patch_inst;
-    std::vector<int> ptr[3];
-    std::vector<QuadPt> pts[3];
-    std::vector<double> charges;  // xyzq
-    std::vector<int> charge_inst;
-    std::vector<int> inst_patch_ptr, inst_charge_ptr;
-    Octree tree;
-    std::vector<int> leaf_ptr, leaf_idx, nb_ptr, nb_idx;
-    Vec3 cube_centre;
-    double cube_edge = 0;
-};
 
 static thread_local std::string g_err;
 
@@ -92,7 +69,7 @@ extern "C" int beep_mesh_set_charges(beep_mesh* m, int n, const double* xyzq) {
         m->mesh.charges.resize(n);
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < 4; ++k) m->mesh.charges[i][k] = xyzq[4 * i + k];
-        m->mesh.finalize();
+        m->mesh.finalize_charges();
     });
 }
 
@@ -101,7 +78,7 @@ extern "C" int beep_mesh_charges_ball(beep_mesh* m, int n, double rmax, double q
     return guarded([&] {
         if (!m || n < 0 || rmax <= 0) throw std::runtime_error("bad ball charges");
         charges_uniform_ball(m->mesh, n, rmax, qmin, qmax, seed);
-        m->mesh.finalize();
+        m->mesh.finalize_charges();
     });
 }
 
@@ -111,7 +88,7 @@ extern "C" int beep_mesh_charges_walk(beep_mesh* m, int n, double step, double c
         if (!m || !m->has_shape) throw std::runtime_error("random-walk charges need a star mesh");
         if (n < 0 || step <= 0) throw std::runtime_error("bad walk charges");
         charges_random_walk(m->mesh, m->shape, n, step, clearance, qsigma, seed);
-        m->mesh.finalize();
+        m->mesh.finalize_charges();
     });
 }
 
@@ -143,6 +120,13 @@ extern "C" int beep_mesh_vertices(const beep_mesh* m, double* xyz, int* tri3) {
         if (tri3)
             for (size_t t = 0; t < m->mesh.tris.size(); ++t)
                 for (int k = 0; k < 3; ++k) tri3[3 * t + k] = m->mesh.tris[t][k];
+    });
+}
+
+extern "C" int beep_mesh_clone(const beep_mesh* m, beep_mesh** out) {
+    return guarded([&] {
+        if (!m || !out) throw std::runtime_error("null argument");
+        *out = new beep_mesh(*m);
     });
 }
 
@@ -188,57 +172,58 @@ static Vec3 place(const Instance& in, const Vec3& p) {
     return in.rot.rotate(p - in.mesh->mesh.centre) + in.offset;
 }
 
+// worker threads for the host build: OMP_NUM_THREADS if set, else the
+// hardware concurrency, at most 16
+static int host_threads() {
+    int n = int(std::thread::hardware_concurrency());
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) n = std::atoi(e);
+    return std::max(1, std::min(16, n));
+}
+
+template <typename F>
+static void parallel_ranges(int n, F&& f) {
+    const int nt = std::min(host_threads(), std::max(1, n / 2048));
+    if (nt <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::exception_ptr err;
+    std::mutex mu;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            try {
+                f(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt));
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu);
+                err = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
 extern "C" int beep_scene_build(beep_scene* s) {
     return guarded([&] {
         if (!s || s->inst.empty()) throw std::runtime_error("scene has no instances");
-        s->centroid.clear(); s->normal.clear(); s->area.clear(); s->eps.clear(); s->gc.clear();
-        s->patch_inst.clear(); s->charges.clear(); s->charge_inst.clear();
+        const int orders[3] = {s->qual, s->quad, 4};
+        // patch p = vertex v of instance ii, instances in order
+        std::vector<std::pair<int, int>> plist;
         s->inst_patch_ptr.assign(1, 0);
         s->inst_charge_ptr.assign(1, 0);
-        for (int k = 0; k < 3; ++k) { s->ptr[k].assign(1, 0); s->pts[k].clear(); }
-        const int orders[3] = {s->qual, s->quad, 4};
-        std::vector<QuadPt> tmp;
+        s->charges.clear();
+        s->charge_inst.clear();
         Vec3 lo(1e99, 1e99, 1e99), hi(-1e99, -1e99, -1e99);
         for (size_t ii = 0; ii < s->inst.size(); ++ii) {
             const Instance& in = s->inst[ii];
             const Mesh& m = in.mesh->mesh;
-            const double eps = s->Dsolvent / in.Dprotein;
-            for (size_t v = 0; v < m.verts.size(); ++v) {
-                const Vec3 c = place(in, m.centroid[v]);
-                const Vec3 n = in.rot.rotate(m.normal[v]);
-                s->centroid.push_back(c);
-                s->normal.push_back(n);
-                s->area.push_back(m.area[v]);
-                s->eps.push_back(eps);
-                s->gc.push_back(0.5);
-                s->patch_inst.push_back(int(ii));
-                for (int k = 0; k < 3; ++k) {
-                    tmp.clear();
-                    if (orders[k] == 0) {
-                        // single centroid point: weight 1 for qualocation,
-                        // patch area for quadrature (node_patch.cpp:111-117)
-                        tmp.push_back(QuadPt{c, n, k == 0 ? 1.0 : m.area[v]});
-                    } else {
-                        m.patch_points(int(v), rule_for_order(orders[k]), tmp);
-                        for (auto& q : tmp) {
-                            q.p = place(in, q.p);
-                            q.n = in.rot.rotate(q.n);
-                        }
-                        // qualocation weights are renormalised to sum to 1
-                        // (node_patch.cpp:221-226)
-                        if (k == 0)
-                            for (auto& q : tmp) q.w /= m.area[v];
-                    }
-                    s->pts[k].insert(s->pts[k].end(), tmp.begin(), tmp.end());
-                    s->ptr[k].push_back(int(s->pts[k].size()));
-                }
-            }
+            for (size_t v = 0; v < m.verts.size(); ++v) plist.push_back({int(ii), int(v)});
             for (const auto& q : m.charges) {
                 Vec3 p = place(in, Vec3(q[0], q[1], q[2]));
                 s->charges.insert(s->charges.end(), {p.x, p.y, p.z, q[3]});
                 s->charge_inst.push_back(int(ii));
             }
-            s->inst_patch_ptr.push_back(int(s->centroid.size()));
+            s->inst_patch_ptr.push_back(int(plist.size()));
             s->inst_charge_ptr.push_back(int(s->charge_inst.size()));
             const double r = m.radius;
             lo = Vec3(std::min(lo.x, in.offset.x - r), std::min(lo.y, in.offset.y - r),
@@ -246,6 +231,59 @@ extern "C" int beep_scene_build(beep_scene* s) {
             hi = Vec3(std::max(hi.x, in.offset.x + r), std::max(hi.y, in.offset.y + r),
                       std::max(hi.z, in.offset.z + r));
         }
+        const int N = int(plist.size());
+        // point counts: one centroid point for rule 0, else k points on each of
+        // the 2 x valence sub-triangles of the patch
+        for (int k = 0; k < 3; ++k) {
+            s->ptr[k].assign(size_t(N) + 1, 0);
+            const size_t per = orders[k] == 0 ? 0 : rule_for_order(orders[k]).w.size();
+            for (int p = 0; p < N; ++p) {
+                const Mesh& m = s->inst[plist[p].first].mesh->mesh;
+                const size_t c = per == 0 ? 1 : 2 * m.vert_tris[plist[p].second].size() * per;
+                if (size_t(s->ptr[k][p]) + c > size_t(0x7fffffff))
+                    throw std::runtime_error("too many points for 32-bit offsets");
+                s->ptr[k][p + 1] = s->ptr[k][p] + int(c);
+            }
+            s->pts[k].resize(size_t(s->ptr[k][N]));
+        }
+        s->centroid.resize(N); s->normal.resize(N); s->area.resize(N);
+        s->eps.resize(N); s->gc.resize(N); s->patch_inst.resize(N);
+        parallel_ranges(N, [&](int p0, int p1) {
+            std::vector<QuadPt> tmp;
+            for (int p = p0; p < p1; ++p) {
+                const Instance& in = s->inst[plist[p].first];
+                const Mesh& m = in.mesh->mesh;
+                const int v = plist[p].second;
+                const Vec3 c = place(in, m.centroid[v]);
+                const Vec3 n = in.rot.rotate(m.normal[v]);
+                s->centroid[p] = c;
+                s->normal[p] = n;
+                s->area[p] = m.area[v];
+                s->eps[p] = s->Dsolvent / in.Dprotein;
+                s->gc[p] = 0.5;
+                s->patch_inst[p] = plist[p].first;
+                for (int k = 0; k < 3; ++k) {
+                    QuadPt* out = s->pts[k].data() + s->ptr[k][p];
+                    if (orders[k] == 0) {
+                        // single centroid point: weight 1 for qualocation,
+                        // patch area for quadrature (node_patch.cpp:111-117)
+                        out[0] = QuadPt{c, n, k == 0 ? 1.0 : m.area[v]};
+                        continue;
+                    }
+                    tmp.clear();
+                    m.patch_points(v, rule_for_order(orders[k]), tmp);
+                    for (size_t j = 0; j < tmp.size(); ++j) {
+                        QuadPt q = tmp[j];
+                        q.p = place(in, q.p);
+                        q.n = in.rot.rotate(q.n);
+                        // qualocation weights are renormalised to sum to 1
+                        // (node_patch.cpp:221-226)
+                        if (k == 0) q.w /= m.area[v];
+                        out[j] = q;
+                    }
+                }
+            }
+        });
         Vec3 d = hi - lo;
         s->cube_edge = std::max(d.x, std::max(d.y, d.z));
         if (!(s->cube_edge > 0)) throw std::runtime_error("degenerate bounding cube");

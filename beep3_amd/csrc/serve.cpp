@@ -1,0 +1,372 @@
+// Solve server (include/beep_serve.h): worker slots with resident device
+// contexts, admission by a device-memory budget, continuous batching of
+// requests as they arrive.  Host C++ only; the device work goes through the
+// C ABI of include/beep_hip.h.
+#include "../../include/beep_serve.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/beep_hip.h"
+#include "host.hpp"
+
+namespace {
+
+// src/bem/constants.h:14-51; energy in kJ/mol (mesh.cpp:1373-1403)
+constexpr double AVOGADRO = 6.02214179e23;
+constexpr double EPSILON0 = 8.85418782e-12;
+constexpr double ELEMENTARY_CHARGE = 1.60217646e-19;
+constexpr double ANGSTROM = 1.0e-10;
+constexpr double KJ_PER_MOL =
+    AVOGADRO * ELEMENTARY_CHARGE * ELEMENTARY_CHARGE / (EPSILON0 * ANGSTROM * 1000.0);
+
+struct Job {
+    long long id;
+    beep_scene* scene;
+    double t_submit;
+};
+
+struct Done {
+    beep_result r{};
+    std::vector<double> f, h;
+};
+
+void check_hip(beep_hip_ctx* c, int rc, const char* what) {
+    if (rc != 0)
+        throw std::runtime_error(std::string(what) + ": " + beep_hip_last_error(c));
+}
+
+// device bytes a scene will need in a context (upper estimate, reconciled
+// with beep_hip_memory_held after the request): point sets in both
+// precisions, coefficients of the solve's precisions, GMRES basis and
+// per-patch FMM storage (tables, expansions, M2L slots: about 12 KB/patch)
+double estimate_bytes(const beep_scene* s, const beep_server_opts& o) {
+    const double N = double(s->centroid.size());
+    const double nq = double(s->pts[0].size()), nd = double(s->pts[1].size()),
+                 ng = double(s->pts[2].size());
+    double nnz = 0;
+    for (size_t L = 0; L + 1 < s->leaf_ptr.size(); ++L)
+        nnz += double(s->leaf_ptr[L + 1] - s->leaf_ptr[L]) * double(s->nb_ptr[L + 1] - s->nb_ptr[L]);
+    double b = 96.0 * (nq + nd) + 64.0 * ng + 64.0 * N;
+    if (o.solve_mode != BEEP_SOLVE_FP64) b += 16.0 * nnz;
+    if (o.solve_mode == BEEP_SOLVE_FP64) b += 32.0 * nnz;
+    b += 16.0 * N * double(o.max_iter + 1) + 16.0 * N * 8.0;
+    b += 12288.0 * N;
+    return b;
+}
+
+}  // namespace
+
+struct beep_server {
+    beep_server_opts o{};
+    std::chrono::steady_clock::time_point t0;
+    mutable std::mutex mu;
+    std::condition_variable cv_job, cv_done, cv_mem;
+    std::deque<Job> queue;
+    std::deque<long long> done_order;
+    std::unordered_map<long long, Done> done;
+    std::vector<std::thread> workers;
+    std::vector<double> held;  // bytes charged to each slot
+    double used = 0;
+    int waiting_for_memory = 0;
+    long long next_id = 1;
+    int pending = 0;  // submitted, not yet polled
+    bool stop = false;
+    std::string err = "no error";
+
+    double clock() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void worker(int slot);
+    void solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d);
+    void release_slot(int slot, beep_hip_ctx*& ctx);  // caller holds mu
+};
+
+void beep_server::release_slot(int slot, beep_hip_ctx*& ctx) {
+    if (ctx) beep_hip_destroy(ctx);
+    ctx = nullptr;
+    used -= held[slot];
+    held[slot] = 0;
+    cv_mem.notify_all();
+}
+
+void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
+    beep_result& r = d.r;
+    beep_scene* sc = j.scene;
+    const double th = clock();
+    if (beep_scene_build(sc) != 0)
+        throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
+    r.host_ms = (clock() - th) * 1e3;
+    const int N = int(sc->centroid.size());
+    r.n_patches = N;
+    // admission: a free slot (this thread) and room in the memory budget
+    const double need_total = estimate_bytes(sc, o);
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        if (need_total > o.hbm_budget)
+            throw std::runtime_error("request needs more device memory than the budget");
+        double grow = std::max(0.0, need_total - held[slot]);
+        if (used + grow > o.hbm_budget && ctx) {
+            // give back what this slot holds before waiting, so that waiting
+            // slots never hold memory the others need
+            release_slot(slot, ctx);
+            grow = need_total;
+        }
+        ++waiting_for_memory;
+        cv_job.notify_all();  // idle slots release their contexts under pressure
+        cv_mem.wait(lk, [&] { return used + grow <= o.hbm_budget || used == 0; });
+        --waiting_for_memory;
+        used += grow;
+        held[slot] += grow;
+    }
+    r.t_admit = clock();
+    const double td = clock();
+    if (!ctx) {
+        if (beep_hip_create(o.device, &ctx) != 0)
+            throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
+    }
+    check_hip(ctx, beep_hip_set_far_field(ctx, o.far_mode, o.fmm_order, o.fmm_leaf), "far field");
+    beep_hip_geometry g{};
+    g.n_patches = N;
+    g.centroid = &sc->centroid[0].x;
+    g.normal = &sc->normal[0].x;
+    g.area = sc->area.data();
+    g.eps_ratio = sc->eps.data();
+    g.gc = sc->gc.data();
+    g.qual_ptr = sc->ptr[0].data();
+    g.qual_pts = &sc->pts[0][0].p.x;
+    g.quad_ptr = sc->ptr[1].data();
+    g.quad_pts = &sc->pts[1][0].p.x;
+    g.gal_ptr = sc->ptr[2].data();
+    g.gal_pts = &sc->pts[2][0].p.x;
+    check_hip(ctx, beep_hip_set_geometry(ctx, &g), "geometry");
+    beep_hip_tree t{};
+    t.n_leaves = int(sc->leaf_ptr.size()) - 1;
+    t.leaf_ptr = sc->leaf_ptr.data();
+    t.leaf_idx = sc->leaf_idx.data();
+    t.nb_ptr = sc->nb_ptr.data();
+    t.nb_idx = sc->nb_idx.data();
+    check_hip(ctx, beep_hip_set_tree(ctx, &t), "tree");
+    if (o.solve_mode != BEEP_SOLVE_FP64)
+        check_hip(ctx, beep_hip_precompute(ctx, sc->kappa, BEEP_FP32), "precompute");
+    else
+        check_hip(ctx, beep_hip_precompute(ctx, sc->kappa, BEEP_FP64), "precompute");
+    std::vector<double> b(size_t(2) * N), x(size_t(2) * N, 0.0);
+    const int nch = int(sc->charge_inst.size());
+    check_hip(ctx, beep_hip_rhs(ctx, nch, nch ? sc->charges.data() : nullptr, sc->Dsolvent,
+                                b.data()), "rhs");
+    beep_solve_opts so{o.tol, o.max_iter, o.solve_mode, o.max_refine, o.inner_tol, 0};
+    beep_solve_stats st{};
+    check_hip(ctx, beep_hip_solve(ctx, b.data(), x.data(), &so, &st), "solve");
+    // energy: Gauss-law renormalised coefficients (mesh.cpp:891-925), then
+    // 1/2 sum (h eps ec_h - f ec_f) per instance (mesh.cpp:1373-1403)
+    const int ni = int(sc->inst.size());
+    std::vector<double> ef(N), eh(N);
+    check_hip(ctx, beep_hip_energy_coefficients(ctx, ni, sc->inst_patch_ptr.data(),
+                                                sc->inst_charge_ptr.data(),
+                                                nch ? sc->charges.data() : nullptr, ef.data(),
+                                                eh.data()), "energy coefficients");
+    double E = 0;
+    for (int k = 0; k < ni; ++k) {
+        const int p0 = sc->inst_patch_ptr[k], p1 = sc->inst_patch_ptr[k + 1];
+        double net = 0, total_f = 0, area = 0;
+        for (int c = sc->inst_charge_ptr[k]; c < sc->inst_charge_ptr[k + 1]; ++c)
+            net += sc->charges[4 * size_t(c) + 3];
+        for (int i = p0; i < p1; ++i) {
+            total_f -= ef[i];
+            area += sc->area[i];
+        }
+        const double e = net - total_f;
+        double s = 0;
+        for (int i = p0; i < p1; ++i) {
+            const double efi = ef[i] - sc->area[i] / area * e;
+            s += x[size_t(N) + i] * sc->eps[i] * eh[i] - x[i] * efi;
+        }
+        E += 0.5 * s * KJ_PER_MOL;
+    }
+    r.energy = E;
+    r.iterations = st.iterations;
+    r.matvecs = st.matvecs;
+    r.refinements = st.refinements;
+    r.converged = st.converged;
+    r.rel_residual = st.rel_residual;
+    if (o.keep_fh) {
+        d.f.assign(x.begin(), x.begin() + N);
+        d.h.assign(x.begin() + N, x.end());
+    }
+    size_t hb = 0;
+    check_hip(ctx, beep_hip_memory_held(ctx, &hb), "memory");
+    r.bytes = double(hb);
+    r.device_ms = (clock() - td) * 1e3;
+    {
+        // charge the slot what its context really holds
+        std::lock_guard<std::mutex> lk(mu);
+        used += double(hb) - held[slot];
+        held[slot] = double(hb);
+        cv_mem.notify_all();
+    }
+}
+
+void beep_server::worker(int slot) {
+    beep_hip_ctx* ctx = nullptr;
+    for (;;) {
+        Job j{};
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_job.wait(lk, [&] {
+                if (!queue.empty() || stop) return true;
+                // an idle slot gives its memory back when another slot waits
+                if (waiting_for_memory > 0 && held[slot] > 0) release_slot(slot, ctx);
+                return false;
+            });
+            if (queue.empty()) break;  // stop requested and nothing left
+            j = queue.front();
+            queue.pop_front();
+        }
+        Done d;
+        d.r.id = j.id;
+        d.r.slot = slot;
+        d.r.t_submit = j.t_submit;
+        try {
+            solve_one(slot, ctx, j, d);
+            d.r.status = 0;
+            std::snprintf(d.r.message, sizeof(d.r.message), "ok");
+        } catch (const std::exception& e) {
+            d.r.status = -1;
+            std::snprintf(d.r.message, sizeof(d.r.message), "%s", e.what());
+            std::lock_guard<std::mutex> lk(mu);
+            release_slot(slot, ctx);  // a failed context is not reused
+        }
+        beep_scene_destroy(j.scene);
+        d.r.t_done = clock();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            done_order.push_back(j.id);
+            done.emplace(j.id, std::move(d));
+            cv_done.notify_all();
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    release_slot(slot, ctx);
+}
+
+extern "C" {
+
+void beep_server_default_opts(beep_server_opts* o) {
+    if (!o) return;
+    *o = beep_server_opts{};
+    o->device = 0;
+    o->slots = 4;
+    o->hbm_budget = 64.0 * 1024 * 1024 * 1024;
+    o->solve_mode = BEEP_SOLVE_MIXED;
+    o->tol = 1e-6;
+    o->max_iter = 100;
+    o->max_refine = 4;
+    o->inner_tol = 1e-3;
+    o->far_mode = BEEP_FAR_AUTO;
+    o->fmm_order = 5;
+    o->fmm_leaf = 40;
+}
+
+int beep_server_create(const beep_server_opts* opts, beep_server** out) {
+    if (!out) return -2;
+    *out = nullptr;
+    try {
+        auto s = std::make_unique<beep_server>();
+        if (opts) s->o = *opts;
+        else beep_server_default_opts(&s->o);
+        if (s->o.slots < 1 || s->o.slots > 64) throw std::runtime_error("slots must be 1..64");
+        if (!(s->o.hbm_budget > 0)) throw std::runtime_error("budget must be positive");
+        s->t0 = std::chrono::steady_clock::now();
+        s->held.assign(size_t(s->o.slots), 0.0);
+        for (int k = 0; k < s->o.slots; ++k) s->workers.emplace_back([p = s.get(), k] { p->worker(k); });
+        *out = s.release();
+        return 0;
+    } catch (const std::exception& e) {
+        beep_host_set_error(e.what());
+        return -1;
+    }
+}
+
+int beep_server_submit(beep_server* s, beep_scene* scene, long long* id) {
+    if (!s || !scene) return -2;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) {
+        s->err = "server is stopping";
+        return -1;
+    }
+    const long long k = s->next_id++;
+    s->queue.push_back(Job{k, scene, s->clock()});
+    ++s->pending;
+    if (id) *id = k;
+    s->cv_job.notify_one();
+    return 0;
+}
+
+int beep_server_poll(beep_server* s, double timeout_s, beep_result* out) {
+    if (!s || !out) return -2;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return !s->done_order.empty(); };
+    if (!ready()) {
+        if (timeout_s <= 0) return 1;
+        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) return 1;
+    }
+    const long long k = s->done_order.front();
+    s->done_order.pop_front();
+    *out = s->done.at(k).r;
+    if (!s->o.keep_fh || s->done.at(k).f.empty()) s->done.erase(k);
+    --s->pending;
+    return 0;
+}
+
+int beep_server_take_fh(beep_server* s, long long id, double* f, double* h) {
+    if (!s) return -2;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->done.find(id);
+    if (it == s->done.end() || it->second.f.empty()) {
+        s->err = "no surface values for this request";
+        return -1;
+    }
+    if (f) std::copy(it->second.f.begin(), it->second.f.end(), f);
+    if (h) std::copy(it->second.h.begin(), it->second.h.end(), h);
+    s->done.erase(it);
+    return 0;
+}
+
+double beep_server_clock(const beep_server* s) { return s ? s->clock() : 0.0; }
+
+int beep_server_pending(const beep_server* s) {
+    if (!s) return -2;
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->pending;
+}
+
+const char* beep_server_last_error(const beep_server* s) {
+    return s ? s->err.c_str() : beep_host_last_error();
+}
+
+int beep_server_destroy(beep_server* s) {
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_job.notify_all();
+    }
+    for (auto& t : s->workers) t.join();
+    delete s;  // results not taken are dropped with it
+    return 0;
+}
+
+}  // extern "C"

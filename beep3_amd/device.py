@@ -73,6 +73,10 @@ class Device:
         m = self.FAR_MODES[mode] if isinstance(mode, str) else int(mode)
         self._check(lib().beep_hip_set_far_field(self._h, m, int(order), int(leaf_target)))
 
+    def set_overlap(self, on: bool = True):
+        """Far field on a second stream, concurrent with the near SpMV."""
+        self._check(lib().beep_hip_set_overlap(self._h, 1 if on else 0))
+
     def far_info(self) -> dict:
         f = FarInfo()
         self._check(lib().beep_hip_far_info(self._h, C.byref(f)))
@@ -149,6 +153,12 @@ class Device:
         f, t = C.c_size_t(), C.c_size_t()
         self._check(lib().beep_hip_memory_info(self._h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def memory_held(self) -> int:
+        """Device bytes held by this context's buffers."""
+        b = C.c_size_t()
+        self._check(lib().beep_hip_memory_held(self._h, C.byref(b)))
+        return b.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -50,6 +50,11 @@ class Mesh:
             check_host(lib().beep_mesh_read_charges(m._h, str(charges).encode()))
         return m
 
+    def clone(self) -> "Mesh":
+        h = vp()
+        check_host(lib().beep_mesh_clone(self._h, C.byref(h)))
+        return Mesh(h)
+
     def write_off(self, path: str):
         check_host(lib().beep_mesh_write_off(self._h, str(path).encode()))
 

@@ -53,19 +53,35 @@ def time_matvecs(b, d_x, d_y, precision, warmup, steps):
     dev.synchronize()
     t1 = time.perf_counter()
     tm = dev.timing()
-    return (t1 - t0) * 1e3 / steps, tm.near_ms_sum / max(tm.count, 1), tm.far_ms_sum / max(tm.count, 1)
+    k = max(tm.count, 1)
+    return ((t1 - t0) * 1e3 / steps, tm.near_ms_sum / k, tm.far_ms_sum / k, tm.total_ms_sum / k)
 
 
 def near_bytes(a, coef_bytes):
-    """Compulsory bytes of one near-field SpMV launch set: every coefficient
-    once, every leaf's column list once, leaf rows + diagonal once, x read
-    once and y written once."""
+    """Bytes of one near-field SpMV launch set.
+
+    algorithmic: the per-pair figure of the reference layout (4 coefficients
+    and one 4-byte column index per pair, local_integrations.h:17-25) plus x
+    read once (2N values of 8 B) and y written once (2N x 8 B);
+    compulsory: what this block layout must move (every coefficient once,
+    one column index per (leaf, column) instead of per pair, leaf rows and
+    the diagonal once, x once, y once)."""
     lp, npt = a["leaf_ptr"], a["nb_ptr"]
     R = np.diff(lp).astype(np.int64)
     C = np.diff(npt).astype(np.int64)
     nnz = int(np.sum(R * C))
     N = int(a["area"].shape[0])
-    return nnz * coef_bytes + int(C.sum()) * 4 + N * (4 + 32) + 2 * N * 8 + 2 * N * 8, nnz
+    algorithmic = nnz * (coef_bytes + 4) + 2 * N * 8 + 2 * N * 8
+    compulsory = nnz * coef_bytes + int(C.sum()) * 4 + N * (4 + 32) + 2 * N * 8 + 2 * N * 8
+    return algorithmic, compulsory, nnz
+
+
+# HBM bytes per FP64 near-SpMV launch set (k_near_spmv_part<double,double,4> +
+# k_near_spmv_sum<double>) on the 163,842-patch globular surface, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/near_spmv_globular_163842_pmc.md):
+# FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors that make the counters
+# read 1 GiB for a 1 GiB device copy.
+MEASURED_TRAFFIC = {("globular", "fp64"): 8.240e9}
 
 
 def cpu_baseline(b, seconds=15.0):
@@ -96,6 +112,80 @@ def cpu_baseline(b, seconds=15.0):
                       f"1-point far sum, diagonal), {t_used:.1f} s"}
 
 
+def serving_bench(seconds, slots, seq_requests, rate=50.0):
+    """Poisson stream of single-molecule solves (rate per second over
+    `seconds`) through the continuous-batching server, then the first
+    `seq_requests` of the same stream as sequential plain FP64 solves (one
+    slot, one at a time)."""
+    from beep3_amd import configs
+    from beep3_amd.serve import Server
+
+    sched = configs.serving_schedule(rate=rate, duration=seconds, seed=11)
+    t0 = time.perf_counter()
+    meshes = configs.serving_meshes()
+    gen_s = time.perf_counter() - t0
+    print(f"[bench] serving: {len(sched)} requests over {seconds:.0f}s, mesh pool {gen_s:.1f}s",
+          file=sys.stderr, flush=True)
+
+    def run(srv, reqs, paced):
+        ids, res = {}, []
+        t_start = srv.clock()
+        for q in reqs:
+            if paced:
+                wait = t_start + q["t"] - srv.clock()
+                if wait > 0:
+                    time.sleep(wait)
+            sc, _ = configs.serving_request(meshes, q)
+            ids[srv.submit(sc)] = q
+            if not paced:  # sequential: one request at a time
+                r = None
+                while r is None:
+                    r = srv.poll(timeout=5.0)
+                res.append(r)
+            else:
+                r = srv.poll(timeout=0.0)
+                while r is not None:
+                    res.append(r)
+                    r = srv.poll(timeout=0.0)
+        while len(res) < len(reqs):
+            r = srv.poll(timeout=5.0)
+            if r is not None:
+                res.append(r)
+        t_end = max(r["t_done"] for r in res)
+        return res, t_end - t_start
+
+    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="auto", fmm_order=5,
+                 fmm_leaf=40)
+    res, span = run(srv, sched, paced=True)
+    srv.close()
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["latency_s"] for r in ok]) * 1e3
+    by_level = {}
+    for r in ok:
+        by_level.setdefault(r["n_patches"], []).append(r["latency_s"] * 1e3)
+    out = {"requests": len(sched), "solved": len(ok), "failed": len(res) - len(ok),
+           "slots": slots, "arrival_rate_per_s": rate, "span_s": span,
+           "solves_per_s": len(ok) / span, "latency_ms_p50": float(np.percentile(lat, 50)),
+           "latency_ms_p99": float(np.percentile(lat, 99)),
+           "latency_ms_p50_by_patches": {str(k): float(np.percentile(v, 50)) for k, v in sorted(by_level.items())},
+           "host_ms_mean": float(np.mean([r["host_ms"] for r in ok])),
+           "device_ms_mean": float(np.mean([r["device_ms"] for r in ok])),
+           "max_device_bytes_per_slot": float(max(r["bytes"] for r in ok)),
+           "unconverged": sum(1 for r in ok if not r["converged"])}
+    print(f"[bench] serving: {out}", file=sys.stderr, flush=True)
+    srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=40)
+    sres, sspan = run(srv, sched[:seq_requests], paced=False)
+    srv.close()
+    sok = [r for r in sres if r["status"] == 0]
+    service = np.array([(r["t_done"] - r["t_submit"]) * 1e3 for r in sok])
+    out["sequential_plain_fp64"] = {
+        "requests": len(sres), "solves_per_s": len(sok) / sspan,
+        "service_ms_p50": float(np.percentile(service, 50)),
+        "service_ms_p99": float(np.percentile(service, 99))}
+    print(f"[bench] sequential plain: {out['sequential_plain_fp64']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +197,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--far", default="fmm", choices=["fmm", "direct", "auto"])
     ap.add_argument("--order", type=int, default=5)
-    ap.add_argument("--leaf", type=int, default=160)
+    ap.add_argument("--leaf", type=int, default=40)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-seq", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
@@ -150,19 +244,28 @@ def main():
     rates = {}
     for p, steps in (("fp64", args.steps), ("fp32", max(3, args.steps // 2)),
                      ("fp32_resid", max(3, args.steps // 2))):
-        ms, near_ms, far_ms = time_matvecs(b, d_x, d_y, p, args.warmup, steps)
+        ms, near_ms, far_ms, tot_ms = time_matvecs(b, d_x, d_y, p, args.warmup, steps)
         rates[p] = {"ms_per_matvec": ms, "near_ms": near_ms, "far_ms": far_ms,
-                    "matvecs_per_s": 1e3 / ms, "steps": steps}
-        print(f"[bench] {p}: {ms:.3f} ms/matvec (near {near_ms:.3f}, far {far_ms:.3f})",
-              file=sys.stderr, flush=True)
+                    "device_ms": tot_ms, "matvecs_per_s": 1e3 / ms, "steps": steps}
+        print(f"[bench] {p}: {ms:.3f} ms/matvec (device {tot_ms:.3f}: near {near_ms:.3f} "
+              f"concurrent with far {far_ms:.3f})", file=sys.stderr, flush=True)
+    # the near SpMV alone (far field serialised after it), for its roofline
+    dev.set_overlap(False)
+    _, near_alone, far_alone, _ = time_matvecs(b, d_x, d_y, "fp64", 1, max(3, args.steps // 2))
+    _, near32_alone, far32_alone, _ = time_matvecs(b, d_x, d_y, "fp32", 1, max(3, args.steps // 2))
+    dev.set_overlap(True)
 
-    nb64, nnz = near_bytes(a, 32)
-    nb32, _ = near_bytes(a, 16)
+    nb64, cb64, nnz = near_bytes(a, 32)
+    nb32, cb32, _ = near_bytes(a, 16)
     head = rates["fp64"]
-    roofline = {"bound": "hbm", "achieved": nb64 / (head["near_ms"] * 1e-3) / 1e9,
+    roofline = {"bound": "hbm", "achieved": nb64 / (near_alone * 1e-3) / 1e9,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": nb64 / (head["near_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_near_spmv<double,double>", "bytes_per_launch": nb64}
+                "frac": nb64 / (near_alone * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
+                "kernel": "k_near_spmv_part<double,double,4> + k_near_spmv_sum<double>",
+                "bytes_per_launch": nb64, "launch_ms": near_alone,
+                "compulsory_bytes": cb64,
+                "compulsory_GBs": cb64 / (near_alone * 1e-3) / 1e9}
     extra = {
         "nnz_near_pairs": nnz, "leaves": int(len(lp) - 1), "integrand_evals": n_evals,
         "precompute_ms": pre,
@@ -171,7 +274,9 @@ def main():
             "fp64": n_evals * FLOP_PER_EVAL / (pre["fp64"] * 1e-3) / 1e12 / FP64_VECTOR_TFLOPS,
             "fp32": n_evals * FLOP_PER_EVAL / (pre["fp32"] * 1e-3) / 1e12 / FP32_VECTOR_TFLOPS},
         "matvec": rates,
-        "near_spmv_fp32_GBs": nb32 / (rates["fp32"]["near_ms"] * 1e-3) / 1e9,
+        "serial_phases_ms": {"fp64": {"near": near_alone, "far": far_alone},
+                             "fp32": {"near": near32_alone, "far": far32_alone}},
+        "near_spmv_fp32_GBs": nb32 / (near32_alone * 1e-3) / 1e9,
         "host_setup_s": setup_s,
     }
 
@@ -186,6 +291,12 @@ def main():
                             "rel_residual": ri["rel_residual"], "energy_kJ_mol": E}
             print(f"[bench] solve {mode}: {solves[mode]}", file=sys.stderr, flush=True)
         solves["mixed_speedup_vs_fp64"] = solves["fp64"]["solve_ms"] / solves["mixed"]["solve_ms"]
+        # per-solve setup on the device: singular terms + near coefficients of
+        # the precision the solve needs (FP64 for the plain solve, FP32 for mixed)
+        for mode, p in (("fp64", "fp64"), ("mixed", "fp32")):
+            solves[mode]["with_precompute_ms"] = solves[mode]["solve_ms"] + pre[p]
+        solves["mixed_speedup_vs_fp64_with_precompute"] = (
+            solves["fp64"]["with_precompute_ms"] / solves["mixed"]["with_precompute_ms"])
         # reference energy: FP64 coefficients, exact direct far field (the
         # explicit-pair + 1-point operator the oracle defines), FP64 GMRES
         if args.far != "direct":
@@ -201,13 +312,17 @@ def main():
                 solves[mode]["energy_kJ_mol"] - E_ref) / abs(E_ref)
         extra["solve"] = solves
 
+    dev.free(d_x)
+    dev.free(d_y)
+    if not args.no_serve:
+        b.close()  # free the headline workload's device memory first
+        extra["serving"] = serving_bench(args.serve_seconds, args.serve_slots, args.serve_seq)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(b, args.cpu_seconds)
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
-    dev.free(d_x)
-    dev.free(d_y)
     out = {
         "metric": METRIC, "value": head["matvecs_per_s"], "unit": "mat-vecs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_matvec"],

@@ -109,6 +109,10 @@ typedef struct {
     double matvec_ms;
 } beep_solve_stats;
 
+/* Device time of the phases of a mat-vec.  With overlap on (the default) the
+ * far field runs on a second stream while the near SpMV runs on the context
+ * stream: near_ms and far_ms are then both measured from the start of the
+ * mat-vec and total_ms is the whole mat-vec (max of the two + the final add). */
 typedef struct {
     double near_ms;     /* last mat-vec: near-field SpMV + diagonal */
     double far_ms;      /* last mat-vec: far field */
@@ -116,6 +120,7 @@ typedef struct {
     int count;          /* mat-vecs since the last beep_hip_timing_reset */
     double near_ms_sum; /* summed over those mat-vecs */
     double far_ms_sum;
+    double total_ms_sum;
 } beep_matvec_timing;
 
 const char* beep_hip_last_error(const beep_hip_ctx* ctx);
@@ -138,6 +143,14 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
 #define BEEP_FAR_DIRECT 0
 #define BEEP_FAR_FMM 1
 int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target);
+
+/* Far field on a second HIP stream, concurrent with the near-field SpMV
+ * (on = 1, the default), or both on the context stream (on = 0).  The same
+ * kernels run in the same order on each stream, so y is bit-identical.  This
+ * replaces the reference's "near field, then FMM" sequence
+ * (beep.cpp:1517-1535), whose GPU near field finishes before the CPU FMM
+ * starts (beep.cpp:1041-1045). */
+int beep_hip_set_overlap(beep_hip_ctx* ctx, int on);
 
 /* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
  * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs
@@ -193,6 +206,9 @@ int beep_hip_device_free(beep_hip_ctx* ctx, void* p);
 int beep_hip_memcpy(beep_hip_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 int beep_hip_synchronize(beep_hip_ctx* ctx);
 int beep_hip_memory_info(beep_hip_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+/* device bytes held by the context (its buffers only grow: a later geometry
+ * of no larger size reuses them) */
+int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,8 @@ int beep_mesh_vertices(const beep_mesh* m, double* xyz, int* tri3);
 int beep_mesh_read(const char* path, beep_mesh** out);
 int beep_mesh_write_off(const beep_mesh* m, const char* path);
 int beep_mesh_read_charges(beep_mesh* m, const char* path);
+/* independent copy (surface, patches, charges, star shape) */
+int beep_mesh_clone(const beep_mesh* m, beep_mesh** out);
 void beep_mesh_destroy(beep_mesh* m);
 
 /* ---- scene: BEEP(Dsolvent, kappa, quad, qual, nbsize) + instances ---- */

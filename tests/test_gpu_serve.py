@@ -1,0 +1,81 @@
+"""GPU: the solve server (include/beep_serve.h).  A request's result is
+bit-identical whether it is solved alone or in a mix of concurrent requests
+(continuous batching over several slots), and a Born ion solved through the
+server matches the Born-Kirkwood energy."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from beep3_amd import configs
+from beep3_amd.scene import Mesh, Scene
+from beep3_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+# (icosphere level, shape seed, charges, charge seed)
+REQUESTS = [(3, 11, 50, 1), (4, 12, 200, 2), (5, 13, 500, 3), (4, 14, 80, 4), (3, 15, 120, 5),
+            (5, 16, 300, 6)]
+
+
+def _meshes():
+    return [Mesh.star(lvl, 25.0, 8, 0.04, s).charges_walk(n, 3.8, 2.0, 0.4, cs)
+            for lvl, s, n, cs in REQUESTS]
+
+
+def _scene(m):
+    return Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 1500).add_instance(m, 2.0)
+
+
+def _collect(srv, ids, timeout=300):
+    out, t0 = {}, time.time()
+    while len(out) < len(ids) and time.time() - t0 < timeout:
+        r = srv.poll(timeout=2.0)
+        if r is None:
+            continue
+        assert r["status"] == 0, r["message"]
+        f, h = srv.take_fh(r["id"], r["n_patches"])
+        out[ids[r["id"]]] = (r, f, h)
+    assert len(out) == len(ids), "requests did not finish"
+    return out
+
+
+@pytest.mark.parametrize("far", ["fmm", "direct"])
+def test_alone_and_in_mix_are_identical(far):
+    meshes = _meshes()
+    opts = dict(keep_fh=True, far_field=far, fmm_order=5, fmm_leaf=40, budget_gib=32)
+    alone = {}
+    srv = Server(slots=1, **opts)
+    for k, m in enumerate(meshes):
+        alone.update(_collect(srv, {srv.submit(_scene(m)): k}))
+    srv.close()
+    # the mix: every request twice, all submitted at once onto 4 slots
+    srv = Server(slots=4, **opts)
+    ids = {}
+    for rep in range(2):
+        for k, m in enumerate(meshes):
+            ids[srv.submit(_scene(m))] = (k, rep)
+    mix = _collect(srv, ids)
+    srv.close()
+    slots_used = {r["slot"] for r, _, _ in mix.values()}
+    print(f"{far}: energies", [round(alone[k][0]["energy"], 6) for k in range(len(meshes))],
+          "slots used in the mix", sorted(slots_used))
+    assert len(slots_used) > 1
+    for (k, rep), (r, f, h) in mix.items():
+        ra, fa, ha = alone[k]
+        assert r["converged"] == 1
+        assert r["energy"] == ra["energy"], (k, rep, r["energy"], ra["energy"])
+        assert np.array_equal(f, fa) and np.array_equal(h, ha), (k, rep)
+
+
+def test_born_ion_through_the_server():
+    m = Mesh.icosphere(4, 2.0).set_charges([[0.0, 0.0, 0.0, 1.0]])
+    srv = Server(slots=2, keep_fh=False, mode="mixed")
+    rid = srv.submit(_scene(m))
+    r = srv.poll(timeout=120.0)
+    srv.close()
+    assert r is not None and r["id"] == rid and r["status"] == 0, r
+    E_ref = oracle.born_energy(1.0, 2.0, 2.0, 80.0, 0.125)
+    print("server Born energy", r["energy"], "analytic", E_ref)
+    assert abs(r["energy"] - E_ref) / abs(E_ref) < 5e-3
