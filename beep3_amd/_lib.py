@@ -165,6 +165,8 @@ def _declare_hip(L):
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.beep_hip_set_overlap.argtypes = [vp, C.c_int]
+    L.beep_hip_set_graphs.argtypes = [vp, C.c_int]
+    L.beep_hip_autotune_leaf.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
     L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
@@ -174,6 +176,7 @@ def _declare_hip(L):
     L.beep_hip_energy_coefficients.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp]
     L.beep_hip_solve.argtypes = [vp, dp, dp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.beep_hip_selftest_mfma.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.beep_hip_fmm_symmetry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip]
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.beep_hip_device_free.argtypes = [vp, vp]
     L.beep_hip_memcpy.argtypes = [vp, vp, vp, C.c_size_t, C.c_int]

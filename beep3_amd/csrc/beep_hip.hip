@@ -69,6 +69,7 @@ struct beep_hip_ctx {
     DevBuf<int> qual_ptr, quad_ptr, gal_ptr;
     DevBuf<Pt64> qual64, quad64, gal64;
     DevBuf<Pt32> qual32, quad32;
+    DevBuf<double> stage;  // raw point upload staging (7 doubles per point)
     bool have_geom = false;
 
     // tree
@@ -109,7 +110,17 @@ struct beep_hip_ctx {
 
     // GMRES workspace
     DevBuf<double> V, hvec, partial, wtmp, bvec, rvec, dvec;
+    DevBuf<double> xin, wvec;   // fixed mat-vec input / output of an Arnoldi step
+    DevBuf<int> jstep;          // Arnoldi step index on the device
+    double* hh_pinned = nullptr;  // step results read by the host (pinned)
+    size_t hh_cap = 0;
     int part_blocks = 0;
+    // one Arnoldi step captured as a HIP graph, replayed for every step
+    bool use_graphs = true;
+    bool capturing = false;
+    hipGraphExec_t step_exec = nullptr;
+    std::vector<const void*> step_key;
+    unsigned long long gen = 0;  // bumped whenever buffers or plans may change
 
     // ring of event sets (start, near done, end, far done), one per mat-vec
     static constexpr int RING = 64;
@@ -150,6 +161,7 @@ void sync_streams(beep_hip_ctx* c) {
 // (re)build the far-field plan for the current geometry and options
 void far_setup(beep_hip_ctx* c) {
     sync_streams(c);
+    c->gen++;
     if (c->fmm) {
         fmm_plan_destroy(c->fmm);
         c->fmm = nullptr;
@@ -158,28 +170,6 @@ void far_setup(beep_hip_ctx* c) {
     if (c->use_fmm)
         c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
                                  c->fmm_arena, c->stream);
-}
-
-// Convert 7-double points of a CSR point set into local-frame records.
-template <typename T>
-void to_local(int N, const int* ptr, const double* pts, const double* centroid,
-              std::vector<Pt<T>>& out) {
-    const int total = ptr[N];
-    out.resize(total);
-    for (int i = 0; i < N; ++i)
-        for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
-            const double* d = pts + 7 * size_t(k);
-            Pt<T> p;
-            p.x = T(d[0] - centroid[3 * i]);
-            p.y = T(d[1] - centroid[3 * i + 1]);
-            p.z = T(d[2] - centroid[3 * i + 2]);
-            p.w = T(d[6]);
-            p.nx = T(d[3]);
-            p.ny = T(d[4]);
-            p.nz = T(d[5]);
-            p.pad = T(0);
-            out[k] = p;
-        }
 }
 
 void check_ptr(int N, const int* ptr, const char* what) {
@@ -194,10 +184,11 @@ void timing_pop(beep_hip_ctx* c) {
     hipEvent_t* e = c->ev[slot];
     float a = 0, b = 0, t = 0;
     hip_check(hipEventSynchronize(e[2]), "hipEventSynchronize");
-    hip_check(hipEventElapsedTime(&a, e[0], e[1]), "hipEventElapsedTime");
+    // e0 start, e3 far done (far stream), e1 near done, e2 end; serially
+    // (one stream) the far part runs first and the near part after e3
     hip_check(hipEventElapsedTime(&b, e[0], e[3]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&a, c->overlap ? e[0] : e[3], e[1]), "hipEventElapsedTime");
     hip_check(hipEventElapsedTime(&t, e[0], e[2]), "hipEventElapsedTime");
-    if (!c->overlap) b -= a;  // serial: the far part starts when the near part ends
     c->timing.near_ms = a;
     c->timing.far_ms = b;
     c->timing.total_ms = t;
@@ -223,11 +214,13 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     // kernel tables are built on first use for the current kappa
     if (c->use_fmm)
         fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, s);
-    if (c->ring_pending == beep_hip_ctx::RING) timing_pop(c);
+    const bool timed = !c->capturing;
+    if (timed && c->ring_pending == beep_hip_ctx::RING) timing_pop(c);
     hipEvent_t* ev = c->ev[c->ring_head];
     const int n2 = 2 * c->N;
     c->yfar.alloc(size_t(n2));
-    hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
+    if (!c->use_fmm) c->src.alloc(size_t(c->N));
+    if (timed) hip_check(hipEventRecord(ev[0], s), "hipEventRecord");
     hipStream_t sf = s;
     if (c->overlap) {
         sf = c->far_stream;
@@ -240,21 +233,21 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
         fmm_evaluate(c->fmm, sf, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa,
                      c->yfar.p);
     } else {
-        c->src.alloc(size_t(c->N));
         launch_far_pack(sf, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
         launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
                           c->src.p, c->kappa, c->yfar.p);
     }
-    hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
+    if (timed) hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
     if (c->overlap) hip_check(hipEventRecord(c->ev_join, sf), "hipEventRecord");
     // near field (explicit pairs) + diagonal into y
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
     launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
                      c->spmv_part.p, dy);
-    hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->overlap) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     launch_axpby(s, n2, 1.0, c->yfar.p, 1.0, dy);
+    if (!timed) return;
     hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
     c->ring_head = (c->ring_head + 1) % beep_hip_ctx::RING;
     c->ring_pending += 1;
@@ -272,6 +265,70 @@ double dev_dot(beep_hip_ctx* c, const double* a, const double* b, int n) {
     return r;
 }
 
+// The launches of one Arnoldi step: w = K V[j] through the fixed buffers xin
+// and wvec, Gram-Schmidt against V[0..j], V[j+1] = w / |w|, the j + 2
+// coefficients to pinned host memory, j += 1 (all indices read from the
+// device, so the same launches serve every step).
+void arnoldi_launches(beep_hip_ctx* c, int precision, int m) {
+    const int n = 2 * c->N;
+    hipStream_t s = c->stream;
+    launch_copy_col(s, n, c->V.p, c->jstep.p, c->xin.p);
+    matvec_dev(c, c->xin.p, c->wvec.p, precision);
+    launch_arnoldi_step(s, n, m, c->part_blocks, c->V.p, c->jstep.p, c->partial.p, c->hvec.p,
+                        c->wvec.p);
+    hip_check(hipMemcpyAsync(c->hh_pinned, c->hvec.p, sizeof(double) * (m + 2),
+                             hipMemcpyDeviceToHost, s), "D2H arnoldi");
+    launch_inc_int(s, c->jstep.p);
+}
+
+void drop_step_graph(beep_hip_ctx* c) {
+    if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+    c->step_exec = nullptr;
+    c->step_key.clear();
+}
+
+void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
+    if (!c->use_graphs) {
+        arnoldi_launches(c, precision, m);
+        return;
+    }
+    // everything the captured launches bake in: buffers, plan, sizes
+    if (c->use_fmm)
+        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, c->stream);
+    c->yfar.alloc(size_t(2) * c->N);
+    if (!c->use_fmm) c->src.alloc(size_t(c->N));
+    const std::vector<const void*> key = {
+        reinterpret_cast<const void*>(uintptr_t(c->gen)), reinterpret_cast<const void*>(intptr_t(precision)),
+        reinterpret_cast<const void*>(intptr_t(m)), reinterpret_cast<const void*>(intptr_t(c->N)),
+        c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
+        c->yfar.p, c->src.p, c->coef32.p, c->coef64.p, c->diag.p, c->spmv_part.p, c->fmm,
+        reinterpret_cast<const void*>(intptr_t(c->overlap))};
+    if (!c->step_exec || key != c->step_key) {
+        drop_step_graph(c);
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        c->capturing = true;
+        try {
+            arnoldi_launches(c, precision, m);
+        } catch (...) {
+            c->capturing = false;
+            (void)hipStreamEndCapture(c->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        c->capturing = false;
+        hip_check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+        hipGraphExec_t ex = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        hip_check(e, "hipGraphInstantiate");
+        c->step_exec = ex;
+        c->step_key = key;
+    }
+    hip_check(hipGraphLaunch(c->step_exec, c->stream), "hipGraphLaunch");
+}
+
 // GMRES without restart (BEEP::gmres, beep.cpp:1343-1432) on device vectors:
 // r = K x - b, classical Gram-Schmidt, Givens on the host, stop when
 // |y_j| <= eps |b|, x -= V y.  Returns inner iterations; adds mat-vecs.
@@ -283,6 +340,17 @@ int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double
     c->hvec.alloc(size_t(m + 2));
     c->partial.alloc(size_t(m + 2) * c->part_blocks);
     c->wtmp.alloc(size_t(m + 1));
+    c->xin.alloc(size_t(n));
+    c->wvec.alloc(size_t(n));
+    c->jstep.alloc(1);
+    if (c->hh_cap < size_t(m + 2)) {
+        if (c->hh_pinned) hip_check(hipHostFree(c->hh_pinned), "hipHostFree");
+        c->hh_pinned = nullptr;
+        c->hh_cap = 0;
+        hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->hh_pinned), sizeof(double) * (m + 2)),
+                  "hipHostMalloc");
+        c->hh_cap = size_t(m + 2);
+    }
     auto NEAR0 = [](double v) { return v == 0 ? 2.2250738585072014e-308 : v; };
     const double nrm2b = std::sqrt(dev_dot(c, d_b, d_b, n));
     double* v0 = c->V.p;
@@ -305,25 +373,16 @@ int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double
         *est_resid = 0;
         return 0;
     }
+    launch_set_int(s, c->jstep.p, 0);
     do {
         const int u0j = uij;
-        double* vj = c->V.p + size_t(j) * n;
-        double* vn = c->V.p + size_t(j + 1) * n;
-        matvec_dev(c, vj, vn, precision);
+        // V[j+1] from V[j]: mat-vec + Gram-Schmidt + normalisation on the
+        // device (one graph launch when graphs are on), then h to the host
+        arnoldi_step(c, precision, m);
         ++*matvecs;
-        launch_multi_dot(s, n, c->V.p, j + 1, vn, c->partial.p, c->part_blocks);
-        launch_reduce_partials(s, j + 1, c->partial.p, c->part_blocks, c->hvec.p);
-        launch_gemv_sub(s, n, c->V.p, j + 1, c->hvec.p, vn);
-        launch_multi_dot(s, n, vn, 1, vn, c->partial.p + size_t(j + 1) * c->part_blocks,
-                         c->part_blocks);
-        launch_reduce_partials(s, 1, c->partial.p + size_t(j + 1) * c->part_blocks,
-                               c->part_blocks, c->hvec.p + j + 1);
-        hip_check(hipMemcpyAsync(hh.data(), c->hvec.p, sizeof(double) * (j + 2),
-                                 hipMemcpyDeviceToHost, s),
-                  "D2H arnoldi");
         hip_check(hipStreamSynchronize(s), "sync arnoldi");
+        for (int q = 0; q <= j + 1; ++q) hh[q] = c->hh_pinned[q];
         const double h = std::sqrt(std::max(0.0, hh[j + 1]));
-        launch_scale_copy(s, n, vn, nullptr, 1.0 / NEAR0(h), vn);
         for (int q = 0; q <= j; ++q) U[u0j + q] = hh[q];
         for (int i = 0; i < j; ++i) {
             const double tmp = cs[i] * U[uij] - sn[i] * U[uij + 1];
@@ -409,6 +468,8 @@ int beep_hip_destroy(beep_hip_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->far_stream) (void)hipStreamSynchronize(ctx->far_stream);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    drop_step_graph(ctx);
+    if (ctx->hh_pinned) (void)hipHostFree(ctx->hh_pinned);
     if (ctx->fmm) fmm_plan_destroy(ctx->fmm);
     for (auto& set : ctx->ev)
         for (auto& e : set)
@@ -443,24 +504,32 @@ int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* g) {
             pr[i] = Patch64{g->centroid[3 * i], g->centroid[3 * i + 1], g->centroid[3 * i + 2],
                             g->area[i], g->normal[3 * i], g->normal[3 * i + 1],
                             g->normal[3 * i + 2], 1.0 / g->eps_ratio[i]};
-        std::vector<Pt64> q64, d64, g64;
-        std::vector<Pt32> q32, d32;
-        to_local<double>(N, g->qual_ptr, g->qual_pts, g->centroid, q64);
-        to_local<double>(N, g->quad_ptr, g->quad_pts, g->centroid, d64);
-        to_local<double>(N, g->gal_ptr, g->gal_pts, g->centroid, g64);
-        to_local<float>(N, g->qual_ptr, g->qual_pts, g->centroid, q32);
-        to_local<float>(N, g->quad_ptr, g->quad_pts, g->centroid, d32);
         hipStream_t s = ctx->stream;
         ctx->patches.upload(pr.data(), N, s);
         ctx->gc.upload(g->gc, N, s);
         ctx->qual_ptr.upload(g->qual_ptr, N + 1, s);
         ctx->quad_ptr.upload(g->quad_ptr, N + 1, s);
         ctx->gal_ptr.upload(g->gal_ptr, N + 1, s);
-        ctx->qual64.upload(q64.data(), q64.size(), s);
-        ctx->quad64.upload(d64.data(), d64.size(), s);
-        ctx->gal64.upload(g64.data(), g64.size(), s);
-        ctx->qual32.upload(q32.data(), q32.size(), s);
-        ctx->quad32.upload(d32.data(), d32.size(), s);
+        // raw points go up in chunks through a staging buffer and are put
+        // in the local frame on the device (FP64 and FP32 copies)
+        constexpr int CHUNK = 1 << 20;
+        ctx->stage.alloc(size_t(7) * CHUNK);
+        auto points = [&](const int* hptr, const int* dptr, const double* raw, DevBuf<Pt64>& o64,
+                          DevBuf<Pt32>* o32) {
+            const int total = hptr[N];
+            o64.alloc(size_t(total));
+            if (o32) o32->alloc(size_t(total));
+            for (int k0 = 0; k0 < total; k0 += CHUNK) {
+                const int n = std::min(CHUNK, total - k0);
+                hip_check(hipMemcpyAsync(ctx->stage.p, raw + 7 * size_t(k0), sizeof(double) * 7 * n,
+                                         hipMemcpyHostToDevice, s), "H2D points");
+                launch_to_local(s, n, k0, ctx->stage.p, dptr, N, ctx->patches.p, o64.p,
+                                o32 ? o32->p : nullptr);
+            }
+        };
+        points(g->qual_ptr, ctx->qual_ptr.p, g->qual_pts, ctx->qual64, &ctx->qual32);
+        points(g->quad_ptr, ctx->quad_ptr.p, g->quad_pts, ctx->quad64, &ctx->quad32);
+        points(g->gal_ptr, ctx->gal_ptr.p, g->gal_pts, ctx->gal64, nullptr);
         hip_check(hipStreamSynchronize(s), "sync geometry upload");
         ctx->N = N;
         ctx->h_centroid.assign(g->centroid, g->centroid + 3 * size_t(N));
@@ -477,6 +546,8 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         require(ctx->have_geom, "set the geometry before the tree");
         require(t && t->n_leaves > 0, "empty tree");
         set_device(ctx);
+        sync_streams(ctx);
+        ctx->gen++;
         const int N = ctx->N, L = t->n_leaves;
         check_ptr(L, t->leaf_ptr, "bad leaf pointer array");
         check_ptr(L, t->nb_ptr, "bad neighbourhood pointer array");
@@ -544,6 +615,8 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
         require(ctx->have_geom && ctx->have_tree, "geometry and tree must be set");
         require(kappa >= 0, "kappa must be >= 0");
         set_device(ctx);
+        sync_streams(ctx);
+        ctx->gen++;
         if (ctx->have_diag && kappa != ctx->kappa) {
             ctx->have32 = ctx->have64 = false;
             ctx->coef32.reset();
@@ -608,7 +681,18 @@ int beep_hip_set_overlap(beep_hip_ctx* ctx, int on) {
     return guarded(ctx, [&] {
         set_device(ctx);
         sync_streams(ctx);
+        timing_collect(ctx);  // event sets recorded in the old mode
         ctx->overlap = on != 0;
+        ctx->gen++;
+    });
+}
+
+int beep_hip_set_graphs(beep_hip_ctx* ctx, int on) {
+    return guarded(ctx, [&] {
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->use_graphs = on != 0;
+        drop_step_graph(ctx);
     });
 }
 
@@ -633,6 +717,58 @@ int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* dx, double* dy, int 
         require(dx && dy && dx != dy, "mat-vec needs distinct device vectors");
         set_device(ctx);
         matvec_dev(ctx, dx, dy, precision);
+    });
+}
+
+int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_targets, int n,
+                           int reps, double* ms_out, int* best) {
+    return guarded(ctx, [&] {
+        require(leaf_targets && n > 0 && reps > 0, "bad autotune arguments");
+        require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
+        set_device(ctx);
+        const int saved_mode = ctx->far_mode;
+        ctx->far_mode = BEEP_FAR_FMM;
+        const size_t n2 = size_t(2) * ctx->N;
+        ctx->xbuf.alloc(n2);
+        ctx->ybuf.alloc(n2);
+        launch_fill(ctx->stream, int(n2), 1.0, ctx->xbuf.p);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        double best_ms = 1e300;
+        int best_leaf = ctx->fmm_leaf;
+        try {
+            for (int k = 0; k < n; ++k) {
+                require(leaf_targets[k] >= 1, "leaf targets must be positive");
+                ctx->fmm_leaf = leaf_targets[k];
+                far_setup(ctx);
+                matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // tables + warm-up
+                hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+                for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
+                hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                const double per = double(ms) / reps;
+                if (ms_out) ms_out[k] = per;
+                if (per < best_ms) {
+                    best_ms = per;
+                    best_leaf = leaf_targets[k];
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            ctx->far_mode = saved_mode;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        timing_collect(ctx);
+        ctx->timing = beep_matvec_timing{};
+        ctx->fmm_leaf = best_leaf;
+        far_setup(ctx);
+        if (best) *best = best_leaf;
     });
 }
 
@@ -795,6 +931,19 @@ int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_sol
     });
 }
 
+int beep_hip_fmm_symmetry(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map) {
+    try {
+        if (!canon_xyz || !node_map) throw std::runtime_error("null output");
+        if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) < 2 ||
+            std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) > 3)
+            throw std::runtime_error("offset is not an M2L offset");
+        return fmm_canonical(tx, ty, tz, order, canon_xyz, node_map);
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return -1;
+    }
+}
+
 int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
                            double* d) {
     return guarded(ctx, [&] {
@@ -853,12 +1002,14 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         auto add = [&](const auto& buf) { b += buf.n * sizeof(*buf.p); };
         add(ctx->patches); add(ctx->gc); add(ctx->qual_ptr); add(ctx->quad_ptr); add(ctx->gal_ptr);
         add(ctx->qual64); add(ctx->quad64); add(ctx->gal64); add(ctx->qual32); add(ctx->quad32);
+        add(ctx->stage);
         add(ctx->leaf_ptr); add(ctx->leaf_idx); add(ctx->nb_ptr); add(ctx->nb_idx);
         add(ctx->coef_off); add(ctx->coef_blk); add(ctx->spmv_work); add(ctx->spmv_groups);
         add(ctx->spmv_group_nblk); add(ctx->spmv_part); add(ctx->diag); add(ctx->coef32);
         add(ctx->coef64); add(ctx->src); add(ctx->xbuf); add(ctx->ybuf); add(ctx->yfar);
         add(ctx->qbuf); add(ctx->rhsbuf); add(ctx->ecf); add(ctx->ech); add(ctx->pinst_d);
         add(ctx->icp_d); add(ctx->V); add(ctx->hvec); add(ctx->partial); add(ctx->wtmp);
+        add(ctx->xin); add(ctx->wvec); add(ctx->jstep);
         add(ctx->bvec); add(ctx->rvec); add(ctx->dvec);
         *bytes = b + ctx->fmm_arena.held;
     });

@@ -153,6 +153,16 @@ void launch_gemv_sub(hipStream_t s, int n, const double* V, int ncols, const dou
                      double* w);
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
 void launch_fill(hipStream_t s, int n, double v, double* y);
+// Arnoldi step (classical Gram-Schmidt against V[0..j], norm, V[j+1]) on
+// the mat-vec output w, with j read from *jp; h receives the j + 2 values.
+void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const int* jp,
+                         double* partial, double* h, double* w);
+void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out);
+void launch_set_int(hipStream_t s, int* p, int v);
+void launch_inc_int(hipStream_t s, int* p);
+// raw CSR points k0 .. k0+n-1 (7 doubles each) -> local-frame records
+void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
+                     const Patch64* patches, Pt64* out64, Pt32* out32);
 void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out);
 void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
                        double scale, double* y);

@@ -25,6 +25,7 @@
 // with S_n(x) = prod_d S_P(xi_{n_d}, u_d), S_P(xi, u) = 1/P + 2/P sum_{k>=1}
 // T_k(xi) T_k(u), u = box-local coordinate in [-1, 1].
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <unordered_map>
 #include <vector>
@@ -196,12 +197,19 @@ __global__ __launch_bounds__(256) void k_m2m(const FmmDev d, int first, T* __res
     }
 }
 
-// M2L on the matrix cores: one block per chunk of up to 16 pairs that share
-// one offset at one level.  Per chunk, C_e (P^3 x 16) = K_e^T (P^3 x P^3) .
-// M_e (P^3 x 16) for e = Laplace (K_L) and the two Yukawa sets (K_Y), as
-// 16x16x4 MFMA tiles: A = kernel table rows (read from the L2-resident table),
-// B = source expansions staged in LDS as [e][m][g].  Results are added to the
-// target locals atomically (several offsets reach one target).
+// M2L on the matrix cores.  The kernel between the node sets of a source
+// box at offset t (box units, max |t_k| >= 2) from its target depends on t
+// only through |t + (xi_m - xi_n)/2|; the Chebyshev node set is symmetric
+// (xi_{P-1-a} = -xi_a), so for t = sigma t_c with sigma a signed permutation
+// of the axes and t_c one of 16 canonical offsets (t_c0 >= t_c1 >= t_c2 >= 0)
+//   K_t[m][n] = K_c[phi(m)][phi(n)],
+// phi = the same signed permutation acting on node indices.  The plan groups
+// the pairs of a level by canonical offset into chunks of 64; a block stages
+// the source expansions of its chunk through LDS with each pair's node
+// permutation applied (B operands), multiplies by the one canonical table
+// (A operands, 16 tables per level: they stay in L2) with 16x16x4 MFMA tiles,
+// and writes each pair's result, permuted back, to that pair's slot; the
+// slots of a target are summed by k_m2l_sum (deterministic, no atomics).
 typedef double mf64x4 __attribute__((ext_vector_type(4)));
 typedef float mf32x4 __attribute__((ext_vector_type(4)));
 
@@ -226,79 +234,132 @@ struct Mfma<float> {
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l / 16) + r; }
 };
 
+// actual node index of canonical node index mc for a pair with symmetry code
+// sym: bits 0-5 the actual axis of canonical axes 0, 1, 2 (2 bits each),
+// bits 6-8 a reflection of canonical axis 0, 1, 2
+template <int P>
+__host__ __device__ __forceinline__ int node_of(int mc, int sym) {
+    const int c[3] = {mc / (P * P), (mc / P) % P, mc % P};
+    int a[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int ax = (sym >> (2 * k)) & 3;
+        const int v = ((sym >> (6 + k)) & 1) ? P - 1 - c[k] : c[k];
+        a[ax] = v;
+    }
+    return (a[0] * P + a[1]) * P + a[2];
+}
+
+constexpr int M2L_G = 64;     // pairs per block (4 MFMA column tiles)
+constexpr int M2L_WAVES = 8;  // waves per block, each owning output row tiles
+constexpr int M2L_MS = 32;    // K-dimension slab staged in LDS
+
 template <typename T, int P>
-__global__ __launch_bounds__(256) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
+__global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
                                              const T* __restrict__ kl,
                                              const T* const* __restrict__ ytabs,
                                              const T* __restrict__ M, T* __restrict__ Cout) {
     constexpr int NP = P * P * P;
     constexpr int NPP = (NP + 15) / 16 * 16;
     constexpr int NTILE = NPP / 16;
-    constexpr int NTW = (NTILE + 3) / 4;  // n tiles per wave
-    constexpr int G = 16;
+    constexpr int NTW = (NTILE + M2L_WAVES - 1) / M2L_WAVES;  // row tiles per wave
+    constexpr int G = M2L_G, CT = G / 16, MS = M2L_MS, MSP = MS + 1;
     using acc_t = typename Mfma<T>::acc_t;
-    __shared__ T sM[3][NPP][G];  // B operands, then the result tile [3][G][NPP]
-    const int4 ch = chunks[xcd_remap(blockIdx.x, gridDim.x)];  // offset, pair begin/end, level
+    // B slab [e][g][m] (padded), reused for the result tile [g][n] of one
+    // (potential, column tile)
+    constexpr int SB = 3 * G * MSP > 16 * NPP ? 3 * G * MSP : 16 * NPP;
+    __shared__ T sbuf[SB];
+    __shared__ int psrc[G], psym[G];
+    const int4 ch = chunks[xcd_remap(blockIdx.x, gridDim.x)];  // canonical offset, pairs, level
     const int off = ch.x, p0 = ch.y, np = ch.z - ch.y, level = ch.w;
     const int t = threadIdx.x;
-    for (int q = t; q < 3 * NPP * G; q += 256) {
-        const int e = q / (NPP * G), rem = q - e * NPP * G, m = rem / G, g = rem - m * G;
-        T v = T(0);
-        if (g < np && m < NP) v = M[size_t(d.pair_src[p0 + g]) * 3 * NP + e * NP + m];
-        sM[e][m][g] = v;
+    if (t < G) {
+        psrc[t] = t < np ? d.pair_src[p0 + t] : 0;
+        psym[t] = t < np ? d.pair_sym[p0 + t] : 0;
     }
-    __syncthreads();
     const int w = t >> 6, l = t & 63;
+    const int kl16 = l / 16, il = l % 16;
     const T* KL = kl + size_t(off) * NP * NP;
     const T* KY = ytabs[level] + size_t(off) * NP * NP;
-    acc_t aL[NTW], a1[NTW], a2[NTW];
+    acc_t acc[NTW][3][CT];
 #pragma unroll
-    for (int k = 0; k < NTW; ++k) {
-        aL[k] = acc_t{0, 0, 0, 0};
-        a1[k] = acc_t{0, 0, 0, 0};
-        a2[k] = acc_t{0, 0, 0, 0};
-    }
-    const int kl16 = l / 16, il = l % 16;
-    for (int m0 = 0; m0 < NPP; m0 += 4) {
-        const int m = m0 + kl16;
-        const T bL = sM[0][m][il], b1 = sM[1][m][il], b2 = sM[2][m][il];
+    for (int k = 0; k < NTW; ++k)
 #pragma unroll
-        for (int k = 0; k < NTW; ++k) {
-            const int nt = w + 4 * k;
-            if (nt < NTILE) {
-                const int n = nt * 16 + il;
-                T xa = T(0), ya = T(0);
-                if (m < NP && n < NP) {
-                    xa = KL[size_t(m) * NP + n];
-                    ya = KY[size_t(m) * NP + n];
+        for (int e = 0; e < 3; ++e)
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[k][e][c] = acc_t{0, 0, 0, 0};
+    for (int m0 = 0; m0 < NPP; m0 += MS) {
+        __syncthreads();  // psrc/psym written; previous slab consumed
+        // stage B[e][g][mm] = M_src(g)[e][node_of(m0 + mm)] for this slab;
+        // consecutive threads walk mm of one (g, e), a run inside one expansion
+        for (int q = t; q < 3 * G * MS; q += 512) {
+            const int g = q / (3 * MS), rem = q - g * 3 * MS, e = rem / MS, mm = rem - e * MS;
+            const int m = m0 + mm;
+            T v = T(0);
+            if (g < np && m < NP)
+                v = M[size_t(psrc[g]) * 3 * NP + e * NP + node_of<P>(m, psym[g])];
+            sbuf[(e * G + g) * MSP + mm] = v;
+        }
+        __syncthreads();
+        for (int k0 = 0; k0 < MS; k0 += 4) {
+            const int m = m0 + k0 + kl16;  // K-dimension index (canonical source node)
+            T bL[CT], b1[CT], b2[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const int g = c * 16 + il;
+                bL[c] = sbuf[(0 * G + g) * MSP + k0 + kl16];
+                b1[c] = sbuf[(1 * G + g) * MSP + k0 + kl16];
+                b2[c] = sbuf[(2 * G + g) * MSP + k0 + kl16];
+            }
+#pragma unroll
+            for (int k = 0; k < NTW; ++k) {
+                const int nt = w + M2L_WAVES * k;
+                if (nt < NTILE) {
+                    const int n = nt * 16 + il;  // output row (canonical target node)
+                    T xa = T(0), ya = T(0);
+                    if (m < NP && n < NP) {
+                        xa = KL[size_t(m) * NP + n];
+                        ya = KY[size_t(m) * NP + n];
+                    }
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) {
+                        acc[k][0][c] = Mfma<T>::op(xa, bL[c], acc[k][0][c]);
+                        acc[k][1][c] = Mfma<T>::op(ya, b1[c], acc[k][1][c]);
+                        acc[k][2][c] = Mfma<T>::op(ya, b2[c], acc[k][2][c]);
+                    }
                 }
-                aL[k] = Mfma<T>::op(xa, bL, aL[k]);
-                a1[k] = Mfma<T>::op(ya, b1, a1[k]);
-                a2[k] = Mfma<T>::op(ya, b2, a2[k]);
             }
         }
     }
-    __syncthreads();  // sM is reused for the result tile
-    T* sC = &sM[0][0][0];  // [3][G][NPP]
-    const T sl = T(1.0 / d.box_h[level]);
+    // results: per (potential, column tile) through LDS as [g][n] with the
+    // pair's node permutation applied, then contiguous slot writes
+    const T sl = T(1.0 / d.box_h[level]);  // Laplace tables are built at h = 1
 #pragma unroll
-    for (int k = 0; k < NTW; ++k) {
-        const int nt = w + 4 * k;
-        if (nt >= NTILE) continue;
+    for (int e = 0; e < 3; ++e)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int n = nt * 16 + Mfma<T>::row(l, r);
-            sC[(0 * G + il) * NPP + n] = aL[k][r] * sl;
-            sC[(1 * G + il) * NPP + n] = a1[k][r];
-            sC[(2 * G + il) * NPP + n] = a2[k][r];
+        for (int c = 0; c < CT; ++c) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NTW; ++k) {
+                const int nt = w + M2L_WAVES * k;
+                if (nt >= NTILE) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int n = nt * 16 + Mfma<T>::row(l, r);
+                    if (n < NP) {
+                        const int gl = il;  // pair within the column tile
+                        const T v = e == 0 ? acc[k][e][c][r] * sl : acc[k][e][c][r];
+                        sbuf[gl * NPP + node_of<P>(n, psym[c * 16 + gl])] = v;
+                    }
+                }
+            }
+            __syncthreads();
+            const int ng = min(16, np - c * 16);
+            for (int q = t; q < ng * NP; q += 512) {
+                const int gl = q / NP, n = q - gl * NP;
+                Cout[size_t(p0 + c * 16 + gl) * 3 * NP + e * NP + n] = sbuf[gl * NPP + n];
+            }
         }
-    }
-    __syncthreads();
-    // per-pair slots C[pair][e][n], written contiguously
-    for (int q = t; q < np * 3 * NP; q += 256) {
-        const int g = q / (3 * NP), rem = q - g * 3 * NP, e = rem / NP, n = rem - e * NP;
-        Cout[size_t(p0 + g) * 3 * NP + rem] = sC[(e * G + g) * NPP + n];
-    }
 }
 
 // Sum the M2L slots of every target box into its local expansion (one block
@@ -544,19 +605,20 @@ __global__ void k_zero(size_t n, T* __restrict__ p) {
     if (k < n) p[k] = T(0);
 }
 
-// Kernel tables between node sets: out[off][m][n] = G(|t + (xi_m - xi_n)/2| * h)
-// for offsets t in [-3, 3]^3 (off = (tx+3)*49 + (ty+3)*7 + (tz+3)); laplace:
-// 1/r with h = 1 (scaled by 1/h at use), else e^{-kappa r}/r at box size h.
+// Kernel tables between node sets: out[c][m][n] = G(|t_c + (xi_m - xi_n)/2| * h)
+// (m = source node, n = target node) for the canonical offsets t_c (box
+// units, t_c0 >= t_c1 >= t_c2 >= 0, t_c0 in {2, 3}); laplace: 1/r with h = 1
+// (scaled by 1/h at use), else e^{-kappa r}/r at box size h.
 template <typename T, int P>
-__global__ void k_ktab(const double* __restrict__ xi, double h, double kappa, int laplace,
-                       T* __restrict__ out) {
+__global__ void k_ktab(const double* __restrict__ xi, const int4* __restrict__ canon, int ncanon,
+                       double h, double kappa, int laplace, T* __restrict__ out) {
     constexpr int NP = P * P * P;
     const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= size_t(343) * NP * NP) return;
+    if (idx >= size_t(ncanon) * NP * NP) return;
     const int off = int(idx / (size_t(NP) * NP));
     const int rem = int(idx - size_t(off) * NP * NP);
     const int m = rem / NP, nn = rem - m * NP;
-    const int tx = off / 49 - 3, ty = (off / 7) % 7 - 3, tz = off % 7 - 3;
+    const int tx = canon[off].x, ty = canon[off].y, tz = canon[off].z;
     const int ma = m / (P * P), mb = (m / P) % P, mc = m % P;
     const int na = nn / (P * P), nb = (nn / P) % P, nc = nn % P;
     const double vx = tx + 0.5 * (xi[ma] - xi[na]);
@@ -580,6 +642,36 @@ namespace {
 
 uint64_t bkey(uint32_t x, uint32_t y, uint32_t z) {
     return (uint64_t(x) << 42) | (uint64_t(y) << 21) | uint64_t(z);
+}
+
+// The 16 canonical M2L offsets (a, b, c), 3 >= a >= b >= c >= 0, a >= 2.
+const std::vector<std::array<int, 3>>& canonical_list() {
+    static const std::vector<std::array<int, 3>> list = [] {
+        std::vector<std::array<int, 3>> v;
+        for (int a = 2; a <= 3; ++a)
+            for (int b = 0; b <= a; ++b)
+                for (int c = 0; c <= b; ++c) v.push_back({a, b, c});
+        return v;
+    }();
+    return list;
+}
+
+// Canonical index of offset t and the symmetry code of the node map (see
+// node_of): canonical axis k is actual axis ax[k] with |t| sorted
+// descending, reflected where that component of t is negative.
+int canonical_offset(int tx, int ty, int tz, int* sym) {
+    const int t[3] = {tx, ty, tz};
+    int ax[3] = {0, 1, 2};
+    std::stable_sort(ax, ax + 3, [&](int p, int q) { return std::abs(t[p]) > std::abs(t[q]); });
+    const std::array<int, 3> c = {std::abs(t[ax[0]]), std::abs(t[ax[1]]), std::abs(t[ax[2]])};
+    int code = ax[0] | (ax[1] << 2) | (ax[2] << 4);
+    for (int k = 0; k < 3; ++k)
+        if (t[ax[k]] < 0) code |= 1 << (6 + k);
+    *sym = code;
+    const auto& L = canonical_list();
+    for (size_t k = 0; k < L.size(); ++k)
+        if (L[k] == c) return int(k);
+    throw std::runtime_error("M2L offset outside the interaction range");
 }
 
 template <typename T>
@@ -721,13 +813,14 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     }
     // interaction lists per level (l >= 2), grouped by offset into chunks
     constexpr int G = FmmPlan::CHUNK;
-    std::vector<int> pair_src, pair_tgt;
+    static_assert(FmmPlan::CHUNK == M2L_G, "plan chunks match the M2L block");
+    std::vector<int> pair_src, pair_tgt, pair_sym;
     std::vector<int4> chunks;
     pl->lvl_chunk.assign(D + 2, 0);
     for (int l = 0; l <= D; ++l) {
         pl->lvl_chunk[l] = int(chunks.size());
         if (l < 2) continue;
-        std::vector<std::vector<std::pair<int, int>>> byoff(343);
+        std::vector<std::vector<std::array<int, 3>>> byoff(FmmPlan::NCANON);
         for (size_t q = 0; q < lev[l].size(); ++q) {
             const Box& b = lev[l][q];
             const long px = b.x >> 1, py = b.y >> 1, pz = b.z >> 1;
@@ -746,18 +839,20 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                             const int tx = int(long(sb.x) - long(b.x)), ty = int(long(sb.y) - long(b.y)),
                                       tz = int(long(sb.z) - long(b.z));
                             if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) <= 1) continue;
-                            const int off = (tx + 3) * 49 + (ty + 3) * 7 + (tz + 3);
-                            byoff[off].push_back({pl->lvl_first[l] + int(q), sg});
+                            int sym = 0;
+                            const int c = canonical_offset(tx, ty, tz, &sym);
+                            byoff[c].push_back({pl->lvl_first[l] + int(q), sg, sym});
                         }
                     }
         }
-        for (int off = 0; off < 343; ++off) {
+        for (int off = 0; off < FmmPlan::NCANON; ++off) {
             auto& v = byoff[off];
             for (size_t k = 0; k < v.size(); k += G) {
                 const int b0 = int(pair_src.size());
                 for (size_t j = k; j < std::min(v.size(), k + G); ++j) {
-                    pair_tgt.push_back(v[j].first);
-                    pair_src.push_back(v[j].second);
+                    pair_tgt.push_back(v[j][0]);
+                    pair_src.push_back(v[j][1]);
+                    pair_sym.push_back(v[j][2]);
                 }
                 chunks.push_back(make_int4(off, b0, int(pair_src.size()), l));
             }
@@ -811,6 +906,15 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     upload(arena, "nb_leaf", pl->dv.nb_leaf, nb_leaf, s);
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
     upload(arena, "pair_tgt", pl->dv.pair_tgt, pair_tgt, s);
+    upload(arena, "pair_sym", pl->dv.pair_sym, pair_sym, s);
+    {
+        std::vector<int4> canon(FmmPlan::NCANON);
+        for (int c = 0; c < FmmPlan::NCANON; ++c) {
+            const auto t = canonical_list()[c];
+            canon[c] = make_int4(t[0], t[1], t[2], 0);
+        }
+        upload(arena, "canon", pl->d_canon, canon, s);
+    }
     upload(arena, "tgt_ptr", pl->dv.tgt_ptr, tgt_ptr, s);
     upload(arena, "tgt_pair", pl->dv.tgt_pair, tgt_pair, s);
     upload(arena, "tn", pl->dv.tn, tn, s);
@@ -833,18 +937,18 @@ template <typename T, int P>
 static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& ytab,
                         hipStream_t s) {
     constexpr int NP = P * P * P;
-    const size_t n = size_t(343) * NP * NP;
+    const size_t n = size_t(FmmPlan::NCANON) * NP * NP;
     const int blocks = int((n + 255) / 256);
     const std::string pre = sizeof(T) == 8 ? "ktab64_" : "ktab32_";
     tabs.assign(pl->D + 2, nullptr);
     // slot D+1: Laplace (level independent, h = 1); slot l: Yukawa at level l
     tabs[pl->D + 1] = pl->arena->get<T>(pre + "laplace", n);
-    hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, 1.0, 0.0, 1,
-                       tabs[pl->D + 1]);
+    hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, pl->d_canon,
+                       FmmPlan::NCANON, 1.0, 0.0, 1, tabs[pl->D + 1]);
     for (int l = 2; l <= pl->D; ++l) {
         tabs[l] = pl->arena->get<T>(pre + std::to_string(l), n);
-        hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, pl->box_h[l],
-                           kappa, 0, tabs[l]);
+        hipLaunchKernelGGL((k_ktab<T, P>), dim3(blocks), dim3(256), 0, s, pl->d_xi, pl->d_canon,
+                           FmmPlan::NCANON, pl->box_h[l], kappa, 0, tabs[l]);
     }
     hip_check(hipGetLastError(), "k_ktab launch");
     ytab = pl->arena->get<T*>(pre + "ptrs", tabs.size());
@@ -907,7 +1011,7 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     {
         const int c0 = pl->lvl_chunk[2], c1 = pl->lvl_chunk[D + 1];
         if (D >= 2 && c1 > c0) {
-            hipLaunchKernelGGL((k_m2l<T, P>), dim3(c1 - c0), dim3(256), 0, s, d, pl->d_chunks + c0,
+            hipLaunchKernelGGL((k_m2l<T, P>), dim3(c1 - c0), dim3(512), 0, s, d, pl->d_chunks + c0,
                                tabs[D + 1], ytab, M, slots);
             const int b0 = pl->lvl_first[2], nb = pl->lvl_first[D + 1] - b0;
             hipLaunchKernelGGL((k_m2l_sum<T, P>), dim3(nb), dim3(256), 0, s, d, b0, slots, L);
@@ -923,6 +1027,24 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     hipLaunchKernelGGL((k_p2p<T>), dim3(pl->nleaf), dim3(256), 0, s, d, pl->src, patches,
                        T(kappa), y, pl->N);
     hip_check(hipGetLastError(), "fmm launch");
+}
+
+int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map) {
+    int sym = 0;
+    const int c = canonical_offset(tx, ty, tz, &sym);
+    const auto t = canonical_list()[c];
+    for (int k = 0; k < 3; ++k) canon_xyz[k] = t[k];
+    const int NP = order * order * order;
+    for (int m = 0; m < NP; ++m) {
+        switch (order) {
+            case 3: node_map[m] = node_of<3>(m, sym); break;
+            case 4: node_map[m] = node_of<4>(m, sym); break;
+            case 5: node_map[m] = node_of<5>(m, sym); break;
+            case 6: node_map[m] = node_of<6>(m, sym); break;
+            default: throw std::runtime_error("FMM order must be 3..6");
+        }
+    }
+    return c;
 }
 
 void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, const double* x,

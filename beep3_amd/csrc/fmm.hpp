@@ -26,6 +26,7 @@ struct FmmDev {
     int* nb_leaf = nullptr;
     int* pair_src = nullptr;    // M2L pairs grouped by offset
     int* pair_tgt = nullptr;
+    int* pair_sym = nullptr;    // node map of each pair to its canonical offset
     int* tgt_ptr = nullptr;     // per box: its M2L pairs (indices into pair_*)
     int* tgt_pair = nullptr;
     double* tn = nullptr;       // T_j(xi_k), P x P
@@ -33,7 +34,8 @@ struct FmmDev {
 };
 
 struct FmmPlan {
-    static constexpr int CHUNK = 16;  // M2L pairs per block
+    static constexpr int CHUNK = 64;  // M2L pairs per block
+    static constexpr int NCANON = 16; // canonical M2L offsets
     int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0;
     long long n_p2p = 0;              // target-source pairs of the leaf pass (self included)
     double edge = 0;
@@ -43,6 +45,7 @@ struct FmmPlan {
     std::vector<double> box_h;
     FmmDev dv;
     int4* d_chunks = nullptr;
+    int4* d_canon = nullptr;          // canonical offsets (x, y, z, 0)
     double* d_xi = nullptr;
     std::vector<double*> ktab64;      // [l] Yukawa at level l, [D+1] Laplace
     std::vector<float*> ktab32;
@@ -64,6 +67,9 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                          DevArena& arena, hipStream_t s);
 void fmm_plan_destroy(FmmPlan* p);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
+// canonical offset of the M2L offset t (index returned, components in
+// canon_xyz) and the map canonical node -> actual node (order^3 entries)
+int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);
 // y += far field (1-point interaction of every pair) of x
 void fmm_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, const double* x,
                   double kappa, double* y);

@@ -256,6 +256,166 @@ void launch_fill(hipStream_t s, int n, double v, double* y) {
     hip_check(hipGetLastError(), "k_fill launch");
 }
 
+// ---- Arnoldi step with the step index j read from device memory, so that
+// one captured HIP graph serves every step of a GMRES cycle ----
+__global__ void k_set_int(int* __restrict__ p, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+__global__ void k_inc_int(int* __restrict__ p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p += 1;
+}
+
+// out = V[j]
+__global__ void k_copy_col(int n, const double* __restrict__ V, const int* __restrict__ jp,
+                           double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = V[size_t(*jp) * n + k];
+}
+
+// partial[q * nb + blk] = block sum of V[q] . w for q <= j (grid y = m + 1)
+__global__ __launch_bounds__(256) void k_multi_dot_j(int n, const double* __restrict__ V,
+                                                     const int* __restrict__ jp,
+                                                     const double* __restrict__ w,
+                                                     double* __restrict__ partial) {
+    const int q = blockIdx.y;
+    if (q > *jp) return;
+    __shared__ double red[256];
+    const int nb = gridDim.x;
+    double acc = 0;
+    const double* v = V + size_t(q) * n;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += nb * 256) acc += v[k] * w[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[size_t(q) * nb + blockIdx.x] = red[0];
+}
+
+// partial[(j + 1) * nb + blk] = block sum of w . w
+__global__ __launch_bounds__(256) void k_dot_self_j(int n, const double* __restrict__ w,
+                                                    const int* __restrict__ jp,
+                                                    double* __restrict__ partial) {
+    __shared__ double red[256];
+    const int nb = gridDim.x;
+    double acc = 0;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < n; k += nb * 256) acc += w[k] * w[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[size_t(*jp + 1) * nb + blockIdx.x] = red[0];
+}
+
+// out[q] = sum of partial row q, for q <= j + extra (grid = m + 2)
+__global__ __launch_bounds__(256) void k_reduce_j(const double* __restrict__ partial, int nb,
+                                                  const int* __restrict__ jp, int first_extra,
+                                                  double* __restrict__ out) {
+    const int j = *jp;
+    // first_extra = 0: rows 0..j; first_extra = 1: row j + 1 only
+    const int q = first_extra ? j + 1 : blockIdx.x;
+    if (first_extra ? blockIdx.x != 0 : q > j) return;
+    __shared__ double red[256];
+    double acc = 0;
+    for (int k = threadIdx.x; k < nb; k += 256) acc += partial[size_t(q) * nb + k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = red[0];
+}
+
+// w -= sum_{q <= j} h[q] V[q]
+__global__ __launch_bounds__(256) void k_gemv_sub_j(int n, const double* __restrict__ V,
+                                                    const int* __restrict__ jp,
+                                                    const double* __restrict__ h,
+                                                    double* __restrict__ w) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int nc = *jp + 1;
+    double acc = 0;
+    for (int q = 0; q < nc; ++q) acc += h[q] * V[size_t(q) * n + k];
+    w[k] -= acc;
+}
+
+// V[j + 1] = w / |w| with |w| = sqrt(max(0, h[j + 1])) and a zero norm
+// replaced by the smallest normal double (the NEAR0 guard of beep.cpp:1366)
+__global__ void k_scale_store_j(int n, const double* __restrict__ w, const double* __restrict__ h,
+                                const int* __restrict__ jp, double* __restrict__ V) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = *jp;
+    const double nv = sqrt(fmax(0.0, h[j + 1]));
+    const double sc = 1.0 / (nv == 0 ? 2.2250738585072014e-308 : nv);
+    V[size_t(j + 1) * n + k] = w[k] * sc;
+}
+
+void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const int* jp,
+                         double* partial, double* h, double* w) {
+    const dim3 rows((n + 255) / 256);
+    hipLaunchKernelGGL(k_multi_dot_j, dim3(nb, m + 1), dim3(256), 0, s, n, V, jp, w, partial);
+    hipLaunchKernelGGL(k_reduce_j, dim3(m + 1), dim3(256), 0, s, partial, nb, jp, 0, h);
+    hipLaunchKernelGGL(k_gemv_sub_j, rows, dim3(256), 0, s, n, V, jp, h, w);
+    hipLaunchKernelGGL(k_dot_self_j, dim3(nb), dim3(256), 0, s, n, w, jp, partial);
+    hipLaunchKernelGGL(k_reduce_j, dim3(1), dim3(256), 0, s, partial, nb, jp, 1, h);
+    hipLaunchKernelGGL(k_scale_store_j, rows, dim3(256), 0, s, n, w, h, jp, V);
+    hip_check(hipGetLastError(), "arnoldi step launch");
+}
+
+void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out) {
+    hipLaunchKernelGGL(k_copy_col, dim3((n + 255) / 256), dim3(256), 0, s, n, V, jp, out);
+    hip_check(hipGetLastError(), "k_copy_col launch");
+}
+
+void launch_set_int(hipStream_t s, int* p, int v) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
+    hip_check(hipGetLastError(), "k_set_int launch");
+}
+
+void launch_inc_int(hipStream_t s, int* p) {
+    hipLaunchKernelGGL(k_inc_int, dim3(1), dim3(64), 0, s, p);
+    hip_check(hipGetLastError(), "k_inc_int launch");
+}
+
+// Points k0 .. k0+n-1 of a CSR point set (raw: 7 doubles x, y, z, nx, ny,
+// nz, w each) into the local frame of the patch that owns them (offsets from
+// its centroid, formed in FP64 and rounded once for the FP32 copy).
+__global__ void k_to_local(int n, int k0, const double* __restrict__ raw,
+                           const int* __restrict__ ptr, int N,
+                           const Patch64* __restrict__ patches, Pt64* __restrict__ out64,
+                           Pt32* __restrict__ out32) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int k = k0 + t;
+    int lo = 0, hi = N;  // patch i with ptr[i] <= k < ptr[i + 1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (ptr[mid] <= k) lo = mid;
+        else hi = mid;
+    }
+    const Patch64 p = patches[lo];
+    const double* d = raw + 7 * size_t(t);
+    const double x = d[0] - p.cx, y = d[1] - p.cy, z = d[2] - p.cz;
+    if (out64) out64[k] = Pt64{x, y, z, d[6], d[3], d[4], d[5], 0.0};
+    if (out32)
+        out32[k] = Pt32{float(x), float(y), float(z), float(d[6]), float(d[3]), float(d[4]),
+                        float(d[5]), 0.0f};
+}
+
+void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
+                     const Patch64* patches, Pt64* out64, Pt32* out32) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_local, dim3((n + 255) / 256), dim3(256), 0, s, n, k0, raw, ptr, N,
+                       patches, out64, out32);
+    hip_check(hipGetLastError(), "k_to_local launch");
+}
+
 }  // namespace beepdev
 
 namespace beepdev {

@@ -77,6 +77,22 @@ class Device:
         """Far field on a second stream, concurrent with the near SpMV."""
         self._check(lib().beep_hip_set_overlap(self._h, 1 if on else 0))
 
+    def set_graphs(self, on: bool = True):
+        """GMRES Arnoldi steps as one replayed HIP graph (default) or kernel
+        by kernel."""
+        self._check(lib().beep_hip_set_graphs(self._h, 1 if on else 0))
+
+    def autotune_leaf(self, precision="fp64", candidates=(24, 32, 40, 64, 96, 160), reps=3):
+        """Time mat-vecs for each FMM leaf target, keep the fastest; returns
+        (best, {leaf: ms per mat-vec})."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int()
+        self._check(lib().beep_hip_autotune_leaf(
+            self._h, _prec(precision), c.ctypes.data_as(C.POINTER(C.c_int)), len(c), int(reps),
+            ms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(best)))
+        return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
     def far_info(self) -> dict:
         f = FarInfo()
         self._check(lib().beep_hip_far_info(self._h, C.byref(f)))

@@ -30,6 +30,9 @@ FP64_VECTOR_TFLOPS = 78.6
 FLOP_PER_EVAL = 92  # integrand evaluation (qual point x quad point)
 
 
+TUNE_LEAVES = (24, 32, 40, 64, 96, 160)
+
+
 def build_workload(name, **kw):
     from beep3_amd import configs
     if name == "globular":
@@ -199,6 +202,7 @@ def main():
     ap.add_argument("--order", type=int, default=5)
     ap.add_argument("--leaf", type=int, default=40)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-seq", type=int, default=40)
@@ -218,6 +222,13 @@ def main():
     a = b.scene.arrays()
     N = b.n
     dev = b.dev
+    tune = None
+    if args.far == "fmm" and not args.no_tune:
+        # leaf-size autotuner on the FP64 mat-vec at this order
+        best, table = dev.autotune_leaf("fp64", TUNE_LEAVES, reps=3)
+        tune = {"leaf_ms_fp64": table, "chosen_leaf": best}
+        args.leaf = best
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
 
     # precompute timings (integrand throughput)
     pre = {}
@@ -278,6 +289,7 @@ def main():
                              "fp32": {"near": near32_alone, "far": far32_alone}},
         "near_spmv_fp32_GBs": nb32 / (near32_alone * 1e-3) / 1e9,
         "host_setup_s": setup_s,
+        "autotune": tune,
     }
 
     if not args.no_solve:

@@ -152,6 +152,24 @@ int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_targ
  * starts (beep.cpp:1041-1045). */
 int beep_hip_set_overlap(beep_hip_ctx* ctx, int on);
 
+/* GMRES Arnoldi steps (mat-vec, Gram-Schmidt, normalisation, coefficient
+ * read-back) as one captured HIP graph replayed for every step (on = 1, the
+ * default; re-captured when buffers or plans change), or launched kernel by
+ * kernel (on = 0).  The same kernels run either way, so results are
+ * bit-identical.  The reference launches nothing (CPU BLAS, beep.cpp:
+ * 1391-1428). */
+int beep_hip_set_graphs(beep_hip_ctx* ctx, int on);
+
+/* Autotuner for the octree leaf size of the FMM far field: times `reps`
+ * mat-vecs of the given precision with each candidate leaf target (mean
+ * patches per leaf box) at the current order, writes the mean time per
+ * mat-vec of each to ms_out (n values, may be NULL), keeps the fastest and
+ * returns it in *best.  Needs the geometry, tree and coefficients of that
+ * precision; switches the far field to BEEP_FAR_FMM.  No counterpart in the
+ * reference, whose leaf size follows from nbsize (beep.h:67-74). */
+int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_targets, int n,
+                           int reps, double* ms_out, int* best);
+
 /* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
  * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs
  * and P2P target-source pairs (zeros for the direct sum). */
@@ -198,6 +216,14 @@ int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_sol
  * result layout the M2L kernels rely on. */
 int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
                            double* d);
+
+/* Diagnostic (host only): the M2L symmetry used by the FMM.  For a source
+ * box at offset t = (tx, ty, tz) (box units, 2 <= max |t_k| <= 3) returns
+ * the index of its canonical offset (components in canon_xyz) and the node
+ * map psi (order^3 entries, canonical node -> actual node) with
+ * K_t[psi(m)][psi(n)] = K_canonical[m][n]; -1 on bad input (message from
+ * beep_hip_last_error(NULL)). */
+int beep_hip_fmm_symmetry(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);
 
 /* device memory helpers for callers that keep vectors resident */
 int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out);

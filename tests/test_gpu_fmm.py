@@ -88,3 +88,22 @@ def test_overlap_bit_identical(off_centre_arrays, mode):
         assert np.array_equal(y1, y0), p
         assert np.array_equal(y1, y1b), p
     d.close()
+
+
+def test_leaf_autotuner(off_centre_arrays):
+    """The tuner times each candidate, keeps the fastest, and the tuned plan
+    still reproduces the direct far field to the FMM accuracy."""
+    a, kappa = off_centre_arrays
+    x = np.random.default_rng(7).normal(size=2 * a["area"].shape[0])
+    dd = _device(a, kappa, "direct", precs=("fp32",))
+    y_dir = dd.matvec(x, "fp32")
+    dd.close()
+    d = _device(a, kappa, "fmm", 5, 40, precs=("fp32",))
+    best, table = d.autotune_leaf("fp32", (16, 48, 128), reps=2)
+    print("leaf autotune (ms per FP32 mat-vec):", table, "best", best)
+    assert best in (16, 48, 128)
+    assert all(v > 0 for v in table.values())
+    assert table[best] == min(table.values())
+    assert d.far_info()["mode"] == 1
+    assert rel(d.matvec(x, "fp32"), y_dir) < 1e-4
+    d.close()
