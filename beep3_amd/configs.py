@@ -104,3 +104,27 @@ def serving_request(meshes, req, nbsize=1500):
     m = meshes[(req["level"], req["shape"])].clone()
     m.charges_walk(req["n_charges"], 3.8, 2.0, 0.4, req["charge_seed"])
     return Scene(80.0, 0.125, QUAD, QUAL, nbsize).add_instance(m, 2.0), m
+
+
+# ---- batched poses: two instances of one surface in many rigid poses ----
+def pose_mesh(subdivisions=6, shape_seed=7, n_charges=2000, charge_seed=7):
+    """The 163,842-patch globular generator at 40,962 patches with a_lm seed 7
+    (r = 25 A star surface, 3.8 A random-walk charges, q ~ N(0, 0.4))."""
+    return globular_mesh(subdivisions, shape_seed, n_charges, charge_seed)
+
+
+def pose_scenes(mesh: Mesh, n_poses=256, seed=7, nbsize=1500):
+    """Scenes of n_poses seeded rigid poses: instance 1 at the origin,
+    instance 2 rotated by a random unit quaternion with its centre at a
+    uniform 45-80 A distance in a random direction (eps_in 2, eps_out 80,
+    kappa 0.125).  All scenes share the one mesh."""
+    from .scene import Scene
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_poses):
+        q, off = random_pose(rng)
+        sc = Scene(80.0, 0.125, QUAD, QUAL, nbsize)
+        sc.add_instance(mesh, 2.0, quat=None, offset=[0.0, 0.0, 0.0])
+        sc.add_instance(mesh, 2.0, quat=q, offset=off)
+        out.append(sc)
+    return out

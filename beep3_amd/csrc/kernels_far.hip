@@ -43,12 +43,19 @@ template <typename T>
 struct FarMath;
 template <>
 struct FarMath<float> {
-    static __device__ __forceinline__ float rsqrt_(float v) { return __frsqrt_rn(v); }
+    // hardware reciprocal square root (1 ulp)
+    static __device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_rsqf(v); }
     static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
 };
 template <>
 struct FarMath<double> {
-    static __device__ __forceinline__ double rsqrt_(double v) { return 1.0 / sqrt(v); }
+    // hardware estimate refined by two Newton steps y <- y + y (1/2 - (v/2) y^2)
+    static __device__ __forceinline__ double rsqrt_(double v) {
+        double y = __builtin_amdgcn_rsq(v);
+        const double h = 0.5 * v;
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        return fma(y, fma(-h * y, y, 0.5), y);
+    }
     static __device__ __forceinline__ double exp_(double v) { return exp(v); }
 };
 

@@ -20,45 +20,48 @@ template <typename T>
 struct Math;
 template <>
 struct Math<float> {
-    static __device__ __forceinline__ float rsqrt_(float v) { return __frsqrt_rn(v); }
+    // hardware reciprocal square root (1 ulp)
+    static __device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_rsqf(v); }
     static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
 };
 template <>
 struct Math<double> {
-    static __device__ __forceinline__ double rsqrt_(double v) { return 1.0 / sqrt(v); }
+    // hardware estimate refined by two Newton steps y <- y + y (1/2 - (v/2) y^2)
+    static __device__ __forceinline__ double rsqrt_(double v) {
+        double y = __builtin_amdgcn_rsq(v);
+        const double h = 0.5 * v;
+        y = fma(y, fma(-h * y, y, 0.5), y);
+        return fma(y, fma(-h * y, y, 0.5), y);
+    }
     static __device__ __forceinline__ double exp_(double v) { return exp(v); }
 };
 
 // Accumulate A..D of one (qual point, quad point) pair into the per-qual-point
-// partial sums.  Returns without effect for coincident points (r^2 < 1e-9,
-// bem_kernels.h:185).
+// partial sums, without the common factor 1/(4 pi) (applied once per patch
+// pair by the caller).  Coincident points (r^2 < 1e-9, bem_kernels.h:185)
+// contribute nothing: 1/r is set to 0, which zeroes every term, so the loop
+// carries no branch.
 template <typename T>
 __device__ __forceinline__ void integrand(T kappa, T inv_eps, T dx, T dy, T dz, T n0x, T n0y,
                                           T n0z, T nx, T ny, T nz, T wt, T& A, T& B, T& C,
                                           T& D) {
     const T r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < T(1e-9)) return;
-    const T ir = Math<T>::rsqrt_(r2);
+    const T ir = r2 < T(1e-9) ? T(0) : Math<T>::rsqrt_(r2);
     const T r = r2 * ir;
     const T ir2 = ir * ir;
     const T ir3 = ir2 * ir;
-    const T K4 = T(FOUR_PI_INV);
-    const T e = Math<T>::exp_(-kappa * r) * K4;
-    const T gr1 = K4 * ir;
-    const T gr3 = K4 * ir3;
-    const T ur1 = e * ir;
-    const T ur3ur2 = e * (ir3 + kappa * ir2);
+    const T e = Math<T>::exp_(-kappa * r);
+    const T u3 = e * (ir3 + kappa * ir2);  // e^{-kr}(1 + kr)/r^3
     const T nd = nx * dx + ny * dy + nz * dz;
     const T n0d = n0x * dx + n0y * dy + n0z * dz;
-    A += wt * (gr1 - ur1);
-    B -= wt * nd * (inv_eps * gr3 - ur3ur2);
-    C += wt * n0d * (gr3 - ur3ur2 * inv_eps);
+    A += wt * (ir - e * ir);
+    B -= wt * nd * (inv_eps * ir3 - u3);
+    C += wt * n0d * (ir3 - u3 * inv_eps);
     if (kappa != T(0)) {
         const T ir5 = ir3 * ir2;
-        const T gr5 = K4 * ir5;
         const T p = e * (kappa * (T(3) * ir2 * ir2 + kappa * ir3) + T(3) * ir5);
         const T nn0 = nx * n0x + ny * n0y + nz * n0z;
-        D -= wt * ((T(3) * gr5 - p) * nd * n0d - (gr3 - ur3ur2) * nn0);
+        D -= wt * ((T(3) * ir5 - p) * nd * n0d - (ir3 - u3) * nn0);
     }
 }
 
@@ -100,10 +103,11 @@ __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restric
     }
     if (threadIdx.x == 0) {
         const double e = 1.0 / inv_eps, g = gc[i];
-        diag4[4 * i + 0] = A / wsum;
-        diag4[4 * i + 1] = B / wsum + ((1.0 - g) + g / e);
-        diag4[4 * i + 2] = C / wsum - (g + (1.0 - g) / e);
-        diag4[4 * i + 3] = D * inv_eps / wsum;
+        const double k4 = FOUR_PI_INV / wsum;
+        diag4[4 * i + 0] = A * k4;
+        diag4[4 * i + 1] = B * k4 + ((1.0 - g) + g / e);
+        diag4[4 * i + 2] = C * k4 - (g + (1.0 - g) / e);
+        diag4[4 * i + 3] = D * inv_eps * k4;
     }
 }
 
@@ -146,7 +150,7 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
         const Pt<T> q = qual[qb + a];
         T a_ = 0, b_ = 0, c_ = 0, d_ = 0;
         const T ox = dcx - q.x, oy = dcy - q.y, oz = dcz - q.z;
-#pragma unroll 4
+#pragma unroll(sizeof(T) == 4 ? 4 : 2)
         for (int k = 0; k < pn; ++k) {
             const Pt<T> p = quad[pb + k];
             integrand<T>(kappa, inv_eps, ox + p.x, oy + p.y, oz + p.z, q.nx, q.ny, q.nz, p.nx,
@@ -166,10 +170,11 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
         C -= c_;
         D -= d_;
     }
-    out[0] = A;
-    out[1] = B;
-    out[2] = C;
-    out[3] = D * inv_eps;
+    const T k4 = T(FOUR_PI_INV);
+    out[0] = A * k4;
+    out[1] = B * k4;
+    out[2] = C * k4;
+    out[3] = D * inv_eps * k4;
 }
 
 template <typename T>

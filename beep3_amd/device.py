@@ -162,6 +162,9 @@ class Device:
         self._check(lib().beep_hip_memcpy(self._h, arr.ctypes.data_as(vp), vp(d_ptr), arr.nbytes, 2))
         return arr
 
+    def copy_device(self, d_dst: int, d_src: int, nbytes: int):
+        self._check(lib().beep_hip_memcpy(self._h, vp(d_dst), vp(d_src), int(nbytes), 3))
+
     def synchronize(self):
         self._check(lib().beep_hip_synchronize(self._h))
 

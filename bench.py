@@ -189,6 +189,58 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
     return out
 
 
+def pose_bench(n_poses, slots, seq_poses):
+    """Two instances of the 40,962-patch surface (a_lm seed 7) in n_poses
+    seeded rigid poses, all submitted at once to the server (FP32 mat-vecs
+    inside FP64 refinement), then seq_poses of them as sequential plain FP64
+    solves."""
+    from beep3_amd import configs
+    from beep3_amd.serve import Server
+
+    mesh = configs.pose_mesh()
+    scenes = configs.pose_scenes(mesh, n_poses)
+
+    def run(srv, scs, one_at_a_time):
+        res = []
+        t0 = srv.clock()
+        for sc in scs:
+            srv.submit(sc)
+            if one_at_a_time:
+                r = None
+                while r is None:
+                    r = srv.poll(timeout=5.0)
+                res.append(r)
+        while len(res) < len(scs):
+            r = srv.poll(timeout=5.0)
+            if r is not None:
+                res.append(r)
+        return res, max(r["t_done"] for r in res) - t0
+
+    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="fmm", fmm_order=5,
+                 fmm_leaf=40)
+    res, span = run(srv, scenes, False)
+    srv.close()
+    ok = [r for r in res if r["status"] == 0]
+    lat = np.array([r["latency_s"] for r in ok]) * 1e3
+    E = np.array([r["energy"] for r in ok])
+    out = {"poses": n_poses, "solved": len(ok), "patches_per_pose": ok[0]["n_patches"] if ok else 0,
+           "slots": slots, "span_s": span, "poses_per_s": len(ok) / span,
+           "latency_ms_p50": float(np.percentile(lat, 50)),
+           "latency_ms_p99": float(np.percentile(lat, 99)),
+           "iterations_mean": float(np.mean([r["iterations"] for r in ok])),
+           "unconverged": sum(1 for r in ok if not r["converged"]),
+           "energy_kJ_mol_range": [float(E.min()), float(E.max())]}
+    print(f"[bench] poses: {out}", file=sys.stderr, flush=True)
+    seq = configs.pose_scenes(mesh, seq_poses)
+    srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    sres, sspan = run(srv, seq, True)
+    srv.close()
+    out["sequential_plain_fp64"] = {"poses": seq_poses, "poses_per_s": len(sres) / sspan}
+    print(f"[bench] poses sequential plain: {out['sequential_plain_fp64']}", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +258,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-seq", type=int, default=40)
+    ap.add_argument("--poses", type=int, default=256)
+    ap.add_argument("--copy-gib", type=float, default=0.0,
+                    help="one device-to-device copy of this many GiB first (a plain copy of "
+                         "known size, to calibrate hardware byte counters)")
+    ap.add_argument("--pose-seq", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
@@ -222,6 +279,13 @@ def main():
     a = b.scene.arrays()
     N = b.n
     dev = b.dev
+    if args.copy_gib > 0:
+        nbytes = int(args.copy_gib * 2**30)
+        src, dst = dev.alloc(nbytes), dev.alloc(nbytes)
+        dev.copy_device(dst, src, nbytes)
+        dev.synchronize()
+        dev.free(src)
+        dev.free(dst)
     tune = None
     if args.far == "fmm" and not args.no_tune:
         # leaf-size autotuner on the FP64 mat-vec at this order
@@ -329,6 +393,10 @@ def main():
     if not args.no_serve:
         b.close()  # free the headline workload's device memory first
         extra["serving"] = serving_bench(args.serve_seconds, args.serve_slots, args.serve_seq)
+    if args.poses > 0:
+        if args.no_serve:
+            b.close()
+        extra["poses"] = pose_bench(args.poses, args.serve_slots, args.pose_seq)
 
     cpu = None
     if not args.no_cpu:

@@ -79,3 +79,26 @@ def test_born_ion_through_the_server():
     E_ref = oracle.born_energy(1.0, 2.0, 2.0, 80.0, 0.125)
     print("server Born energy", r["energy"], "analytic", E_ref)
     assert abs(r["energy"] - E_ref) / abs(E_ref) < 5e-3
+
+
+def test_pose_through_server_matches_single_solve():
+    """A two-instance pose solved by the server gives the same f, h bits as
+    the same pose solved on its own through the BEEP object."""
+    rng = np.random.default_rng(5)
+    mesh = configs.pose_mesh(4)
+    q, off = configs.random_pose(rng)
+    b = configs.pose_pair(mesh, q, off)
+    b.solve(1e-6, 100, mode="mixed")
+    E_single = b.calculate_energies()
+    srv = Server(slots=2, keep_fh=True, mode="mixed")
+    sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 1500)
+    sc.add_instance(mesh, 2.0, offset=[0.0, 0.0, 0.0])
+    sc.add_instance(mesh, 2.0, quat=q, offset=off)
+    rid = srv.submit(sc)
+    out = _collect(srv, {rid: 0})
+    srv.close()
+    r, f, h = out[0]
+    print("pose energy server", r["energy"], "single", E_single)
+    assert np.array_equal(f, b.f) and np.array_equal(h, b.h)
+    assert abs(r["energy"] - E_single) <= 1e-12 * abs(E_single)
+    b.close()
