@@ -82,7 +82,7 @@ class MatvecTiming(C.Structure):
     _fields_ = [
         ("near_ms", C.c_double), ("far_ms", C.c_double), ("total_ms", C.c_double),
         ("count", C.c_int), ("near_ms_sum", C.c_double), ("far_ms_sum", C.c_double),
-        ("total_ms_sum", C.c_double),
+        ("total_ms_sum", C.c_double), ("m2l_ms", C.c_double), ("m2l_ms_sum", C.c_double),
     ]
 
 

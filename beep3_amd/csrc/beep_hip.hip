@@ -122,10 +122,12 @@ struct beep_hip_ctx {
     std::vector<const void*> step_key;
     unsigned long long gen = 0;  // bumped whenever buffers or plans may change
 
-    // ring of event sets (start, near done, end, far done), one per mat-vec
+    // ring of event sets (start, near done, end, far done, M2L start, M2L
+    // done), one per mat-vec
     static constexpr int RING = 64;
-    hipEvent_t ev[RING][4] = {};
+    hipEvent_t ev[RING][6] = {};
     int ring_head = 0, ring_pending = 0;  // next slot, slots not yet read
+    bool slot_fmm[RING] = {};             // the slot's mat-vec recorded M2L events
     beep_matvec_timing timing{};
 };
 
@@ -189,6 +191,10 @@ void timing_pop(beep_hip_ctx* c) {
     hip_check(hipEventElapsedTime(&b, e[0], e[3]), "hipEventElapsedTime");
     hip_check(hipEventElapsedTime(&a, c->overlap ? e[0] : e[3], e[1]), "hipEventElapsedTime");
     hip_check(hipEventElapsedTime(&t, e[0], e[2]), "hipEventElapsedTime");
+    float m2l = 0;
+    if (c->slot_fmm[slot]) hip_check(hipEventElapsedTime(&m2l, e[4], e[5]), "hipEventElapsedTime");
+    c->timing.m2l_ms = m2l;
+    c->timing.m2l_ms_sum += m2l;
     c->timing.near_ms = a;
     c->timing.far_ms = b;
     c->timing.total_ms = t;
@@ -231,7 +237,7 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     launch_fill(sf, n2, 0.0, c->yfar.p);
     if (c->use_fmm) {
         fmm_evaluate(c->fmm, sf, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa,
-                     c->yfar.p);
+                     c->yfar.p, timed ? &ev[4] : nullptr);
     } else {
         launch_far_pack(sf, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
         launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
@@ -249,6 +255,7 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     launch_axpby(s, n2, 1.0, c->yfar.p, 1.0, dy);
     if (!timed) return;
     hip_check(hipEventRecord(ev[2], s), "hipEventRecord");
+    c->slot_fmm[c->ring_head] = c->use_fmm;
     c->ring_head = (c->ring_head + 1) % beep_hip_ctx::RING;
     c->ring_pending += 1;
 }

@@ -992,7 +992,7 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
 template <typename T, int P>
 static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const double* x,
                      double kappa, T* M, T* L, const std::vector<T*>& tabs, T* const* ytab,
-                     T* slots, double* y) {
+                     T* slots, double* y, hipEvent_t* m2l_ev) {
     constexpr int NP = P * P * P;
     constexpr int NT = ((NP + 63) / 64) * 64;
     const FmmDev& d = pl->dv;
@@ -1010,12 +1010,14 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     }
     {
         const int c0 = pl->lvl_chunk[2], c1 = pl->lvl_chunk[D + 1];
+        if (m2l_ev) hip_check(hipEventRecord(m2l_ev[0], s), "hipEventRecord");
         if (D >= 2 && c1 > c0) {
             hipLaunchKernelGGL((k_m2l<T, P>), dim3(c1 - c0), dim3(512), 0, s, d, pl->d_chunks + c0,
                                tabs[D + 1], ytab, M, slots);
             const int b0 = pl->lvl_first[2], nb = pl->lvl_first[D + 1] - b0;
             hipLaunchKernelGGL((k_m2l_sum<T, P>), dim3(nb), dim3(256), 0, s, d, b0, slots, L);
         }
+        if (m2l_ev) hip_check(hipEventRecord(m2l_ev[1], s), "hipEventRecord");
     }
     for (int l = 2; l < D; ++l) {
         const int first = pl->lvl_first[l + 1], cnt = pl->lvl_first[l + 2] - first;
@@ -1048,26 +1050,26 @@ int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_m
 }
 
 void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, const double* x,
-                  double kappa, double* y) {
+                  double kappa, double* y, hipEvent_t* m2l_ev) {
     if (fp32) {
         if (!pl->have32) throw std::runtime_error("FMM FP32 tables not prepared");
         float* M = pl->exp32;
         float* L = pl->exp32 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
-            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
-            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
-            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y); break;
+            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
+            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
+            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
+            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
         }
     } else {
         if (!pl->have64) throw std::runtime_error("FMM FP64 tables not prepared");
         double* M = pl->exp64;
         double* L = pl->exp64 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
-            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
-            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
-            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y); break;
+            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
+            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
+            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
+            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
         }
     }
 }

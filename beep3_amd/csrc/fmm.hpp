@@ -71,7 +71,8 @@ void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 // canon_xyz) and the map canonical node -> actual node (order^3 entries)
 int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);
 // y += far field (1-point interaction of every pair) of x
+// (m2l_ev: optional pair of events recorded around the M2L launches)
 void fmm_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, const double* x,
-                  double kappa, double* y);
+                  double kappa, double* y, hipEvent_t* m2l_ev = nullptr);
 
 }  // namespace beepdev

@@ -28,6 +28,8 @@ HBM_PEAK_GBS = 8000.0
 FP32_VECTOR_TFLOPS = 157.3
 FP64_VECTOR_TFLOPS = 78.6
 FLOP_PER_EVAL = 92  # integrand evaluation (qual point x quad point)
+MFMA_FP64_TFLOPS = 78.6   # dense matrix peaks of one MI355X
+MFMA_FP32_TFLOPS = 157.3
 
 
 TUNE_LEAVES = (24, 32, 40, 64, 96, 160)
@@ -57,6 +59,7 @@ def time_matvecs(b, d_x, d_y, precision, warmup, steps):
     t1 = time.perf_counter()
     tm = dev.timing()
     k = max(tm.count, 1)
+    time_matvecs.m2l_ms = tm.m2l_ms_sum / k
     return ((t1 - t0) * 1e3 / steps, tm.near_ms_sum / k, tm.far_ms_sum / k, tm.total_ms_sum / k)
 
 
@@ -321,7 +324,8 @@ def main():
                      ("fp32_resid", max(3, args.steps // 2))):
         ms, near_ms, far_ms, tot_ms = time_matvecs(b, d_x, d_y, p, args.warmup, steps)
         rates[p] = {"ms_per_matvec": ms, "near_ms": near_ms, "far_ms": far_ms,
-                    "device_ms": tot_ms, "matvecs_per_s": 1e3 / ms, "steps": steps}
+                    "device_ms": tot_ms, "m2l_ms": time_matvecs.m2l_ms,
+                    "matvecs_per_s": 1e3 / ms, "steps": steps}
         print(f"[bench] {p}: {ms:.3f} ms/matvec (device {tot_ms:.3f}: near {near_ms:.3f} "
               f"concurrent with far {far_ms:.3f})", file=sys.stderr, flush=True)
     # the near SpMV alone (far field serialised after it), for its roofline
@@ -341,7 +345,22 @@ def main():
                 "bytes_per_launch": nb64, "launch_ms": near_alone,
                 "compulsory_bytes": cb64,
                 "compulsory_GBs": cb64 / (near_alone * 1e-3) / 1e9}
+    fi = dev.far_info()
+    m2l = None
+    if fi["mode"] == 1 and fi["m2l_pairs"] > 0:
+        # real GEMM flops of M2L: 3 products of the P^3 x P^3 table with a
+        # source expansion per box pair (k_m2l + k_m2l_sum time)
+        NP = fi["order"] ** 3
+        flops = 2.0 * 3 * NP * NP * fi["m2l_pairs"]
+        m2l = {"pairs": fi["m2l_pairs"], "gflop_per_launch": flops / 1e9}
+        for p, peak in (("fp64", MFMA_FP64_TFLOPS), ("fp32", MFMA_FP32_TFLOPS)):
+            ms = rates[p]["m2l_ms"]
+            if ms > 0:
+                m2l[p] = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                          "frac_of_mfma_peak": flops / (ms * 1e-3) / 1e12 / peak}
     extra = {
+        "far_plan": fi,
+        "m2l_mfma": m2l,
         "nnz_near_pairs": nnz, "leaves": int(len(lp) - 1), "integrand_evals": n_evals,
         "precompute_ms": pre,
         "integrand_tflops": {p: n_evals * FLOP_PER_EVAL / (pre[p] * 1e-3) / 1e12 for p in pre},

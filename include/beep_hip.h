@@ -121,6 +121,8 @@ typedef struct {
     double near_ms_sum; /* summed over those mat-vecs */
     double far_ms_sum;
     double total_ms_sum;
+    double m2l_ms;      /* last mat-vec: FMM M2L (k_m2l + k_m2l_sum), 0 for the direct far field */
+    double m2l_ms_sum;
 } beep_matvec_timing;
 
 const char* beep_hip_last_error(const beep_hip_ctx* ctx);
