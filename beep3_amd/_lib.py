@@ -43,6 +43,7 @@ class SceneSizes(C.Structure):
         ("n_qual", C.c_int), ("n_quad", C.c_int), ("n_galerkin", C.c_int),
         ("n_leaves", C.c_int), ("n_near_cols", C.c_int), ("n_near_pairs", C.c_longlong),
         ("tree_depth", C.c_int), ("cube_centre", C.c_double * 3), ("cube_edge", C.c_double),
+        ("n_triangles", C.c_int), ("n_vertex_triangles", C.c_int),
     ]
 
 
@@ -53,6 +54,15 @@ class HipGeometry(C.Structure):
         ("qual_ptr", vp), ("qual_pts", vp),
         ("quad_ptr", vp), ("quad_pts", vp),
         ("gal_ptr", vp), ("gal_pts", vp),
+    ]
+
+
+class HipSurface(C.Structure):
+    _fields_ = [
+        ("n_patches", C.c_int),
+        ("centroid", vp), ("normal", vp), ("area", vp), ("eps_ratio", vp), ("gc", vp),
+        ("verts", vp), ("n_tris", C.c_int), ("tris", vp), ("vt_ptr", vp), ("vt_tri", vp),
+        ("vt_corner", vp), ("qual_rule", C.c_int), ("quad_rule", C.c_int),
     ]
 
 
@@ -108,7 +118,8 @@ class ServerResult(C.Structure):
         ("energy", C.c_double), ("iterations", C.c_int), ("matvecs", C.c_int),
         ("refinements", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
         ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
-        ("host_ms", C.c_double), ("device_ms", C.c_double), ("bytes", C.c_double),
+        ("host_ms", C.c_double), ("device_ms", C.c_double), ("upload_ms", C.c_double),
+        ("precompute_ms", C.c_double), ("solve_ms", C.c_double), ("bytes", C.c_double),
         ("slot", C.c_int), ("message", C.c_char * 128),
     ]
 
@@ -139,6 +150,8 @@ def _declare(L):
     L.beep_scene_instances.argtypes = [vp, dp, ip, ip]
     L.beep_scene_destroy.argtypes = [vp]
     L.beep_scene_destroy.restype = None
+    L.beep_scene_set_host_points.argtypes = [vp, C.c_int]
+    L.beep_scene_surface.argtypes = [vp, dp, ip, ip, ip, ip]
     L.beep_server_default_opts.argtypes = [C.POINTER(ServerOpts)]
     L.beep_server_default_opts.restype = None
     L.beep_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -162,11 +175,14 @@ def _declare_hip(L):
     L.beep_hip_destroy.argtypes = [vp]
     L.beep_hip_set_geometry.argtypes = [vp, C.POINTER(HipGeometry)]
     L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
+    L.beep_hip_set_surface.argtypes = [vp, C.POINTER(HipSurface)]
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.beep_hip_set_overlap.argtypes = [vp, C.c_int]
     L.beep_hip_set_graphs.argtypes = [vp, C.c_int]
     L.beep_hip_autotune_leaf.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
+    L.beep_hip_autotune_spmv.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
+    L.beep_hip_set_spmv_residency.argtypes = [vp, C.c_int]
     L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]

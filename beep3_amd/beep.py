@@ -32,12 +32,15 @@ CONVERT_ENERGY_TO_KJ_PER_MOL = AVOGADRO * ELEMENTARY_CHARGE**2 / (EPSILON0 * ANG
 
 class BEEP:
     def __init__(self, Dsolvent=80.0, kappa=0.0, quad_points=0, qual_points=4, nbsize=1500,
-                 device=0, precision="fp32", far_field="auto", fmm_order=5, fmm_leaf=48):
+                 device=0, precision="fp32", far_field="auto", fmm_order=5, fmm_leaf=48,
+                 device_points=True):
         self.Dsolvent, self.kappa = float(Dsolvent), float(kappa)
         self.quad, self.qual, self.nbsize = int(quad_points), int(qual_points), int(nbsize)
         self.precision = precision  # coefficient set precomputed at init
         self.device_id = device
         self.far_field, self.fmm_order, self.fmm_leaf = far_field, int(fmm_order), int(fmm_leaf)
+        # point sets made on the device from the surface (else on the host and uploaded)
+        self.device_points = bool(device_points)
         self.library: list[Mesh] = []
         self.instances: list[tuple] = []
         self.scene: Scene | None = None
@@ -66,7 +69,8 @@ class BEEP:
         """Build patches and octree (host), upload to the GPU and precompute
         the explicit-pair coefficients (precalc_neighbour_interactions)."""
         t0 = time.perf_counter()
-        sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize)
+        sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize,
+                   host_points=not self.device_points)
         for lib_id, off, rot, Dp in self.instances:
             sc.add_instance(self.library[lib_id], Dp, quat=rot, offset=off)
         sc.build()
@@ -75,7 +79,10 @@ class BEEP:
         if self.dev is None:
             self.dev = Device(self.device_id)
         self.dev.set_far_field(self.far_field, self.fmm_order, self.fmm_leaf)
-        self.dev.set_geometry(a)
+        if self.device_points:
+            self.dev.set_surface(a, sc.surface(), self.qual, self.quad)
+        else:
+            self.dev.set_geometry(a)
         self.dev.set_tree(a)
         t2 = time.perf_counter()
         for p in (precisions or [self.precision]):

@@ -12,6 +12,7 @@
 #include "../../include/beep_hip.h"
 #include "device.hpp"
 #include "fmm.hpp"
+#include "host.hpp"
 
 using namespace beepdev;
 
@@ -60,6 +61,8 @@ struct beep_hip_ctx {
     hipStream_t far_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool overlap = true;
+    int n_cu = 256;               // compute units of the device
+    int spmv_blocks_per_cu = 0;   // near-SpMV residency cap (0: one block per work item)
     std::string err;
 
     // geometry
@@ -70,6 +73,8 @@ struct beep_hip_ctx {
     DevBuf<Pt64> qual64, quad64, gal64;
     DevBuf<Pt32> qual32, quad32;
     DevBuf<double> stage;  // raw point upload staging (7 doubles per point)
+    DevBuf<double> sverts; // surface for point generation
+    DevBuf<int> stris, svt_ptr, svt_tri, svt_corner;
     bool have_geom = false;
 
     // tree
@@ -169,9 +174,40 @@ void far_setup(beep_hip_ctx* c) {
         c->fmm = nullptr;
     }
     c->use_fmm = c->far_mode == BEEP_FAR_FMM || (c->far_mode == BEEP_FAR_AUTO && c->N > 20000);
-    if (c->use_fmm)
+    // the plan itself is built on first use (ensure_far), so its host work
+    // can overlap the coefficient precompute already running on the device
+}
+
+// Build the far-field plan if it is missing, on the far stream (which holds
+// no other work then), and the kernel tables of the given precision for
+// the current kappa.  Tables change only through a new kappa (precompute,
+// which drains both streams first) or a precision not built yet (buffers of
+// their own), so no pending far-field work reads what is written here.
+void ensure_far(beep_hip_ctx* c, int precision = -1) {
+    if (!c->use_fmm) return;
+    if (!c->fmm) {
+        hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
         c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
-                                 c->fmm_arena, c->stream);
+                                 c->fmm_arena, c->far_stream);
+        c->gen++;
+    }
+    if (precision < 0 || !c->have_diag) return;
+    const bool f64 = precision != BEEP_FP32, f32 = precision != BEEP_FP64;
+    if (!fmm_tables_ready(c->fmm, c->kappa, f64, f32)) {
+        fmm_prepare(c->fmm, c->kappa, f64, f32, c->far_stream);
+        c->gen++;
+    }
+}
+
+// state common to both ways of giving the geometry
+void geometry_done(beep_hip_ctx* c, int N, const double* centroid) {
+    c->N = N;
+    c->h_centroid.assign(centroid, centroid + 3 * size_t(N));
+    far_setup(c);
+    c->part_blocks = std::max(1, std::min(256, (2 * N + 1023) / 1024));
+    c->have_geom = true;
+    c->have_tree = false;
+    c->have32 = c->have64 = c->have_diag = false;
 }
 
 void check_ptr(int N, const int* ptr, const char* what) {
@@ -217,9 +253,9 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
     else require(c->have32, "FP32 coefficients not precomputed");
     hipStream_t s = c->stream;
-    // kernel tables are built on first use for the current kappa
-    if (c->use_fmm)
-        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, s);
+    // plan and kernel tables are built on first use for the current kappa
+    // (before any capture: a captured step finds them ready)
+    if (!c->capturing) ensure_far(c, precision == BEEP_FP32 ? BEEP_FP32 : BEEP_FP64);
     const bool timed = !c->capturing;
     if (timed && c->ring_pending == beep_hip_ctx::RING) timing_pop(c);
     hipEvent_t* ev = c->ev[c->ring_head];
@@ -249,7 +285,7 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
     launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
-                     c->spmv_part.p, dy);
+                     c->spmv_part.p, dy, c->spmv_blocks_per_cu * c->n_cu);
     if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->overlap) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     launch_axpby(s, n2, 1.0, c->yfar.p, 1.0, dy);
@@ -300,8 +336,7 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         return;
     }
     // everything the captured launches bake in: buffers, plan, sizes
-    if (c->use_fmm)
-        fmm_prepare(c->fmm, c->kappa, precision != BEEP_FP32, precision == BEEP_FP32, c->stream);
+    ensure_far(c, precision == BEEP_FP32 ? BEEP_FP32 : BEEP_FP64);
     c->yfar.alloc(size_t(2) * c->N);
     if (!c->use_fmm) c->src.alloc(size_t(c->N));
     const std::vector<const void*> key = {
@@ -309,7 +344,8 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         reinterpret_cast<const void*>(intptr_t(m)), reinterpret_cast<const void*>(intptr_t(c->N)),
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
         c->yfar.p, c->src.p, c->coef32.p, c->coef64.p, c->diag.p, c->spmv_part.p, c->fmm,
-        reinterpret_cast<const void*>(intptr_t(c->overlap))};
+        reinterpret_cast<const void*>(intptr_t(c->overlap)),
+        reinterpret_cast<const void*>(intptr_t(c->spmv_blocks_per_cu))};
     if (!c->step_exec || key != c->step_key) {
         drop_step_graph(c);
         hipGraph_t g = nullptr;
@@ -449,6 +485,8 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
         set_device(c.get());
         hip_check(hipStreamCreate(&c->stream), "hipStreamCreate");
         hip_check(hipStreamCreate(&c->far_stream), "hipStreamCreate");
+        hip_check(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device),
+                  "hipDeviceGetAttribute");
         hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
         for (auto& set : c->ev)
@@ -538,13 +576,78 @@ int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* g) {
         points(g->quad_ptr, ctx->quad_ptr.p, g->quad_pts, ctx->quad64, &ctx->quad32);
         points(g->gal_ptr, ctx->gal_ptr.p, g->gal_pts, ctx->gal64, nullptr);
         hip_check(hipStreamSynchronize(s), "sync geometry upload");
-        ctx->N = N;
-        ctx->h_centroid.assign(g->centroid, g->centroid + 3 * size_t(N));
-        far_setup(ctx);
-        ctx->part_blocks = std::max(1, std::min(256, (2 * N + 1023) / 1024));
-        ctx->have_geom = true;
-        ctx->have_tree = false;
-        ctx->have32 = ctx->have64 = ctx->have_diag = false;
+        geometry_done(ctx, N, g->centroid);
+    });
+}
+
+int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
+    return guarded(ctx, [&] {
+        require(g && g->n_patches > 0 && g->n_tris > 0, "empty surface");
+        set_device(ctx);
+        sync_streams(ctx);
+        const int N = g->n_patches, T = g->n_tris;
+        require(g->centroid && g->normal && g->area && g->eps_ratio && g->gc, "null patch array");
+        require(g->verts && g->tris && g->vt_ptr && g->vt_tri && g->vt_corner, "null surface array");
+        auto rule_ok = [](int r) { return r == 0 || r == 1 || r == 4 || r == 6 || r == 7 || r == 16; };
+        require(rule_ok(g->qual_rule) && rule_ok(g->quad_rule), "rule must be 0, 1, 4, 6, 7 or 16");
+        check_ptr(N, g->vt_ptr, "bad vertex-triangle pointer array");
+        for (int t = 0; t < 3 * T; ++t) require(g->tris[t] >= 0 && g->tris[t] < N, "triangle vertex out of range");
+        for (int p = 0; p < N; ++p) {
+            require(g->vt_ptr[p + 1] > g->vt_ptr[p], "vertex without triangles");
+            require(g->eps_ratio[p] > 0 && g->area[p] > 0, "non-positive area or eps ratio");
+            for (int k = g->vt_ptr[p]; k < g->vt_ptr[p + 1]; ++k) {
+                const int t = g->vt_tri[k], c = g->vt_corner[k];
+                require(t >= 0 && t < T && c >= 0 && c < 3 && g->tris[3 * t + c] == p,
+                        "vertex-triangle list does not match the triangles");
+            }
+        }
+        std::vector<Patch64> pr(N);
+        for (int i = 0; i < N; ++i)
+            pr[i] = Patch64{g->centroid[3 * i], g->centroid[3 * i + 1], g->centroid[3 * i + 2],
+                            g->area[i], g->normal[3 * i], g->normal[3 * i + 1],
+                            g->normal[3 * i + 2], 1.0 / g->eps_ratio[i]};
+        hipStream_t s = ctx->stream;
+        ctx->patches.upload(pr.data(), N, s);
+        ctx->gc.upload(g->gc, N, s);
+        ctx->sverts.upload(g->verts, size_t(3) * N, s);
+        ctx->stris.upload(g->tris, size_t(3) * T, s);
+        const int nvt = g->vt_ptr[N];
+        ctx->svt_ptr.upload(g->vt_ptr, N + 1, s);
+        ctx->svt_tri.upload(g->vt_tri, nvt, s);
+        ctx->svt_corner.upload(g->vt_corner, nvt, s);
+        // one point set per rule: counts (1, or 2 sub-triangles x k points per
+        // triangle around the vertex), then the points on the device
+        std::vector<int> ptr(size_t(N) + 1);
+        auto make = [&](int order, DevBuf<int>& dptr, DevBuf<Pt64>& o64, DevBuf<Pt32>* o32,
+                        int per_area) {
+            RuleDev rd{};
+            if (order != 0) {
+                const beep::Rule& r = beep::rule_for_order(order);
+                rd.k = int(r.w.size());
+                for (int j = 0; j < rd.k; ++j) {
+                    rd.u[j] = r.u[j];
+                    rd.v[j] = r.v[j];
+                    rd.w[j] = r.w[j];
+                }
+            }
+            ptr[0] = 0;
+            for (int p = 0; p < N; ++p) {
+                const int64_t c = rd.k == 0 ? 1 : int64_t(2) * (g->vt_ptr[p + 1] - g->vt_ptr[p]) * rd.k;
+                require(ptr[p] + c < int64_t(0x7fffffff), "too many points for 32-bit offsets");
+                ptr[p + 1] = ptr[p] + int(c);
+            }
+            dptr.upload(ptr.data(), N + 1, s);
+            o64.alloc(size_t(ptr[N]));
+            if (o32) o32->alloc(size_t(ptr[N]));
+            launch_gen_points(s, N, ctx->sverts.p, ctx->stris.p, ctx->svt_ptr.p, ctx->svt_tri.p,
+                              ctx->svt_corner.p, ctx->patches.p, dptr.p, rd, per_area, o64.p,
+                              o32 ? o32->p : nullptr);
+            hip_check(hipStreamSynchronize(s), "sync point generation");  // ptr is reused
+        };
+        make(g->qual_rule, ctx->qual_ptr, ctx->qual64, &ctx->qual32, 1);
+        make(g->quad_rule, ctx->quad_ptr, ctx->quad64, &ctx->quad32, 0);
+        make(4, ctx->gal_ptr, ctx->gal64, nullptr, 0);
+        geometry_done(ctx, N, g->centroid);
     });
 }
 
@@ -648,7 +751,6 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
                                     ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
                                     ctx->quad64.p, kappa, 1, ctx->coef64.p);
             ctx->have64 = true;
-            if (ctx->use_fmm) fmm_prepare(ctx->fmm, kappa, true, false, s);
         } else if (precision == BEEP_FP32 || precision == BEEP_FP32_RESID) {
             const size_t need = size_t(ctx->nnz) * sizeof(float4);
             require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
@@ -660,13 +762,16 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
                                     ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
                                     ctx->quad32.p, float(kappa), 1, ctx->coef32.p);
             ctx->have32 = true;
-            // FP32 coefficients serve the FP32 mat-vec (FP32 far field) and the
-            // refinement residual (FP64 far field)
-            if (ctx->use_fmm) fmm_prepare(ctx->fmm, kappa, true, true, s);
         } else {
             throw std::runtime_error("unknown precision");
         }
-        hip_check(hipStreamSynchronize(s), "sync precompute");
+        // the far-field plan and tables are built on the host and the far
+        // stream while the coefficient kernels run (no wait here: the
+        // precompute is stream-ordered); FP32 coefficients serve the FP32
+        // mat-vec (FP32 far field) and the refinement residual (FP64 far field)
+        ctx->kappa = kappa;
+        ctx->have_diag = true;
+        ensure_far(ctx, precision);
     });
 }
 
@@ -708,6 +813,7 @@ int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
         require(out != nullptr, "null output");
         *out = beep_far_info{};
         out->mode = ctx->use_fmm ? BEEP_FAR_FMM : BEEP_FAR_DIRECT;
+        ensure_far(ctx);
         if (ctx->use_fmm && ctx->fmm) {
             out->order = ctx->fmm->P;
             out->leaf_level = ctx->fmm->D;
@@ -776,6 +882,66 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
         ctx->fmm_leaf = best_leaf;
         far_setup(ctx);
         if (best) *best = best_leaf;
+    });
+}
+
+int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu) {
+    return guarded(ctx, [&] {
+        require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per CU must be 0..64");
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->spmv_blocks_per_cu = blocks_per_cu;
+        ctx->gen++;
+    });
+}
+
+int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
+                           int reps, double* ms_out, int* best) {
+    return guarded(ctx, [&] {
+        require(blocks_per_cu && n > 0 && reps > 0, "bad autotune arguments");
+        require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
+        set_device(ctx);
+        const size_t n2 = size_t(2) * ctx->N;
+        ctx->xbuf.alloc(n2);
+        ctx->ybuf.alloc(n2);
+        launch_fill(ctx->stream, int(n2), 1.0, ctx->xbuf.p);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int saved = ctx->spmv_blocks_per_cu;
+        double best_ms = 1e300;
+        int best_v = saved;
+        try {
+            for (int k = 0; k < n; ++k) {
+                require(blocks_per_cu[k] >= 0 && blocks_per_cu[k] <= 64, "blocks per CU must be 0..64");
+                ctx->spmv_blocks_per_cu = blocks_per_cu[k];
+                matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // warm-up
+                hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+                for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
+                hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                const double per = double(ms) / reps;
+                if (ms_out) ms_out[k] = per;
+                if (per < best_ms) {
+                    best_ms = per;
+                    best_v = blocks_per_cu[k];
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            ctx->spmv_blocks_per_cu = saved;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        timing_collect(ctx);
+        ctx->timing = beep_matvec_timing{};
+        ctx->spmv_blocks_per_cu = best_v;
+        ctx->gen++;
+        if (best) *best = best_v;
     });
 }
 
@@ -1009,7 +1175,8 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         auto add = [&](const auto& buf) { b += buf.n * sizeof(*buf.p); };
         add(ctx->patches); add(ctx->gc); add(ctx->qual_ptr); add(ctx->quad_ptr); add(ctx->gal_ptr);
         add(ctx->qual64); add(ctx->quad64); add(ctx->gal64); add(ctx->qual32); add(ctx->quad32);
-        add(ctx->stage);
+        add(ctx->stage); add(ctx->sverts); add(ctx->stris); add(ctx->svt_ptr); add(ctx->svt_tri);
+        add(ctx->svt_corner);
         add(ctx->leaf_ptr); add(ctx->leaf_idx); add(ctx->nb_ptr); add(ctx->nb_idx);
         add(ctx->coef_off); add(ctx->coef_blk); add(ctx->spmv_work); add(ctx->spmv_groups);
         add(ctx->spmv_group_nblk); add(ctx->spmv_part); add(ctx->diag); add(ctx->coef32);

@@ -127,7 +127,8 @@ struct SpmvPlan {
 void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
                       const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
-                      const double* diag4, const double* x, void* part, double* y);
+                      const double* diag4, const double* x, void* part, double* y,
+                      int max_blocks = 0);
 
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,
@@ -160,6 +161,17 @@ void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const i
 void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out);
 void launch_set_int(hipStream_t s, int* p, int v);
 void launch_inc_int(hipStream_t s, int* p);
+// A triangle rule for point generation: k points per sub-triangle at
+// barycentric (u, v) with weights w (k = 0: one centroid point per patch).
+struct RuleDev {
+    int k;
+    double u[16], v[16], w[16];
+};
+// the point set of every patch from the surface, in the local frame
+void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tris,
+                       const int* vt_ptr, const int* vt_tri, const int* vt_corner,
+                       const Patch64* patches, const int* ptr, const RuleDev& rule, int per_area,
+                       Pt64* out64, Pt32* out32);
 // raw CSR points k0 .. k0+n-1 (7 doubles each) -> local-frame records
 void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
                      const Patch64* patches, Pt64* out64, Pt32* out32);

@@ -223,6 +223,9 @@ struct Mfma<double> {
     }
     // accumulator element r of lane l holds row l/16 + 4r (column l % 16)
     static __device__ __forceinline__ int row(int l, int r) { return l / 16 + 4 * r; }
+    // lane group (l / 16) and element of row rr
+    static __device__ __forceinline__ int lane_group(int rr) { return rr & 3; }
+    static __device__ __forceinline__ int elem(int rr) { return rr >> 2; }
 };
 template <>
 struct Mfma<float> {
@@ -232,13 +235,15 @@ struct Mfma<float> {
     }
     // accumulator element r of lane l holds row 4 (l/16) + r (column l % 16)
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l / 16) + r; }
+    static __device__ __forceinline__ int lane_group(int rr) { return rr >> 2; }
+    static __device__ __forceinline__ int elem(int rr) { return rr & 3; }
 };
 
 // actual node index of canonical node index mc for a pair with symmetry code
 // sym: bits 0-5 the actual axis of canonical axes 0, 1, 2 (2 bits each),
 // bits 6-8 a reflection of canonical axis 0, 1, 2
 template <int P>
-__host__ __device__ __forceinline__ int node_of(int mc, int sym) {
+inline int node_of(int mc, int sym) {
     const int c[3] = {mc / (P * P), (mc / P) % P, mc % P};
     int a[3];
 #pragma unroll
@@ -253,6 +258,7 @@ __host__ __device__ __forceinline__ int node_of(int mc, int sym) {
 constexpr int M2L_G = 64;     // pairs per block (4 MFMA column tiles)
 constexpr int M2L_WAVES = 8;  // waves per block, each owning output row tiles
 constexpr int M2L_MS = 32;    // K-dimension slab staged in LDS
+constexpr int M2L_MSP = 36;   // padded slab row (spreads the B reads over the banks)
 
 template <typename T, int P>
 __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
@@ -263,19 +269,18 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     constexpr int NPP = (NP + 15) / 16 * 16;
     constexpr int NTILE = NPP / 16;
     constexpr int NTW = (NTILE + M2L_WAVES - 1) / M2L_WAVES;  // row tiles per wave
-    constexpr int G = M2L_G, CT = G / 16, MS = M2L_MS, MSP = MS + 1;
+    constexpr int G = M2L_G, CT = G / 16, MS = M2L_MS, MSP = M2L_MSP;
     using acc_t = typename Mfma<T>::acc_t;
-    // B slab [e][g][m] (padded), reused for the result tile [g][n] of one
-    // (potential, column tile)
-    constexpr int SB = 3 * G * MSP > 16 * NPP ? 3 * G * MSP : 16 * NPP;
-    __shared__ T sbuf[SB];
-    __shared__ int psrc[G], psym[G];
-    const int4 ch = chunks[xcd_remap(blockIdx.x, gridDim.x)];  // canonical offset, pairs, level
+    __shared__ T sb[3 * G * MSP];  // B slab [e][g][m]
+    __shared__ int psrc[G];
+    __shared__ const unsigned short* pperm[G];
+    const int cid = xcd_remap(blockIdx.x, gridDim.x);
+    const int4 ch = chunks[cid];  // canonical offset, pairs, level
     const int off = ch.x, p0 = ch.y, np = ch.z - ch.y, level = ch.w;
     const int t = threadIdx.x;
     if (t < G) {
         psrc[t] = t < np ? d.pair_src[p0 + t] : 0;
-        psym[t] = t < np ? d.pair_sym[p0 + t] : 0;
+        pperm[t] = d.psi + size_t(t < np ? d.pair_sym[p0 + t] : 0) * NP;
     }
     const int w = t >> 6, l = t & 63;
     const int kl16 = l / 16, il = l % 16;
@@ -289,27 +294,28 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
 #pragma unroll
             for (int c = 0; c < CT; ++c) acc[k][e][c] = acc_t{0, 0, 0, 0};
     for (int m0 = 0; m0 < NPP; m0 += MS) {
-        __syncthreads();  // psrc/psym written; previous slab consumed
-        // stage B[e][g][mm] = M_src(g)[e][node_of(m0 + mm)] for this slab;
-        // consecutive threads walk mm of one (g, e), a run inside one expansion
+        __syncthreads();  // pair lists written; previous slab consumed
+        // B[e][g][mm] = M_src(g)[e][psi_g(m0 + mm)]: the source expansion in
+        // the canonical node order of its pair; consecutive threads walk mm
+        // of one (g, e), a run inside one expansion
         for (int q = t; q < 3 * G * MS; q += 512) {
             const int g = q / (3 * MS), rem = q - g * 3 * MS, e = rem / MS, mm = rem - e * MS;
             const int m = m0 + mm;
             T v = T(0);
-            if (g < np && m < NP)
-                v = M[size_t(psrc[g]) * 3 * NP + e * NP + node_of<P>(m, psym[g])];
-            sbuf[(e * G + g) * MSP + mm] = v;
+            if (g < np && m < NP) v = M[size_t(psrc[g]) * 3 * NP + e * NP + pperm[g][m]];
+            sb[(e * G + g) * MSP + mm] = v;
         }
         __syncthreads();
+#pragma unroll 2
         for (int k0 = 0; k0 < MS; k0 += 4) {
             const int m = m0 + k0 + kl16;  // K-dimension index (canonical source node)
             T bL[CT], b1[CT], b2[CT];
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
                 const int g = c * 16 + il;
-                bL[c] = sbuf[(0 * G + g) * MSP + k0 + kl16];
-                b1[c] = sbuf[(1 * G + g) * MSP + k0 + kl16];
-                b2[c] = sbuf[(2 * G + g) * MSP + k0 + kl16];
+                bL[c] = sb[(0 * G + g) * MSP + k0 + kl16];
+                b1[c] = sb[(1 * G + g) * MSP + k0 + kl16];
+                b2[c] = sb[(2 * G + g) * MSP + k0 + kl16];
             }
 #pragma unroll
             for (int k = 0; k < NTW; ++k) {
@@ -318,8 +324,8 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
                     const int n = nt * 16 + il;  // output row (canonical target node)
                     T xa = T(0), ya = T(0);
                     if (m < NP && n < NP) {
-                        xa = KL[size_t(m) * NP + n];
-                        ya = KY[size_t(m) * NP + n];
+                        xa = KL[m * NP + n];
+                        ya = KY[m * NP + n];
                     }
 #pragma unroll
                     for (int c = 0; c < CT; ++c) {
@@ -331,35 +337,26 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
             }
         }
     }
-    // results: per (potential, column tile) through LDS as [g][n] with the
-    // pair's node permutation applied, then contiguous slot writes
+    // the accumulators go out as they sit in the registers: chunk slot
+    // [e][row tile][column tile][lane][element], 32 contiguous bytes per lane
+    // (FP64) and one contiguous run per wave store; k_m2l_sum finds a
+    // (pair, node) value through the inverse of the MFMA layout and the
+    // pair's node map
     const T sl = T(1.0 / d.box_h[level]);  // Laplace tables are built at h = 1
+    T* out = Cout + size_t(cid) * 3 * NPP * G;
 #pragma unroll
-    for (int e = 0; e < 3; ++e)
+    for (int k = 0; k < NTW; ++k) {
+        const int nt = w + M2L_WAVES * k;
+        if (nt >= NTILE) continue;
 #pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            __syncthreads();
+        for (int e = 0; e < 3; ++e)
 #pragma unroll
-            for (int k = 0; k < NTW; ++k) {
-                const int nt = w + M2L_WAVES * k;
-                if (nt >= NTILE) continue;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int n = nt * 16 + Mfma<T>::row(l, r);
-                    if (n < NP) {
-                        const int gl = il;  // pair within the column tile
-                        const T v = e == 0 ? acc[k][e][c][r] * sl : acc[k][e][c][r];
-                        sbuf[gl * NPP + node_of<P>(n, psym[c * 16 + gl])] = v;
-                    }
-                }
+            for (int c = 0; c < CT; ++c) {
+                acc_t v = acc[k][e][c];
+                if (e == 0) v = v * sl;
+                *reinterpret_cast<acc_t*>(out + (((size_t(e) * NTILE + nt) * CT + c) * 64 + l) * 4) = v;
             }
-            __syncthreads();
-            const int ng = min(16, np - c * 16);
-            for (int q = t; q < ng * NP; q += 512) {
-                const int gl = q / NP, n = q - gl * NP;
-                Cout[size_t(p0 + c * 16 + gl) * 3 * NP + e * NP + n] = sbuf[gl * NPP + n];
-            }
-        }
+    }
 }
 
 // Sum the M2L slots of every target box into its local expansion (one block
@@ -371,9 +368,20 @@ __global__ __launch_bounds__(256) void k_m2l_sum(const FmmDev d, int first_box,
     const int box = first_box + blockIdx.x;
     const int k0 = d.tgt_ptr[box], k1 = d.tgt_ptr[box + 1];
     if (k0 == k1) return;
+    constexpr int NPP = (NP + 15) / 16 * 16, NTILE = NPP / 16, CT = M2L_G / 16;
     for (int q = threadIdx.x; q < 3 * NP; q += 256) {
+        const int e = q / NP, n = q - e * NP;
         T acc = T(0);
-        for (int k = k0; k < k1; ++k) acc += Cin[size_t(d.tgt_pair[k]) * 3 * NP + q];
+        for (int k = k0; k < k1; ++k) {
+            const int pr = d.tgt_pair[k];
+            const int code = d.pair_slot[pr];  // chunk * 64 + position in the chunk
+            const int g = code & 63;
+            const int nc = d.phi[size_t(d.pair_sym[pr]) * NP + n];  // canonical node
+            const int nt = nc >> 4, rr = nc & 15;
+            const int l = Mfma<T>::lane_group(rr) * 16 + (g & 15);
+            acc += Cin[size_t(code >> 6) * 3 * NPP * M2L_G +
+                       (((size_t(e) * NTILE + nt) * CT + (g >> 4)) * 64 + l) * 4 + Mfma<T>::elem(rr)];
+        }
         L[size_t(box) * 3 * NP + q] += acc;
     }
 }
@@ -659,6 +667,28 @@ const std::vector<std::array<int, 3>>& canonical_list() {
 // Canonical index of offset t and the symmetry code of the node map (see
 // node_of): canonical axis k is actual axis ax[k] with |t| sorted
 // descending, reflected where that component of t is negative.
+int canonical_offset(int tx, int ty, int tz, int* sym);
+
+// The 48 signed axis permutations as node-map codes (see node_of), in a
+// fixed order; a pair stores the index of its code.
+const std::vector<int>& symmetry_codes() {
+    static const std::vector<int> codes = [] {
+        std::vector<int> v;
+        const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+        for (const auto& pm : perms)
+            for (int sgn = 0; sgn < 8; ++sgn) v.push_back(pm[0] | (pm[1] << 2) | (pm[2] << 4) | (sgn << 6));
+        return v;
+    }();
+    return codes;
+}
+
+int symmetry_index(int code) {
+    const auto& c = symmetry_codes();
+    for (size_t k = 0; k < c.size(); ++k)
+        if (c[k] == code) return int(k);
+    throw std::runtime_error("bad M2L symmetry code");
+}
+
 int canonical_offset(int tx, int ty, int tz, int* sym) {
     const int t[3] = {tx, ty, tz};
     int ax[3] = {0, 1, 2};
@@ -814,7 +844,7 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     // interaction lists per level (l >= 2), grouped by offset into chunks
     constexpr int G = FmmPlan::CHUNK;
     static_assert(FmmPlan::CHUNK == M2L_G, "plan chunks match the M2L block");
-    std::vector<int> pair_src, pair_tgt, pair_sym;
+    std::vector<int> pair_src, pair_tgt, pair_sym, pair_slot;
     std::vector<int4> chunks;
     pl->lvl_chunk.assign(D + 2, 0);
     for (int l = 0; l <= D; ++l) {
@@ -841,7 +871,7 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                             if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) <= 1) continue;
                             int sym = 0;
                             const int c = canonical_offset(tx, ty, tz, &sym);
-                            byoff[c].push_back({pl->lvl_first[l] + int(q), sg, sym});
+                            byoff[c].push_back({pl->lvl_first[l] + int(q), sg, symmetry_index(sym)});
                         }
                     }
         }
@@ -853,12 +883,14 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                     pair_tgt.push_back(v[j][0]);
                     pair_src.push_back(v[j][1]);
                     pair_sym.push_back(v[j][2]);
+                    pair_slot.push_back(int(chunks.size()) * G + int(j - k));
                 }
                 chunks.push_back(make_int4(off, b0, int(pair_src.size()), l));
             }
         }
     }
     pl->lvl_chunk[D + 1] = int(chunks.size());
+    if (D >= 2 && pl->lvl_chunk[2] != 0) throw std::runtime_error("M2L chunks must start at level 2");
     pl->n_pairs = int(pair_src.size());
     std::vector<int> tgt_ptr(nbox + 1, 0), tgt_pair(pair_tgt.size());
     for (int tg : pair_tgt) tgt_ptr[tg + 1]++;
@@ -907,8 +939,30 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
     upload(arena, "pair_tgt", pl->dv.pair_tgt, pair_tgt, s);
     upload(arena, "pair_sym", pl->dv.pair_sym, pair_sym, s);
+    upload(arena, "pair_slot", pl->dv.pair_slot, pair_slot, s);
+    // node maps of the 48 symmetries: psi (canonical -> actual) and phi
+    // (actual -> canonical), order^3 entries each (host copies live until the
+    // stream is synchronised below)
+    const int NPn = order * order * order;
+    std::vector<unsigned short> psi(size_t(48) * NPn), phi(size_t(48) * NPn);
     {
-        std::vector<int4> canon(FmmPlan::NCANON);
+        for (int k = 0; k < 48; ++k)
+            for (int m = 0; m < NPn; ++m) {
+                int a = 0;
+                switch (order) {
+                    case 3: a = node_of<3>(m, symmetry_codes()[k]); break;
+                    case 4: a = node_of<4>(m, symmetry_codes()[k]); break;
+                    case 5: a = node_of<5>(m, symmetry_codes()[k]); break;
+                    default: a = node_of<6>(m, symmetry_codes()[k]); break;
+                }
+                psi[size_t(k) * NPn + m] = (unsigned short)a;
+                phi[size_t(k) * NPn + a] = (unsigned short)m;
+            }
+        upload(arena, "psi", pl->dv.psi, psi, s);
+        upload(arena, "phi", pl->dv.phi, phi, s);
+    }
+    std::vector<int4> canon(FmmPlan::NCANON);
+    {
         for (int c = 0; c < FmmPlan::NCANON; ++c) {
             const auto t = canonical_list()[c];
             canon[c] = make_int4(t[0], t[1], t[2], 0);
@@ -956,6 +1010,14 @@ static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& y
               "H2D ytab");
 }
 
+// M2L result slots: every chunk keeps 3 x NPP x 64 values in the MFMA
+// accumulator layout
+static size_t slot_count(const FmmPlan* pl) {
+    const int NP = pl->P * pl->P * pl->P, NPP = (NP + 15) / 16 * 16;
+    const size_t nch = size_t(pl->lvl_chunk.empty() ? 0 : pl->lvl_chunk.back());
+    return std::max<size_t>(1, nch * 3 * NPP * M2L_G);
+}
+
 void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s) {
     if (pl->tables_kappa == kappa && (!fp64 || pl->have64) && (!fp32 || pl->have32)) return;
     if (pl->tables_kappa != kappa) {
@@ -969,7 +1031,7 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
             case 5: make_tables<double, 5>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
             case 6: make_tables<double, 6>(pl, kappa, pl->ktab64, pl->ytab64, s); break;
         }
-        const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+        const size_t ns = slot_count(pl);
         pl->exp64 = pl->arena->get<double>("exp64", 2 * pl->n_exp);
         pl->slots64 = pl->arena->get<double>("slots64", ns);
         pl->have64 = true;
@@ -981,7 +1043,7 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
             case 5: make_tables<float, 5>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
             case 6: make_tables<float, 6>(pl, kappa, pl->ktab32, pl->ytab32, s); break;
         }
-        const size_t ns = std::max<size_t>(1, size_t(pl->n_pairs) * 3 * pl->P * pl->P * pl->P);
+        const size_t ns = slot_count(pl);
         pl->exp32 = pl->arena->get<float>("exp32", 2 * pl->n_exp);
         pl->slots32 = pl->arena->get<float>("slots32", ns);
         pl->have32 = true;
@@ -1009,10 +1071,12 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
         if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, M);
     }
     {
-        const int c0 = pl->lvl_chunk[2], c1 = pl->lvl_chunk[D + 1];
+        // chunks exist from level 2 on and are numbered from 0 (the slot
+        // index of a pair is its chunk number)
+        const int nch = D >= 2 ? pl->lvl_chunk[D + 1] : 0;
         if (m2l_ev) hip_check(hipEventRecord(m2l_ev[0], s), "hipEventRecord");
-        if (D >= 2 && c1 > c0) {
-            hipLaunchKernelGGL((k_m2l<T, P>), dim3(c1 - c0), dim3(512), 0, s, d, pl->d_chunks + c0,
+        if (nch > 0) {
+            hipLaunchKernelGGL((k_m2l<T, P>), dim3(nch), dim3(512), 0, s, d, pl->d_chunks,
                                tabs[D + 1], ytab, M, slots);
             const int b0 = pl->lvl_first[2], nb = pl->lvl_first[D + 1] - b0;
             hipLaunchKernelGGL((k_m2l_sum<T, P>), dim3(nb), dim3(256), 0, s, d, b0, slots, L);

@@ -26,7 +26,10 @@ struct FmmDev {
     int* nb_leaf = nullptr;
     int* pair_src = nullptr;    // M2L pairs grouped by offset
     int* pair_tgt = nullptr;
-    int* pair_sym = nullptr;    // node map of each pair to its canonical offset
+    int* pair_sym = nullptr;    // symmetry (0..47) taking each pair to its canonical offset
+    int* pair_slot = nullptr;   // chunk * 64 + position of each pair in its M2L chunk
+    unsigned short* psi = nullptr;  // [48][P^3] canonical node -> actual node
+    unsigned short* phi = nullptr;  // [48][P^3] actual node -> canonical node
     int* tgt_ptr = nullptr;     // per box: its M2L pairs (indices into pair_*)
     int* tgt_pair = nullptr;
     double* tn = nullptr;       // T_j(xi_k), P x P
@@ -67,6 +70,9 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                          DevArena& arena, hipStream_t s);
 void fmm_plan_destroy(FmmPlan* p);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
+inline bool fmm_tables_ready(const FmmPlan* p, double kappa, bool fp64, bool fp32) {
+    return p->tables_kappa == kappa && (!fp64 || p->have64) && (!fp32 || p->have32);
+}
 // canonical offset of the M2L offset t (index returned, components in
 // canon_xyz) and the map canonical node -> actual node (order^3 entries)
 int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);

@@ -195,6 +195,7 @@ struct beep_scene {
     int quad, qual, nbsize;
     std::vector<Instance> inst;
     bool built = false;
+    bool host_points = true;  // false: point sets are generated on the device
     std::vector<beep::Vec3> centroid, normal;
     std::vector<double> area, eps, gc;
     std::vector<int> patch_inst;

@@ -244,7 +244,8 @@ extern "C" int beep_scene_build(beep_scene* s) {
                     throw std::runtime_error("too many points for 32-bit offsets");
                 s->ptr[k][p + 1] = s->ptr[k][p] + int(c);
             }
-            s->pts[k].resize(size_t(s->ptr[k][N]));
+            s->pts[k].clear();
+            if (s->host_points) s->pts[k].resize(size_t(s->ptr[k][N]));
         }
         s->centroid.resize(N); s->normal.resize(N); s->area.resize(N);
         s->eps.resize(N); s->gc.resize(N); s->patch_inst.resize(N);
@@ -262,7 +263,7 @@ extern "C" int beep_scene_build(beep_scene* s) {
                 s->eps[p] = s->Dsolvent / in.Dprotein;
                 s->gc[p] = 0.5;
                 s->patch_inst[p] = plist[p].first;
-                for (int k = 0; k < 3; ++k) {
+                for (int k = 0; k < 3 && s->host_points; ++k) {
                     QuadPt* out = s->pts[k].data() + s->ptr[k][p];
                     if (orders[k] == 0) {
                         // single centroid point: weight 1 for qualocation,
@@ -307,7 +308,7 @@ extern "C" int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* o) {
         o->n_charges = int(s->charge_inst.size());
         o->n_qual = int(s->pts[0].size());
         o->n_quad = int(s->pts[1].size());
-        o->n_galerkin = int(s->pts[2].size());
+        o->n_galerkin = s->ptr[2].empty() ? 0 : s->ptr[2].back();
         o->n_leaves = int(s->leaf_ptr.size()) - 1;
         o->n_near_cols = int(s->nb_idx.size());
         long long pairs = 0;
@@ -319,6 +320,10 @@ extern "C" int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* o) {
         o->cube_centre[1] = s->cube_centre.y;
         o->cube_centre[2] = s->cube_centre.z;
         o->cube_edge = s->cube_edge;
+        for (const Instance& in : s->inst) {
+            o->n_triangles += int(in.mesh->mesh.tris.size());
+            for (const auto& vt : in.mesh->mesh.vert_tris) o->n_vertex_triangles += int(vt.size());
+        }
     });
 }
 
@@ -341,6 +346,8 @@ extern "C" int beep_scene_points(const beep_scene* s, int kind, int* ptr, double
     return guarded([&] {
         need_built(s);
         if (kind < 0 || kind > 2) throw std::runtime_error("point kind must be 0, 1 or 2");
+        if (!s->host_points && pts)
+            throw std::runtime_error("this scene leaves its point sets to the device");
         if (ptr) std::memcpy(ptr, s->ptr[kind].data(), s->ptr[kind].size() * sizeof(int));
         if (pts)
             for (size_t i = 0; i < s->pts[kind].size(); ++i) {
@@ -350,6 +357,47 @@ extern "C" int beep_scene_points(const beep_scene* s, int kind, int* ptr, double
                 d[3] = q.n.x; d[4] = q.n.y; d[5] = q.n.z;
                 d[6] = q.w;
             }
+    });
+}
+
+extern "C" int beep_scene_set_host_points(beep_scene* s, int on) {
+    return guarded([&] {
+        if (!s) throw std::runtime_error("null scene");
+        s->host_points = on != 0;
+        s->built = false;
+    });
+}
+
+extern "C" int beep_scene_surface(const beep_scene* s, double* verts3, int* tris3, int* vt_ptr,
+                                  int* vt_tri, int* vt_corner) {
+    return guarded([&] {
+        need_built(s);
+        size_t vbase = 0, tbase = 0, k = 0;
+        if (vt_ptr) vt_ptr[0] = 0;
+        for (const Instance& in : s->inst) {
+            const Mesh& m = in.mesh->mesh;
+            for (size_t v = 0; v < m.verts.size(); ++v) {
+                if (verts3) {
+                    const Vec3 q = place(in, m.verts[v]);
+                    verts3[3 * (vbase + v)] = q.x;
+                    verts3[3 * (vbase + v) + 1] = q.y;
+                    verts3[3 * (vbase + v) + 2] = q.z;
+                }
+                // triangles around the vertex in the order its points are made
+                for (int t : m.vert_tris[v]) {
+                    const auto& tr = m.tris[t];
+                    if (vt_tri) vt_tri[k] = int(tbase) + t;
+                    if (vt_corner) vt_corner[k] = tr[0] == int(v) ? 0 : (tr[1] == int(v) ? 1 : 2);
+                    ++k;
+                }
+                if (vt_ptr) vt_ptr[vbase + v + 1] = int(k);
+            }
+            if (tris3)
+                for (size_t t = 0; t < m.tris.size(); ++t)
+                    for (int c = 0; c < 3; ++c) tris3[3 * (tbase + t) + c] = int(vbase) + m.tris[t][c];
+            vbase += m.verts.size();
+            tbase += m.tris.size();
+        }
     });
 }
 

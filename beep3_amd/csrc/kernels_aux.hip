@@ -383,6 +383,87 @@ void launch_inc_int(hipStream_t s, int* p) {
     hip_check(hipGetLastError(), "k_inc_int launch");
 }
 
+// Node-patch point sets generated on the device from the surface
+// (BasicNodePatch point lists, node_patch.cpp:55-273; sub-triangle points,
+// triangle.h:387-479): patch p is vertex p, and for every triangle around
+// it (in the listed order) the two sub-triangles (centre, vertex, edge
+// midpoint) carry the points of the rule, weight w * (triangle area) / 6.
+// One thread per patch writes its run of points in the local frame (offsets
+// from the patch centroid, FP64, and the FP32 copy rounded once).
+__global__ __launch_bounds__(256) void k_gen_points(
+    int n, const double* __restrict__ verts, const int* __restrict__ tris,
+    const int* __restrict__ vt_ptr, const int* __restrict__ vt_tri,
+    const int* __restrict__ vt_corner, const Patch64* __restrict__ patches,
+    const int* __restrict__ ptr, const RuleDev rule, int per_area, Pt64* __restrict__ out64,
+    Pt32* __restrict__ out32) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const Patch64 pa = patches[p];
+    int k = ptr[p];
+    if (rule.k == 0) {
+        const double w = per_area ? 1.0 : pa.area;
+        if (out64) out64[k] = Pt64{0.0, 0.0, 0.0, w, pa.nx, pa.ny, pa.nz, 0.0};
+        if (out32)
+            out32[k] = Pt32{0.f, 0.f, 0.f, float(w), float(pa.nx), float(pa.ny), float(pa.nz), 0.f};
+        return;
+    }
+    const double wscale = per_area ? 1.0 / pa.area : 1.0;
+    for (int t = vt_ptr[p]; t < vt_ptr[p + 1]; ++t) {
+        const int tri = vt_tri[t], corner = vt_corner[t];
+        const int i1 = tris[3 * tri], i2 = tris[3 * tri + 1], i3 = tris[3 * tri + 2];
+        const double x1 = verts[3 * i1], y1 = verts[3 * i1 + 1], z1 = verts[3 * i1 + 2];
+        const double x2 = verts[3 * i2], y2 = verts[3 * i2 + 1], z2 = verts[3 * i2 + 2];
+        const double x3 = verts[3 * i3], y3 = verts[3 * i3 + 1], z3 = verts[3 * i3 + 2];
+        const double ax = x2 - x1, ay = y2 - y1, az = z2 - z1;
+        const double bx = x3 - x1, by = y3 - y1, bz = z3 - z1;
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double cn = sqrt(cx * cx + cy * cy + cz * cz);
+        const double tri_area = 0.5 * cn;
+        const double nx = cx / cn, ny = cy / cn, nz = cz / cn;
+        // barycentric (u, v, w) of the vertex, of the midpoints of its two edges
+        // and of the triangle centre; a point is w*v1 + u*v2 + v*v3
+        double o[3], m1[3], m2[3];
+        const double third = 1.0 / 3.0;
+        if (corner == 0) {
+            o[0] = 0; o[1] = 0; o[2] = 1; m1[0] = 0.5; m1[1] = 0; m1[2] = 0.5; m2[0] = 0; m2[1] = 0.5; m2[2] = 0.5;
+        } else if (corner == 1) {
+            o[0] = 1; o[1] = 0; o[2] = 0; m1[0] = 0.5; m1[1] = 0.5; m1[2] = 0; m2[0] = 0.5; m2[1] = 0; m2[2] = 0.5;
+        } else {
+            o[0] = 0; o[1] = 1; o[2] = 0; m1[0] = 0; m1[1] = 0.5; m1[2] = 0.5; m2[0] = 0.5; m2[1] = 0.5; m2[2] = 0;
+        }
+        const double c3[3] = {third, third, third};
+        for (int s = 0; s < 2; ++s) {
+            const double* P1 = s == 0 ? c3 : m2;
+            const double* P3 = s == 0 ? m1 : c3;
+            for (int j = 0; j < rule.k; ++j) {
+                const double ru = rule.u[j], rv = rule.v[j], r0 = 1.0 - ru - rv;
+                const double bu = P1[0] * r0 + o[0] * ru + P3[0] * rv;
+                const double bv = P1[1] * r0 + o[1] * ru + P3[1] * rv;
+                const double bw = P1[2] * r0 + o[2] * ru + P3[2] * rv;
+                const double px = x1 * bw + x2 * bu + x3 * bv - pa.cx;
+                const double py = y1 * bw + y2 * bu + y3 * bv - pa.cy;
+                const double pz = z1 * bw + z2 * bu + z3 * bv - pa.cz;
+                const double wt = rule.w[j] * tri_area / 6.0 * wscale;
+                if (out64) out64[k] = Pt64{px, py, pz, wt, nx, ny, nz, 0.0};
+                if (out32)
+                    out32[k] = Pt32{float(px), float(py), float(pz), float(wt), float(nx), float(ny),
+                                    float(nz), 0.f};
+                ++k;
+            }
+        }
+    }
+}
+
+void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tris,
+                       const int* vt_ptr, const int* vt_tri, const int* vt_corner,
+                       const Patch64* patches, const int* ptr, const RuleDev& rule, int per_area,
+                       Pt64* out64, Pt32* out32) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gen_points, dim3((n + 255) / 256), dim3(256), 0, s, n, verts, tris,
+                       vt_ptr, vt_tri, vt_corner, patches, ptr, rule, per_area, out64, out32);
+    hip_check(hipGetLastError(), "k_gen_points launch");
+}
+
 // Points k0 .. k0+n-1 of a CSR point set (raw: 7 doubles x, y, z, nx, ny,
 // nz, w each) into the local frame of the patch that owns them (offsets from
 // its centroid, formed in FP64 and rounded once for the FP32 copy).

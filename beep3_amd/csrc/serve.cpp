@@ -106,6 +106,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     beep_result& r = d.r;
     beep_scene* sc = j.scene;
     const double th = clock();
+    sc->host_points = false;  // the device generates the point sets
     if (beep_scene_build(sc) != 0)
         throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
     r.host_ms = (clock() - th) * 1e3;
@@ -138,20 +139,32 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
             throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
     }
     check_hip(ctx, beep_hip_set_far_field(ctx, o.far_mode, o.fmm_order, o.fmm_leaf), "far field");
-    beep_hip_geometry g{};
+    // the surface: placed vertices, triangles, triangles around each vertex
+    beep_scene_sizes sz{};
+    if (beep_scene_get_sizes(sc, &sz) != 0)
+        throw std::runtime_error(std::string("scene sizes: ") + beep_host_last_error());
+    std::vector<double> verts(size_t(3) * N);
+    std::vector<int> tris(size_t(3) * sz.n_triangles), vt_ptr(size_t(N) + 1),
+        vt_tri(size_t(sz.n_vertex_triangles)), vt_corner(size_t(sz.n_vertex_triangles));
+    if (beep_scene_surface(sc, verts.data(), tris.data(), vt_ptr.data(), vt_tri.data(),
+                           vt_corner.data()) != 0)
+        throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
+    beep_hip_surface g{};
     g.n_patches = N;
     g.centroid = &sc->centroid[0].x;
     g.normal = &sc->normal[0].x;
     g.area = sc->area.data();
     g.eps_ratio = sc->eps.data();
     g.gc = sc->gc.data();
-    g.qual_ptr = sc->ptr[0].data();
-    g.qual_pts = &sc->pts[0][0].p.x;
-    g.quad_ptr = sc->ptr[1].data();
-    g.quad_pts = &sc->pts[1][0].p.x;
-    g.gal_ptr = sc->ptr[2].data();
-    g.gal_pts = &sc->pts[2][0].p.x;
-    check_hip(ctx, beep_hip_set_geometry(ctx, &g), "geometry");
+    g.verts = verts.data();
+    g.n_tris = sz.n_triangles;
+    g.tris = tris.data();
+    g.vt_ptr = vt_ptr.data();
+    g.vt_tri = vt_tri.data();
+    g.vt_corner = vt_corner.data();
+    g.qual_rule = sc->qual;
+    g.quad_rule = sc->quad;
+    check_hip(ctx, beep_hip_set_surface(ctx, &g), "geometry");
     beep_hip_tree t{};
     t.n_leaves = int(sc->leaf_ptr.size()) - 1;
     t.leaf_ptr = sc->leaf_ptr.data();
@@ -159,10 +172,15 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     t.nb_ptr = sc->nb_ptr.data();
     t.nb_idx = sc->nb_idx.data();
     check_hip(ctx, beep_hip_set_tree(ctx, &t), "tree");
+    const double tp = clock();
+    r.upload_ms = (tp - td) * 1e3;
     if (o.solve_mode != BEEP_SOLVE_FP64)
         check_hip(ctx, beep_hip_precompute(ctx, sc->kappa, BEEP_FP32), "precompute");
     else
         check_hip(ctx, beep_hip_precompute(ctx, sc->kappa, BEEP_FP64), "precompute");
+    check_hip(ctx, beep_hip_synchronize(ctx), "precompute");
+    const double ts = clock();
+    r.precompute_ms = (ts - tp) * 1e3;
     std::vector<double> b(size_t(2) * N), x(size_t(2) * N, 0.0);
     const int nch = int(sc->charge_inst.size());
     check_hip(ctx, beep_hip_rhs(ctx, nch, nch ? sc->charges.data() : nullptr, sc->Dsolvent,
@@ -210,6 +228,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     check_hip(ctx, beep_hip_memory_held(ctx, &hb), "memory");
     r.bytes = double(hb);
     r.device_ms = (clock() - td) * 1e3;
+    r.solve_ms = (clock() - ts) * 1e3;
     {
         // charge the slot what its context really holds
         std::lock_guard<std::mutex> lk(mu);

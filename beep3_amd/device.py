@@ -5,8 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (BeepError, FarInfo, HipGeometry, HipTree, MatvecTiming, SolveOpts, SolveStats,
-                   check_hip, dp, ip, lib, vp)
+from ._lib import (BeepError, FarInfo, HipGeometry, HipSurface, HipTree, MatvecTiming,
+                   SolveOpts, SolveStats, check_hip, dp, ip, lib, vp)
 
 FP64, FP32, FP32_RESID = 0, 1, 2
 SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
@@ -58,6 +58,22 @@ class Device:
         g.gal_ptr, g.gal_pts = keep["gal_ptr"].ctypes.data, keep["gal"].ctypes.data
         self._check(lib().beep_hip_set_geometry(self._h, C.byref(g)))
 
+    def set_surface(self, a: dict, surf: dict, qual_rule: int, quad_rule: int):
+        """Geometry from the surface: the device generates the point sets
+        (a: patch arrays of Scene.arrays(), surf: Scene.surface())."""
+        self.n = n = int(a["area"].shape[0])
+        keep = {k: np.ascontiguousarray(a[k]) for k in ("centroid", "normal", "area", "eps", "gc")}
+        keep.update({k: np.ascontiguousarray(surf[k]) for k in surf})
+        g = HipSurface()
+        g.n_patches = n
+        g.centroid, g.normal = keep["centroid"].ctypes.data, keep["normal"].ctypes.data
+        g.area, g.eps_ratio, g.gc = keep["area"].ctypes.data, keep["eps"].ctypes.data, keep["gc"].ctypes.data
+        g.verts, g.n_tris, g.tris = keep["verts"].ctypes.data, int(keep["tris"].shape[0]), keep["tris"].ctypes.data
+        g.vt_ptr, g.vt_tri, g.vt_corner = (keep["vt_ptr"].ctypes.data, keep["vt_tri"].ctypes.data,
+                                           keep["vt_corner"].ctypes.data)
+        g.qual_rule, g.quad_rule = int(qual_rule), int(quad_rule)
+        self._check(lib().beep_hip_set_surface(self._h, C.byref(g)))
+
     def set_tree(self, a: dict):
         keep = {k: np.ascontiguousarray(a[k], dtype=np.int32) for k in (
             "leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx")}
@@ -92,6 +108,20 @@ class Device:
             self._h, _prec(precision), c.ctypes.data_as(C.POINTER(C.c_int)), len(c), int(reps),
             ms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(best)))
         return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
+    def autotune_spmv(self, precision="fp64", candidates=(0, 1, 2, 3, 4, 6), reps=3):
+        """Time whole mat-vecs for each near-SpMV residency cap (blocks per
+        CU, 0 = uncapped), keep the fastest; returns (best, {cap: ms})."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int()
+        self._check(lib().beep_hip_autotune_spmv(
+            self._h, _prec(precision), c.ctypes.data_as(C.POINTER(C.c_int)), len(c), int(reps),
+            ms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(best)))
+        return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
+    def set_spmv_residency(self, blocks_per_cu: int):
+        self._check(lib().beep_hip_set_spmv_residency(self._h, int(blocks_per_cu)))
 
     def far_info(self) -> dict:
         f = FarInfo()

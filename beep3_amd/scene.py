@@ -98,10 +98,13 @@ class Mesh:
 class Scene:
     """Solver parameters + rigid mesh instances + octree."""
 
-    def __init__(self, Dsolvent=80.0, kappa=0.0, quad=0, qual=4, nbsize=1500):
+    def __init__(self, Dsolvent=80.0, kappa=0.0, quad=0, qual=4, nbsize=1500, host_points=True):
         h = vp()
         check_host(lib().beep_scene_create(Dsolvent, kappa, quad, qual, nbsize, C.byref(h)))
         self._h = h
+        self.host_points = bool(host_points)
+        if not self.host_points:
+            check_host(lib().beep_scene_set_host_points(h, 0))
         self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize = Dsolvent, kappa, quad, qual, nbsize
         self._meshes = []
         self._arrays = None
@@ -140,8 +143,9 @@ class Scene:
                                             _ip(a["instance"])))
         for kind, name, n in ((0, "qual", s.n_qual), (1, "quad", s.n_quad), (2, "gal", s.n_galerkin)):
             ptr = np.zeros(N + 1, dtype=np.int32)
-            pts = np.zeros((n, 7))
-            check_host(lib().beep_scene_points(self._h, kind, _ip(ptr), _dp(pts)))
+            pts = np.zeros((n if self.host_points else 0, 7))
+            check_host(lib().beep_scene_points(self._h, kind, _ip(ptr),
+                                               _dp(pts) if self.host_points else None))
             a[name + "_ptr"], a[name] = ptr, pts
         L = s.n_leaves
         a["leaf_ptr"] = np.zeros(L + 1, dtype=np.int32)
@@ -160,6 +164,19 @@ class Scene:
                                               _ip(a["inst_charge_ptr"])))
         self._arrays = a
         return a
+
+    def surface(self) -> dict:
+        """Placed vertices, triangles and the triangles around every vertex
+        (with the vertex's corner in each), as the device generates points."""
+        s = self.sizes()
+        out = {"verts": np.zeros((s.n_patches, 3)), "tris": np.zeros((s.n_triangles, 3), np.int32),
+               "vt_ptr": np.zeros(s.n_patches + 1, np.int32),
+               "vt_tri": np.zeros(s.n_vertex_triangles, np.int32),
+               "vt_corner": np.zeros(s.n_vertex_triangles, np.int32)}
+        check_host(lib().beep_scene_surface(self._h, _dp(out["verts"]), _ip(out["tris"]),
+                                            _ip(out["vt_ptr"]), _ip(out["vt_tri"]),
+                                            _ip(out["vt_corner"])))
+        return out
 
     def __del__(self):
         try:

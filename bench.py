@@ -170,10 +170,15 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
     for r in ok:
         by_level.setdefault(r["n_patches"], []).append(r["latency_s"] * 1e3)
     out = {"requests": len(sched), "solved": len(ok), "failed": len(res) - len(ok),
+           "failures": sorted({r["message"] for r in res if r["status"] != 0}),
            "slots": slots, "arrival_rate_per_s": rate, "span_s": span,
            "solves_per_s": len(ok) / span, "latency_ms_p50": float(np.percentile(lat, 50)),
            "latency_ms_p99": float(np.percentile(lat, 99)),
            "latency_ms_p50_by_patches": {str(k): float(np.percentile(v, 50)) for k, v in sorted(by_level.items())},
+           "phase_ms_median_by_patches": {
+               str(k): {f: float(np.median([r[f] for r in ok if r["n_patches"] == k]))
+                        for f in ("host_ms", "upload_ms", "precompute_ms", "solve_ms")}
+               for k in sorted(by_level)},
            "host_ms_mean": float(np.mean([r["host_ms"] for r in ok])),
            "device_ms_mean": float(np.mean([r["device_ms"] for r in ok])),
            "max_device_bytes_per_slot": float(max(r["bytes"] for r in ok)),
@@ -231,8 +236,11 @@ def pose_bench(n_poses, slots, seq_poses):
            "latency_ms_p50": float(np.percentile(lat, 50)),
            "latency_ms_p99": float(np.percentile(lat, 99)),
            "iterations_mean": float(np.mean([r["iterations"] for r in ok])),
+           "phase_ms_median": {f: float(np.median([r[f] for r in ok]))
+                               for f in ("host_ms", "upload_ms", "precompute_ms", "solve_ms")},
            "unconverged": sum(1 for r in ok if not r["converged"]),
-           "energy_kJ_mol_range": [float(E.min()), float(E.max())]}
+           "energy_kJ_mol_range": [float(E.min()), float(E.max())],
+           "failures": sorted({r["message"] for r in res if r["status"] != 0})}
     print(f"[bench] poses: {out}", file=sys.stderr, flush=True)
     seq = configs.pose_scenes(mesh, seq_poses)
     srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=40)
@@ -295,6 +303,9 @@ def main():
         best, table = dev.autotune_leaf("fp64", TUNE_LEAVES, reps=3)
         tune = {"leaf_ms_fp64": table, "chosen_leaf": best}
         args.leaf = best
+        # near-SpMV residency cap (room for the concurrent far field)
+        cap, ctable = dev.autotune_spmv("fp64", (0, 1, 2, 3, 4, 6), reps=3)
+        tune.update(spmv_blocks_per_cu_ms_fp64=ctable, chosen_spmv_blocks_per_cu=cap)
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
 
     # precompute timings (integrand throughput)
@@ -377,11 +388,14 @@ def main():
 
     if not args.no_solve:
         solves = {}
+        for mode in ("fp64", "mixed"):  # untimed: GMRES buffers and step graphs
+            b.solve(1e-6, 100, mode=mode)
         for mode in ("fp64", "mixed"):
             b.solve(1e-6, 100, mode=mode)
             E = b.calculate_energies()
             ri = dict(b.run_info)
-            solves[mode] = {"solve_ms": ri["rhs_ms"] + ri["gmres_ms"], "iterations": ri["iterations"],
+            solves[mode] = {"solve_ms": ri["rhs_ms"] + ri["gmres_ms"], "rhs_ms": ri["rhs_ms"],
+                            "gmres_ms": ri["gmres_ms"], "iterations": ri["iterations"],
                             "matvecs": ri["matvecs"], "refinements": ri["refinements"],
                             "rel_residual": ri["rel_residual"], "energy_kJ_mol": E}
             print(f"[bench] solve {mode}: {solves[mode]}", file=sys.stderr, flush=True)

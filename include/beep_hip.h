@@ -78,6 +78,29 @@ typedef struct {
     const double* gal_pts;
 } beep_hip_geometry;
 
+/* The same node patches given by their surface: the device generates the
+ * qualocation, quadrature and galerkin point sets itself (the sub-triangle
+ * points of node_patch.cpp:55-273 / triangle.h:387-479) instead of
+ * receiving them.  Patch p is vertex p; for every triangle around it, in the
+ * order of vt_tri, the two sub-triangles (centre, vertex, edge midpoint)
+ * carry the points of the rule (0 = one centroid point per patch). */
+typedef struct {
+    int n_patches;
+    const double* centroid;  /* 3N */
+    const double* normal;    /* 3N, unit */
+    const double* area;      /* N */
+    const double* eps_ratio; /* N */
+    const double* gc;        /* N */
+    const double* verts;     /* 3N: vertex positions, universe frame */
+    int n_tris;
+    const int* tris;         /* 3 n_tris: vertex indices (outward orientation) */
+    const int* vt_ptr;       /* N + 1 */
+    const int* vt_tri;       /* triangles around each vertex */
+    const int* vt_corner;    /* corner 0, 1, 2 of the vertex in each of them */
+    int qual_rule;           /* rule order of the qualocation points */
+    int quad_rule;           /* rule order of the quadrature points */
+} beep_hip_surface;
+
 /* Explicit-pair structure from the octree: every leaf's rows interact
  * explicitly with the columns of its 27-box neighbourhood
  * (octree.h:627-648, beep.cpp:859-950). */
@@ -132,6 +155,9 @@ int beep_hip_destroy(beep_hip_ctx* ctx);
 
 int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* geom);
 int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
+/* geometry from the surface (point sets generated on the device); replaces
+ * beep_hip_set_geometry, after which the tree is set as usual */
+int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* surface);
 
 /* Far-field engine for the 1-point interaction of all pairs (the role of
  * FMM_BEM_Octree::solve, fmm_bem_hybrid.h:184-279):
@@ -170,6 +196,17 @@ int beep_hip_set_graphs(beep_hip_ctx* ctx, int on);
  * precision; switches the far field to BEEP_FAR_FMM.  No counterpart in the
  * reference, whose leaf size follows from nbsize (beep.h:67-74). */
 int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_targets, int n,
+                           int reps, double* ms_out, int* best);
+
+/* Residency cap of the near-field SpMV: at most blocks_per_cu blocks per
+ * compute unit walk the work items (0: one block per work item, the
+ * default).  A cap leaves room on every CU for the far-field kernels that
+ * run on the other stream at the same time. */
+int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu);
+/* Autotuner for that cap: times `reps` whole mat-vecs of the given precision
+ * (far field overlapped) for each candidate, writes the mean times to ms_out
+ * (may be NULL), keeps the fastest and returns it in *best. */
+int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
                            int reps, double* ms_out, int* best);
 
 /* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or

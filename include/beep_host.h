@@ -76,6 +76,8 @@ typedef struct {
     int tree_depth;    /* bottom level of the octree */
     double cube_centre[3];
     double cube_edge;
+    int n_triangles;         /* surface triangles over all instances */
+    int n_vertex_triangles;  /* sum over vertices of the triangles around them */
 } beep_scene_sizes;
 int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* out);
 
@@ -86,6 +88,17 @@ int beep_scene_patches(const beep_scene* s, double* centroid3, double* normal3, 
 /* kind 0 = qualocation, 1 = quadrature, 2 = galerkin;
    ptr: N+1, pts: n x 7 (x, y, z, nx, ny, nz, w) */
 int beep_scene_points(const beep_scene* s, int kind, int* ptr, double* pts);
+/* Leave the point sets to the device (on = 0): build() then skips them and
+ * beep_scene_points returns only the pointer arrays; the device makes the
+ * points from beep_scene_surface (beep_hip_set_surface).  Default on. */
+int beep_scene_set_host_points(beep_scene* s, int on);
+/* The surface in universe coordinates: placed vertices (3 x n_patches: patch
+ * p is vertex p), triangles (3 x n_triangles, vertex indices), and for every
+ * vertex the triangles around it (vt_ptr: n_patches + 1, vt_tri and the
+ * corner 0/1/2 of the vertex in each: n_vertex_triangles), in the order the
+ * patch's points are generated (node_patch.cpp:55-273). */
+int beep_scene_surface(const beep_scene* s, double* verts3, int* tris3, int* vt_ptr, int* vt_tri,
+                       int* vt_corner);
 /* leaf_ptr: L+1, leaf_idx: N, nb_ptr: L+1, nb_idx: n_near_cols */
 int beep_scene_tree(const beep_scene* s, int* leaf_ptr, int* leaf_idx, int* nb_ptr, int* nb_idx);
 /* posed charges: xyzq n x 4, charge_instance n */

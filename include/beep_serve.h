@@ -64,6 +64,9 @@ typedef struct {
     double t_done;
     double host_ms;         /* scene build (patches, point sets, octree) */
     double device_ms;       /* upload, precompute, RHS, solve, energy */
+    double upload_ms;       /* of which: geometry, tree and far-field plan */
+    double precompute_ms;   /*           singular terms and near coefficients */
+    double solve_ms;        /*           RHS, GMRES, energy */
     double bytes;           /* device bytes held by the slot after the request */
     int slot;
     char message[128];

@@ -107,3 +107,20 @@ def test_leaf_autotuner(off_centre_arrays):
     assert d.far_info()["mode"] == 1
     assert rel(d.matvec(x, "fp32"), y_dir) < 1e-4
     d.close()
+
+
+def test_spmv_residency_cap_is_bit_identical(off_centre_arrays):
+    """Capping the near-SpMV blocks per CU changes only which block handles
+    which work item: y is bit-identical, and the tuner keeps the fastest."""
+    a, kappa = off_centre_arrays
+    x = np.random.default_rng(8).normal(size=2 * a["area"].shape[0])
+    d = _device(a, kappa, "fmm", 5, 40, precs=("fp64", "fp32"))
+    ref = {p: d.matvec(x, p) for p in ("fp64", "fp32")}
+    for cap in (1, 3):
+        d.set_spmv_residency(cap)
+        for p in ("fp64", "fp32"):
+            assert np.array_equal(d.matvec(x, p), ref[p]), (cap, p)
+    best, table = d.autotune_spmv("fp32", (0, 2), reps=2)
+    print("SpMV residency autotune (ms per FP32 mat-vec):", table, "best", best)
+    assert table[best] == min(table.values())
+    d.close()

@@ -126,3 +126,39 @@ def test_off_centre_energy_vs_kirkwood(off_centre_case):
     print("E", E, "Kirkwood", ref, "rel", abs(E - ref) / abs(ref))
     assert abs(E - ref) / abs(ref) < 5e-3
     b.close()
+
+
+def test_device_generated_points_match_host_points():
+    """Point sets generated on the device from the surface give the operator
+    of the host-built point sets (a posed two-instance scene): FP64 mat-vecs
+    agree to rounding, FP32 to FP32 rounding, energy coefficients to rounding."""
+    from beep3_amd import Mesh
+    from beep3_amd.scene import Scene
+
+    m = Mesh.star(4, 25.0, 8, 0.04, 5).charges_walk(100, 3.8, 2.0, 0.4, 5)
+    sc = Scene(80.0, 0.125, 6, 0, 1500)
+    sc.add_instance(m, 2.0, offset=[0.0, 0.0, 0.0])
+    sc.add_instance(m, 2.0, quat=[0.6, -0.2, 0.7, 0.3], offset=[52.0, -4.0, 6.0])
+    sc.build()
+    a, surf = sc.arrays(), sc.surface()
+    devs = []
+    for mode in ("host", "device"):
+        d = Device(0)
+        d.set_far_field("direct")
+        if mode == "host":
+            d.set_geometry(a)
+        else:
+            d.set_surface(a, surf, 0, 6)
+        d.set_tree(a)
+        d.precompute(0.125, "fp64")
+        d.precompute(0.125, "fp32")
+        devs.append(d)
+    x = np.random.default_rng(11).normal(size=2 * a["area"].shape[0])
+    e64 = rel(devs[1].matvec(x, "fp64"), devs[0].matvec(x, "fp64"))
+    e32 = rel(devs[1].matvec(x, "fp32"), devs[0].matvec(x, "fp32"))
+    ef = [d.energy_coefficients(a["inst_patch_ptr"], a["inst_charge_ptr"], a["charges"]) for d in devs]
+    ee = max(rel(ef[1][k], ef[0][k]) for k in range(2))
+    print(f"device vs host points: fp64 {e64:.2e}, fp32 {e32:.2e}, energy coefficients {ee:.2e}")
+    assert e64 < 1e-13 and e32 < 1e-6 and ee < 1e-13
+    for d in devs:
+        d.close()

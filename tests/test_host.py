@@ -219,3 +219,38 @@ def test_bad_mesh_file(tmp_path):
     ply.write_text("ply\n")
     with pytest.raises(BeepError, match="off or .gts"):
         Mesh.read(str(ply))
+
+
+def test_surface_export_matches_the_host_point_sets():
+    """The surface handed to the device (placed vertices, triangles, the
+    triangles around each vertex with its corner) regenerates, through the
+    numpy restatement, the point sets the host builds for a posed two-instance
+    scene; a scene left to device points exports the same surface."""
+    m = Mesh.star(2, 25.0, 8, 0.04, 3).charges_walk(20, 3.8, 2.0, 0.4, 3)
+    q = np.array([0.8, 0.2, -0.3, 0.4])
+    scenes = []
+    for host in (True, False):
+        sc = Scene(80.0, 0.125, 6, 0, 1500, host_points=host)
+        sc.add_instance(m, 2.0, offset=[0.0, 0.0, 0.0]).add_instance(m, 2.0, quat=q,
+                                                                   offset=[55.0, 3.0, -2.0])
+        scenes.append(sc.build())
+    a, surf = scenes[0].arrays(), scenes[0].surface()
+    surf2 = scenes[1].surface()
+    for k in surf:
+        assert np.array_equal(surf[k], surf2[k]), k
+    N = a["area"].shape[0]
+    xyz, tri = surf["verts"], surf["tris"]
+    assert xyz.shape == (N, 3) and surf["vt_ptr"][-1] == surf["vt_tri"].shape[0]
+    for v in range(N):
+        ts = surf["vt_tri"][surf["vt_ptr"][v]:surf["vt_ptr"][v + 1]]
+        cs = surf["vt_corner"][surf["vt_ptr"][v]:surf["vt_ptr"][v + 1]]
+        assert np.all(tri[ts, cs] == v)
+        assert sorted(ts) == sorted(np.nonzero(np.any(tri == v, axis=1))[0])
+    for v in range(0, N, 7):
+        mine = a["quad"][a["quad_ptr"][v]:a["quad_ptr"][v + 1]]
+        ref = np.array([np.concatenate([p, n, [w]]) for p, n, w in dual_points(xyz, tri, v, R6)])
+        key = lambda arr: np.lexsort(np.round(arr[:, :3], 8).T)
+        assert np.allclose(mine[key(mine)], ref[key(ref)], atol=1e-11)
+    # the device-point scene exports the pointer arrays but no points
+    b = scenes[1].arrays()
+    assert np.array_equal(b["quad_ptr"], a["quad_ptr"]) and b["quad"].shape[0] == 0
