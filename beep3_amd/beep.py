@@ -98,6 +98,14 @@ class BEEP:
                              precalc_ms=(t3 - t2) * 1e3)
         self._ready = True
 
+    def host_scene(self) -> Scene:
+        """The same scene with its point sets built on the host (for CPU
+        checks of the operator; the device path generates its own)."""
+        sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize, host_points=True)
+        for lib_id, off, rot, Dp in self.instances:
+            sc.add_instance(self.library[lib_id], Dp, quat=rot, offset=off)
+        return sc.build()
+
     def _need(self):
         if not self._ready:
             self.init()

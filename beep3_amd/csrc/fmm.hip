@@ -293,19 +293,43 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
         for (int e = 0; e < 3; ++e)
 #pragma unroll
             for (int c = 0; c < CT; ++c) acc[k][e][c] = acc_t{0, 0, 0, 0};
-    for (int m0 = 0; m0 < NPP; m0 += MS) {
-        __syncthreads();  // pair lists written; previous slab consumed
-        // B[e][g][mm] = M_src(g)[e][psi_g(m0 + mm)]: the source expansion in
-        // the canonical node order of its pair; consecutive threads walk mm
-        // of one (g, e), a run inside one expansion
-        for (int q = t; q < 3 * G * MS; q += 512) {
-            const int g = q / (3 * MS), rem = q - g * 3 * MS, e = rem / MS, mm = rem - e * MS;
-            const int m = m0 + mm;
-            T v = T(0);
-            if (g < np && m < NP) v = M[size_t(psrc[g]) * 3 * NP + e * NP + pperm[g][m]];
-            sb[(e * G + g) * MSP + mm] = v;
+    // B slabs: B[e][g][mm] = M_src(g)[e][psi_g(m0 + mm)], the source
+    // expansions in the canonical node order of their pairs.  Thread t owns
+    // the same (g, e, mm) entries of every slab (consecutive threads walk mm
+    // of one (g, e), a run inside one expansion); the next slab's values are
+    // loaded into registers while the current slab feeds the matrix cores.
+    constexpr int NQ = (3 * G * MS + 511) / 512;  // entries per thread and slab
+    constexpr int NSLAB = (NPP + MS - 1) / MS;
+    __syncthreads();  // pair lists written
+    int qsrc[NQ], qmm[NQ];
+    const unsigned short* qperm[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int q = t + 512 * i;
+        const int g = q / (3 * MS), rem = q - g * 3 * MS, e = rem / MS, mm = rem - e * MS;
+        const bool ok = q < 3 * G * MS && g < np;
+        qsrc[i] = ok ? psrc[g] * 3 * NP + e * NP : -1;
+        qperm[i] = ok ? pperm[g] : nullptr;
+        qmm[i] = q < 3 * G * MS ? (e * G + g) * MSP + mm : -1;  // LDS slot
+    }
+    auto fetch = [&](int m0, T* v) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int q = t + 512 * i;
+            const int mm = q % MS, m = m0 + mm;
+            v[i] = (qsrc[i] >= 0 && m < NP) ? M[size_t(qsrc[i]) + qperm[i][m]] : T(0);
         }
+    };
+    T nxt[NQ];
+    fetch(0, nxt);
+    for (int sl = 0; sl < NSLAB; ++sl) {
+        const int m0 = sl * MS;
+        __syncthreads();  // previous slab consumed
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            if (qmm[i] >= 0) sb[qmm[i]] = nxt[i];
         __syncthreads();
+        if (sl + 1 < NSLAB) fetch(m0 + MS, nxt);  // in flight during the MFMAs below
 #pragma unroll 2
         for (int k0 = 0; k0 < MS; k0 += 4) {
             const int m = m0 + k0 + kl16;  // K-dimension index (canonical source node)

@@ -94,7 +94,7 @@ def cpu_baseline(b, seconds=15.0):
     """Oracle (oracle/oracle.c, FP64, OpenMP) on a sample of rows of the same
     workload; rate scaled to whole mat-vecs."""
     import oracle
-    a = b.scene.arrays()
+    a = b.host_scene().arrays()
     o = oracle.Oracle(a, b.kappa, b.Dsolvent)
     N = o.n
     rng = np.random.default_rng(5)
@@ -390,10 +390,14 @@ def main():
         solves = {}
         for mode in ("fp64", "mixed"):  # untimed: GMRES buffers and step graphs
             b.solve(1e-6, 100, mode=mode)
-        for mode in ("fp64", "mixed"):
-            b.solve(1e-6, 100, mode=mode)
+        fh = {}
+        # mixed: inner FP32 GMRES to 1e-3 (default) and tighter inner tolerances
+        for mode, kw in (("fp64", {}), ("mixed", {}), ("mixed_inner1e-4", {"inner_tol": 1e-4}),
+                         ("mixed_inner5e-7", {"inner_tol": 5e-7})):
+            b.solve(1e-6, 100, mode=mode.split("_")[0], **kw)
             E = b.calculate_energies()
             ri = dict(b.run_info)
+            fh[mode] = (b.f.copy(), b.h.copy())
             solves[mode] = {"solve_ms": ri["rhs_ms"] + ri["gmres_ms"], "rhs_ms": ri["rhs_ms"],
                             "gmres_ms": ri["gmres_ms"], "iterations": ri["iterations"],
                             "matvecs": ri["matvecs"], "refinements": ri["refinements"],
@@ -412,13 +416,19 @@ def main():
             dev.set_far_field("direct")
             b.solve(1e-8, 100, mode="fp64")
             E_ref = b.calculate_energies()
+            f_ref, h_ref = b.f.copy(), b.h.copy()
             dev.set_far_field(args.far, args.order, args.leaf)
         else:
             E_ref = solves["fp64"]["energy_kJ_mol"]
+            f_ref, h_ref = fh["fp64"]
         solves["reference_energy_kJ_mol"] = E_ref
-        for mode in ("fp64", "mixed"):
+        for mode in fh:
             solves[mode]["energy_rel_err_vs_reference"] = abs(
                 solves[mode]["energy_kJ_mol"] - E_ref) / abs(E_ref)
+            solves[mode]["f_rel_l2_err_vs_reference"] = float(
+                np.linalg.norm(fh[mode][0] - f_ref) / np.linalg.norm(f_ref))
+            solves[mode]["h_rel_l2_err_vs_reference"] = float(
+                np.linalg.norm(fh[mode][1] - h_ref) / np.linalg.norm(h_ref))
         extra["solve"] = solves
 
     dev.free(d_x)

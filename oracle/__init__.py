@@ -99,6 +99,10 @@ class Oracle:
         self.Dsolvent = float(Dsolvent)
         a = self.a
         self.n = int(a["area"].shape[0])
+        for kind in ("qual", "quad", "gal"):
+            if a[kind].shape[0] != int(a[kind + "_ptr"][-1]):
+                raise ValueError(f"the oracle needs the host point sets ({kind} is missing: "
+                                 "build the scene with host points)")
         g = OrcGeom()
         g.n = self.n
         for name in ("centroid", "normal", "area", "eps", "gc", "qual_ptr", "qual", "quad_ptr",

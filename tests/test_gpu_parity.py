@@ -69,8 +69,7 @@ def test_solve_born_ion():
         assert abs(E - born) / abs(born) < 1e-2
         if K is None:
             import oracle as O
-            sc = b.scene
-            o = O.Oracle(sc.arrays(), b.kappa, b.Dsolvent)
+            o = O.Oracle(b.host_scene().arrays(), b.kappa, b.Dsolvent)
             K = o.assemble()
             x_ref = np.linalg.solve(K, o.rhs())
             E_ref = o.energies(x_ref).sum()
