@@ -1061,7 +1061,10 @@ int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_sol
             stats.residual = est;
             stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
         } else if (o->mode == BEEP_SOLVE_MIXED) {
-            require(o->max_refine > 0 && o->inner_tol > 0, "bad refinement options");
+            require(o->max_refine > 0, "bad refinement options");
+            // inner FP32 GMRES tolerance; <= 0: half the outer one (one
+            // refinement step then usually suffices)
+            const double inner_tol = o->inner_tol > 0 ? o->inner_tol : 0.5 * o->tol;
             bool done = false;
             double rn = 0;
             for (int k = 0; k <= o->max_refine; ++k) {
@@ -1085,7 +1088,7 @@ int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_sol
                 launch_fill(s, n, 0.0, ctx->dvec.p);
                 bool conv = false;
                 double est = 0;
-                stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, o->inner_tol,
+                stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
                                           o->max_iter, &matvecs, &conv, &est, true);
                 launch_axpby(s, n, 1.0, ctx->dvec.p, 1.0, ctx->xbuf.p);
                 stats.refinements = k + 1;

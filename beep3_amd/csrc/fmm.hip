@@ -322,42 +322,55 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     };
     T nxt[NQ];
     fetch(0, nxt);
-    for (int sl = 0; sl < NSLAB; ++sl) {
-        const int m0 = sl * MS;
-        __syncthreads();  // previous slab consumed
+    // A operands (kernel table values) one k-step ahead: the loads of step
+    // k+1 are in flight while the MFMAs of step k run
+    constexpr int NSTEP = NPP / 4, SPS = MS / 4;  // k-steps, k-steps per slab
+    T xa_n[NTW], ya_n[NTW];
+    auto load_a = [&](int step, T* xa, T* ya) {
+        const int m = 4 * step + kl16;  // K-dimension index (canonical source node)
 #pragma unroll
-        for (int i = 0; i < NQ; ++i)
-            if (qmm[i] >= 0) sb[qmm[i]] = nxt[i];
-        __syncthreads();
-        if (sl + 1 < NSLAB) fetch(m0 + MS, nxt);  // in flight during the MFMAs below
-#pragma unroll 2
-        for (int k0 = 0; k0 < MS; k0 += 4) {
-            const int m = m0 + k0 + kl16;  // K-dimension index (canonical source node)
-            T bL[CT], b1[CT], b2[CT];
+        for (int k = 0; k < NTW; ++k) {
+            const int n = (w + M2L_WAVES * k) * 16 + il;  // output row (canonical target node)
+            const bool ok = m < NP && n < NP;
+            xa[k] = ok ? KL[m * NP + n] : T(0);
+            ya[k] = ok ? KY[m * NP + n] : T(0);
+        }
+    };
+    load_a(0, xa_n, ya_n);
+    for (int step = 0; step < NSTEP; ++step) {
+        const int k0 = (step % SPS) * 4;
+        if (k0 == 0) {
+            const int sl = step / SPS;
+            __syncthreads();  // previous slab consumed
+#pragma unroll
+            for (int i = 0; i < NQ; ++i)
+                if (qmm[i] >= 0) sb[qmm[i]] = nxt[i];
+            __syncthreads();
+            if (sl + 1 < NSLAB) fetch((sl + 1) * MS, nxt);  // in flight during the MFMAs
+        }
+        T xa[NTW], ya[NTW];
+#pragma unroll
+        for (int k = 0; k < NTW; ++k) {
+            xa[k] = xa_n[k];
+            ya[k] = ya_n[k];
+        }
+        if (step + 1 < NSTEP) load_a(step + 1, xa_n, ya_n);
+        T bL[CT], b1[CT], b2[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int g = c * 16 + il;
+            bL[c] = sb[(0 * G + g) * MSP + k0 + kl16];
+            b1[c] = sb[(1 * G + g) * MSP + k0 + kl16];
+            b2[c] = sb[(2 * G + g) * MSP + k0 + kl16];
+        }
+#pragma unroll
+        for (int k = 0; k < NTW; ++k) {
+            if (w + M2L_WAVES * k >= NTILE) continue;
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
-                const int g = c * 16 + il;
-                bL[c] = sb[(0 * G + g) * MSP + k0 + kl16];
-                b1[c] = sb[(1 * G + g) * MSP + k0 + kl16];
-                b2[c] = sb[(2 * G + g) * MSP + k0 + kl16];
-            }
-#pragma unroll
-            for (int k = 0; k < NTW; ++k) {
-                const int nt = w + M2L_WAVES * k;
-                if (nt < NTILE) {
-                    const int n = nt * 16 + il;  // output row (canonical target node)
-                    T xa = T(0), ya = T(0);
-                    if (m < NP && n < NP) {
-                        xa = KL[m * NP + n];
-                        ya = KY[m * NP + n];
-                    }
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) {
-                        acc[k][0][c] = Mfma<T>::op(xa, bL[c], acc[k][0][c]);
-                        acc[k][1][c] = Mfma<T>::op(ya, b1[c], acc[k][1][c]);
-                        acc[k][2][c] = Mfma<T>::op(ya, b2[c], acc[k][2][c]);
-                    }
-                }
+                acc[k][0][c] = Mfma<T>::op(xa[k], bL[c], acc[k][0][c]);
+                acc[k][1][c] = Mfma<T>::op(ya[k], b1[c], acc[k][1][c]);
+                acc[k][2][c] = Mfma<T>::op(ya[k], b2[c], acc[k][2][c]);
             }
         }
     }

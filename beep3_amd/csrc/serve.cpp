@@ -293,7 +293,7 @@ void beep_server_default_opts(beep_server_opts* o) {
     o->tol = 1e-6;
     o->max_iter = 100;
     o->max_refine = 4;
-    o->inner_tol = 1e-3;
+    o->inner_tol = 0.0;  // half the outer tolerance
     o->far_mode = BEEP_FAR_AUTO;
     o->fmm_order = 5;
     o->fmm_leaf = 40;

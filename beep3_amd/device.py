@@ -166,7 +166,9 @@ class Device:
         return ef, eh
 
     def solve(self, b, x0=None, tol=1e-6, max_iter=100, mode="fp64", max_refine=10,
-              inner_tol=1e-3, verbose=False):
+              inner_tol=0.0, verbose=False):
+        """inner_tol: tolerance of the inner FP32 solves of the mixed mode
+        (0: half of tol)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
         o = SolveOpts(tol, max_iter, SOLVE_MODES[mode] if isinstance(mode, str) else int(mode),

@@ -117,7 +117,7 @@ typedef struct {
     int max_iter;      /* GMRES inner iterations per cycle (m) */
     int mode;          /* BEEP_SOLVE_* */
     int max_refine;    /* refinement steps (mixed mode) */
-    double inner_tol;  /* relative tolerance of each inner solve (mixed mode) */
+    double inner_tol;  /* relative tolerance of each inner solve (mixed mode); <= 0: tol / 2 */
     int verbose;
 } beep_solve_opts;
 

@@ -39,7 +39,7 @@ typedef struct {
     double tol;             /* GMRES tolerance (relative to |b|) */
     int max_iter;
     int max_refine;         /* mixed: refinement steps */
-    double inner_tol;       /* mixed: inner FP32 GMRES tolerance */
+    double inner_tol;       /* mixed: inner FP32 GMRES tolerance (<= 0: tol / 2) */
     int far_mode;           /* BEEP_FAR_* */
     int fmm_order;
     int fmm_leaf;
@@ -47,7 +47,7 @@ typedef struct {
 } beep_server_opts;
 
 /* fills the defaults: device 0, 4 slots, 64 GiB, mixed, tol 1e-6, 100
- * iterations, 4 refinements, inner 1e-3, far field auto, FMM order 5, leaf
+ * iterations, 4 refinements, inner tol / 2, far field auto, FMM order 5, leaf
  * 40, f and h not kept */
 void beep_server_default_opts(beep_server_opts* o);
 
