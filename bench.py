@@ -184,6 +184,14 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
            "max_device_bytes_per_slot": float(max(r["bytes"] for r in ok)),
            "unconverged": sum(1 for r in ok if not r["converged"])}
     print(f"[bench] serving: {out}", file=sys.stderr, flush=True)
+    # capacity: the same requests submitted at once (closed loop)
+    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="auto", fmm_order=5,
+                 fmm_leaf=40)
+    cres, cspan = run(srv, [dict(q, t=0.0) for q in sched], paced=True)
+    srv.close()
+    out["capacity_solves_per_s"] = sum(1 for r in cres if r["status"] == 0) / cspan
+    print(f"[bench] serving capacity: {out['capacity_solves_per_s']:.1f} solves/s",
+          file=sys.stderr, flush=True)
     srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=40)
     sres, sspan = run(srv, sched[:seq_requests], paced=False)
     srv.close()
@@ -339,10 +347,14 @@ def main():
                     "matvecs_per_s": 1e3 / ms, "steps": steps}
         print(f"[bench] {p}: {ms:.3f} ms/matvec (device {tot_ms:.3f}: near {near_ms:.3f} "
               f"concurrent with far {far_ms:.3f})", file=sys.stderr, flush=True)
-    # the near SpMV alone (far field serialised after it), for its roofline
+    # the near SpMV alone (far field serialised before it, grid uncapped: the
+    # SpMV has the whole chip and memory system), for its roofline
     dev.set_overlap(False)
+    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 0
+    dev.set_spmv_residency(0)
     _, near_alone, far_alone, _ = time_matvecs(b, d_x, d_y, "fp64", 1, max(3, args.steps // 2))
     _, near32_alone, far32_alone, _ = time_matvecs(b, d_x, d_y, "fp32", 1, max(3, args.steps // 2))
+    dev.set_spmv_residency(cap)
     dev.set_overlap(True)
 
     nb64, cb64, nnz = near_bytes(a, 32)
