@@ -59,7 +59,7 @@ struct beep_hip_ctx {
     // far field runs on its own stream, concurrently with the near SpMV, into
     // yfar; the context stream adds it to y (fork / join through events)
     hipStream_t far_stream = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_pack = nullptr;
     bool overlap = true;
     int n_cu = 256;               // compute units of the device
     int spmv_blocks_per_cu = 0;   // near-SpMV residency cap (0: one block per work item)
@@ -273,7 +273,7 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     launch_fill(sf, n2, 0.0, c->yfar.p);
     if (c->use_fmm) {
         fmm_evaluate(c->fmm, sf, precision == BEEP_FP32 ? 1 : 0, c->patches.p, dx, c->kappa,
-                     c->yfar.p, timed ? &ev[4] : nullptr);
+                     c->yfar.p, timed ? &ev[4] : nullptr, c->ev_pack);
     } else {
         launch_far_pack(sf, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
         launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
@@ -287,6 +287,12 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
                      c->spmv_part.p, dy, c->spmv_blocks_per_cu * c->n_cu);
     if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
+    if (c->use_fmm) {
+        // the P2P part (VALU) follows the SpMV on this stream, beside the
+        // MFMA-bound M2L of the far stream; it needs the packed sources
+        if (c->overlap) hip_check(hipStreamWaitEvent(s, c->ev_pack, 0), "hipStreamWaitEvent");
+        fmm_p2p(c->fmm, s, precision == BEEP_FP32 ? 1 : 0, c->patches.p, c->kappa, dy);
+    }
     if (c->overlap) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "hipStreamWaitEvent");
     launch_axpby(s, n2, 1.0, c->yfar.p, 1.0, dy);
     if (!timed) return;
@@ -489,6 +495,7 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
                   "hipDeviceGetAttribute");
         hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&c->ev_pack, hipEventDisableTiming), "hipEventCreate");
         for (auto& set : c->ev)
             for (auto& e : set) hip_check(hipEventCreate(&e), "hipEventCreate");
     });
@@ -500,6 +507,7 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
                 if (e) (void)hipEventDestroy(e);
         if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
         if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+        if (c->ev_pack) (void)hipEventDestroy(c->ev_pack);
         if (c->far_stream) (void)hipStreamDestroy(c->far_stream);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         return rc;
@@ -521,6 +529,7 @@ int beep_hip_destroy(beep_hip_ctx* ctx) {
             if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_pack) (void)hipEventDestroy(ctx->ev_pack);
     if (ctx->far_stream) (void)hipStreamDestroy(ctx->far_stream);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

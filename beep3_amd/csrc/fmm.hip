@@ -1091,11 +1091,12 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
 template <typename T, int P>
 static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const double* x,
                      double kappa, T* M, T* L, const std::vector<T*>& tabs, T* const* ytab,
-                     T* slots, double* y, hipEvent_t* m2l_ev) {
+                     T* slots, double* y, hipEvent_t* m2l_ev, hipEvent_t pack_ev) {
     constexpr int NP = P * P * P;
     constexpr int NT = ((NP + 63) / 64) * 64;
     const FmmDev& d = pl->dv;
     launch_far_pack(s, pl->N, patches, d.perm, x, pl->src);
+    if (pack_ev) hip_check(hipEventRecord(pack_ev, s), "hipEventRecord");
     {
         const size_t n = pl->n_exp;
         hipLaunchKernelGGL((k_zero<T>), dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n, L);
@@ -1127,8 +1128,6 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     if (D >= 2)
         hipLaunchKernelGGL((k_l2p<T, P>), dim3(pl->nleaf), dim3(64), 0, s, d, patches, L, y,
                            pl->N);
-    hipLaunchKernelGGL((k_p2p<T>), dim3(pl->nleaf), dim3(256), 0, s, d, pl->src, patches,
-                       T(kappa), y, pl->N);
     hip_check(hipGetLastError(), "fmm launch");
 }
 
@@ -1150,27 +1149,38 @@ int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_m
     return c;
 }
 
+void fmm_p2p(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, double kappa,
+             double* y) {
+    if (fp32)
+        hipLaunchKernelGGL((k_p2p<float>), dim3(pl->nleaf), dim3(256), 0, s, pl->dv, pl->src,
+                           patches, float(kappa), y, pl->N);
+    else
+        hipLaunchKernelGGL((k_p2p<double>), dim3(pl->nleaf), dim3(256), 0, s, pl->dv, pl->src,
+                           patches, kappa, y, pl->N);
+    hip_check(hipGetLastError(), "k_p2p launch");
+}
+
 void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, const double* x,
-                  double kappa, double* y, hipEvent_t* m2l_ev) {
+                  double kappa, double* y, hipEvent_t* m2l_ev, hipEvent_t pack_ev) {
     if (fp32) {
         if (!pl->have32) throw std::runtime_error("FMM FP32 tables not prepared");
         float* M = pl->exp32;
         float* L = pl->exp32 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
-            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
-            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
-            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev); break;
+            case 3: evaluate<float, 3>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev, pack_ev); break;
+            case 4: evaluate<float, 4>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev, pack_ev); break;
+            case 5: evaluate<float, 5>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev, pack_ev); break;
+            case 6: evaluate<float, 6>(pl, s, patches, x, kappa, M, L, pl->ktab32, pl->ytab32, pl->slots32, y, m2l_ev, pack_ev); break;
         }
     } else {
         if (!pl->have64) throw std::runtime_error("FMM FP64 tables not prepared");
         double* M = pl->exp64;
         double* L = pl->exp64 + pl->n_exp;
         switch (pl->P) {
-            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
-            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
-            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
-            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev); break;
+            case 3: evaluate<double, 3>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev, pack_ev); break;
+            case 4: evaluate<double, 4>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev, pack_ev); break;
+            case 5: evaluate<double, 5>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev, pack_ev); break;
+            case 6: evaluate<double, 6>(pl, s, patches, x, kappa, M, L, pl->ktab64, pl->ytab64, pl->slots64, y, m2l_ev, pack_ev); break;
         }
     }
 }

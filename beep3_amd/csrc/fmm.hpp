@@ -77,8 +77,14 @@ inline bool fmm_tables_ready(const FmmPlan* p, double kappa, bool fp64, bool fp3
 // canon_xyz) and the map canonical node -> actual node (order^3 entries)
 int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);
 // y += far field (1-point interaction of every pair) of x
-// (m2l_ev: optional pair of events recorded around the M2L launches)
+// y += the far field of x without its P2P part (pack, upward pass, M2L,
+// downward pass, L2P); m2l_ev: optional pair of events recorded around the
+// M2L launches; pack_ev: optional event recorded once the sources are packed
 void fmm_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, const double* x,
-                  double kappa, double* y, hipEvent_t* m2l_ev = nullptr);
+                  double kappa, double* y, hipEvent_t* m2l_ev = nullptr,
+                  hipEvent_t pack_ev = nullptr);
+// y += the P2P part (neighbour leaf boxes) of the sources packed by the last
+// fmm_evaluate (stream-ordered after its pack)
+void fmm_p2p(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, double kappa, double* y);
 
 }  // namespace beepdev
