@@ -62,7 +62,7 @@ struct beep_hip_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_pack = nullptr;
     bool overlap = true;
     int n_cu = 256;               // compute units of the device
-    int spmv_blocks_per_cu = 0;   // near-SpMV residency cap (0: one block per work item)
+    int spmv_blocks_per_cu = 1;   // near-SpMV residency cap (0: one block per work item)
     std::string err;
 
     // geometry

@@ -374,13 +374,13 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
             }
         }
     }
-    // the accumulators go out as they sit in the registers: chunk slot
-    // [e][row tile][column tile][lane][element], 32 contiguous bytes per lane
-    // (FP64) and one contiguous run per wave store; k_m2l_sum finds a
-    // (pair, node) value through the inverse of the MFMA layout and the
-    // pair's node map
+    // each pair's result goes to its own slot of 3 x NPP values, the rows of
+    // a 16-row tile kept in the order the accumulators hold them (position
+    // 4 (l / 16) + element), so every lane stores one contiguous run of 4
+    // values and k_m2l_sum reads each slot as one contiguous block; the sum
+    // maps a node to its position through the pair's node map and the MFMA
+    // layout
     const T sl = T(1.0 / d.box_h[level]);  // Laplace tables are built at h = 1
-    T* out = Cout + size_t(cid) * 3 * NPP * G;
 #pragma unroll
     for (int k = 0; k < NTW; ++k) {
         const int nt = w + M2L_WAVES * k;
@@ -391,36 +391,65 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
             for (int c = 0; c < CT; ++c) {
                 acc_t v = acc[k][e][c];
                 if (e == 0) v = v * sl;
-                *reinterpret_cast<acc_t*>(out + (((size_t(e) * NTILE + nt) * CT + c) * 64 + l) * 4) = v;
+                const size_t pair = size_t(cid) * G + c * 16 + il;
+                *reinterpret_cast<acc_t*>(Cout + (pair * 3 + e) * NPP + nt * 16 + kl16 * 4) = v;
             }
     }
 }
 
 // Sum the M2L slots of every target box into its local expansion (one block
-// per box at levels >= 2; pairs listed per target in tgt_ptr / tgt_pair).
+// per box at levels >= 2; pairs listed per target in tgt_ptr / tgt_pair, at
+// most 189 per box).  The slot codes and node maps of the box's pairs are
+// staged in LDS; each thread owns outputs (e, n) and adds the pairs in list
+// order, so the sum is deterministic.
+constexpr int M2L_SUM_K = 192;  // pairs staged per pass
+
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_m2l_sum(const FmmDev d, int first_box,
                                                  const T* __restrict__ Cin, T* __restrict__ L) {
     constexpr int NP = P * P * P;
+    constexpr int NPP = (NP + 15) / 16 * 16;
+    constexpr int NQ = (3 * NP + 255) / 256;
+    __shared__ int s_code[M2L_SUM_K];
+    __shared__ int s_sym[M2L_SUM_K];
     const int box = first_box + blockIdx.x;
     const int k0 = d.tgt_ptr[box], k1 = d.tgt_ptr[box + 1];
     if (k0 == k1) return;
-    constexpr int NPP = (NP + 15) / 16 * 16, NTILE = NPP / 16, CT = M2L_G / 16;
-    for (int q = threadIdx.x; q < 3 * NP; q += 256) {
+    const int t = threadIdx.x;
+    T acc[NQ];
+    int qoff[NQ], qn[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int q = t + 256 * i;
         const int e = q / NP, n = q - e * NP;
-        T acc = T(0);
-        for (int k = k0; k < k1; ++k) {
-            const int pr = d.tgt_pair[k];
-            const int code = d.pair_slot[pr];  // chunk * 64 + position in the chunk
-            const int g = code & 63;
-            const int nc = d.phi[size_t(d.pair_sym[pr]) * NP + n];  // canonical node
-            const int nt = nc >> 4, rr = nc & 15;
-            const int l = Mfma<T>::lane_group(rr) * 16 + (g & 15);
-            acc += Cin[size_t(code >> 6) * 3 * NPP * M2L_G +
-                       (((size_t(e) * NTILE + nt) * CT + (g >> 4)) * 64 + l) * 4 + Mfma<T>::elem(rr)];
-        }
-        L[size_t(box) * 3 * NP + q] += acc;
+        acc[i] = T(0);
+        qoff[i] = q < 3 * NP ? e * NPP : -1;
+        qn[i] = n;
     }
+    for (int kb = k0; kb < k1; kb += M2L_SUM_K) {
+        const int nk = min(M2L_SUM_K, k1 - kb);
+        __syncthreads();  // previous pass consumed
+        if (t < nk) {
+            const int pr = d.tgt_pair[kb + t];
+            s_code[t] = d.pair_slot[pr];
+            s_sym[t] = d.pair_sym[pr] * NP;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            if (qoff[i] < 0) continue;
+#pragma unroll 4
+            for (int k = 0; k < nk; ++k) {
+                const int nc = d.phi[s_sym[k] + qn[i]];  // canonical node
+                const int rr = nc & 15;
+                const int pos = (nc & ~15) + Mfma<T>::lane_group(rr) * 4 + Mfma<T>::elem(rr);
+                acc[i] += Cin[size_t(s_code[k]) * 3 * NPP + qoff[i] + pos];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+        if (qoff[i] >= 0) L[size_t(box) * 3 * NP + t + 256 * i] += acc[i];
 }
 
 // L2L: one block per child box at level l+1 (boxes [first, first+count)).
@@ -1047,8 +1076,8 @@ static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& y
               "H2D ytab");
 }
 
-// M2L result slots: every chunk keeps 3 x NPP x 64 values in the MFMA
-// accumulator layout
+// M2L result slots: 3 x NPP values for each of the 64 pair positions of
+// every chunk
 static size_t slot_count(const FmmPlan* pl) {
     const int NP = pl->P * pl->P * pl->P, NPP = (NP + 15) / 16 * 16;
     const size_t nch = size_t(pl->lvl_chunk.empty() ? 0 : pl->lvl_chunk.back());

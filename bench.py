@@ -350,7 +350,7 @@ def main():
     # the near SpMV alone (far field serialised before it, grid uncapped: the
     # SpMV has the whole chip and memory system), for its roofline
     dev.set_overlap(False)
-    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 0
+    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 1  # the library default
     dev.set_spmv_residency(0)
     _, near_alone, far_alone, _ = time_matvecs(b, d_x, d_y, "fp64", 1, max(3, args.steps // 2))
     _, near32_alone, far32_alone, _ = time_matvecs(b, d_x, d_y, "fp32", 1, max(3, args.steps // 2))

@@ -199,9 +199,9 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
                            int reps, double* ms_out, int* best);
 
 /* Residency cap of the near-field SpMV: at most blocks_per_cu blocks per
- * compute unit walk the work items (0: one block per work item, the
- * default).  A cap leaves room on every CU for the far-field kernels that
- * run on the other stream at the same time. */
+ * compute unit walk the work items (default 1; 0: one block per work item).
+ * A cap leaves room on every CU for the far-field kernels that run on the
+ * other stream at the same time. */
 int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu);
 /* Autotuner for that cap: times `reps` whole mat-vecs of the given precision
  * (far field overlapped) for each candidate, writes the mean times to ms_out
