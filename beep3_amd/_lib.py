@@ -183,6 +183,8 @@ def _declare_hip(L):
     L.beep_hip_autotune_leaf.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
     L.beep_hip_autotune_spmv.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
     L.beep_hip_set_spmv_residency.argtypes = [vp, C.c_int]
+    L.beep_hip_autotune_spmv_depth.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
+    L.beep_hip_set_spmv_depth.argtypes = [vp, C.c_int]
     L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]

@@ -63,6 +63,7 @@ struct beep_hip_ctx {
     bool overlap = true;
     int n_cu = 256;               // compute units of the device
     int spmv_blocks_per_cu = 1;   // near-SpMV residency cap (0: one block per work item)
+    int spmv_depth = 1;           // near-SpMV loads in flight per thread (x 4 FP64, x 8 FP32)
     std::string err;
 
     // geometry
@@ -285,7 +286,7 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
     launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
-                     c->spmv_part.p, dy, c->spmv_blocks_per_cu * c->n_cu);
+                     c->spmv_part.p, dy, c->spmv_blocks_per_cu * c->n_cu, c->spmv_depth);
     if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->use_fmm) {
         // the P2P part (VALU) follows the SpMV on this stream, beside the
@@ -351,7 +352,8 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
         c->yfar.p, c->src.p, c->coef32.p, c->coef64.p, c->diag.p, c->spmv_part.p, c->fmm,
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
-        reinterpret_cast<const void*>(intptr_t(c->spmv_blocks_per_cu))};
+        reinterpret_cast<const void*>(intptr_t(c->spmv_blocks_per_cu)),
+        reinterpret_cast<const void*>(intptr_t(c->spmv_depth))};
     if (!c->step_exec || key != c->step_key) {
         drop_step_graph(c);
         hipGraph_t g = nullptr;
@@ -904,53 +906,82 @@ int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu) {
     });
 }
 
+int beep_hip_set_spmv_depth(beep_hip_ctx* ctx, int depth) {
+    return guarded(ctx, [&] {
+        require(depth == 1 || depth == 2 || depth == 4, "SpMV depth must be 1, 2 or 4");
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->spmv_depth = depth;
+        ctx->gen++;
+    });
+}
+
+// Times `reps` whole mat-vecs (after one warm-up) for each candidate value of
+// one SpMV setting, keeps the fastest.  `field` is the context member tuned.
+static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int beep_hip_ctx::*field,
+                              const int* values, int n, int reps, bool (*valid)(int),
+                              const char* what, double* ms_out, int* best) {
+    require(values && n > 0 && reps > 0, "bad autotune arguments");
+    require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
+    for (int k = 0; k < n; ++k) require(valid(values[k]), what);
+    set_device(ctx);
+    const size_t n2 = size_t(2) * ctx->N;
+    ctx->xbuf.alloc(n2);
+    ctx->ybuf.alloc(n2);
+    launch_fill(ctx->stream, int(n2), 1.0, ctx->xbuf.p);
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    const int saved = ctx->*field;
+    double best_ms = 1e300;
+    int best_v = saved;
+    try {
+        for (int k = 0; k < n; ++k) {
+            ctx->*field = values[k];
+            matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // warm-up
+            hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
+            hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            const double per = double(ms) / reps;
+            if (ms_out) ms_out[k] = per;
+            if (per < best_ms) {
+                best_ms = per;
+                best_v = values[k];
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->*field = saved;
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    timing_collect(ctx);
+    ctx->timing = beep_matvec_timing{};
+    ctx->*field = best_v;
+    ctx->gen++;
+    if (best) *best = best_v;
+}
+
 int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
                            int reps, double* ms_out, int* best) {
     return guarded(ctx, [&] {
-        require(blocks_per_cu && n > 0 && reps > 0, "bad autotune arguments");
-        require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
-        set_device(ctx);
-        const size_t n2 = size_t(2) * ctx->N;
-        ctx->xbuf.alloc(n2);
-        ctx->ybuf.alloc(n2);
-        launch_fill(ctx->stream, int(n2), 1.0, ctx->xbuf.p);
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
-        const int saved = ctx->spmv_blocks_per_cu;
-        double best_ms = 1e300;
-        int best_v = saved;
-        try {
-            for (int k = 0; k < n; ++k) {
-                require(blocks_per_cu[k] >= 0 && blocks_per_cu[k] <= 64, "blocks per CU must be 0..64");
-                ctx->spmv_blocks_per_cu = blocks_per_cu[k];
-                matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // warm-up
-                hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
-                for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
-                hip_check(hipEventRecord(e1, ctx->stream), "hipEventRecord");
-                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-                const double per = double(ms) / reps;
-                if (ms_out) ms_out[k] = per;
-                if (per < best_ms) {
-                    best_ms = per;
-                    best_v = blocks_per_cu[k];
-                }
-            }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            ctx->spmv_blocks_per_cu = saved;
-            throw;
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        timing_collect(ctx);
-        ctx->timing = beep_matvec_timing{};
-        ctx->spmv_blocks_per_cu = best_v;
-        ctx->gen++;
-        if (best) *best = best_v;
+        tune_spmv_setting(ctx, precision, &beep_hip_ctx::spmv_blocks_per_cu, blocks_per_cu, n, reps,
+                          [](int v) { return v >= 0 && v <= 64; }, "blocks per CU must be 0..64",
+                          ms_out, best);
+    });
+}
+
+int beep_hip_autotune_spmv_depth(beep_hip_ctx* ctx, int precision, const int* depths, int n,
+                                 int reps, double* ms_out, int* best) {
+    return guarded(ctx, [&] {
+        tune_spmv_setting(ctx, precision, &beep_hip_ctx::spmv_depth, depths, n, reps,
+                          [](int v) { return v == 1 || v == 2 || v == 4; },
+                          "SpMV depth must be 1, 2 or 4", ms_out, best);
     });
 }
 

@@ -123,12 +123,14 @@ struct SpmvPlan {
 };
 // y = near(x) + diag(x), overwriting y.  mode: 0 FP64 coef/FP64 sums,
 // 1 FP32 coef/FP32 sums, 2 FP32 coef/FP64 sums.  part: scratch of
-// n_work * 256 * 2 doubles.
+// n_work * 256 * 2 doubles.  max_blocks > 0 caps the grid (blocks then walk
+// the work items with a grid stride); depth (1, 2, 4) scales the coefficient
+// loads each thread keeps in flight.
 void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
                       const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
                       const double* diag4, const double* x, void* part, double* y,
-                      int max_blocks = 0);
+                      int max_blocks, int depth);
 
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,

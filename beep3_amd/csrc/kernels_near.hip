@@ -311,6 +311,10 @@ __global__ __launch_bounds__(256) void k_near_spmv_part(
             Coef4<TC> v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = load_coef<TC>(coef, base + int64_t(c + u * S) * R);
+            // all U loads are issued before the first use (the scheduler
+            // would otherwise interleave them with the sums and keep only a
+            // few in flight)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const TA xf = xs[0][c + u * S], xh = xs[1][c + u * S];
@@ -362,29 +366,48 @@ __global__ __launch_bounds__(256) void k_near_spmv_sum(
     y[n + i] = double(ah) + (d[3] * xf - d[2] * xh);
 }
 
+// Loads in flight per thread: U = 4 * depth (FP64 coefficients) or 8 * depth
+// (FP32); the column order each thread sums in is the same for every depth,
+// so y does not depend on it.
+template <typename TC, typename TA, int U>
+static void spmv_part(hipStream_t s, int grid, const SpmvPlan& p, const int* leaf_ptr,
+                      const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
+                      const void* coef, const double* x, int n, TA* part) {
+    hipLaunchKernelGGL((k_near_spmv_part<TC, TA, U>), dim3(grid), dim3(256), 0, s, p.work,
+                       p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off, coef, x, n, part);
+}
+
 void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
                       const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
                       const double* diag4, const double* x, void* part, double* y,
-                      int max_blocks) {
+                      int max_blocks, int depth) {
     if (p.n_work <= 0) return;
+    if (depth != 1 && depth != 2 && depth != 4) throw std::runtime_error("SpMV depth must be 1, 2 or 4");
     const int grid = max_blocks > 0 ? std::min(p.n_work, max_blocks) : p.n_work;
+    const int d = depth == 1 ? 0 : (depth == 2 ? 1 : 2);
     if (mode == 0) {
-        hipLaunchKernelGGL((k_near_spmv_part<double, double, 4>), dim3(grid), dim3(256), 0, s,
-                           p.work, p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef64, x, n, (double*)part);
+        using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
+                           const int64_t*, const void*, const double*, int, double*);
+        constexpr F f[3] = {spmv_part<double, double, 4>, spmv_part<double, double, 8>,
+                            spmv_part<double, double, 16>};
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef64, x, n, (double*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
                            p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_near_spmv_part<float, float, 8>), dim3(grid), dim3(256), 0, s,
-                           p.work, p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef32, x, n, (float*)part);
+        using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
+                           const int64_t*, const void*, const double*, int, float*);
+        constexpr F f[3] = {spmv_part<float, float, 8>, spmv_part<float, float, 16>,
+                            spmv_part<float, float, 32>};
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, n, (float*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<float>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
                            p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const float*)part, y);
     } else {
-        hipLaunchKernelGGL((k_near_spmv_part<float, double, 8>), dim3(grid), dim3(256), 0, s,
-                           p.work, p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off,
-                           (const void*)coef32, x, n, (double*)part);
+        using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
+                           const int64_t*, const void*, const double*, int, double*);
+        constexpr F f[3] = {spmv_part<float, double, 8>, spmv_part<float, double, 16>,
+                            spmv_part<float, double, 32>};
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, n, (double*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
                            p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
     }

@@ -123,6 +123,20 @@ class Device:
     def set_spmv_residency(self, blocks_per_cu: int):
         self._check(lib().beep_hip_set_spmv_residency(self._h, int(blocks_per_cu)))
 
+    def autotune_spmv_depth(self, precision="fp64", candidates=(1, 2, 4), reps=3):
+        """Time whole mat-vecs for each near-SpMV load depth, keep the
+        fastest; returns (best, {depth: ms})."""
+        c = np.ascontiguousarray(candidates, dtype=np.int32)
+        ms = np.zeros(len(c))
+        best = C.c_int()
+        self._check(lib().beep_hip_autotune_spmv_depth(
+            self._h, _prec(precision), c.ctypes.data_as(C.POINTER(C.c_int)), len(c), int(reps),
+            ms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(best)))
+        return best.value, {int(k): float(v) for k, v in zip(c, ms)}
+
+    def set_spmv_depth(self, depth: int):
+        self._check(lib().beep_hip_set_spmv_depth(self._h, int(depth)))
+
     def far_info(self) -> dict:
         f = FarInfo()
         self._check(lib().beep_hip_far_info(self._h, C.byref(f)))

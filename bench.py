@@ -311,9 +311,12 @@ def main():
         best, table = dev.autotune_leaf("fp64", TUNE_LEAVES, reps=3)
         tune = {"leaf_ms_fp64": table, "chosen_leaf": best}
         args.leaf = best
-        # near-SpMV residency cap (room for the concurrent far field)
+        # near-SpMV load depth at the default cap, then the residency cap at
+        # that depth (room for the concurrent far field)
+        depth, dtable = dev.autotune_spmv_depth("fp64", (1, 2, 4), reps=3)
         cap, ctable = dev.autotune_spmv("fp64", (0, 1, 2, 3, 4, 6), reps=3)
-        tune.update(spmv_blocks_per_cu_ms_fp64=ctable, chosen_spmv_blocks_per_cu=cap)
+        tune.update(spmv_depth_ms_fp64=dtable, chosen_spmv_depth=depth,
+                    spmv_blocks_per_cu_ms_fp64=ctable, chosen_spmv_blocks_per_cu=cap)
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
 
     # precompute timings (integrand throughput)
@@ -349,12 +352,23 @@ def main():
               f"concurrent with far {far_ms:.3f})", file=sys.stderr, flush=True)
     # the near SpMV alone (far field serialised before it, grid uncapped: the
     # SpMV has the whole chip and memory system), for its roofline
+    # at each load depth; the fastest depth gives the roofline figure
     dev.set_overlap(False)
-    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 1  # the library default
+    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 1  # the library defaults
+    depth = tune["chosen_spmv_depth"] if tune else 1
     dev.set_spmv_residency(0)
-    _, near_alone, far_alone, _ = time_matvecs(b, d_x, d_y, "fp64", 1, max(3, args.steps // 2))
-    _, near32_alone, far32_alone, _ = time_matvecs(b, d_x, d_y, "fp32", 1, max(3, args.steps // 2))
+    alone = {}
+    for dd in (1, 2):
+        dev.set_spmv_depth(dd)
+        _, n64, f64, _ = time_matvecs(b, d_x, d_y, "fp64", 1, max(3, args.steps // 2))
+        _, n32, f32, _ = time_matvecs(b, d_x, d_y, "fp32", 1, max(3, args.steps // 2))
+        alone[dd] = (n64, f64, n32, f32)
+    best_d64 = min(alone, key=lambda k: alone[k][0])
+    best_d32 = min(alone, key=lambda k: alone[k][2])
+    near_alone, far_alone = alone[best_d64][0], alone[best_d64][1]
+    near32_alone, far32_alone = alone[best_d32][2], alone[best_d32][3]
     dev.set_spmv_residency(cap)
+    dev.set_spmv_depth(depth)
     dev.set_overlap(True)
 
     nb64, cb64, nnz = near_bytes(a, 32)
@@ -364,7 +378,7 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": nb64 / (near_alone * 1e-3) / 1e9 / HBM_PEAK_GBS,
                 "traffic": MEASURED_TRAFFIC.get((args.workload, "fp64")),
-                "kernel": "k_near_spmv_part<double,double,4> + k_near_spmv_sum<double>",
+                "kernel": f"k_near_spmv_part<double,double,{4 * best_d64}> + k_near_spmv_sum<double>",
                 "bytes_per_launch": nb64, "launch_ms": near_alone,
                 "compulsory_bytes": cb64,
                 "compulsory_GBs": cb64 / (near_alone * 1e-3) / 1e9}
@@ -393,6 +407,7 @@ def main():
         "matvec": rates,
         "serial_phases_ms": {"fp64": {"near": near_alone, "far": far_alone},
                              "fp32": {"near": near32_alone, "far": far32_alone}},
+        "near_spmv_alone_ms_by_depth": {dd: {"fp64": v[0], "fp32": v[2]} for dd, v in alone.items()},
         "near_spmv_fp32_GBs": nb32 / (near32_alone * 1e-3) / 1e9,
         "host_setup_s": setup_s,
         "autotune": tune,

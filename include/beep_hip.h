@@ -208,6 +208,14 @@ int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu);
  * (may be NULL), keeps the fastest and returns it in *best. */
 int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
                            int reps, double* ms_out, int* best);
+/* Coefficient loads each SpMV thread keeps in flight: 4 x depth (FP64
+ * coefficients) or 8 x depth (FP32), depth 1 (default), 2 or 4.  Deeper
+ * lets fewer resident blocks cover the memory latency (with a residency
+ * cap).  y is bit-identical for every depth. */
+int beep_hip_set_spmv_depth(beep_hip_ctx* ctx, int depth);
+/* Autotuner for the depth, as beep_hip_autotune_spmv. */
+int beep_hip_autotune_spmv_depth(beep_hip_ctx* ctx, int precision, const int* depths, int n,
+                                 int reps, double* ms_out, int* best);
 
 /* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
  * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs

@@ -111,15 +111,23 @@ def test_leaf_autotuner(off_centre_arrays):
 
 def test_spmv_residency_cap_is_bit_identical(off_centre_arrays):
     """Capping the near-SpMV blocks per CU changes only which block handles
-    which work item: y is bit-identical, and the tuner keeps the fastest."""
+    which work item, and the load depth only how many loads are in flight:
+    y is bit-identical, and the tuners keep the fastest."""
     a, kappa = off_centre_arrays
     x = np.random.default_rng(8).normal(size=2 * a["area"].shape[0])
     d = _device(a, kappa, "fmm", 5, 40, precs=("fp64", "fp32"))
-    ref = {p: d.matvec(x, p) for p in ("fp64", "fp32")}
+    ref = {p: d.matvec(x, p) for p in ("fp64", "fp32", "fp32_resid")}
     for cap in (0, 3):  # the default is 1
         d.set_spmv_residency(cap)
-        for p in ("fp64", "fp32"):
+        for p in ("fp64", "fp32", "fp32_resid"):
             assert np.array_equal(d.matvec(x, p), ref[p]), (cap, p)
+    # deeper coefficient loads per thread: the same column order per thread
+    for depth in (2, 4):
+        d.set_spmv_depth(depth)
+        for p in ("fp64", "fp32", "fp32_resid"):
+            assert np.array_equal(d.matvec(x, p), ref[p]), (depth, p)
+    best_d, dtable = d.autotune_spmv_depth("fp64", (1, 2), reps=2)
+    assert dtable[best_d] == min(dtable.values())
     best, table = d.autotune_spmv("fp32", (0, 2), reps=2)
     print("SpMV residency autotune (ms per FP32 mat-vec):", table, "best", best)
     assert table[best] == min(table.values())
