@@ -270,11 +270,22 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     constexpr int NTILE = NPP / 16;
     constexpr int NTW = (NTILE + M2L_WAVES - 1) / M2L_WAVES;  // row tiles per wave
     constexpr int G = M2L_G, CT = G / 16, MS = M2L_MS, MSP = M2L_MSP;
+    constexpr int NQ = (G * MS + 511) / 512;  // slab entries per thread
+    constexpr int NSLAB = (NPP + MS - 1) / MS;
+    constexpr int NSTEP = NPP / 4, SPS = MS / 4;  // k-steps, k-steps per slab
+    constexpr int RV = 16 / int(sizeof(T));       // k-steps per 16-byte LDS read
+    static_assert(SPS % RV == 0 && NSTEP % RV == 0, "slab / vector shape");
     using acc_t = typename Mfma<T>::acc_t;
-    __shared__ T sb[3 * G * MSP];  // B slab [e][g][m]
+    typedef T vec_t __attribute__((ext_vector_type(RV)));
+    // two B slab buffers [g][position of m]: slab s + 1 is stored while slab
+    // s feeds the matrix cores
+    __shared__ alignas(16) T sb[2][G * MSP];
     __shared__ int psrc[G];
     __shared__ const unsigned short* pperm[G];
-    const int cid = xcd_remap(blockIdx.x, gridDim.x);
+    // block = (chunk, potential): potential 0 uses the Laplace table,
+    // potentials 1 and 2 the Yukawa table of the level
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int cid = bid / 3, e = bid - 3 * cid;
     const int4 ch = chunks[cid];  // canonical offset, pairs, level
     const int off = ch.x, p0 = ch.y, np = ch.z - ch.y, level = ch.w;
     const int t = threadIdx.x;
@@ -284,93 +295,86 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     }
     const int w = t >> 6, l = t & 63;
     const int kl16 = l / 16, il = l % 16;
-    const T* KL = kl + size_t(off) * NP * NP;
-    const T* KY = ytabs[level] + size_t(off) * NP * NP;
-    acc_t acc[NTW][3][CT];
+    const T* KT = (e == 0 ? kl : ytabs[level]) + size_t(off) * NP * NP;
+    acc_t acc[NTW][CT];
 #pragma unroll
     for (int k = 0; k < NTW; ++k)
 #pragma unroll
-        for (int e = 0; e < 3; ++e)
-#pragma unroll
-            for (int c = 0; c < CT; ++c) acc[k][e][c] = acc_t{0, 0, 0, 0};
-    // B slabs: B[e][g][mm] = M_src(g)[e][psi_g(m0 + mm)], the source
-    // expansions in the canonical node order of their pairs.  Thread t owns
-    // the same (g, e, mm) entries of every slab (consecutive threads walk mm
-    // of one (g, e), a run inside one expansion); the next slab's values are
-    // loaded into registers while the current slab feeds the matrix cores.
-    constexpr int NQ = (3 * G * MS + 511) / 512;  // entries per thread and slab
-    constexpr int NSLAB = (NPP + MS - 1) / MS;
+        for (int c = 0; c < CT; ++c) acc[k][c] = acc_t{0, 0, 0, 0};
+    // B slabs: B[g][mm] = M_src(g)[e][psi_g(m0 + mm)], the source expansions
+    // in the canonical node order of their pairs.  Thread t owns the same
+    // (g, mm) entries of every slab (consecutive threads walk mm of one g, a
+    // run inside one expansion).  In each g row, m = 4 s + k (k-step s of the
+    // slab, K index k of the MFMA operand) sits at k * SPS + s, so a lane
+    // finds the operands of RV consecutive k-steps side by side.
     __syncthreads();  // pair lists written
     int qsrc[NQ], qmm[NQ];
     const unsigned short* qperm[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
         const int q = t + 512 * i;
-        const int g = q / (3 * MS), rem = q - g * 3 * MS, e = rem / MS, mm = rem - e * MS;
-        const bool ok = q < 3 * G * MS && g < np;
+        const int g = q / MS, mm = q - g * MS;
+        const bool ok = q < G * MS && g < np;
         qsrc[i] = ok ? psrc[g] * 3 * NP + e * NP : -1;
         qperm[i] = ok ? pperm[g] : nullptr;
-        qmm[i] = q < 3 * G * MS ? (e * G + g) * MSP + mm : -1;  // LDS slot
+        qmm[i] = q < G * MS ? g * MSP + (mm & 3) * SPS + (mm >> 2) : -1;
     }
     auto fetch = [&](int m0, T* v) {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            const int q = t + 512 * i;
-            const int mm = q % MS, m = m0 + mm;
+            const int m = m0 + (t + 512 * i) % MS;
             v[i] = (qsrc[i] >= 0 && m < NP) ? M[size_t(qsrc[i]) + qperm[i][m]] : T(0);
         }
     };
     T nxt[NQ];
     fetch(0, nxt);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+        if (qmm[i] >= 0) sb[0][qmm[i]] = nxt[i];
+    if (NSLAB > 1) fetch(MS, nxt);  // in flight during the first slab
     // A operands (kernel table values) one k-step ahead: the loads of step
     // k+1 are in flight while the MFMAs of step k run
-    constexpr int NSTEP = NPP / 4, SPS = MS / 4;  // k-steps, k-steps per slab
-    T xa_n[NTW], ya_n[NTW];
-    auto load_a = [&](int step, T* xa, T* ya) {
+    T xa_n[NTW];
+    auto load_a = [&](int step, T* xa) {
         const int m = 4 * step + kl16;  // K-dimension index (canonical source node)
 #pragma unroll
         for (int k = 0; k < NTW; ++k) {
             const int n = (w + M2L_WAVES * k) * 16 + il;  // output row (canonical target node)
-            const bool ok = m < NP && n < NP;
-            xa[k] = ok ? KL[m * NP + n] : T(0);
-            ya[k] = ok ? KY[m * NP + n] : T(0);
+            xa[k] = (m < NP && n < NP) ? KT[m * NP + n] : T(0);
         }
     };
-    load_a(0, xa_n, ya_n);
-    for (int step = 0; step < NSTEP; ++step) {
-        const int k0 = (step % SPS) * 4;
-        if (k0 == 0) {
-            const int sl = step / SPS;
-            __syncthreads();  // previous slab consumed
-#pragma unroll
-            for (int i = 0; i < NQ; ++i)
-                if (qmm[i] >= 0) sb[qmm[i]] = nxt[i];
+    load_a(0, xa_n);
+#pragma unroll 1
+    for (int sg = 0; sg < NSTEP; sg += RV) {
+        const int ss = sg % SPS, slab = sg / SPS;  // first k-step of the group in its slab
+        if (ss == 0) {
+            // slab `slab` is complete in its buffer, and every read of the
+            // previous slab (the other buffer) is done
             __syncthreads();
-            if (sl + 1 < NSLAB) fetch((sl + 1) * MS, nxt);  // in flight during the MFMAs
+            if (slab + 1 < NSLAB) {
+#pragma unroll
+                for (int i = 0; i < NQ; ++i)
+                    if (qmm[i] >= 0) sb[(slab + 1) & 1][qmm[i]] = nxt[i];
+                if (slab + 2 < NSLAB) fetch((slab + 2) * MS, nxt);
+            }
         }
-        T xa[NTW], ya[NTW];
+        const T* sbuf = sb[slab & 1];
+        vec_t bv[CT];
 #pragma unroll
-        for (int k = 0; k < NTW; ++k) {
-            xa[k] = xa_n[k];
-            ya[k] = ya_n[k];
-        }
-        if (step + 1 < NSTEP) load_a(step + 1, xa_n, ya_n);
-        T bL[CT], b1[CT], b2[CT];
+        for (int c = 0; c < CT; ++c)
+            bv[c] = *reinterpret_cast<const vec_t*>(&sbuf[(c * 16 + il) * MSP + kl16 * SPS + ss]);
 #pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int g = c * 16 + il;
-            bL[c] = sb[(0 * G + g) * MSP + k0 + kl16];
-            b1[c] = sb[(1 * G + g) * MSP + k0 + kl16];
-            b2[c] = sb[(2 * G + g) * MSP + k0 + kl16];
-        }
+        for (int r = 0; r < RV; ++r) {
+            const int step = sg + r;
+            T xa[NTW];
 #pragma unroll
-        for (int k = 0; k < NTW; ++k) {
-            if (w + M2L_WAVES * k >= NTILE) continue;
+            for (int k = 0; k < NTW; ++k) xa[k] = xa_n[k];
+            if (step + 1 < NSTEP) load_a(step + 1, xa_n);
 #pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                acc[k][0][c] = Mfma<T>::op(xa[k], bL[c], acc[k][0][c]);
-                acc[k][1][c] = Mfma<T>::op(ya[k], b1[c], acc[k][1][c]);
-                acc[k][2][c] = Mfma<T>::op(ya[k], b2[c], acc[k][2][c]);
+            for (int k = 0; k < NTW; ++k) {
+                if (w + M2L_WAVES * k >= NTILE) continue;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) acc[k][c] = Mfma<T>::op(xa[k], bv[c][r], acc[k][c]);
             }
         }
     }
@@ -380,20 +384,18 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     // values and k_m2l_sum reads each slot as one contiguous block; the sum
     // maps a node to its position through the pair's node map and the MFMA
     // layout
-    const T sl = T(1.0 / d.box_h[level]);  // Laplace tables are built at h = 1
+    const T scale = e == 0 ? T(1.0 / d.box_h[level]) : T(1);  // Laplace tables are built at h = 1
 #pragma unroll
     for (int k = 0; k < NTW; ++k) {
         const int nt = w + M2L_WAVES * k;
         if (nt >= NTILE) continue;
 #pragma unroll
-        for (int e = 0; e < 3; ++e)
-#pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                acc_t v = acc[k][e][c];
-                if (e == 0) v = v * sl;
-                const size_t pair = size_t(cid) * G + c * 16 + il;
-                *reinterpret_cast<acc_t*>(Cout + (pair * 3 + e) * NPP + nt * 16 + kl16 * 4) = v;
-            }
+        for (int c = 0; c < CT; ++c) {
+            acc_t v = acc[k][c];
+            if (e == 0) v = v * scale;
+            const size_t pair = size_t(cid) * G + c * 16 + il;
+            *reinterpret_cast<acc_t*>(Cout + (pair * 3 + e) * NPP + nt * 16 + kl16 * 4) = v;
+        }
     }
 }
 
@@ -1143,7 +1145,7 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
         const int nch = D >= 2 ? pl->lvl_chunk[D + 1] : 0;
         if (m2l_ev) hip_check(hipEventRecord(m2l_ev[0], s), "hipEventRecord");
         if (nch > 0) {
-            hipLaunchKernelGGL((k_m2l<T, P>), dim3(nch), dim3(512), 0, s, d, pl->d_chunks,
+            hipLaunchKernelGGL((k_m2l<T, P>), dim3(3 * nch), dim3(512), 0, s, d, pl->d_chunks,
                                tabs[D + 1], ytab, M, slots);
             const int b0 = pl->lvl_first[2], nb = pl->lvl_first[D + 1] - b0;
             hipLaunchKernelGGL((k_m2l_sum<T, P>), dim3(nb), dim3(256), 0, s, d, b0, slots, L);
