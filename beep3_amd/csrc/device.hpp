@@ -47,6 +47,16 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
     return x * q + min(x, r) + k;
 }
 
+// 1/sqrt(v) in FP64: the hardware estimate refined by two Newton steps
+// y <- y + y (1/2 - (v/2) y^2), within an ulp or two of the correctly
+// rounded value and several times cheaper than 1.0 / sqrt(v)
+__device__ __forceinline__ double rsqrt_nr(double v) {
+    double y = __builtin_amdgcn_rsq(v);
+    const double h = 0.5 * v;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return fma(y, fma(-h * y, y, 0.5), y);
+}
+
 inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));

@@ -54,8 +54,8 @@ __global__ __launch_bounds__(256) void k_rhs(int n, const int* __restrict__ qual
                     const double4 c = ch[k];
                     const double rx = px[a] - c.x, ry = py[a] - c.y, rz = pz[a] - c.z;
                     const double d2 = rx * rx + ry * ry + rz * rz;
-                    if (d2 <= 2.220446049250313e-16) continue;
-                    const double id = 1.0 / sqrt(d2);
+                    // a charge on the point contributes nothing (id = 0)
+                    const double id = d2 <= 2.220446049250313e-16 ? 0.0 : rsqrt_nr(d2);
                     const double pt = c.w * id;
                     pa += pt;
                     const double ft = -pt * id * id;
@@ -89,7 +89,8 @@ void launch_rhs(hipStream_t s, int n, const int* qual_ptr, const Pt64* qual,
 // the Gauss-law renormalisation: over the galerkin points of each patch,
 //   ec_f = sum w (grad phi . n) / 4pi,  ec_h = sum w phi / 4pi,
 // with phi from the charges of the patch's own instance.  One wave per patch,
-// lane = galerkin point.
+// lane = charge (a patch has a few dozen galerkin points and an instance
+// thousands of charges); the points are read by every lane alike.
 __global__ __launch_bounds__(64) void k_energy_coeffs(int n, const int* __restrict__ gal_ptr,
                                                       const Pt64* __restrict__ gal,
                                                       const Patch64* __restrict__ patches,
@@ -105,30 +106,29 @@ __global__ __launch_bounds__(64) void k_energy_coeffs(int n, const int* __restri
     const int c0 = inst_charge_ptr[inst], c1 = inst_charge_ptr[inst + 1];
     const Patch64 p = patches[i];
     double ef = 0, eh = 0;
-    for (int a = threadIdx.x; a < ng; a += 64) {
-        const Pt64 q = gal[g0 + a];
-        const double x = p.cx + q.x, y = p.cy + q.y, z = p.cz + q.z;
-        double pot = 0, fx = 0, fy = 0, fz = 0;
-        for (int c = c0; c < c1; ++c) {
-            const double rx = x - xyzq[4 * c], ry = y - xyzq[4 * c + 1], rz = z - xyzq[4 * c + 2];
+    for (int c = c0 + int(threadIdx.x); c < c1; c += 64) {
+        const double cx = xyzq[4 * c], cy = xyzq[4 * c + 1], cz = xyzq[4 * c + 2];
+        const double cq = xyzq[4 * c + 3];
+        for (int a = 0; a < ng; ++a) {
+            const Pt64 q = gal[g0 + a];
+            const double x = p.cx + q.x, y = p.cy + q.y, z = p.cz + q.z;
+            const double rx = x - cx, ry = y - cy, rz = z - cz;
             const double d2 = rx * rx + ry * ry + rz * rz;
-            if (d2 <= 2.220446049250313e-16) continue;
-            const double id = 1.0 / sqrt(d2);
-            const double pt = xyzq[4 * c + 3] * id;
-            pot += pt;
+            // a charge on the point contributes nothing (id = 0)
+            const double id = d2 <= 2.220446049250313e-16 ? 0.0 : rsqrt_nr(d2);
+            const double pt = cq * id;
             const double ft = -pt * id * id;
-            fx += ft * rx; fy += ft * ry; fz += ft * rz;
+            eh += pt * q.w;
+            ef += ft * (rx * q.nx + ry * q.ny + rz * q.nz) * q.w;
         }
-        ef += (fx * q.nx + fy * q.ny + fz * q.nz) * q.w * FOUR_PI_INV;
-        eh += pot * q.w * FOUR_PI_INV;
     }
     for (int off = 32; off > 0; off >>= 1) {
         ef += __shfl_down(ef, off, 64);
         eh += __shfl_down(eh, off, 64);
     }
     if (threadIdx.x == 0) {
-        ec_f[i] = ef;
-        ec_h[i] = eh;
+        ec_f[i] = ef * FOUR_PI_INV;
+        ec_h[i] = eh * FOUR_PI_INV;
     }
 }
 
