@@ -124,8 +124,13 @@ struct beep_hip_ctx {
     // one Arnoldi step captured as a HIP graph, replayed for every step
     bool use_graphs = true;
     bool capturing = false;
-    hipGraphExec_t step_exec = nullptr;
-    std::vector<const void*> step_key;
+    // captured Arnoldi steps, one per distinct key (precision, buffers, ...),
+    // most recently used first; mixed solves alternate two precisions
+    struct StepGraph {
+        std::vector<const void*> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<StepGraph> step_graphs;
     unsigned long long gen = 0;  // bumped whenever buffers or plans may change
 
     // ring of event sets (start, near done, end, far done, M2L start, M2L
@@ -331,10 +336,12 @@ void arnoldi_launches(beep_hip_ctx* c, int precision, int m) {
     launch_inc_int(s, c->jstep.p);
 }
 
+constexpr size_t STEP_GRAPHS = 4;  // captured steps kept per context
+
 void drop_step_graph(beep_hip_ctx* c) {
-    if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
-    c->step_exec = nullptr;
-    c->step_key.clear();
+    for (auto& g : c->step_graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c->step_graphs.clear();
 }
 
 void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
@@ -354,8 +361,23 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
         reinterpret_cast<const void*>(intptr_t(c->spmv_blocks_per_cu)),
         reinterpret_cast<const void*>(intptr_t(c->spmv_depth))};
-    if (!c->step_exec || key != c->step_key) {
-        drop_step_graph(c);
+    // graphs of an older generation bake in buffers or plans that changed
+    for (size_t k = c->step_graphs.size(); k-- > 0;)
+        if (c->step_graphs[k].key[0] != key[0]) {
+            (void)hipGraphExecDestroy(c->step_graphs[k].exec);
+            c->step_graphs.erase(c->step_graphs.begin() + k);
+        }
+    size_t hit = 0;
+    while (hit < c->step_graphs.size() && c->step_graphs[hit].key != key) ++hit;
+    if (hit < c->step_graphs.size()) {
+        // most recently used first
+        std::rotate(c->step_graphs.begin(), c->step_graphs.begin() + hit,
+                    c->step_graphs.begin() + hit + 1);
+    } else {
+        if (c->step_graphs.size() >= STEP_GRAPHS) {
+            (void)hipGraphExecDestroy(c->step_graphs.back().exec);
+            c->step_graphs.pop_back();
+        }
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
                   "hipStreamBeginCapture");
@@ -374,10 +396,9 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         hip_check(e, "hipGraphInstantiate");
-        c->step_exec = ex;
-        c->step_key = key;
+        c->step_graphs.insert(c->step_graphs.begin(), beep_hip_ctx::StepGraph{key, ex});
     }
-    hip_check(hipGraphLaunch(c->step_exec, c->stream), "hipGraphLaunch");
+    hip_check(hipGraphLaunch(c->step_graphs.front().exec, c->stream), "hipGraphLaunch");
 }
 
 // GMRES without restart (BEEP::gmres, beep.cpp:1343-1432) on device vectors:

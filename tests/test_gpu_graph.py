@@ -1,7 +1,8 @@
-"""GPU: GMRES Arnoldi steps replayed from one captured HIP graph give the same
+"""GPU: GMRES Arnoldi steps replayed from captured HIP graphs give the same
 bits as the same kernels launched one by one, on every replay (each solve
-replays the graph once per iteration, with eager refinement work and host
-synchronisation in between), for the direct and the FMM far field."""
+replays a graph once per iteration, with eager refinement work and host
+synchronisation in between), for the direct and the FMM far field; solves
+that alternate precisions reuse their cached graphs."""
 import numpy as np
 import pytest
 
@@ -47,4 +48,20 @@ def test_graph_replay_matches_eager(case):
         assert np.array_equal(x0, x1), mode
         assert np.array_equal(x1, x2), mode
     print(case, "iterations (eager, graph):", out)
+    d.close()
+
+
+def test_cached_graphs_across_alternating_solves():
+    """FP64 and mixed solves in turn keep one cached step graph per precision;
+    every solve matches the eager solve of its mode bit for bit."""
+    b = configs.off_centre()
+    d, sc, a = _device(b, 4.0, "fmm", ("fp64", "fp32"))
+    rhs = d.rhs(a["charges"], b.Dsolvent)
+    d.set_graphs(False)
+    eager = {m: d.solve(rhs, tol=1e-6, max_iter=100, mode=m)[0] for m in ("fp64", "mixed", "fp32")}
+    d.set_graphs(True)
+    for m in ("fp64", "mixed", "fp64", "fp32", "mixed", "fp64", "fp32"):
+        x, st = d.solve(rhs, tol=1e-6, max_iter=100, mode=m)
+        assert st["converged"], m
+        assert np.array_equal(x, eager[m]), m
     d.close()
