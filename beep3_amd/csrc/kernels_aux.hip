@@ -90,7 +90,11 @@ void launch_rhs(hipStream_t s, int n, const int* qual_ptr, const Pt64* qual,
 //   ec_f = sum w (grad phi . n) / 4pi,  ec_h = sum w phi / 4pi,
 // with phi from the charges of the patch's own instance.  One wave per patch,
 // lane = charge (a patch has a few dozen galerkin points and an instance
-// thousands of charges); the points are read by every lane alike.
+// thousands of charges).  The points, as position and w n, are staged in
+// LDS (read by every lane alike); each lane takes two charges per pass so a
+// point read serves two pairs.
+constexpr int EC_PTS = 64;  // galerkin points staged per pass
+
 __global__ __launch_bounds__(64) void k_energy_coeffs(int n, const int* __restrict__ gal_ptr,
                                                       const Pt64* __restrict__ gal,
                                                       const Patch64* __restrict__ patches,
@@ -99,34 +103,59 @@ __global__ __launch_bounds__(64) void k_energy_coeffs(int n, const int* __restri
                                                       const double* __restrict__ xyzq,
                                                       double* __restrict__ ec_f,
                                                       double* __restrict__ ec_h) {
+    __shared__ double4 pxw[EC_PTS];  // point position, w
+    __shared__ double4 pwn[EC_PTS];  // w n
     const int i = blockIdx.x;
     if (i >= n) return;
+    const int t = threadIdx.x;
     const int g0 = gal_ptr[i], ng = gal_ptr[i + 1] - g0;
     const int inst = patch_inst[i];
     const int c0 = inst_charge_ptr[inst], c1 = inst_charge_ptr[inst + 1];
     const Patch64 p = patches[i];
     double ef = 0, eh = 0;
-    for (int c = c0 + int(threadIdx.x); c < c1; c += 64) {
-        const double cx = xyzq[4 * c], cy = xyzq[4 * c + 1], cz = xyzq[4 * c + 2];
-        const double cq = xyzq[4 * c + 3];
-        for (int a = 0; a < ng; ++a) {
-            const Pt64 q = gal[g0 + a];
-            const double x = p.cx + q.x, y = p.cy + q.y, z = p.cz + q.z;
-            const double rx = x - cx, ry = y - cy, rz = z - cz;
-            const double d2 = rx * rx + ry * ry + rz * rz;
-            // a charge on the point contributes nothing (id = 0)
-            const double id = d2 <= 2.220446049250313e-16 ? 0.0 : rsqrt_nr(d2);
-            const double pt = cq * id;
-            const double ft = -pt * id * id;
-            eh += pt * q.w;
-            ef += ft * (rx * q.nx + ry * q.ny + rz * q.nz) * q.w;
+    for (int a0 = 0; a0 < ng; a0 += EC_PTS) {
+        const int na = min(EC_PTS, ng - a0);
+        __syncthreads();  // previous points consumed
+        if (t < na) {
+            const Pt64 q = gal[g0 + a0 + t];
+            pxw[t] = make_double4(p.cx + q.x, p.cy + q.y, p.cz + q.z, q.w);
+            pwn[t] = make_double4(q.w * q.nx, q.w * q.ny, q.w * q.nz, 0.0);
+        }
+        __syncthreads();
+        for (int c = c0 + 2 * t; c < c1; c += 128) {
+            // two charges (the second may be past the end: zero charge)
+            const bool two = c + 1 < c1;
+            const double ax = xyzq[4 * c], ay = xyzq[4 * c + 1], az = xyzq[4 * c + 2];
+            const double aq = xyzq[4 * c + 3];
+            const double bx = two ? xyzq[4 * c + 4] : ax, by = two ? xyzq[4 * c + 5] : ay;
+            const double bz = two ? xyzq[4 * c + 6] : az, bq = two ? xyzq[4 * c + 7] : 0.0;
+            for (int k = 0; k < na; ++k) {
+                const double4 xw = pxw[k], wn = pwn[k];
+                {
+                    const double rx = xw.x - ax, ry = xw.y - ay, rz = xw.z - az;
+                    const double d2 = rx * rx + ry * ry + rz * rz;
+                    // a charge on the point contributes nothing (id = 0)
+                    const double id = d2 <= 2.220446049250313e-16 ? 0.0 : rsqrt_nr(d2);
+                    const double pt = aq * id;
+                    eh += pt * xw.w;
+                    ef -= pt * id * id * (rx * wn.x + ry * wn.y + rz * wn.z);
+                }
+                {
+                    const double rx = xw.x - bx, ry = xw.y - by, rz = xw.z - bz;
+                    const double d2 = rx * rx + ry * ry + rz * rz;
+                    const double id = d2 <= 2.220446049250313e-16 ? 0.0 : rsqrt_nr(d2);
+                    const double pt = bq * id;
+                    eh += pt * xw.w;
+                    ef -= pt * id * id * (rx * wn.x + ry * wn.y + rz * wn.z);
+                }
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
         ef += __shfl_down(ef, off, 64);
         eh += __shfl_down(eh, off, 64);
     }
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         ec_f[i] = ef * FOUR_PI_INV;
         ec_h[i] = eh * FOUR_PI_INV;
     }

@@ -20,50 +20,56 @@ namespace beepdev {
 
 template <typename T>
 struct Math;
+// e^{-kappa r} = expk(ek * r) with ek = ek_of(kappa) formed once per kernel
 template <>
 struct Math<float> {
     // hardware reciprocal square root (1 ulp)
     static __device__ __forceinline__ float rsqrt_(float v) { return __builtin_amdgcn_rsqf(v); }
-    static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
+    // hardware 2^x with the factor log2(e) folded into ek
+    static __host__ __device__ __forceinline__ float ek_of(float kappa) {
+        return -kappa * 1.44269504088896341f;
+    }
+    static __device__ __forceinline__ float expk(float v) { return __builtin_amdgcn_exp2f(v); }
 };
 template <>
 struct Math<double> {
     // hardware estimate refined by two Newton steps y <- y + y (1/2 - (v/2) y^2)
-    static __device__ __forceinline__ double rsqrt_(double v) {
-        double y = __builtin_amdgcn_rsq(v);
-        const double h = 0.5 * v;
-        y = fma(y, fma(-h * y, y, 0.5), y);
-        return fma(y, fma(-h * y, y, 0.5), y);
-    }
-    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
+    static __device__ __forceinline__ double rsqrt_(double v) { return rsqrt_nr(v); }
+    static __host__ __device__ __forceinline__ double ek_of(double kappa) { return -kappa; }
+    static __device__ __forceinline__ double expk(double v) { return exp(v); }
 };
 
 // Accumulate A..D of one (qual point, quad point) pair into the per-qual-point
 // partial sums, without the common factor 1/(4 pi) (applied once per patch
 // pair by the caller).  Coincident points (r^2 < 1e-9, bem_kernels.h:185)
 // contribute nothing: 1/r is set to 0, which zeroes every term, so the loop
-// carries no branch.
-template <typename T>
-__device__ __forceinline__ void integrand(T kappa, T inv_eps, T dx, T dy, T dz, T n0x, T n0y,
-                                          T n0z, T nx, T ny, T nz, T wt, T& A, T& B, T& C,
-                                          T& D) {
+// carries no branch.  KAPPA = (kappa != 0), fixed per launch.
+template <typename T, bool KAPPA>
+__device__ __forceinline__ void integrand(T kappa, T ek, T inv_eps, T dx, T dy, T dz, T n0x,
+                                          T n0y, T n0z, T nx, T ny, T nz, T wt, T& A, T& B,
+                                          T& C, T& D) {
     const T r2 = dx * dx + dy * dy + dz * dz;
     const T ir = r2 < T(1e-9) ? T(0) : Math<T>::rsqrt_(r2);
-    const T r = r2 * ir;
     const T ir2 = ir * ir;
     const T ir3 = ir2 * ir;
-    const T e = Math<T>::exp_(-kappa * r);
-    const T u3 = e * (ir3 + kappa * ir2);  // e^{-kr}(1 + kr)/r^3
     const T nd = nx * dx + ny * dy + nz * dz;
     const T n0d = n0x * dx + n0y * dy + n0z * dz;
-    A += wt * (ir - e * ir);
-    B -= wt * nd * (inv_eps * ir3 - u3);
-    C += wt * n0d * (ir3 - u3 * inv_eps);
-    if (kappa != T(0)) {
+    if constexpr (KAPPA) {
+        const T r = r2 * ir;
+        const T e = Math<T>::expk(ek * r);
+        const T u3 = e * (ir3 + kappa * ir2);  // e^{-kr}(1 + kr)/r^3
+        A += wt * (ir - e * ir);
+        B -= wt * nd * (inv_eps * ir3 - u3);
+        C += wt * n0d * (ir3 - u3 * inv_eps);
         const T ir5 = ir3 * ir2;
         const T p = e * (kappa * (T(3) * ir2 * ir2 + kappa * ir3) + T(3) * ir5);
         const T nn0 = nx * n0x + ny * n0y + nz * n0z;
         D -= wt * ((T(3) * ir5 - p) * nd * n0d - (ir3 - u3) * nn0);
+    } else {
+        // e^{-kr} = 1: A and D vanish (D for kappa = 0 is not computed,
+        // bem_kernels.h:210)
+        B -= wt * nd * (inv_eps * ir3 - ir3);
+        C += wt * n0d * (ir3 - ir3 * inv_eps);
     }
 }
 
@@ -72,6 +78,7 @@ __device__ __forceinline__ void integrand(T kappa, T inv_eps, T dx, T dy, T dz, 
 // over the galerkin points of each patch, normalised by the summed weights,
 // plus the geometric correction (bem_kernels.cpp:123-136; beep.cpp:836-841).
 // One wave per patch; lane = source point.
+template <bool KAPPA>
 __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restrict__ patches,
                                                  const double* __restrict__ gc,
                                                  const int* __restrict__ gal_ptr,
@@ -87,8 +94,8 @@ __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restric
         double a = 0, b = 0, c = 0, d = 0;
         for (int t = 0; t < np; ++t) {
             const Pt64 p = gal[p0 + t];
-            integrand<double>(kappa, inv_eps, p.x - q.x, p.y - q.y, p.z - q.z, q.nx, q.ny, q.nz,
-                              p.nx, p.ny, p.nz, p.w, a, b, c, d);
+            integrand<double, KAPPA>(kappa, -kappa, inv_eps, p.x - q.x, p.y - q.y, p.z - q.z,
+                                     q.nx, q.ny, q.nz, p.nx, p.ny, p.nz, p.w, a, b, c, d);
         }
         A += a * q.w;
         B += b * q.w;
@@ -116,8 +123,12 @@ __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restric
 void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,
                      const int* gal_ptr, const Pt64* gal, double kappa, double* diag4) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_singular, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr, gal, kappa,
-                       diag4);
+    if (kappa != 0.0)
+        hipLaunchKernelGGL(k_singular<true>, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr, gal,
+                           kappa, diag4);
+    else
+        hipLaunchKernelGGL(k_singular<false>, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr,
+                           gal, kappa, diag4);
     hip_check(hipGetLastError(), "k_singular launch");
 }
 
@@ -139,7 +150,7 @@ __device__ __forceinline__ int find_leaf(const int* __restrict__ blk_ptr, int n_
     return lo;
 }
 
-template <typename T>
+template <typename T, bool KAPPA>
 __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj, int qb, int qn,
                                             const Pt<T>* __restrict__ qual, int pb, int pn,
                                             const Pt<T>* __restrict__ quad, T kappa,
@@ -147,6 +158,7 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
     // centroid difference in FP64, then the working precision
     const T dcx = T(pj.cx - pi.cx), dcy = T(pj.cy - pi.cy), dcz = T(pj.cz - pi.cz);
     const T inv_eps = T(pj.inv_eps);
+    const T ek = Math<T>::ek_of(kappa);
     T A = 0, B = 0, C = 0, D = 0;
     for (int a = 0; a < qn; ++a) {
         const Pt<T> q = qual[qb + a];
@@ -155,8 +167,8 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
 #pragma unroll(sizeof(T) == 4 ? 4 : 2)
         for (int k = 0; k < pn; ++k) {
             const Pt<T> p = quad[pb + k];
-            integrand<T>(kappa, inv_eps, ox + p.x, oy + p.y, oz + p.z, q.nx, q.ny, q.nz, p.nx,
-                         p.ny, p.nz, p.w, a_, b_, c_, d_);
+            integrand<T, KAPPA>(kappa, ek, inv_eps, ox + p.x, oy + p.y, oz + p.z, q.nx, q.ny,
+                                q.nz, p.nx, p.ny, p.nz, p.w, a_, b_, c_, d_);
         }
         A += a_ * q.w;
         B += b_ * q.w;
@@ -165,8 +177,8 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
     }
     if (subtract_1pt) {
         T a_ = 0, b_ = 0, c_ = 0, d_ = 0;
-        integrand<T>(kappa, inv_eps, dcx, dcy, dcz, T(pi.nx), T(pi.ny), T(pi.nz), T(pj.nx),
-                     T(pj.ny), T(pj.nz), T(pj.area), a_, b_, c_, d_);
+        integrand<T, KAPPA>(kappa, ek, inv_eps, dcx, dcy, dcz, T(pi.nx), T(pi.ny), T(pi.nz),
+                            T(pj.nx), T(pj.ny), T(pj.nz), T(pj.area), a_, b_, c_, d_);
         A -= a_;
         B -= b_;
         C -= c_;
@@ -179,7 +191,7 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
     out[3] = D * inv_eps * k4;
 }
 
-template <typename T>
+template <typename T, bool KAPPA>
 __global__ __launch_bounds__(256) void k_near_coeffs(
     int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
     const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
@@ -203,7 +215,7 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
         const Patch64 pj = patches[j];
         const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
         const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
-        pair_coeffs<T>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, subtract_1pt, out);
+        pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, subtract_1pt, out);
     }
     const int64_t idx = coef_off[L] + e;
     if constexpr (sizeof(T) == 4) {
@@ -221,9 +233,10 @@ void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
                              const Pt32* quad, float kappa, int subtract_1pt, float4* coef) {
     if (n_blocks <= 0) return;
-    hipLaunchKernelGGL(k_near_coeffs<float>, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr,
-                       leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual,
-                       quad_ptr, quad, kappa, subtract_1pt, (void*)coef);
+    auto k = kappa != 0.0f ? k_near_coeffs<float, true> : k_near_coeffs<float, false>;
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
+                       nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
+                       subtract_1pt, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<float> launch");
 }
 
@@ -233,9 +246,10 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
                              const Pt64* quad, double kappa, int subtract_1pt, double2* coef) {
     if (n_blocks <= 0) return;
-    hipLaunchKernelGGL(k_near_coeffs<double>, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr,
-                       leaf_ptr, leaf_idx, nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual,
-                       quad_ptr, quad, kappa, subtract_1pt, (void*)coef);
+    auto k = kappa != 0.0 ? k_near_coeffs<double, true> : k_near_coeffs<double, false>;
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
+                       nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
+                       subtract_1pt, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<double> launch");
 }
 
