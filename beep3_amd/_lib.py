@@ -100,6 +100,7 @@ class FarInfo(C.Structure):
     _fields_ = [
         ("mode", C.c_int), ("order", C.c_int), ("leaf_level", C.c_int), ("n_leaf_boxes", C.c_int),
         ("n_boxes", C.c_int), ("m2l_pairs", C.c_longlong), ("p2p_pairs", C.c_longlong),
+        ("p2p_residual_pairs", C.c_longlong), ("plan_ms", C.c_double),
     ]
 
 

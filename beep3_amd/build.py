@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libbeep3.so")
 
 HOST_SOURCES = ["host_mesh.cpp", "host_octree.cpp", "host_scene.cpp", "host_io.cpp", "serve.cpp"]
-HIP_SOURCES = ["kernels_near.hip", "kernels_far.hip", "kernels_aux.hip", "fmm.hip", "beep_hip.hip"]
+HIP_SOURCES = ["kernels_near.hip", "kernels_far.hip", "kernels_aux.hip", "fmm.hip",
+               "fmm_residual.hip", "beep_hip.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -90,6 +90,7 @@ struct beep_hip_ctx {
     DevBuf<double> spmv_part;
     SpmvPlan spmv;
     int64_t nnz = 0;
+    long long n_nb = 0;  // neighbourhood entries (nb_ptr[L])
     bool have_tree = false;
 
     // precomputed operator
@@ -97,10 +98,14 @@ struct beep_hip_ctx {
     DevBuf<float4> coef32;
     DevBuf<double2> coef64;
     bool have32 = false, have64 = false, have_diag = false;
+    // far-field plan each coefficient set was built for (0: none, the direct
+    // far field): pairs of the FMM leaf pass keep the full rule
+    unsigned long long uid32 = 0, uid64 = 0;
     double kappa = 0;
 
     // far field
     int far_mode = BEEP_FAR_AUTO, fmm_order = 4, fmm_leaf = 48;
+    double plan_ms = 0;  // host time of the current plan and its residual list
     std::vector<double> h_centroid;
     DevArena fmm_arena;  // device buffers of the FMM plan, reused across plans
     FmmPlan* fmm = nullptr;
@@ -185,18 +190,34 @@ void far_setup(beep_hip_ctx* c) {
 }
 
 // Build the far-field plan if it is missing, on the far stream (which holds
-// no other work then), and the kernel tables of the given precision for
+// no other work then), and with the near list set, its residual leaf-pass
+// list.  Host-synchronous.
+void ensure_plan(beep_hip_ctx* c) {
+    if (!c->use_fmm) return;
+    if (!c->fmm) {
+        hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
+        const auto t0 = std::chrono::steady_clock::now();
+        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
+                                 c->fmm_arena, c->far_stream);
+        c->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c->gen++;
+    }
+    if (c->have_tree && c->fmm->cellkey && c->fmm->n_res < 0) {
+        const auto t0 = std::chrono::steady_clock::now();
+        fmm_build_residual(c->fmm, c->L, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p,
+                           c->n_nb, c->far_stream);
+        c->plan_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c->gen++;
+    }
+}
+
+// The plan (ensure_plan) and the kernel tables of the given precision for
 // the current kappa.  Tables change only through a new kappa (precompute,
 // which drains both streams first) or a precision not built yet (buffers of
 // their own), so no pending far-field work reads what is written here.
 void ensure_far(beep_hip_ctx* c, int precision = -1) {
     if (!c->use_fmm) return;
-    if (!c->fmm) {
-        hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
-        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
-                                 c->fmm_arena, c->far_stream);
-        c->gen++;
-    }
+    ensure_plan(c);
     if (precision < 0 || !c->have_diag) return;
     const bool f64 = precision != BEEP_FP32, f32 = precision != BEEP_FP64;
     if (!fmm_tables_ready(c->fmm, c->kappa, f64, f32)) {
@@ -258,6 +279,12 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
             "unknown precision");
     if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
     else require(c->have32, "FP32 coefficients not precomputed");
+    {
+        const unsigned long long built = precision == BEEP_FP64 ? c->uid64 : c->uid32;
+        require(c->use_fmm ? (c->fmm && c->fmm->uid == built) : built == 0,
+                "near coefficients were precomputed for another far field: call "
+                "beep_hip_precompute again");
+    }
     hipStream_t s = c->stream;
     // plan and kernel tables are built on first use for the current kappa
     // (before any capture: a captured step finds them ready)
@@ -745,65 +772,80 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         ctx->spmv.n_groups = int(groups.size());
         ctx->h_leaf_ptr.assign(t->leaf_ptr, t->leaf_ptr + L + 1);
         ctx->h_nb_ptr.assign(t->nb_ptr, t->nb_ptr + L + 1);
+        ctx->n_nb = nnb;
         ctx->have_tree = true;
         ctx->have32 = ctx->have64 = ctx->have_diag = false;
+        if (ctx->fmm) ctx->fmm->n_res = -1;  // the residual leaf-pass list follows the near list
         ctx->coef32.reset();
         ctx->coef64.reset();
     });
 }
 
+// Singular terms and the near coefficients of one precision.  With the FMM
+// the plan is built first: explicit pairs its leaf pass covers keep the full
+// rule (their packed leaf cells are adjacent) and the leaf pass runs over
+// the residual pairs only.
+void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
+    require(ctx->have_geom && ctx->have_tree, "geometry and tree must be set");
+    require(kappa >= 0, "kappa must be >= 0");
+    require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
+            "unknown precision");
+    sync_streams(ctx);
+    ctx->gen++;
+    if (ctx->have_diag && kappa != ctx->kappa) {
+        ctx->have32 = ctx->have64 = false;
+        ctx->coef32.reset();
+        ctx->coef64.reset();
+    }
+    hipStream_t s = ctx->stream;
+    ctx->kappa = kappa;
+    ctx->diag.alloc(size_t(4) * ctx->N);
+    launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p, kappa,
+                    ctx->diag.p);
+    ctx->have_diag = true;
+    const unsigned* cellkey = nullptr;
+    unsigned long long uid = 0;
+    if (ctx->use_fmm) {
+        ensure_plan(ctx);  // host-synchronous: cell keys are on the device
+        cellkey = ctx->fmm->n_res >= 0 ? ctx->fmm->cellkey : nullptr;
+        uid = ctx->fmm->uid;
+    }
+    size_t freeb = 0, totalb = 0;
+    hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
+    if (precision == BEEP_FP64) {
+        const size_t need = size_t(ctx->nnz) * 2 * sizeof(double2);
+        require(ctx->coef64.p || need + (size_t(1) << 30) < freeb,
+                "not enough device memory for FP64 coefficients");
+        ctx->coef64.alloc(size_t(ctx->nnz) * 2);
+        launch_near_coeffs_fp64(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
+                                ctx->leaf_idx.p, ctx->nb_ptr.p, ctx->nb_idx.p, ctx->coef_off.p,
+                                ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
+                                ctx->quad64.p, kappa, 1, cellkey, ctx->coef64.p);
+        ctx->have64 = true;
+        ctx->uid64 = uid;
+    } else {
+        const size_t need = size_t(ctx->nnz) * sizeof(float4);
+        require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
+                "not enough device memory for FP32 coefficients");
+        ctx->coef32.alloc(size_t(ctx->nnz));
+        launch_near_coeffs_fp32(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
+                                ctx->leaf_idx.p, ctx->nb_ptr.p, ctx->nb_idx.p, ctx->coef_off.p,
+                                ctx->patches.p, ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
+                                ctx->quad32.p, float(kappa), 1, cellkey, ctx->coef32.p);
+        ctx->have32 = true;
+        ctx->uid32 = uid;
+    }
+    // the kernel tables are built on the far stream while the coefficient
+    // kernels run (no wait here: the precompute is stream-ordered); FP32
+    // coefficients serve the FP32 mat-vec (FP32 far field) and the
+    // refinement residual (FP64 far field)
+    ensure_far(ctx, precision);
+}
+
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
     return guarded(ctx, [&] {
-        require(ctx->have_geom && ctx->have_tree, "geometry and tree must be set");
-        require(kappa >= 0, "kappa must be >= 0");
         set_device(ctx);
-        sync_streams(ctx);
-        ctx->gen++;
-        if (ctx->have_diag && kappa != ctx->kappa) {
-            ctx->have32 = ctx->have64 = false;
-            ctx->coef32.reset();
-            ctx->coef64.reset();
-        }
-        hipStream_t s = ctx->stream;
-        ctx->kappa = kappa;
-        ctx->diag.alloc(size_t(4) * ctx->N);
-        launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p, kappa,
-                        ctx->diag.p);
-        ctx->have_diag = true;
-        size_t freeb = 0, totalb = 0;
-        hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
-        if (precision == BEEP_FP64) {
-            const size_t need = size_t(ctx->nnz) * 2 * sizeof(double2);
-            require(ctx->coef64.p || need + (size_t(1) << 30) < freeb,
-                    "not enough device memory for FP64 coefficients");
-            ctx->coef64.alloc(size_t(ctx->nnz) * 2);
-            launch_near_coeffs_fp64(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p,
-                                    ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
-                                    ctx->nb_idx.p, ctx->coef_off.p, ctx->patches.p,
-                                    ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
-                                    ctx->quad64.p, kappa, 1, ctx->coef64.p);
-            ctx->have64 = true;
-        } else if (precision == BEEP_FP32 || precision == BEEP_FP32_RESID) {
-            const size_t need = size_t(ctx->nnz) * sizeof(float4);
-            require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
-                    "not enough device memory for FP32 coefficients");
-            ctx->coef32.alloc(size_t(ctx->nnz));
-            launch_near_coeffs_fp32(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p,
-                                    ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
-                                    ctx->nb_idx.p, ctx->coef_off.p, ctx->patches.p,
-                                    ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
-                                    ctx->quad32.p, float(kappa), 1, ctx->coef32.p);
-            ctx->have32 = true;
-        } else {
-            throw std::runtime_error("unknown precision");
-        }
-        // the far-field plan and tables are built on the host and the far
-        // stream while the coefficient kernels run (no wait here: the
-        // precompute is stream-ordered); FP32 coefficients serve the FP32
-        // mat-vec (FP32 far field) and the refinement residual (FP64 far field)
-        ctx->kappa = kappa;
-        ctx->have_diag = true;
-        ensure_far(ctx, precision);
+        precompute_impl(ctx, kappa, precision);
     });
 }
 
@@ -817,7 +859,7 @@ int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_targ
         ctx->far_mode = mode;
         ctx->fmm_order = order;
         ctx->fmm_leaf = leaf_target;
-        if (ctx->have_geom) far_setup(ctx);  // kernel tables follow on first use
+        if (ctx->have_geom) far_setup(ctx);  // plan and tables follow on first use
     });
 }
 
@@ -853,6 +895,8 @@ int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
             out->n_boxes = ctx->fmm->nbox;
             out->m2l_pairs = ctx->fmm->n_pairs;
             out->p2p_pairs = ctx->fmm->n_p2p;
+            out->p2p_residual_pairs = ctx->fmm->n_res;
+            out->plan_ms = ctx->plan_ms;
         }
     });
 }
@@ -887,6 +931,7 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
                 require(leaf_targets[k] >= 1, "leaf targets must be positive");
                 ctx->fmm_leaf = leaf_targets[k];
                 far_setup(ctx);
+                precompute_impl(ctx, ctx->kappa, precision);  // coefficients follow the plan
                 matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // tables + warm-up
                 hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
                 for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
@@ -913,6 +958,10 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
         ctx->timing = beep_matvec_timing{};
         ctx->fmm_leaf = best_leaf;
         far_setup(ctx);
+        // every coefficient set the context held, for the chosen plan
+        const bool f64 = ctx->have64, f32 = ctx->have32;
+        if (f64) precompute_impl(ctx, ctx->kappa, BEEP_FP64);
+        if (f32) precompute_impl(ctx, ctx->kappa, BEEP_FP32);
         if (best) *best = best_leaf;
     });
 }

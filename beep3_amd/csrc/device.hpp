@@ -57,6 +57,20 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
     return fma(y, fma(-h * y, y, 0.5), y);
 }
 
+// FMM leaf cell of a patch packed as x | y << 10 | z << 20 (leaf level
+// <= 10): two patches meet in the FMM leaf pass iff every field differs by
+// at most 1.
+constexpr int FMM_CELL_BITS = 10;
+__host__ __device__ inline bool fmm_cells_adjacent(unsigned a, unsigned b) {
+    constexpr unsigned M = (1u << FMM_CELL_BITS) - 1;
+    bool ok = true;
+    for (int f = 0; f < 3; ++f) {
+        const int u = int((a >> (FMM_CELL_BITS * f)) & M), v = int((b >> (FMM_CELL_BITS * f)) & M);
+        ok = ok && u - v <= 1 && v - u <= 1;
+    }
+    return ok;
+}
+
 inline void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
         throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
@@ -112,12 +126,14 @@ void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
-                             const Pt32* quad, float kappa, int subtract_1pt, float4* coef);
+                             const Pt32* quad, float kappa, int subtract_1pt,
+                             const unsigned* cellkey, float4* coef);
 void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
                              const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
-                             const Pt64* quad, double kappa, int subtract_1pt, double2* coef);
+                             const Pt64* quad, double kappa, int subtract_1pt,
+                             const unsigned* cellkey, double2* coef);
 
 // Work decomposition of the near-field SpMV (built on the host from the
 // tree): work items (leaf, first row, first column, columns) of <= 256 rows x

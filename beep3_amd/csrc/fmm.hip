@@ -26,6 +26,7 @@
 // T_k(xi) T_k(u), u = box-local coordinate in [-1, 1].
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <unordered_map>
 #include <vector>
@@ -38,19 +39,6 @@ namespace beepdev {
 namespace {
 
 constexpr double PI = 3.14159265358979323846;
-
-template <typename T>
-struct Fm;
-template <>
-struct Fm<float> {
-    static __device__ __forceinline__ float rsqrt_(float v) { return __frsqrt_rn(v); }
-    static __device__ __forceinline__ float exp_(float v) { return __expf(v); }
-};
-template <>
-struct Fm<double> {
-    static __device__ __forceinline__ double rsqrt_(double v) { return 1.0 / sqrt(v); }
-    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
-};
 
 // 1-D interpolation basis and derivative at u for all P nodes.
 // tn[k * P + j] = T_j(xi_k).
@@ -540,23 +528,6 @@ __global__ __launch_bounds__(64) void k_l2p(const FmmDev d, const Patch64* __res
     }
 }
 
-// 1/sqrt(r2), and 0 for r2 == 0 (the self pair: target and source are the
-// same centroid, formed from the same bits, so every term it feeds vanishes).
-__device__ __forceinline__ float rsq_or_zero(float r2) {
-    const float ir = __builtin_amdgcn_rsqf(r2);
-    return r2 > 0.f ? ir : 0.f;
-}
-__device__ __forceinline__ double rsq_or_zero(double r2) {
-    // hardware estimate + two Newton steps y <- y + y (1/2 - (r2/2) y^2)
-    double yv = __builtin_amdgcn_rsq(r2);
-    const double h = 0.5 * r2;
-    double e = fma(-h * yv, yv, 0.5);
-    yv = fma(yv, e, yv);
-    e = fma(-h * yv, yv, 0.5);
-    yv = fma(yv, e, yv);
-    return r2 > 0.0 ? yv : 0.0;
-}
-
 template <typename T>
 struct P2PCap {
     static constexpr int value = 256;  // 16 KB of FP64 sources
@@ -634,25 +605,8 @@ __global__ __launch_bounds__(256) void k_p2p(const FmmDev d, const Src64* __rest
             if (valid) {
                 for (int jj = phase; jj < m; jj += ns) {
                     const SrcR<T> q = tile[jj];
-                    const T dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-                    const T r2 = dx * dx + dy * dy + dz * dz;
-                    const T ir = rsq_or_zero(r2);
-                    const T r = r2 * ir;
-                    const T ir2 = ir * ir;
-                    const T ir3 = ir2 * ir;
-                    const T kr = kappa * r;
-                    const T e = Fm<T>::exp_(-kr);
-                    const T u = e * ir;
-                    const T g1 = e * (T(1) + kr) * ir3;
-                    const T ir5 = ir3 * ir2;
-                    const T gL2 = T(3) * ir5;
-                    const T g2 = e * (kr * kr + T(3) * kr + T(3)) * ir5;
-                    const T dd = q.dx * dx + q.dy * dy + q.dz * dz;
-                    const T nd = n0x * dx + n0y * dy + n0z * dz;
-                    const T ndv = n0x * q.dx + n0y * q.dy + n0z * q.dz;
-                    af += q.sh * (u - ir) + dd * (g1 - q.ie * ir3);
-                    ah += -q.sh * nd * ir3 +
-                          q.ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + q.sh * nd * g1);
+                    p2p_pair<T>(q.x - xi, q.y - yi, q.z - zi, q.ie, q.dx, q.dy, q.dz, q.sh, n0x,
+                                n0y, n0z, kappa, af, ah);
                 }
             }
         }
@@ -810,12 +764,34 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     };
     int D = 0;
     for (; D < 12; ++D) {
-        std::unordered_map<uint64_t, int> occ;
-        occ.reserve(size_t(N));
-        for (int i = 0; i < N; ++i) occ[bkey(cell(i, D, 0), cell(i, D, 1), cell(i, D, 2))]++;
-        if (double(N) / double(occ.size()) <= leaf_target) break;
+        // occupied boxes: a bitmap over the level's cells up to 2^24 of them
+        // (N inserts into a hash map per level cost tens of ms at 10^5 patches)
+        size_t occupied = 0;
+        if (D <= 8) {
+            const size_t cells = size_t(1) << (3 * D);
+            std::vector<uint64_t> bits((cells + 63) / 64, 0);
+            for (int i = 0; i < N; ++i) {
+                const size_t c = (size_t(cell(i, D, 0)) << (2 * D)) | (size_t(cell(i, D, 1)) << D) |
+                                 size_t(cell(i, D, 2));
+                const uint64_t m = uint64_t(1) << (c & 63);
+                if (!(bits[c >> 6] & m)) {
+                    bits[c >> 6] |= m;
+                    ++occupied;
+                }
+            }
+        } else {
+            std::vector<uint64_t> keys(N);
+            for (int i = 0; i < N; ++i) keys[i] = bkey(cell(i, D, 0), cell(i, D, 1), cell(i, D, 2));
+            std::sort(keys.begin(), keys.end());
+            occupied = size_t(std::unique(keys.begin(), keys.end()) - keys.begin());
+        }
+        if (double(N) / double(occupied) <= leaf_target) break;
     }
     pl->D = D;
+    {
+        static std::atomic<unsigned long long> next_uid{1};
+        pl->uid = next_uid++;
+    }
     // sort patches by leaf key (Morton-interleaved for locality)
     auto morton = [&](uint32_t x, uint32_t y, uint32_t z) {
         uint64_t m = 0;
@@ -1002,6 +978,22 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     upload(arena, "child_oct", pl->dv.child_oct, child_oct, s);
     upload(arena, "leaf_src", pl->dv.leaf_src, leaf_src, s);
     upload(arena, "perm", pl->dv.perm, perm, s);
+    // patch -> FMM order, FMM order -> leaf box, and the packed leaf cell of
+    // every patch (the leaf-pass neighbourhood test of the near coefficients);
+    // the host copies live until the stream is synchronised below
+    std::vector<int> iperm(N), tleaf(N);
+    std::vector<unsigned> ck;
+    for (int k = 0; k < N; ++k) iperm[perm[k]] = k;
+    for (int q = 0; q < nleaf; ++q)
+        for (int k = leaf_src[q]; k < leaf_src[q + 1]; ++k) tleaf[k] = q;
+    upload(arena, "iperm", pl->d_iperm, iperm, s);
+    upload(arena, "tleaf", pl->d_tleaf, tleaf, s);
+    if (D <= FMM_CELL_BITS) {
+        ck.resize(N);
+        for (int i = 0; i < N; ++i)
+            ck[i] = cx[i] | (cy[i] << FMM_CELL_BITS) | (cz[i] << (2 * FMM_CELL_BITS));
+        upload(arena, "cellkey", pl->cellkey, ck, s);
+    }
     upload(arena, "nb_ptr", pl->dv.nb_ptr, nb_ptr, s);
     upload(arena, "nb_leaf", pl->dv.nb_leaf, nb_leaf, s);
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
@@ -1182,6 +1174,11 @@ int fmm_canonical(int tx, int ty, int tz, int order, int* canon_xyz, int* node_m
 
 void fmm_p2p(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, double kappa,
              double* y) {
+    if (pl->n_res >= 0) {
+        // the near coefficients hold the explicit pairs of the leaf pass in full
+        if (pl->n_res > 0) launch_p2p_residual(pl, s, fp32, patches, kappa, y);
+        return;
+    }
     if (fp32)
         hipLaunchKernelGGL((k_p2p<float>), dim3(pl->nleaf), dim3(256), 0, s, pl->dv, pl->src,
                            patches, float(kappa), y, pl->N);

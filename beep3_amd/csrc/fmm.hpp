@@ -34,13 +34,62 @@ struct FmmDev {
     int* tgt_pair = nullptr;
     double* tn = nullptr;       // T_j(xi_k), P x P
     double* t_oct = nullptr;    // child-node interpolation factors, 2 x P x P
+    // leaf-pass pairs that are not explicit near pairs (the others are folded
+    // into the near coefficients): per target in FMM order, sources in FMM
+    // order
+    int* res_ptr = nullptr;
+    int* res_src = nullptr;
 };
+
+// 1/sqrt(r2), and 0 for r2 == 0 (the self pair: target and source are the
+// same centroid, formed from the same bits, so every term it feeds vanishes).
+__device__ __forceinline__ float rsq_or_zero(float r2) {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    return r2 > 0.f ? ir : 0.f;
+}
+__device__ __forceinline__ double rsq_or_zero(double r2) {
+    const double ir = rsqrt_nr(r2);
+    return r2 > 0.0 ? ir : 0.0;
+}
+__device__ __forceinline__ float exp_fast(float v) { return __expf(v); }
+__device__ __forceinline__ double exp_fast(double v) { return exp(v); }
+
+// 1-point interaction of one packed source (relative position d = source -
+// target, 1/eps ie, dipole a f n = (sdx, sdy, sdz), monopole a h = sh) with
+// a target of normal n0, added to the target's f (af) and h (ah) sums
+// without the factor 1/4pi (the three potentials of kernels_far.hip).
+template <typename T>
+__device__ __forceinline__ void p2p_pair(T dx, T dy, T dz, T ie, T sdx, T sdy, T sdz, T sh,
+                                         T n0x, T n0y, T n0z, T kappa, T& af, T& ah) {
+    const T r2 = dx * dx + dy * dy + dz * dz;
+    const T ir = rsq_or_zero(r2);
+    const T r = r2 * ir;
+    const T ir2 = ir * ir;
+    const T ir3 = ir2 * ir;
+    const T kr = kappa * r;
+    const T e = exp_fast(-kr);
+    const T u = e * ir;
+    const T g1 = e * (T(1) + kr) * ir3;
+    const T ir5 = ir3 * ir2;
+    const T gL2 = T(3) * ir5;
+    const T g2 = e * (kr * kr + T(3) * kr + T(3)) * ir5;
+    const T dd = sdx * dx + sdy * dy + sdz * dz;
+    const T nd = n0x * dx + n0y * dy + n0z * dz;
+    const T ndv = n0x * sdx + n0y * sdy + n0z * sdz;
+    af += sh * (u - ir) + dd * (g1 - ie * ir3);
+    ah += -sh * nd * ir3 + ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + sh * nd * g1);
+}
 
 struct FmmPlan {
     static constexpr int CHUNK = 64;  // M2L pairs per block
     static constexpr int NCANON = 16; // canonical M2L offsets
+    unsigned long long uid = 0;       // unique per plan built in the process
     int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0;
     long long n_p2p = 0;              // target-source pairs of the leaf pass (self included)
+    unsigned* cellkey = nullptr;      // per patch (original index); null when D > 10
+    int* d_iperm = nullptr;           // patch -> FMM order
+    int* d_tleaf = nullptr;           // FMM order -> leaf box
+    long long n_res = -1;             // residual leaf-pass pairs (-1: not built)
     double edge = 0;
     size_t n_exp = 0;                 // values per expansion set (nbox * 3 * P^3)
     std::vector<int> lvl_first;       // D + 2
@@ -84,7 +133,16 @@ void fmm_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, c
                   double kappa, double* y, hipEvent_t* m2l_ev = nullptr,
                   hipEvent_t pack_ev = nullptr);
 // y += the P2P part (neighbour leaf boxes) of the sources packed by the last
-// fmm_evaluate (stream-ordered after its pack)
+// fmm_evaluate (stream-ordered after its pack): with a residual list, only
+// the pairs the near coefficients do not hold in full
 void fmm_p2p(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, double kappa, double* y);
+// Build the residual leaf-pass list against the explicit near pairs (per
+// near-list leaf L: rows leaf_idx[leaf_ptr[L]..], columns nb_idx[nb_ptr[L]..],
+// device arrays); needs cellkey.  Synchronises the stream.
+void fmm_build_residual(FmmPlan* p, int n_leaves, const int* leaf_ptr, const int* leaf_idx,
+                        const int* nb_ptr, const int* nb_idx, long long n_nb, hipStream_t s);
+// y += the residual leaf pass (fmm_p2p with a residual list)
+void launch_p2p_residual(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches,
+                         double kappa, double* y);
 
 }  // namespace beepdev

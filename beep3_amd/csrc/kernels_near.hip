@@ -140,7 +140,9 @@ void launch_singular(hipStream_t s, int n, const Patch64* patches, const double*
 // thread integrates the qualocation points of row i against the quadrature
 // points of column j; with subtract_1pt it stores the difference to the
 // 1-point centroid interaction, which the far-field kernels add back for
-// every pair.
+// every pair, except for pairs whose packed FMM leaf cells (cellkey, when
+// given) are adjacent: the FMM leaf pass leaves those out, so they keep the
+// full rule.
 __device__ __forceinline__ int find_leaf(const int* __restrict__ blk_ptr, int n_leaves, int b) {
     int lo = 0, hi = n_leaves;  // blk_ptr[lo] <= b < blk_ptr[hi]
     while (hi - lo > 1) {
@@ -198,7 +200,8 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
     const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,
     const Patch64* __restrict__ patches, const int* __restrict__ qual_ptr,
     const Pt<T>* __restrict__ qual, const int* __restrict__ quad_ptr,
-    const Pt<T>* __restrict__ quad, T kappa, int subtract_1pt, void* __restrict__ coef) {
+    const Pt<T>* __restrict__ quad, T kappa, int subtract_1pt,
+    const unsigned* __restrict__ cellkey, void* __restrict__ coef) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int L = find_leaf(blk_ptr, n_leaves, b);
     const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
@@ -215,7 +218,10 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
         const Patch64 pj = patches[j];
         const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
         const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
-        pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, subtract_1pt, out);
+        // pairs the FMM leaf pass covers keep the full rule (that pass then
+        // skips them)
+        const bool sub = subtract_1pt && !(cellkey && fmm_cells_adjacent(cellkey[i], cellkey[j]));
+        pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, sub, out);
     }
     const int64_t idx = coef_off[L] + e;
     if constexpr (sizeof(T) == 4) {
@@ -231,12 +237,13 @@ void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
-                             const Pt32* quad, float kappa, int subtract_1pt, float4* coef) {
+                             const Pt32* quad, float kappa, int subtract_1pt,
+                             const unsigned* cellkey, float4* coef) {
     if (n_blocks <= 0) return;
     auto k = kappa != 0.0f ? k_near_coeffs<float, true> : k_near_coeffs<float, false>;
     hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
                        nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
-                       subtract_1pt, (void*)coef);
+                       subtract_1pt, cellkey, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<float> launch");
 }
 
@@ -244,12 +251,13 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
-                             const Pt64* quad, double kappa, int subtract_1pt, double2* coef) {
+                             const Pt64* quad, double kappa, int subtract_1pt,
+                             const unsigned* cellkey, double2* coef) {
     if (n_blocks <= 0) return;
     auto k = kappa != 0.0 ? k_near_coeffs<double, true> : k_near_coeffs<double, false>;
     hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
                        nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
-                       subtract_1pt, (void*)coef);
+                       subtract_1pt, cellkey, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<double> launch");
 }
 

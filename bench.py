@@ -441,11 +441,15 @@ def main():
         # reference energy: FP64 coefficients, exact direct far field (the
         # explicit-pair + 1-point operator the oracle defines), FP64 GMRES
         if args.far != "direct":
+            # the near coefficients follow the far-field plan
             dev.set_far_field("direct")
+            dev.precompute(b.kappa, "fp64")
             b.solve(1e-8, 100, mode="fp64")
             E_ref = b.calculate_energies()
             f_ref, h_ref = b.f.copy(), b.h.copy()
             dev.set_far_field(args.far, args.order, args.leaf)
+            for p in ("fp64", "fp32"):
+                dev.precompute(b.kappa, p)
         else:
             E_ref = solves["fp64"]["energy_kJ_mol"]
             f_ref, h_ref = fh["fp64"]

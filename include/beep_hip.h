@@ -166,7 +166,10 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* surface);
  *                    leaf boxes holding about leaf_target patches,
  *   BEEP_FAR_AUTO    FMM above 20,000 patches, direct below.
  * Takes effect immediately (kernel tables are built by the first mat-vec or
- * precompute that needs them); default BEEP_FAR_AUTO, order 4, leaf_target 48. */
+ * precompute that needs them); default BEEP_FAR_AUTO, order 4, leaf_target 48.
+ * The near coefficients depend on the FMM plan (explicit pairs that its leaf
+ * pass also covers keep the full rule, and that pass skips them): after a
+ * change call beep_hip_precompute again, or the mat-vec returns an error. */
 #define BEEP_FAR_AUTO (-1)
 #define BEEP_FAR_DIRECT 0
 #define BEEP_FAR_FMM 1
@@ -193,8 +196,11 @@ int beep_hip_set_graphs(beep_hip_ctx* ctx, int on);
  * patches per leaf box) at the current order, writes the mean time per
  * mat-vec of each to ms_out (n values, may be NULL), keeps the fastest and
  * returns it in *best.  Needs the geometry, tree and coefficients of that
- * precision; switches the far field to BEEP_FAR_FMM.  No counterpart in the
- * reference, whose leaf size follows from nbsize (beep.h:67-74). */
+ * precision; switches the far field to BEEP_FAR_FMM.  Each candidate plan
+ * gets its own coefficients (precomputed per candidate), and every
+ * coefficient set the context held is precomputed again for the chosen one.
+ * No counterpart in the reference, whose leaf size follows from nbsize
+ * (beep.h:67-74). */
 int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_targets, int n,
                            int reps, double* ms_out, int* best);
 
@@ -218,8 +224,11 @@ int beep_hip_autotune_spmv_depth(beep_hip_ctx* ctx, int precision, const int* de
                                  int reps, double* ms_out, int* best);
 
 /* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
- * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs
- * and P2P target-source pairs (zeros for the direct sum). */
+ * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs,
+ * P2P target-source pairs of the leaf pass, the pairs among them that are not
+ * explicit near pairs (the only ones the leaf pass evaluates once the near
+ * list is set; -1 before), and the host milliseconds spent building the plan
+ * and that residual list (zeros for the direct sum). */
 typedef struct {
     int mode;
     int order;
@@ -228,11 +237,15 @@ typedef struct {
     int n_boxes;
     long long m2l_pairs;
     long long p2p_pairs;
+    long long p2p_residual_pairs;
+    double plan_ms;
 } beep_far_info;
 int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out);
 
 /* Singular self terms (FP64 always) and near-field coefficients of the given
- * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set). */
+ * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set).
+ * With the FMM far field, builds its plan first (the coefficients follow
+ * it). */
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);
 
 /* y = K x on host arrays (do_bem_fmm_iteration); blocks until y is written. */

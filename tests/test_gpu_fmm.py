@@ -132,3 +132,27 @@ def test_spmv_residency_cap_is_bit_identical(off_centre_arrays):
     print("SpMV residency autotune (ms per FP32 mat-vec):", table, "best", best)
     assert table[best] == min(table.values())
     d.close()
+
+
+def test_leaf_pass_folded_into_near_coefficients(off_centre_arrays):
+    """With the FMM, explicit pairs that its leaf pass covers keep the full
+    rule and the leaf pass runs over the other neighbouring-leaf pairs only:
+    that residual is a small part of the leaf pass, the product still meets
+    the order-5 bound against the direct far field, and a far-field change
+    without a new precompute is refused."""
+    a, kappa = off_centre_arrays
+    x = np.random.default_rng(13).normal(size=2 * a["area"].shape[0])
+    d = _device(a, kappa, "fmm", 5, 40, precs=("fp64", "fp32"))
+    fi = d.far_info()
+    print("leaf pass:", fi["p2p_pairs"], "pairs,", fi["p2p_residual_pairs"], "not explicit;",
+          f"plan + residual list {fi['plan_ms']:.1f} ms")
+    assert 0 <= fi["p2p_residual_pairs"] < 0.5 * fi["p2p_pairs"]
+    y64, y32 = d.matvec(x, "fp64"), d.matvec(x, "fp32")
+    d.set_far_field("direct")
+    with pytest.raises(RuntimeError, match="precompute"):
+        d.matvec(x, "fp64")
+    d.precompute(kappa, "fp64")
+    y_dir = d.matvec(x, "fp64")
+    assert rel(y64, y_dir) < 4e-5
+    assert rel(y32, y_dir) < 4e-5
+    d.close()
