@@ -82,9 +82,14 @@ __global__ __launch_bounds__(256) void k_res_scan(int n, const FmmDev d,
     if (!EMIT) out[k] = c;
 }
 
-// Residual leaf pass: one thread per target (FMM order), sources from the
-// packed list of the mat-vec (k_far_pack), positions relative to the target
-// formed in FP64.
+// Residual leaf pass: 16 lanes per target (FMM order), each lane taking
+// every 16th source of the target's list, then a fixed-order shuffle
+// reduction (lists are short and uneven: a thread per target would leave
+// most lanes waiting on the longest list).  Sources come from the packed
+// list of the mat-vec (k_far_pack), positions relative to the target formed
+// in FP64.
+constexpr int RES_LANES = 16;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_p2p_res(int n, const int* __restrict__ perm,
                                                  const int* __restrict__ res_ptr,
@@ -92,21 +97,32 @@ __global__ __launch_bounds__(256) void k_p2p_res(int n, const int* __restrict__ 
                                                  const Src64* __restrict__ src,
                                                  const Patch64* __restrict__ patches, T kappa,
                                                  double* __restrict__ y) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    const int r0 = res_ptr[k], r1 = res_ptr[k + 1];
-    if (r0 == r1) return;
-    const int i = perm[k];
-    const Patch64 p = patches[i];
-    const T n0x = T(p.nx), n0y = T(p.ny), n0z = T(p.nz);
+    const int k = blockIdx.x * (256 / RES_LANES) + threadIdx.x / RES_LANES;
+    const int lane = threadIdx.x % RES_LANES;
+    const bool valid = k < n;
+    const int r0 = valid ? res_ptr[k] : 0, r1 = valid ? res_ptr[k + 1] : 0;
     T af = 0, ah = 0;
-    for (int r = r0; r < r1; ++r) {
-        const Src64 q = src[res_src[r]];
-        p2p_pair<T>(T(q.x - p.cx), T(q.y - p.cy), T(q.z - p.cz), T(q.inv_eps), T(q.dx), T(q.dy),
-                    T(q.dz), T(q.sh), n0x, n0y, n0z, kappa, af, ah);
+    int i = 0;
+    if (r0 < r1) {
+        i = perm[k];
+        const Patch64 p = patches[i];
+        const T n0x = T(p.nx), n0y = T(p.ny), n0z = T(p.nz);
+        for (int r = r0 + lane; r < r1; r += RES_LANES) {
+            const Src64 q = src[res_src[r]];
+            p2p_pair<T>(T(q.x - p.cx), T(q.y - p.cy), T(q.z - p.cz), T(q.inv_eps), T(q.dx),
+                        T(q.dy), T(q.dz), T(q.sh), n0x, n0y, n0z, kappa, af, ah);
+        }
     }
-    y[i] += double(af) * FOUR_PI_INV;
-    y[n + i] += double(ah) * FOUR_PI_INV;
+    // every lane of the wave takes part in the shuffles
+#pragma unroll
+    for (int off = RES_LANES / 2; off > 0; off >>= 1) {
+        af += __shfl_down(af, off, RES_LANES);
+        ah += __shfl_down(ah, off, RES_LANES);
+    }
+    if (r0 < r1 && lane == 0) {
+        y[i] += double(af) * FOUR_PI_INV;
+        y[n + i] += double(ah) * FOUR_PI_INV;
+    }
 }
 
 }  // namespace
@@ -167,7 +183,8 @@ void fmm_build_residual(FmmPlan* pl, int n_leaves, const int* leaf_ptr, const in
 void launch_p2p_residual(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches,
                          double kappa, double* y) {
     const int N = pl->N;
-    const unsigned blocks = unsigned((N + 255) / 256);
+    constexpr int per_block = 256 / RES_LANES;
+    const unsigned blocks = unsigned((N + per_block - 1) / per_block);
     if (fp32)
         hipLaunchKernelGGL((k_p2p_res<float>), dim3(blocks), dim3(256), 0, s, N, pl->dv.perm,
                            pl->dv.res_ptr, pl->dv.res_src, pl->src, patches, float(kappa), y);
