@@ -62,8 +62,11 @@ struct beep_hip_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_pack = nullptr;
     bool overlap = true;
     int n_cu = 256;               // compute units of the device
-    int spmv_blocks_per_cu = 1;   // near-SpMV residency cap (0: one block per work item)
-    int spmv_depth = 1;           // near-SpMV loads in flight per thread (x 4 FP64, x 8 FP32)
+    // near-SpMV settings per coefficient mode (0: FP64, 1: FP32, 2: FP32
+    // coefficients with FP64 sums): residency cap in blocks per CU (0: one
+    // block per work item) and loads in flight per thread (x 4 FP64, x 8 FP32)
+    int spmv_cap[3] = {1, 1, 1};
+    int spmv_depth[3] = {1, 1, 1};
     std::string err;
 
     // geometry
@@ -272,6 +275,11 @@ void timing_collect(beep_hip_ctx* c) {
     while (c->ring_pending > 0) timing_pop(c);
 }
 
+// near-SpMV coefficient mode of a mat-vec precision
+int spmv_mode(int precision) {
+    return precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
+}
+
 void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     require(c->have_geom && c->have_tree, "geometry and tree must be set before a mat-vec");
     require(c->have_diag, "precompute before the mat-vec");
@@ -315,10 +323,10 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     if (timed) hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
     if (c->overlap) hip_check(hipEventRecord(c->ev_join, sf), "hipEventRecord");
     // near field (explicit pairs) + diagonal into y
-    const int mode = precision == BEEP_FP64 ? 0 : (precision == BEEP_FP32 ? 1 : 2);
+    const int mode = spmv_mode(precision);
     launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                      c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
-                     c->spmv_part.p, dy, c->spmv_blocks_per_cu * c->n_cu, c->spmv_depth);
+                     c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
     if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->use_fmm) {
         // the P2P part (VALU) follows the SpMV on this stream, beside the
@@ -386,8 +394,8 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
         c->yfar.p, c->src.p, c->coef32.p, c->coef64.p, c->diag.p, c->spmv_part.p, c->fmm,
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
-        reinterpret_cast<const void*>(intptr_t(c->spmv_blocks_per_cu)),
-        reinterpret_cast<const void*>(intptr_t(c->spmv_depth))};
+        reinterpret_cast<const void*>(intptr_t(c->spmv_cap[spmv_mode(precision)])),
+        reinterpret_cast<const void*>(intptr_t(c->spmv_depth[spmv_mode(precision)]))};
     // graphs of an older generation bake in buffers or plans that changed
     for (size_t k = c->step_graphs.size(); k-- > 0;)
         if (c->step_graphs[k].key[0] != key[0]) {
@@ -971,7 +979,7 @@ int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu) {
         require(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks per CU must be 0..64");
         set_device(ctx);
         sync_streams(ctx);
-        ctx->spmv_blocks_per_cu = blocks_per_cu;
+        for (int& v : ctx->spmv_cap) v = blocks_per_cu;
         ctx->gen++;
     });
 }
@@ -981,16 +989,20 @@ int beep_hip_set_spmv_depth(beep_hip_ctx* ctx, int depth) {
         require(depth == 1 || depth == 2 || depth == 4, "SpMV depth must be 1, 2 or 4");
         set_device(ctx);
         sync_streams(ctx);
-        ctx->spmv_depth = depth;
+        for (int& v : ctx->spmv_depth) v = depth;
         ctx->gen++;
     });
 }
 
 // Times `reps` whole mat-vecs (after one warm-up) for each candidate value of
-// one SpMV setting, keeps the fastest.  `field` is the context member tuned.
-static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int beep_hip_ctx::*field,
+// one SpMV setting, keeps the fastest.  `table` is the per-mode setting; the
+// entry of the precision's mode is tuned.
+static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int (beep_hip_ctx::*table)[3],
                               const int* values, int n, int reps, bool (*valid)(int),
                               const char* what, double* ms_out, int* best) {
+    require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
+            "unknown precision");
+    int* field = &(ctx->*table)[spmv_mode(precision)];
     require(values && n > 0 && reps > 0, "bad autotune arguments");
     require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
     for (int k = 0; k < n; ++k) require(valid(values[k]), what);
@@ -1002,12 +1014,12 @@ static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int beep_hip_ctx
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
-    const int saved = ctx->*field;
+    const int saved = *field;
     double best_ms = 1e300;
     int best_v = saved;
     try {
         for (int k = 0; k < n; ++k) {
-            ctx->*field = values[k];
+            *field = values[k];
             matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);  // warm-up
             hip_check(hipEventRecord(e0, ctx->stream), "hipEventRecord");
             for (int r = 0; r < reps; ++r) matvec_dev(ctx, ctx->xbuf.p, ctx->ybuf.p, precision);
@@ -1025,14 +1037,14 @@ static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int beep_hip_ctx
     } catch (...) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        ctx->*field = saved;
+        *field = saved;
         throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     timing_collect(ctx);
     ctx->timing = beep_matvec_timing{};
-    ctx->*field = best_v;
+    *field = best_v;
     ctx->gen++;
     if (best) *best = best_v;
 }
@@ -1040,7 +1052,7 @@ static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int beep_hip_ctx
 int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
                            int reps, double* ms_out, int* best) {
     return guarded(ctx, [&] {
-        tune_spmv_setting(ctx, precision, &beep_hip_ctx::spmv_blocks_per_cu, blocks_per_cu, n, reps,
+        tune_spmv_setting(ctx, precision, &beep_hip_ctx::spmv_cap, blocks_per_cu, n, reps,
                           [](int v) { return v >= 0 && v <= 64; }, "blocks per CU must be 0..64",
                           ms_out, best);
     });

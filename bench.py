@@ -312,11 +312,12 @@ def main():
         tune = {"leaf_ms_fp64": table, "chosen_leaf": best}
         args.leaf = best
         # near-SpMV load depth at the default cap, then the residency cap at
-        # that depth (room for the concurrent far field)
-        depth, dtable = dev.autotune_spmv_depth("fp64", (1, 2, 4), reps=3)
-        cap, ctable = dev.autotune_spmv("fp64", (0, 1, 2, 3, 4, 6), reps=3)
-        tune.update(spmv_depth_ms_fp64=dtable, chosen_spmv_depth=depth,
-                    spmv_blocks_per_cu_ms_fp64=ctable, chosen_spmv_blocks_per_cu=cap)
+        # that depth (room for the concurrent far field), per precision
+        for p in ("fp64", "fp32", "fp32_resid"):
+            depth, dtable = dev.autotune_spmv_depth(p, (1, 2, 4), reps=3)
+            cap, ctable = dev.autotune_spmv(p, (0, 1, 2, 3, 4, 6), reps=3)
+            tune[f"spmv_{p}"] = {"depth_ms": dtable, "depth": depth,
+                                 "blocks_per_cu_ms": ctable, "blocks_per_cu": cap}
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
 
     # precompute timings (integrand throughput)
@@ -354,8 +355,6 @@ def main():
     # SpMV has the whole chip and memory system), for its roofline
     # at each load depth; the fastest depth gives the roofline figure
     dev.set_overlap(False)
-    cap = tune["chosen_spmv_blocks_per_cu"] if tune else 1  # the library defaults
-    depth = tune["chosen_spmv_depth"] if tune else 1
     dev.set_spmv_residency(0)
     alone = {}
     for dd in (1, 2):
@@ -367,8 +366,13 @@ def main():
     best_d32 = min(alone, key=lambda k: alone[k][2])
     near_alone, far_alone = alone[best_d64][0], alone[best_d64][1]
     near32_alone, far32_alone = alone[best_d32][2], alone[best_d32][3]
-    dev.set_spmv_residency(cap)
-    dev.set_spmv_depth(depth)
+    # back to the tuned settings of every precision (or the library defaults);
+    # a tuner run over a single candidate sets one precision's value
+    dev.set_spmv_residency(1)
+    dev.set_spmv_depth(1)
+    for p in (("fp64", "fp32", "fp32_resid") if tune else ()):
+        dev.autotune_spmv_depth(p, (tune[f"spmv_{p}"]["depth"],), reps=1)
+        dev.autotune_spmv(p, (tune[f"spmv_{p}"]["blocks_per_cu"],), reps=1)
     dev.set_overlap(True)
 
     nb64, cb64, nnz = near_bytes(a, 32)

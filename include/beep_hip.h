@@ -207,19 +207,22 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
 /* Residency cap of the near-field SpMV: at most blocks_per_cu blocks per
  * compute unit walk the work items (default 1; 0: one block per work item).
  * A cap leaves room on every CU for the far-field kernels that run on the
- * other stream at the same time. */
+ * other stream at the same time.  Sets the cap of every precision. */
 int beep_hip_set_spmv_residency(beep_hip_ctx* ctx, int blocks_per_cu);
 /* Autotuner for that cap: times `reps` whole mat-vecs of the given precision
  * (far field overlapped) for each candidate, writes the mean times to ms_out
- * (may be NULL), keeps the fastest and returns it in *best. */
+ * (may be NULL), keeps the fastest for mat-vecs of that precision (BEEP_FP64,
+ * BEEP_FP32 and BEEP_FP32_RESID each keep their own) and returns it in
+ * *best. */
 int beep_hip_autotune_spmv(beep_hip_ctx* ctx, int precision, const int* blocks_per_cu, int n,
                            int reps, double* ms_out, int* best);
 /* Coefficient loads each SpMV thread keeps in flight: 4 x depth (FP64
  * coefficients) or 8 x depth (FP32), depth 1 (default), 2 or 4.  Deeper
  * lets fewer resident blocks cover the memory latency (with a residency
- * cap).  y is bit-identical for every depth. */
+ * cap).  y is bit-identical for every depth.  Sets the depth of every
+ * precision. */
 int beep_hip_set_spmv_depth(beep_hip_ctx* ctx, int depth);
-/* Autotuner for the depth, as beep_hip_autotune_spmv. */
+/* Autotuner for the depth, as beep_hip_autotune_spmv (per precision). */
 int beep_hip_autotune_spmv_depth(beep_hip_ctx* ctx, int precision, const int* depths, int n,
                                  int reps, double* ms_out, int* best);
 
