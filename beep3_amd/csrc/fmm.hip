@@ -726,6 +726,56 @@ int canonical_offset(int tx, int ty, int tz, int* sym) {
     throw std::runtime_error("M2L offset outside the interaction range");
 }
 
+// canonical offset and symmetry index of every M2L offset t in [-3, 3]^3
+// (max |t_k| >= 2), looked up per pair while the plan is built
+struct OffsetClass {
+    int canon = -1, sym = -1;
+};
+const std::array<OffsetClass, 343>& offset_classes() {
+    static const std::array<OffsetClass, 343> tab = [] {
+        std::array<OffsetClass, 343> a{};
+        for (int tx = -3; tx <= 3; ++tx)
+            for (int ty = -3; ty <= 3; ++ty)
+                for (int tz = -3; tz <= 3; ++tz) {
+                    if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) < 2) continue;
+                    int code = 0;
+                    const int c = canonical_offset(tx, ty, tz, &code);
+                    a[size_t(((tx + 3) * 7 + (ty + 3)) * 7 + (tz + 3))] = {c, symmetry_index(code)};
+                }
+        return a;
+    }();
+    return tab;
+}
+
+// Boxes of one level: cell -> box number, a dense table when the level has
+// at most 2^21 cells, else a hash map
+struct BoxIndex {
+    int lev = 0;
+    std::vector<int> dense;
+    std::unordered_map<uint64_t, int> sparse;
+    explicit BoxIndex(int l = 0) : lev(l) {
+        if (3 * l <= 21) dense.assign(size_t(1) << (3 * l), -1);
+    }
+    size_t cell(uint32_t x, uint32_t y, uint32_t z) const {
+        return (size_t(x) << (2 * lev)) | (size_t(y) << lev) | size_t(z);
+    }
+    int find(uint32_t x, uint32_t y, uint32_t z) const {
+        if (!dense.empty()) return dense[cell(x, y, z)];
+        const auto it = sparse.find(bkey(x, y, z));
+        return it == sparse.end() ? -1 : it->second;
+    }
+    // true if the cell had no box yet (it now has v)
+    bool insert(uint32_t x, uint32_t y, uint32_t z, int v) {
+        if (!dense.empty()) {
+            int& r = dense[cell(x, y, z)];
+            if (r >= 0) return false;
+            r = v;
+            return true;
+        }
+        return sparse.emplace(bkey(x, y, z), v).second;
+    }
+};
+
 template <typename T>
 void upload(DevArena& ar, const char* name, T*& dst, const std::vector<T>& v, hipStream_t s) {
     dst = ar.get<T>(name, std::max<size_t>(v.size(), 1));
@@ -812,13 +862,12 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     // boxes per level; leaf level first
     struct Box { uint32_t x, y, z; };
     std::vector<std::vector<Box>> lev(D + 1);
-    std::vector<std::unordered_map<uint64_t, int>> idx(D + 1);
+    std::vector<BoxIndex> idx;
+    for (int l = 0; l <= D; ++l) idx.emplace_back(l);
     std::vector<int> leaf_src;
     for (int k = 0; k < N; ++k) {
         const int i = perm[k];
-        const uint64_t key = bkey(cx[i], cy[i], cz[i]);
-        if (!idx[D].count(key)) {
-            idx[D][key] = int(lev[D].size());
+        if (idx[D].insert(cx[i], cy[i], cz[i], int(lev[D].size()))) {
             lev[D].push_back({cx[i], cy[i], cz[i]});
             leaf_src.push_back(k);
         }
@@ -827,11 +876,7 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     for (int l = D - 1; l >= 0; --l)
         for (const Box& b : lev[l + 1]) {
             const Box pb{b.x >> 1, b.y >> 1, b.z >> 1};
-            const uint64_t key = bkey(pb.x, pb.y, pb.z);
-            if (!idx[l].count(key)) {
-                idx[l][key] = int(lev[l].size());
-                lev[l].push_back(pb);
-            }
+            if (idx[l].insert(pb.x, pb.y, pb.z, int(lev[l].size()))) lev[l].push_back(pb);
         }
     // global box numbering: level 0 first
     pl->lvl_first.assign(D + 2, 0);
@@ -850,7 +895,7 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
             box_c[g] = make_double4(x0[0] + (b.x + 0.5) * h, x0[1] + (b.y + 0.5) * h,
                                     x0[2] + (b.z + 0.5) * h, h);
             if (l > 0) {
-                const int par = pl->lvl_first[l - 1] + idx[l - 1].at(bkey(b.x >> 1, b.y >> 1, b.z >> 1));
+                const int par = pl->lvl_first[l - 1] + idx[l - 1].find(b.x >> 1, b.y >> 1, b.z >> 1);
                 const int oct = int(b.x & 1) | (int(b.y & 1) << 1) | (int(b.z & 1) << 2);
                 box_parent[g] = par;
                 box_oct[g] = oct;
@@ -874,8 +919,8 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                     const long x = long(b.x) + dx, y = long(b.y) + dy, z = long(b.z) + dz;
                     const long cells = 1L << D;
                     if (x < 0 || y < 0 || z < 0 || x >= cells || y >= cells || z >= cells) continue;
-                    auto it = idx[D].find(bkey(uint32_t(x), uint32_t(y), uint32_t(z)));
-                    if (it != idx[D].end()) nb_leaf.push_back(it->second);
+                    const int q2 = idx[D].find(uint32_t(x), uint32_t(y), uint32_t(z));
+                    if (q2 >= 0) nb_leaf.push_back(q2);
                 }
         nb_ptr[q + 1] = int(nb_leaf.size());
     }
@@ -904,18 +949,18 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
                     for (int dx = -1; dx <= 1; ++dx) {
                         const long X = px + dx, Y = py + dy, Z = pz + dz;
                         if (X < 0 || Y < 0 || Z < 0 || X >= pcells || Y >= pcells || Z >= pcells) continue;
-                        auto pit = idx[l - 1].find(bkey(uint32_t(X), uint32_t(Y), uint32_t(Z)));
-                        if (pit == idx[l - 1].end()) continue;
-                        const int pg = pl->lvl_first[l - 1] + pit->second;
+                        const int pq = idx[l - 1].find(uint32_t(X), uint32_t(Y), uint32_t(Z));
+                        if (pq < 0) continue;
+                        const int pg = pl->lvl_first[l - 1] + pq;
                         for (auto& kc : kids[pg]) {
                             const int sg = kc.first;
                             const Box& sb = lev[l][sg - pl->lvl_first[l]];
                             const int tx = int(long(sb.x) - long(b.x)), ty = int(long(sb.y) - long(b.y)),
                                       tz = int(long(sb.z) - long(b.z));
                             if (std::max(std::abs(tx), std::max(std::abs(ty), std::abs(tz))) <= 1) continue;
-                            int sym = 0;
-                            const int c = canonical_offset(tx, ty, tz, &sym);
-                            byoff[c].push_back({pl->lvl_first[l] + int(q), sg, symmetry_index(sym)});
+                            const OffsetClass& oc =
+                                offset_classes()[size_t(((tx + 3) * 7 + (ty + 3)) * 7 + (tz + 3))];
+                            byoff[oc.canon].push_back({pl->lvl_first[l] + int(q), sg, oc.sym});
                         }
                     }
         }

@@ -54,7 +54,11 @@ __device__ __forceinline__ bool sorted_contains(const int* __restrict__ a, int n
 }
 
 // EMIT = false: count[k] = residual sources of target k; EMIT = true: write
-// them at out[res_ptr[k]..] in neighbour-list order, increasing index
+// them at out[res_ptr[k]..] in neighbour-list order, increasing index.  16
+// lanes per target walk its neighbour boxes' sources side by side; a ballot
+// over the group orders each step's finds.
+constexpr int SCAN_LANES = 16;
+
 template <bool EMIT>
 __global__ __launch_bounds__(256) void k_res_scan(int n, const FmmDev d,
                                                   const int* __restrict__ tleaf,
@@ -63,23 +67,29 @@ __global__ __launch_bounds__(256) void k_res_scan(int n, const FmmDev d,
                                                   const int* __restrict__ keys,
                                                   const int* __restrict__ res_ptr,
                                                   int* __restrict__ out) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
+    const int k = blockIdx.x * (256 / SCAN_LANES) + int(threadIdx.x) / SCAN_LANES;
+    const int lane = int(threadIdx.x) % SCAN_LANES;
+    const int wl = int(threadIdx.x) % 64;  // lane in the wave
+    const unsigned long long gmask = 0xFFFFull << (wl & ~(SCAN_LANES - 1));
+    if (k >= n) return;  // the whole group: every lane of it has this k
     const int A = tleaf[k];
     const int L = leaf_of[d.perm[k]];
     const int* S = keys + nb_ptr[L];
     const int C = nb_ptr[L + 1] - nb_ptr[L];
-    int c = 0;
     const int o = EMIT ? res_ptr[k] : 0;
+    int c = 0;
     for (int q = d.nb_ptr[A]; q < d.nb_ptr[A + 1]; ++q) {
         const int B = d.nb_leaf[q];
-        for (int s = d.leaf_src[B]; s < d.leaf_src[B + 1]; ++s) {
-            if (s == k || sorted_contains(S, C, s)) continue;
-            if (EMIT) out[o + c] = s;
-            ++c;
+        const int s0 = d.leaf_src[B], s1 = d.leaf_src[B + 1];
+        for (int sb = s0; sb < s1; sb += SCAN_LANES) {
+            const int s = sb + lane;
+            const bool keep = s < s1 && s != k && !sorted_contains(S, C, s);
+            const unsigned long long m = __ballot(keep) & gmask;
+            if (EMIT && keep) out[o + c + __popcll(m & ((1ull << wl) - 1))] = s;
+            c += __popcll(m);
         }
     }
-    if (!EMIT) out[k] = c;
+    if (!EMIT && lane == 0) out[k] = c;
 }
 
 // Residual leaf pass: 16 lanes per target (FMM order), each lane taking
@@ -154,7 +164,7 @@ void fmm_build_residual(FmmPlan* pl, int n_leaves, const int* leaf_ptr, const in
     hip_check(hipGetLastError(), "k_res_leaf_of launch");
     // 2. count, scan, emit
     int* count = ar.get<int>("res_count", size_t(N));
-    const unsigned blocks = unsigned((N + 255) / 256);
+    const unsigned blocks = unsigned((N + 256 / SCAN_LANES - 1) / (256 / SCAN_LANES));
     hipLaunchKernelGGL((k_res_scan<false>), dim3(blocks), dim3(256), 0, s, N, pl->dv,
                        pl->d_tleaf, leaf_of, nb_ptr, keys, nullptr, count);
     hip_check(hipGetLastError(), "k_res_scan launch");
