@@ -194,6 +194,8 @@ def _declare_hip(L):
     L.beep_hip_rhs.argtypes = [vp, C.c_int, dp, C.c_double, dp]
     L.beep_hip_energy_coefficients.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp]
     L.beep_hip_solve.argtypes = [vp, dp, dp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.beep_hip_solve_device.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
+    L.beep_hip_rhs_device.argtypes = [vp, C.c_int, dp, C.c_double, vp]
     L.beep_hip_selftest_mfma.argtypes = [vp, C.c_int, dp, dp, dp]
     L.beep_hip_fmm_symmetry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip]
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]

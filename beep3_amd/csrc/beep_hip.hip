@@ -1119,6 +1119,21 @@ int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Ds
     });
 }
 
+int beep_hip_rhs_device(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
+                        double* d_b) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "geometry not set");
+        require(n_charges >= 0 && (n_charges == 0 || xyzq) && d_b && Dsolvent > 0,
+                "bad rhs args");
+        set_device(ctx);
+        DevBuf<double>& q = ctx->qbuf;
+        if (n_charges > 0) q.upload(xyzq, size_t(4) * n_charges, ctx->stream);
+        launch_rhs(ctx->stream, ctx->N, ctx->qual_ptr.p, ctx->qual64.p, ctx->patches.p, n_charges,
+                   n_charges > 0 ? q.p : nullptr, FOUR_PI_INV / Dsolvent, d_b);
+        hip_check(hipStreamSynchronize(ctx->stream), "sync rhs");  // the charges buffer is reused
+    });
+}
+
 int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_inst, const int* ipp, const int* icp,
                                  const double* xyzq, double* ec_f, double* ec_h) {
     return guarded(ctx, [&] {
@@ -1153,79 +1168,123 @@ int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_inst, const int* ipp, 
     });
 }
 
+// The solve on device vectors: b read-only, x the initial guess on entry and
+// the solution on exit (x_zero: the guess is zero, a cold start).
+void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
+                const beep_solve_opts* o, beep_solve_stats* st) {
+    require(o->max_iter > 0 && o->tol > 0, "bad solve options");
+    const int n = 2 * ctx->N;
+    hipStream_t s = ctx->stream;
+    beep_solve_stats stats{};
+    const double t0 = now_ms();
+    ctx->dvec.alloc(n);
+    ctx->rvec.alloc(n);
+    const double bnorm = std::sqrt(dev_dot(ctx, d_b, d_b, n));
+    int matvecs = 0;
+    if (o->mode == BEEP_SOLVE_FP64 || o->mode == BEEP_SOLVE_FP32) {
+        const int prec = o->mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32;
+        bool conv = false;
+        double est = 0;
+        stats.iterations = gmres(ctx, prec, d_b, d_x, o->tol, o->max_iter, &matvecs, &conv, &est,
+                                 x_zero);
+        stats.converged = conv;
+        stats.residual = est;
+        stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
+    } else if (o->mode == BEEP_SOLVE_MIXED) {
+        require(o->max_refine > 0, "bad refinement options");
+        // inner FP32 GMRES tolerance; <= 0: half the outer one (one
+        // refinement step then usually suffices)
+        const double inner_tol = o->inner_tol > 0 ? o->inner_tol : 0.5 * o->tol;
+        bool done = false;
+        double rn = 0;
+        for (int k = 0; k <= o->max_refine; ++k) {
+            // r = b - K x with the FP64 residual operator
+            if (k == 0 && x_zero) {
+                hip_check(hipMemcpyAsync(ctx->rvec.p, d_b, sizeof(double) * n,
+                                         hipMemcpyDeviceToDevice, s), "D2D r");
+            } else {
+                matvec_dev(ctx, d_x, ctx->rvec.p, BEEP_FP32_RESID);
+                ++matvecs;
+                launch_axpby(s, n, 1.0, d_b, -1.0, ctx->rvec.p);
+            }
+            rn = std::sqrt(dev_dot(ctx, ctx->rvec.p, ctx->rvec.p, n));
+            if (o->verbose)
+                fprintf(stderr, "refine %d: |r|/|b| = %.3e\n", k, bnorm > 0 ? rn / bnorm : 0.0);
+            if (rn <= o->tol * bnorm) {
+                done = true;
+                break;
+            }
+            if (k == o->max_refine) break;
+            launch_fill(s, n, 0.0, ctx->dvec.p);
+            bool conv = false;
+            double est = 0;
+            stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
+                                      o->max_iter, &matvecs, &conv, &est, true);
+            launch_axpby(s, n, 1.0, ctx->dvec.p, 1.0, d_x);
+            stats.refinements = k + 1;
+        }
+        stats.converged = done;
+        stats.residual = rn;
+        stats.rel_residual = bnorm > 0 ? rn / bnorm : 0;
+    } else {
+        throw std::runtime_error("unknown solve mode");
+    }
+    hip_check(hipStreamSynchronize(s), "sync solve");
+    stats.matvecs = matvecs;
+    stats.time_ms = now_ms() - t0;
+    if (st) *st = stats;
+}
+
 int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_solve_opts* o,
                    beep_solve_stats* st) {
     return guarded(ctx, [&] {
         require(b && x && o, "bad solve args");
-        require(o->max_iter > 0 && o->tol > 0, "bad solve options");
         set_device(ctx);
         const int n = 2 * ctx->N;
         hipStream_t s = ctx->stream;
-        beep_solve_stats stats{};
-        const double t0 = now_ms();
         ctx->bvec.alloc(n);
-        ctx->dvec.alloc(n);
-        ctx->rvec.alloc(n);
         ctx->xbuf.alloc(n);
         hip_check(hipMemcpyAsync(ctx->bvec.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s), "H2D b");
         hip_check(hipMemcpyAsync(ctx->xbuf.p, x, sizeof(double) * n, hipMemcpyHostToDevice, s), "H2D x");
         bool x_zero = true;
         for (int k = 0; k < n && x_zero; ++k) x_zero = (x[k] == 0.0);
-        const double bnorm = std::sqrt(dev_dot(ctx, ctx->bvec.p, ctx->bvec.p, n));
-        int matvecs = 0;
-        if (o->mode == BEEP_SOLVE_FP64 || o->mode == BEEP_SOLVE_FP32) {
-            const int prec = o->mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32;
-            bool conv = false;
-            double est = 0;
-            stats.iterations = gmres(ctx, prec, ctx->bvec.p, ctx->xbuf.p, o->tol, o->max_iter,
-                                     &matvecs, &conv, &est, x_zero);
-            stats.converged = conv;
-            stats.residual = est;
-            stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
-        } else if (o->mode == BEEP_SOLVE_MIXED) {
-            require(o->max_refine > 0, "bad refinement options");
-            // inner FP32 GMRES tolerance; <= 0: half the outer one (one
-            // refinement step then usually suffices)
-            const double inner_tol = o->inner_tol > 0 ? o->inner_tol : 0.5 * o->tol;
-            bool done = false;
-            double rn = 0;
-            for (int k = 0; k <= o->max_refine; ++k) {
-                // r = b - K x with the FP64 residual operator
-                if (k == 0 && x_zero) {
-                    hip_check(hipMemcpyAsync(ctx->rvec.p, ctx->bvec.p, sizeof(double) * n,
-                                             hipMemcpyDeviceToDevice, s), "D2D r");
-                } else {
-                    matvec_dev(ctx, ctx->xbuf.p, ctx->rvec.p, BEEP_FP32_RESID);
-                    ++matvecs;
-                    launch_axpby(s, n, 1.0, ctx->bvec.p, -1.0, ctx->rvec.p);
-                }
-                rn = std::sqrt(dev_dot(ctx, ctx->rvec.p, ctx->rvec.p, n));
-                if (o->verbose)
-                    fprintf(stderr, "refine %d: |r|/|b| = %.3e\n", k, bnorm > 0 ? rn / bnorm : 0.0);
-                if (rn <= o->tol * bnorm) {
-                    done = true;
-                    break;
-                }
-                if (k == o->max_refine) break;
-                launch_fill(s, n, 0.0, ctx->dvec.p);
-                bool conv = false;
-                double est = 0;
-                stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
-                                          o->max_iter, &matvecs, &conv, &est, true);
-                launch_axpby(s, n, 1.0, ctx->dvec.p, 1.0, ctx->xbuf.p);
-                stats.refinements = k + 1;
-            }
-            stats.converged = done;
-            stats.residual = rn;
-            stats.rel_residual = bnorm > 0 ? rn / bnorm : 0;
-        } else {
-            throw std::runtime_error("unknown solve mode");
-        }
+        solve_impl(ctx, ctx->bvec.p, ctx->xbuf.p, x_zero, o, st);
         hip_check(hipMemcpyAsync(x, ctx->xbuf.p, sizeof(double) * n, hipMemcpyDeviceToHost, s), "D2H x");
         hip_check(hipStreamSynchronize(s), "sync solve");
-        stats.matvecs = matvecs;
-        stats.time_ms = now_ms() - t0;
-        if (st) *st = stats;
+    });
+}
+
+int beep_hip_solve_from_charges(beep_hip_ctx* ctx, int n_charges, const double* xyzq,
+                                double Dsolvent, const beep_solve_opts* o, double* x,
+                                beep_solve_stats* st) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "geometry not set");
+        require(n_charges >= 0 && (n_charges == 0 || xyzq) && x && o && Dsolvent > 0,
+                "bad solve args");
+        set_device(ctx);
+        const int n = 2 * ctx->N;
+        hipStream_t s = ctx->stream;
+        ctx->bvec.alloc(n);
+        ctx->xbuf.alloc(n);
+        DevBuf<double>& q = ctx->qbuf;
+        if (n_charges > 0) q.upload(xyzq, size_t(4) * n_charges, s);
+        launch_rhs(s, ctx->N, ctx->qual_ptr.p, ctx->qual64.p, ctx->patches.p, n_charges,
+                   n_charges > 0 ? q.p : nullptr, FOUR_PI_INV / Dsolvent, ctx->bvec.p);
+        launch_fill(s, n, 0.0, ctx->xbuf.p);
+        solve_impl(ctx, ctx->bvec.p, ctx->xbuf.p, true, o, st);
+        hip_check(hipMemcpyAsync(x, ctx->xbuf.p, sizeof(double) * n, hipMemcpyDeviceToHost, s), "D2H x");
+        hip_check(hipStreamSynchronize(s), "sync solve");
+    });
+}
+
+int beep_hip_solve_device(beep_hip_ctx* ctx, const double* d_b, double* d_x,
+                          const beep_solve_opts* o, beep_solve_stats* st) {
+    return guarded(ctx, [&] {
+        require(d_b && d_x && o && d_b != d_x, "bad solve args");
+        set_device(ctx);
+        const int n = 2 * ctx->N;
+        const bool x_zero = dev_dot(ctx, d_x, d_x, n) == 0.0;
+        solve_impl(ctx, d_b, d_x, x_zero, o, st);
     });
 }
 

@@ -181,13 +181,12 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     check_hip(ctx, beep_hip_synchronize(ctx), "precompute");
     const double ts = clock();
     r.precompute_ms = (ts - tp) * 1e3;
-    std::vector<double> b(size_t(2) * N), x(size_t(2) * N, 0.0);
+    std::vector<double> x(size_t(2) * N, 0.0);
     const int nch = int(sc->charge_inst.size());
-    check_hip(ctx, beep_hip_rhs(ctx, nch, nch ? sc->charges.data() : nullptr, sc->Dsolvent,
-                                b.data()), "rhs");
     beep_solve_opts so{o.tol, o.max_iter, o.solve_mode, o.max_refine, o.inner_tol, 0};
     beep_solve_stats st{};
-    check_hip(ctx, beep_hip_solve(ctx, b.data(), x.data(), &so, &st), "solve");
+    check_hip(ctx, beep_hip_solve_from_charges(ctx, nch, nch ? sc->charges.data() : nullptr,
+                                               sc->Dsolvent, &so, x.data(), &st), "solve");
     // energy: Gauss-law renormalised coefficients (mesh.cpp:891-925), then
     // 1/2 sum (h eps ec_h - f ec_f) per instance (mesh.cpp:1373-1403)
     const int ni = int(sc->inst.size());

@@ -191,6 +191,22 @@ class Device:
         self._check(lib().beep_hip_solve(self._h, _dp(b), _dp(x), C.byref(o), C.byref(st)))
         return x, {f: getattr(st, f) for f, _ in SolveStats._fields_}
 
+    def rhs_device(self, xyzq, Dsolvent: float, d_b: int):
+        """b = [f_rhs; h_rhs] into the device vector d_b (2N doubles)."""
+        q = np.ascontiguousarray(xyzq, dtype=np.float64).reshape(-1, 4)
+        self._check(lib().beep_hip_rhs_device(self._h, q.shape[0], _dp(q), float(Dsolvent),
+                                              vp(d_b)))
+
+    def solve_device(self, d_b: int, d_x: int, tol=1e-6, max_iter=100, mode="fp64",
+                     max_refine=10, inner_tol=0.0, verbose=False):
+        """solve on device vectors (d_x: initial guess in, solution out)."""
+        o = SolveOpts(tol, max_iter, SOLVE_MODES[mode] if isinstance(mode, str) else int(mode),
+                      max_refine, inner_tol, int(verbose))
+        st = SolveStats()
+        self._check(lib().beep_hip_solve_device(self._h, vp(d_b), vp(d_x), C.byref(o),
+                                                C.byref(st)))
+        return {f: getattr(st, f) for f, _ in SolveStats._fields_}
+
     # device memory helpers (bench / serving keep vectors resident)
     def alloc(self, nbytes: int) -> int:
         p = vp()

@@ -263,6 +263,9 @@ int beep_hip_timing_reset(beep_hip_ctx* ctx);
 /* b = [f_rhs; h_rhs] from point charges xyzq (n x 4, universe frame) */
 int beep_hip_rhs(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
                  double* b);
+/* As beep_hip_rhs, into a device vector of 2N doubles (d_b). */
+int beep_hip_rhs_device(beep_hip_ctx* ctx, int n_charges, const double* xyzq, double Dsolvent,
+                        double* d_b);
 /* raw energy coefficients (before the Gauss-law renormalisation) of every
  * patch from the charges of its own instance */
 int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_instances, const int* inst_patch_ptr,
@@ -272,6 +275,17 @@ int beep_hip_energy_coefficients(beep_hip_ctx* ctx, int n_instances, const int* 
 /* Solve K x = b; x holds the initial guess on entry (zeros: cold start). */
 int beep_hip_solve(beep_hip_ctx* ctx, const double* b, double* x, const beep_solve_opts* opts,
                    beep_solve_stats* stats);
+/* The RHS of point charges (as beep_hip_rhs) and the solve from a zero
+ * guess, b kept on the device; x (2N doubles, host) receives the solution.
+ * One PCIe transfer of a vector instead of four (BEEP::set_rhs then
+ * BEEP::gmres, beep.cpp:1124-1483). */
+int beep_hip_solve_from_charges(beep_hip_ctx* ctx, int n_charges, const double* xyzq,
+                                double Dsolvent, const beep_solve_opts* opts, double* x,
+                                beep_solve_stats* stats);
+/* As beep_hip_solve on device vectors of 2N doubles (d_b read, d_x guess in
+ * and solution out): no PCIe transfer of b or x. */
+int beep_hip_solve_device(beep_hip_ctx* ctx, const double* d_b, double* d_x,
+                          const beep_solve_opts* opts, beep_solve_stats* stats);
 
 /* Diagnostic: D = A B through one 16x16x4 MFMA (fp32 = 0: FP64 operands,
  * 1: FP32), A 16x4 and B 4x16 row-major host arrays; d[4 l + r] receives

@@ -65,3 +65,23 @@ def test_cached_graphs_across_alternating_solves():
         assert st["converged"], m
         assert np.array_equal(x, eager[m]), m
     d.close()
+
+
+def test_device_resident_solves_match_host_solve():
+    """RHS and solve on device vectors give the bits of the host-vector calls."""
+    b = configs.born_ion()
+    d, sc, a = _device(b, 2.0, "direct", ("fp64", "fp32"))
+    rhs = d.rhs(a["charges"], b.Dsolvent)
+    n = rhs.shape[0]
+    db, dx = d.alloc(8 * n), d.alloc(8 * n)
+    d.rhs_device(a["charges"], b.Dsolvent, db)
+    assert np.array_equal(d.download(np.empty(n), db), rhs)
+    for mode in ("fp64", "mixed"):
+        x0, st0 = d.solve(rhs, tol=1e-6, max_iter=100, mode=mode)
+        d.upload(dx, np.zeros(n))
+        st1 = d.solve_device(db, dx, tol=1e-6, max_iter=100, mode=mode)
+        assert st1["iterations"] == st0["iterations"], mode
+        assert np.array_equal(d.download(np.empty(n), dx), x0), mode
+    d.free(db)
+    d.free(dx)
+    d.close()
