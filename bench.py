@@ -306,6 +306,22 @@ def main():
         dev.synchronize()
         dev.free(src)
         dev.free(dst)
+    # what a plain device-to-device copy sustains (read + write bytes over its
+    # time, best of 5 copies of 2 GiB): the practical HBM ceiling beside the
+    # 8 TB/s of the data sheet
+    copy_bytes = 2 << 30
+    c_src, c_dst = dev.alloc(copy_bytes), dev.alloc(copy_bytes)
+    dev.copy_device(c_dst, c_src, copy_bytes)
+    dev.synchronize()
+    copy_s = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        dev.copy_device(c_dst, c_src, copy_bytes)
+        dev.synchronize()
+        copy_s.append(time.perf_counter() - t0)
+    dev.free(c_src)
+    dev.free(c_dst)
+    copy_GBs = 2 * copy_bytes / min(copy_s) / 1e9
     tune = None
     if args.far == "fmm" and not args.no_tune:
         # leaf-size autotuner on the FP64 mat-vec at this order
@@ -415,6 +431,12 @@ def main():
         "near_spmv_alone_ms_by_depth": {dd: {"fp64": v[0], "fp32": v[2]} for dd, v in alone.items()},
         "near_spmv_fp32_GBs": nb32 / (near32_alone * 1e-3) / 1e9,
         "host_setup_s": setup_s,
+        "hbm_copy_GBs": copy_GBs,
+        # measured HBM bytes of the FP64 SpMV over its time, against the copy
+        "near_spmv_fp64_hbm_GBs": (MEASURED_TRAFFIC.get((args.workload, "fp64")) or nb64)
+                                  / (near_alone * 1e-3) / 1e9,
+        "near_spmv_fp64_frac_of_copy": (MEASURED_TRAFFIC.get((args.workload, "fp64")) or nb64)
+                                       / (near_alone * 1e-3) / 1e9 / copy_GBs,
         "autotune": tune,
     }
 
