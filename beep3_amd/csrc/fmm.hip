@@ -320,25 +320,18 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     for (int i = 0; i < NQ; ++i)
         if (qmm[i] >= 0) sb[0][qmm[i]] = nxt[i];
     if (NSLAB > 1) fetch(MS, nxt);  // in flight during the first slab
-    // A operands (kernel table values), RV k-steps per group, loaded PF
-    // groups ahead (4-5 k-steps FP64, 4-7 FP32): the table loads hit L2,
-    // whose latency grows when the near SpMV of the other stream floods the
-    // memory system
-    auto load_a = [&](int sg, T (*xa)[NTW]) {
+    // A operands (kernel table values) one k-step ahead: the loads of step
+    // k+1 are in flight while the MFMAs of step k run
+    T xa_n[NTW];
+    auto load_a = [&](int step, T* xa) {
+        const int m = 4 * step + kl16;  // K-dimension index (canonical source node)
 #pragma unroll
-        for (int r = 0; r < RV; ++r) {
-            const int m = 4 * (sg + r) + kl16;  // K-dimension index (canonical source node)
-#pragma unroll
-            for (int k = 0; k < NTW; ++k) {
-                const int n = (w + M2L_WAVES * k) * 16 + il;  // output row (canonical target node)
-                xa[r][k] = (sg + r < NSTEP && m < NP && n < NP) ? KT[m * NP + n] : T(0);
-            }
+        for (int k = 0; k < NTW; ++k) {
+            const int n = (w + M2L_WAVES * k) * 16 + il;  // output row (canonical target node)
+            xa[k] = (m < NP && n < NP) ? KT[m * NP + n] : T(0);
         }
     };
-    constexpr int PF = sizeof(T) == 8 ? 2 : 1;
-    T ab[PF + 1][RV][NTW];
-#pragma unroll
-    for (int g = 0; g <= PF; ++g) load_a(g * RV, ab[g]);
+    load_a(0, xa_n);
 #pragma unroll 1
     for (int sg = 0; sg < NSTEP; sg += RV) {
         const int ss = sg % SPS, slab = sg / SPS;  // first k-step of the group in its slab
@@ -358,24 +351,18 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
 #pragma unroll
         for (int c = 0; c < CT; ++c)
             bv[c] = *reinterpret_cast<const vec_t*>(&sbuf[(c * 16 + il) * MSP + kl16 * SPS + ss]);
-        T xa[RV][NTW];
-#pragma unroll
-        for (int r = 0; r < RV; ++r)
-#pragma unroll
-            for (int k = 0; k < NTW; ++k) {
-                xa[r][k] = ab[0][r][k];
-#pragma unroll
-                for (int g = 0; g < PF; ++g) ab[g][r][k] = ab[g + 1][r][k];
-            }
-        load_a(sg + (PF + 1) * RV, ab[PF]);  // zeros past the end
 #pragma unroll
         for (int r = 0; r < RV; ++r) {
+            const int step = sg + r;
+            T xa[NTW];
+#pragma unroll
+            for (int k = 0; k < NTW; ++k) xa[k] = xa_n[k];
+            if (step + 1 < NSTEP) load_a(step + 1, xa_n);
 #pragma unroll
             for (int k = 0; k < NTW; ++k) {
                 if (w + M2L_WAVES * k >= NTILE) continue;
 #pragma unroll
-                for (int c = 0; c < CT; ++c)
-                    acc[k][c] = Mfma<T>::op(xa[r][k], bv[c][r], acc[k][c]);
+                for (int c = 0; c < CT; ++c) acc[k][c] = Mfma<T>::op(xa[k], bv[c][r], acc[k][c]);
             }
         }
     }
