@@ -18,6 +18,11 @@ import os
 import sys
 import time
 
+# hardware queues of the HIP runtime (read when it starts, before any HIP
+# call): the solve server runs two streams per slot, and with the default 4
+# queues a small request waits behind other slots' long kernels
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
