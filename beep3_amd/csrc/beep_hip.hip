@@ -195,14 +195,23 @@ void far_setup(beep_hip_ctx* c) {
 // Build the far-field plan if it is missing, on the far stream (which holds
 // no other work then), and with the near list set, its residual leaf-pass
 // list.  Host-synchronous.
+void begin_plan(beep_hip_ctx* c) {
+    if (!c->use_fmm || c->fmm) return;
+    hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
+    const auto t0 = std::chrono::steady_clock::now();
+    c->fmm = fmm_plan_begin(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf, c->fmm_arena,
+                            c->far_stream);
+    c->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->gen++;
+}
+
 void ensure_plan(beep_hip_ctx* c) {
     if (!c->use_fmm) return;
-    if (!c->fmm) {
-        hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
+    begin_plan(c);
+    if (c->fmm->stage) {
         const auto t0 = std::chrono::steady_clock::now();
-        c->fmm = fmm_plan_create(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf,
-                                 c->fmm_arena, c->far_stream);
-        c->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        fmm_plan_finish(c->fmm, c->far_stream);
+        c->plan_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         c->gen++;
     }
     if (c->have_tree && c->fmm->cellkey && c->fmm->n_res < 0) {
@@ -814,9 +823,13 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
     const unsigned* cellkey = nullptr;
     unsigned long long uid = 0;
     if (ctx->use_fmm) {
-        ensure_plan(ctx);  // host-synchronous: cell keys are on the device
-        cellkey = ctx->fmm->n_res >= 0 ? ctx->fmm->cellkey : nullptr;
+        // the leaf level and cell keys first (the rest of the plan is built on
+        // the host while the coefficient kernel runs, below)
+        begin_plan(ctx);
+        cellkey = ctx->fmm->cellkey;  // null above leaf level 10: no folding
         uid = ctx->fmm->uid;
+        hip_check(hipEventRecord(ctx->ev_pack, ctx->far_stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(s, ctx->ev_pack, 0), "hipStreamWaitEvent");
     }
     size_t freeb = 0, totalb = 0;
     hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
@@ -843,11 +856,15 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
         ctx->have32 = true;
         ctx->uid32 = uid;
     }
-    // the kernel tables are built on the far stream while the coefficient
-    // kernels run (no wait here: the precompute is stream-ordered); FP32
-    // coefficients serve the FP32 mat-vec (FP32 far field) and the
-    // refinement residual (FP64 far field)
+    // the rest of the plan, its residual leaf-pass list (with cell keys,
+    // the list the coefficients assume) and the kernel tables are built on
+    // the host and the far stream while the coefficient kernels run (no
+    // wait here: the precompute is stream-ordered); FP32 coefficients serve
+    // the FP32 mat-vec (FP32 far field) and the refinement residual (FP64
+    // far field)
     ensure_far(ctx, precision);
+    if (ctx->use_fmm)
+        require(!cellkey || ctx->fmm->n_res >= 0, "residual leaf-pass list missing");
 }
 
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {

@@ -786,14 +786,21 @@ void upload(DevArena& ar, const char* name, T*& dst, const std::vector<T>& v, hi
 
 }  // namespace
 
-FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
-                         DevArena& arena, hipStream_t s) {
+// Host data carried from fmm_plan_begin to fmm_plan_finish.
+struct FmmPlanStage {
+    double x0[3] = {0, 0, 0};
+    std::vector<uint32_t> cx, cy, cz;
+    std::vector<unsigned> ck;  // packed cells, until the upload has completed
+};
+
+FmmPlan* fmm_plan_begin(int N, const double* centroid, int order, int leaf_target,
+                        DevArena& arena, hipStream_t s) {
     if (order < 3 || order > 6) throw std::runtime_error("FMM order must be 3..6");
     auto* pl = new FmmPlan();
     pl->arena = &arena;
     pl->N = N;
     pl->P = order;
-    const int NP = order * order * order;
+    pl->stage = new FmmPlanStage();
     // bounding cube
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
     for (int i = 0; i < N; ++i)
@@ -842,6 +849,39 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
         static std::atomic<unsigned long long> next_uid{1};
         pl->uid = next_uid++;
     }
+    pl->edge = edge;
+    FmmPlanStage& st = *pl->stage;
+    for (int k = 0; k < 3; ++k) st.x0[k] = x0[k];
+    st.cx.resize(N);
+    st.cy.resize(N);
+    st.cz.resize(N);
+    for (int i = 0; i < N; ++i) {
+        st.cx[i] = cell(i, D, 0);
+        st.cy[i] = cell(i, D, 1);
+        st.cz[i] = cell(i, D, 2);
+    }
+    // the packed leaf cell of every patch: the leaf-pass neighbourhood test
+    // of the near coefficients, which can start before the plan is finished
+    if (D <= FMM_CELL_BITS) {
+        st.ck.resize(N);
+        for (int i = 0; i < N; ++i)
+            st.ck[i] = st.cx[i] | (st.cy[i] << FMM_CELL_BITS) | (st.cz[i] << (2 * FMM_CELL_BITS));
+        upload(arena, "cellkey", pl->cellkey, st.ck, s);
+    }
+    return pl;
+}
+
+void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
+    if (!pl->stage) return;
+    DevArena& arena = *pl->arena;
+    const int N = pl->N, D = pl->D;
+    const int NP = pl->P * pl->P * pl->P;
+    const int order = pl->P;
+    const double edge = pl->edge;
+    const double* x0 = pl->stage->x0;
+    const std::vector<uint32_t>& cx = pl->stage->cx;
+    const std::vector<uint32_t>& cy = pl->stage->cy;
+    const std::vector<uint32_t>& cz = pl->stage->cz;
     // sort patches by leaf key (Morton-interleaved for locality)
     auto morton = [&](uint32_t x, uint32_t y, uint32_t z) {
         uint64_t m = 0;
@@ -851,11 +891,7 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
         return m;
     };
     std::vector<uint64_t> code(N);
-    std::vector<uint32_t> cx(N), cy(N), cz(N);
-    for (int i = 0; i < N; ++i) {
-        cx[i] = cell(i, D, 0); cy[i] = cell(i, D, 1); cz[i] = cell(i, D, 2);
-        code[i] = morton(cx[i], cy[i], cz[i]);
-    }
+    for (int i = 0; i < N; ++i) code[i] = morton(cx[i], cy[i], cz[i]);
     std::vector<int> perm(N);
     for (int i = 0; i < N; ++i) perm[i] = i;
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return code[a] < code[b]; });
@@ -1023,22 +1059,14 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     upload(arena, "child_oct", pl->dv.child_oct, child_oct, s);
     upload(arena, "leaf_src", pl->dv.leaf_src, leaf_src, s);
     upload(arena, "perm", pl->dv.perm, perm, s);
-    // patch -> FMM order, FMM order -> leaf box, and the packed leaf cell of
-    // every patch (the leaf-pass neighbourhood test of the near coefficients);
-    // the host copies live until the stream is synchronised below
+    // patch -> FMM order and FMM order -> leaf box; the host copies live
+    // until the stream is synchronised below
     std::vector<int> iperm(N), tleaf(N);
-    std::vector<unsigned> ck;
     for (int k = 0; k < N; ++k) iperm[perm[k]] = k;
     for (int q = 0; q < nleaf; ++q)
         for (int k = leaf_src[q]; k < leaf_src[q + 1]; ++k) tleaf[k] = q;
     upload(arena, "iperm", pl->d_iperm, iperm, s);
     upload(arena, "tleaf", pl->d_tleaf, tleaf, s);
-    if (D <= FMM_CELL_BITS) {
-        ck.resize(N);
-        for (int i = 0; i < N; ++i)
-            ck[i] = cx[i] | (cy[i] << FMM_CELL_BITS) | (cz[i] << (2 * FMM_CELL_BITS));
-        upload(arena, "cellkey", pl->cellkey, ck, s);
-    }
     upload(arena, "nb_ptr", pl->dv.nb_ptr, nb_ptr, s);
     upload(arena, "nb_leaf", pl->dv.nb_leaf, nb_leaf, s);
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
@@ -1086,11 +1114,22 @@ FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_targ
     const size_t ne = size_t(nbox) * 3 * NP;
     pl->n_exp = ne;
     hip_check(hipStreamSynchronize(s), "sync fmm plan");
-    (void)NP;
+    delete pl->stage;  // the cell keys are uploaded too (same stream)
+    pl->stage = nullptr;
+}
+
+FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
+                         DevArena& arena, hipStream_t s) {
+    FmmPlan* pl = fmm_plan_begin(N, centroid, order, leaf_target, arena, s);
+    fmm_plan_finish(pl, s);
     return pl;
 }
 
-void fmm_plan_destroy(FmmPlan* p) { delete p; }
+void fmm_plan_destroy(FmmPlan* p) {
+    if (!p) return;
+    delete p->stage;
+    delete p;
+}
 
 template <typename T, int P>
 static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& ytab,

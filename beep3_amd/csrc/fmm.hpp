@@ -80,6 +80,8 @@ __device__ __forceinline__ void p2p_pair(T dx, T dy, T dz, T ie, T sdx, T sdy, T
     ah += -sh * nd * ir3 + ie * (ndv * (ir3 - g1) - dd * nd * (gL2 - g2) + sh * nd * g1);
 }
 
+struct FmmPlanStage;
+
 struct FmmPlan {
     static constexpr int CHUNK = 64;  // M2L pairs per block
     static constexpr int NCANON = 16; // canonical M2L offsets
@@ -90,6 +92,7 @@ struct FmmPlan {
     int* d_iperm = nullptr;           // patch -> FMM order
     int* d_tleaf = nullptr;           // FMM order -> leaf box
     long long n_res = -1;             // residual leaf-pass pairs (-1: not built)
+    FmmPlanStage* stage = nullptr;    // host data of a plan begun and not finished
     double edge = 0;
     size_t n_exp = 0;                 // values per expansion set (nbox * 3 * P^3)
     std::vector<int> lvl_first;       // D + 2
@@ -117,6 +120,12 @@ struct FmmPlan {
 // by the next plan built on it); one live plan per arena.
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
                          DevArena& arena, hipStream_t s);
+// The same in two halves: begin chooses the leaf level and uploads the cell
+// keys (stream-ordered on s; uid, D, cellkey valid), finish builds the rest
+// and synchronises s.
+FmmPlan* fmm_plan_begin(int N, const double* centroid, int order, int leaf_target,
+                        DevArena& arena, hipStream_t s);
+void fmm_plan_finish(FmmPlan* p, hipStream_t s);
 void fmm_plan_destroy(FmmPlan* p);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 inline bool fmm_tables_ready(const FmmPlan* p, double kappa, bool fp64, bool fp32) {
