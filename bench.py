@@ -19,9 +19,10 @@ import sys
 import time
 
 # hardware queues of the HIP runtime (read when it starts, before any HIP
-# call): the solve server runs two streams per slot, and with the default 4
-# queues a small request waits behind other slots' long kernels
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# call): the solve server runs two streams per slot, and with HIP's default
+# of 4 queues a small request waits behind other slots' long kernels
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import numpy as np
 
