@@ -117,6 +117,7 @@ struct beep_hip_ctx {
     // per mat-vec scratch
     DevBuf<Src64> src;
     DevBuf<double> xbuf, ybuf, yfar;
+    DevBuf<double> far_part;  // partial sums of the split direct far field
     // charges and outputs of the RHS / energy-coefficient calls (kept, so a
     // solve frees nothing: hipFree waits for the whole device)
     DevBuf<double> qbuf, rhsbuf, ecf, ech;
@@ -326,8 +327,10 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
                      c->yfar.p, timed ? &ev[4] : nullptr, c->ev_pack);
     } else {
         launch_far_pack(sf, c->N, c->patches.p, c->leaf_idx.p, dx, c->src.p);
-        launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, c->leaf_idx.p, c->patches.p,
-                          c->src.p, c->kappa, c->yfar.p);
+        const int ns = far_direct_splits(c->N, c->n_cu);
+        if (ns > 1 && !c->capturing) c->far_part.alloc(size_t(2) * c->N * ns);
+        launch_far_direct(sf, precision == BEEP_FP32 ? 1 : 0, c->N, ns, c->leaf_idx.p,
+                          c->patches.p, c->src.p, c->kappa, c->yfar.p, c->far_part.p);
     }
     if (timed) hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
     if (c->overlap) hip_check(hipEventRecord(c->ev_join, sf), "hipEventRecord");
@@ -396,12 +399,17 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
     // everything the captured launches bake in: buffers, plan, sizes
     ensure_far(c, precision == BEEP_FP32 ? BEEP_FP32 : BEEP_FP64);
     c->yfar.alloc(size_t(2) * c->N);
-    if (!c->use_fmm) c->src.alloc(size_t(c->N));
+    if (!c->use_fmm) {
+        c->src.alloc(size_t(c->N));
+        const int ns = far_direct_splits(c->N, c->n_cu);
+        if (ns > 1) c->far_part.alloc(size_t(2) * c->N * ns);
+    }
     const std::vector<const void*> key = {
         reinterpret_cast<const void*>(uintptr_t(c->gen)), reinterpret_cast<const void*>(intptr_t(precision)),
         reinterpret_cast<const void*>(intptr_t(m)), reinterpret_cast<const void*>(intptr_t(c->N)),
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
-        c->yfar.p, c->src.p, c->coef32.p, c->coef64.p, c->diag.p, c->spmv_part.p, c->fmm,
+        c->yfar.p, c->src.p, c->far_part.p, c->coef32.p, c->coef64.p, c->diag.p,
+        c->spmv_part.p, c->fmm,
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
         reinterpret_cast<const void*>(intptr_t(c->spmv_cap[spmv_mode(precision)])),
         reinterpret_cast<const void*>(intptr_t(c->spmv_depth[spmv_mode(precision)]))};
@@ -1382,6 +1390,7 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         add(ctx->coef_off); add(ctx->coef_blk); add(ctx->spmv_work); add(ctx->spmv_groups);
         add(ctx->spmv_group_nblk); add(ctx->spmv_part); add(ctx->diag); add(ctx->coef32);
         add(ctx->coef64); add(ctx->src); add(ctx->xbuf); add(ctx->ybuf); add(ctx->yfar);
+        add(ctx->far_part);
         add(ctx->qbuf); add(ctx->rhsbuf); add(ctx->ecf); add(ctx->ech); add(ctx->pinst_d);
         add(ctx->icp_d); add(ctx->V); add(ctx->hvec); add(ctx->partial); add(ctx->wtmp);
         add(ctx->xin); add(ctx->wvec); add(ctx->jstep);

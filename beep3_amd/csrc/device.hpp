@@ -161,8 +161,12 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,
                      const double* x, Src64* src);
-void launch_far_direct(hipStream_t s, int fp32, int n, const int* order, const Patch64* patches,
-                       const Src64* src, double kappa, double* y);
+// source-run splits of the direct far field that fill the chip (n_cu CUs)
+int far_direct_splits(int n, int n_cu);
+// y += direct far field; n_split > 1 needs part: 2 n n_split doubles
+void launch_far_direct(hipStream_t s, int fp32, int n, int n_split, const int* order,
+                       const Patch64* patches, const Src64* src, double kappa, double* y,
+                       double* part);
 
 // RHS and energy coefficients (FP64 direct sums)
 void launch_rhs(hipStream_t s, int n, const int* qual_ptr, const Pt64* qual,

@@ -17,6 +17,9 @@
 //   h += -s_h nd gL1 + ie [ ndv (gL1 - g1) - dd nd (gL2 - g2) + s_h nd g1 ]
 // with G = 1/r, u = e/r, gL1 = 1/r^3, g1 = e (1 + k r)/r^3, gL2 = 3/r^5,
 // g2 = e (k^2 r^2 + 3 k r + 3)/r^5, e = exp(-k r).
+#include <algorithm>
+#include <stdexcept>
+
 #include "device.hpp"
 
 namespace beepdev {
@@ -67,18 +70,27 @@ struct alignas(16) SrcT {
 
 // Targets and sources both in leaf order (spatially compact runs); positions
 // are taken relative to the first target of the block, formed in FP64, so
-// that nearby pairs keep their relative accuracy in FP32.
+// that nearby pairs keep their relative accuracy in FP32.  A block covers 256
+// targets and one of n_split runs of source tiles (grid.y), so small
+// problems still fill the chip; with n_split > 1 each run's partial sums go
+// to part[run][2n] and k_far_direct_sum adds them in run order.
 constexpr int FAR_TILE = 256;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_far_direct(int n, const int* __restrict__ order,
+__global__ __launch_bounds__(256) void k_far_direct(int n, int n_split,
+                                                    const int* __restrict__ order,
                                                     const Patch64* __restrict__ patches,
                                                     const Src64* __restrict__ src, T kappa,
-                                                    double* __restrict__ y) {
+                                                    double* __restrict__ y,
+                                                    double* __restrict__ part) {
     __shared__ SrcT<T> tile[FAR_TILE];
     __shared__ int tile_idx[FAR_TILE];
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int t0 = blockIdx.x * 256;
+    const int sp = blockIdx.y;
+    const int n_tiles = (n + FAR_TILE - 1) / FAR_TILE;
+    const int s_begin = int((long long)sp * n_tiles / n_split) * FAR_TILE;
+    const int s_end = min(n, int((long long)(sp + 1) * n_tiles / n_split) * FAR_TILE);
     const double ox = src[t0].x, oy = src[t0].y, oz = src[t0].z;
     const bool valid = t < n;
     const int i = valid ? order[t] : -1;
@@ -90,7 +102,7 @@ __global__ __launch_bounds__(256) void k_far_direct(int n, const int* __restrict
     }
     T af = 0, ah = 0;
     const T k = kappa;
-    for (int s0 = 0; s0 < n; s0 += FAR_TILE) {
+    for (int s0 = s_begin; s0 < s_end; s0 += FAR_TILE) {
         __syncthreads();
         const int s = s0 + threadIdx.x;
         if (s < n) {
@@ -128,22 +140,56 @@ __global__ __launch_bounds__(256) void k_far_direct(int n, const int* __restrict
         }
     }
     if (valid) {
-        y[i] += double(af) * FOUR_PI_INV;
-        y[n + i] += double(ah) * FOUR_PI_INV;
+        if (n_split == 1) {
+            y[i] += double(af) * FOUR_PI_INV;
+            y[n + i] += double(ah) * FOUR_PI_INV;
+        } else {
+            part[size_t(2 * sp) * n + i] = double(af);
+            part[size_t(2 * sp + 1) * n + i] = double(ah);
+        }
     }
 }
 
-void launch_far_direct(hipStream_t s, int fp32, int n, const int* order, const Patch64* patches,
-                       const Src64* src, double kappa, double* y) {
+__global__ __launch_bounds__(256) void k_far_direct_sum(int n, int n_split,
+                                                        const double* __restrict__ part,
+                                                        double* __restrict__ y) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double af = 0, ah = 0;
+    for (int sp = 0; sp < n_split; ++sp) {
+        af += part[size_t(2 * sp) * n + i];
+        ah += part[size_t(2 * sp + 1) * n + i];
+    }
+    y[i] += af * FOUR_PI_INV;
+    y[n + i] += ah * FOUR_PI_INV;
+}
+
+int far_direct_splits(int n, int n_cu) {
+    if (n <= 0) return 1;
+    const int tgt_blocks = (n + 255) / 256;
+    const int n_tiles = (n + FAR_TILE - 1) / FAR_TILE;
+    const int want = (4 * n_cu + tgt_blocks - 1) / tgt_blocks;  // about 4 blocks per CU
+    return std::max(1, std::min(want, n_tiles));
+}
+
+void launch_far_direct(hipStream_t s, int fp32, int n, int n_split, const int* order,
+                       const Patch64* patches, const Src64* src, double kappa, double* y,
+                       double* part) {
     if (n <= 0) return;
-    const int blocks = (n + 255) / 256;
+    if (n_split > 1 && !part) throw std::runtime_error("direct far field needs its partial sums");
+    const dim3 grid(unsigned((n + 255) / 256), unsigned(n_split));
     if (fp32)
-        hipLaunchKernelGGL(k_far_direct<float>, dim3(blocks), dim3(256), 0, s, n, order, patches,
-                           src, float(kappa), y);
+        hipLaunchKernelGGL(k_far_direct<float>, grid, dim3(256), 0, s, n, n_split, order, patches,
+                           src, float(kappa), y, part);
     else
-        hipLaunchKernelGGL(k_far_direct<double>, dim3(blocks), dim3(256), 0, s, n, order, patches,
-                           src, kappa, y);
+        hipLaunchKernelGGL(k_far_direct<double>, grid, dim3(256), 0, s, n, n_split, order, patches,
+                           src, kappa, y, part);
     hip_check(hipGetLastError(), "k_far_direct launch");
+    if (n_split > 1) {
+        hipLaunchKernelGGL(k_far_direct_sum, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, n,
+                           n_split, part, y);
+        hip_check(hipGetLastError(), "k_far_direct_sum launch");
+    }
 }
 
 }  // namespace beepdev
