@@ -374,6 +374,15 @@ def main():
                     "matvecs_per_s": 1e3 / ms, "steps": steps}
         print(f"[bench] {p}: {ms:.3f} ms/matvec (device {tot_ms:.3f}: near {near_ms:.3f} "
               f"concurrent with far {far_ms:.3f})", file=sys.stderr, flush=True)
+    # the ABI's host-vector mat-vec (beep_hip_matvec): x up and y down over
+    # PCIe around the same device work (not the headline value)
+    host_ms = {}
+    for p in ("fp64", "fp32"):
+        b.dev.matvec(x, p)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            b.dev.matvec(x, p)
+        host_ms[p] = (time.perf_counter() - t0) / 5 * 1e3
     # the near SpMV alone (far field serialised before it, grid uncapped: the
     # SpMV has the whole chip and memory system), for its roofline
     # at each load depth; the fastest depth gives the roofline figure
@@ -436,6 +445,7 @@ def main():
                              "fp32": {"near": near32_alone, "far": far32_alone}},
         "near_spmv_alone_ms_by_depth": {dd: {"fp64": v[0], "fp32": v[2]} for dd, v in alone.items()},
         "near_spmv_fp32_GBs": nb32 / (near32_alone * 1e-3) / 1e9,
+        "matvec_host_vectors_ms": host_ms,
         "host_setup_s": setup_s,
         "hbm_copy_GBs": copy_GBs,
         # measured HBM bytes of the FP64 SpMV over its time, against the copy
