@@ -245,8 +245,8 @@ inline int node_of(int mc, int sym) {
 
 constexpr int M2L_G = 64;     // pairs per block (4 MFMA column tiles)
 constexpr int M2L_WAVES = 8;  // waves per block, each owning output row tiles
-constexpr int M2L_MS = 32;    // K-dimension slab staged in LDS
-constexpr int M2L_MSP = 36;   // padded slab row (spreads the B reads over the banks)
+constexpr int M2L_MS = 64;    // K-dimension slab staged in LDS
+constexpr int M2L_MSP = 68;   // padded slab row (spreads the B reads over the banks)
 
 template <typename T, int P>
 __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
