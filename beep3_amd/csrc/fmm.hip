@@ -245,8 +245,11 @@ inline int node_of(int mc, int sym) {
 
 constexpr int M2L_G = 64;     // pairs per block (4 MFMA column tiles)
 constexpr int M2L_WAVES = 8;  // waves per block, each owning output row tiles
-constexpr int M2L_MS = 64;    // K-dimension slab staged in LDS
-constexpr int M2L_MSP = 68;   // padded slab row (spreads the B reads over the banks)
+// K-dimension slab staged in LDS per barrier: 32 (FP64) or 64 (FP32)
+// K-indices, measured fastest; rows padded by 4 (spreads the B reads over the
+// banks, keeps 16-byte alignment)
+template <typename T>
+constexpr int m2l_slab() { return sizeof(T) == 8 ? 32 : 64; }
 
 template <typename T, int P>
 __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restrict__ chunks,
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     constexpr int NPP = (NP + 15) / 16 * 16;
     constexpr int NTILE = NPP / 16;
     constexpr int NTW = (NTILE + M2L_WAVES - 1) / M2L_WAVES;  // row tiles per wave
-    constexpr int G = M2L_G, CT = G / 16, MS = M2L_MS, MSP = M2L_MSP;
+    constexpr int G = M2L_G, CT = G / 16, MS = m2l_slab<T>(), MSP = MS + 4;
     constexpr int NQ = (G * MS + 511) / 512;  // slab entries per thread
     constexpr int NSLAB = (NPP + MS - 1) / MS;
     constexpr int NSTEP = NPP / 4, SPS = MS / 4;  // k-steps, k-steps per slab
