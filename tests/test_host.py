@@ -122,6 +122,21 @@ def test_rigid_pose():
     assert np.allclose(dots0, dots1, atol=1e-12)
 
 
+@pytest.mark.parametrize("quat, offset, expect", [
+    ([1, 0, 0, 0], [0, 0, 0], [5, 7, 9]),    # null rotation and translation
+    ([0, 1, 0, 0], [0, 0, 0], [5, -7, -9]),  # half turn about x
+    ([0, 1, 0, 0], [1, 2, 3], [6, -5, -6]),  # then the centre moved to (1, 2, 3)
+])
+def test_pose_known_answers(quat, offset, expect):
+    """The known answers of tools/test.py:1-32 (Vector::change_coordinate_frame
+    with the quaternions (1,0,0,0) and (0,1,0,0)), applied to a charge at
+    (5, 7, 9) of a mesh whose charge centre is the origin."""
+    m = Mesh.icosphere(1, 1.0).set_charges([[5, 7, 9, 1.0], [-5, -7, -9, 1.0]])
+    a = Scene(80.0, 0.1, 0, 0, 1500).add_instance(m, 2.0, quat=quat, offset=offset).build().arrays()
+    assert np.allclose(a["charges"][0, :3], expect, atol=1e-12)
+    assert np.allclose(a["charges"][1, :3], 2 * np.asarray(offset) - np.asarray(expect), atol=1e-12)
+
+
 def test_star_mesh_and_random_walk_charges():
     m = Mesh.star(3, 25.0, 8, 0.04, 3).charges_walk(300, 3.8, 2.0, 0.4, 3)
     xyz, tri = m.vertices()
