@@ -31,13 +31,17 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
-    // grow-only: keeps the buffer when it is already large enough
+    // grow-only: keeps the buffer when it is already large enough, and a
+    // regrowth takes 25 % headroom, so that a context serving problems of
+    // varying size rarely frees (hipFree waits for the whole device and
+    // blocks every other thread's launches meanwhile)
     void alloc(size_t count) {
         if (p && count <= n) return;
+        const size_t want = n ? std::max(count, n + n / 4) : count;
         reset();
-        if (count == 0) return;
-        hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-        n = count;
+        if (want == 0) return;
+        hip_check(hipMalloc(&p, want * sizeof(T)), "hipMalloc");
+        n = want;
     }
     void upload(const T* src, size_t count, hipStream_t s) {
         alloc(count);
@@ -801,8 +805,7 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         ctx->have_tree = true;
         ctx->have32 = ctx->have64 = ctx->have_diag = false;
         if (ctx->fmm) ctx->fmm->n_res = -1;  // the residual leaf-pass list follows the near list
-        ctx->coef32.reset();
-        ctx->coef64.reset();
+        // the coefficient buffers stay allocated for the next precompute
     });
 }
 
@@ -843,7 +846,8 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
     hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
     if (precision == BEEP_FP64) {
         const size_t need = size_t(ctx->nnz) * 2 * sizeof(double2);
-        require(ctx->coef64.p || need + (size_t(1) << 30) < freeb,
+        const size_t have = ctx->coef64.n * sizeof(double2);
+        require(need <= have || need + (size_t(1) << 30) < freeb + have,
                 "not enough device memory for FP64 coefficients");
         ctx->coef64.alloc(size_t(ctx->nnz) * 2);
         launch_near_coeffs_fp64(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
@@ -854,7 +858,8 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
         ctx->uid64 = uid;
     } else {
         const size_t need = size_t(ctx->nnz) * sizeof(float4);
-        require(ctx->coef32.p || need + (size_t(1) << 30) < freeb,
+        const size_t have = ctx->coef32.n * sizeof(float4);
+        require(need <= have || need + (size_t(1) << 30) < freeb + have,
                 "not enough device memory for FP32 coefficients");
         ctx->coef32.alloc(size_t(ctx->nnz));
         launch_near_coeffs_fp32(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
@@ -1243,8 +1248,12 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
             launch_fill(s, n, 0.0, ctx->dvec.p);
             bool conv = false;
             double est = 0;
-            stats.iterations += gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
-                                      o->max_iter, &matvecs, &conv, &est, true);
+            const int it = gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
+                                 o->max_iter, &matvecs, &conv, &est, true);
+            if (o->verbose)
+                fprintf(stderr, "inner %d: %d iterations, estimate |r - K d|/|r| = %.3e%s\n", k,
+                        it, rn > 0 ? est / rn : 0.0, conv ? "" : " (not converged)");
+            stats.iterations += it;
             launch_axpby(s, n, 1.0, ctx->dvec.p, 1.0, d_x);
             stats.refinements = k + 1;
         }

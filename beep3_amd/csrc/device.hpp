@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -91,15 +92,18 @@ struct DevArena {
     DevArena(const DevArena&) = delete;
     DevArena& operator=(const DevArena&) = delete;
     ~DevArena() { release(); }
+    // grow-only, a regrowth with 25 % headroom (plans of varying size reuse
+    // their buffers; a free waits for the whole device)
     void* get(const std::string& name, size_t bytes) {
         Slot& sl = slots[name];
         if (sl.p && bytes <= sl.bytes) return sl.p;
+        size_t b = bytes ? bytes : 16;
         if (sl.p) {
+            b = std::max(b, sl.bytes + sl.bytes / 4);
             (void)hipFree(sl.p);
             held -= sl.bytes;
             sl = Slot{};
         }
-        const size_t b = bytes ? bytes : 16;
         hip_check(hipMalloc(&sl.p, b), "hipMalloc");
         sl.bytes = b;
         held += b;

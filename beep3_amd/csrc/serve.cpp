@@ -80,15 +80,6 @@ struct beep_server {
     std::vector<std::thread> workers;
     std::vector<double> held;  // bytes charged to each slot
     double used = 0;
-    // raw energy coefficients of an instance depend only on its mesh and
-    // charges, not on where the instance is placed (Mesh::init_energy_precalcs
-    // runs once per mesh, mesh.cpp:730-887): kept per mesh stamp
-    struct EnergyCoeffs {
-        std::vector<double> ef, eh;
-    };
-    std::mutex ecache_mu;
-    std::unordered_map<uint64_t, EnergyCoeffs> ecache;
-    static constexpr size_t ECACHE_MAX = 64;  // meshes kept
     int waiting_for_memory = 0;
     long long next_id = 1;
     int pending = 0;  // submitted, not yet polled
@@ -200,33 +191,13 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     // 1/2 sum (h eps ec_h - f ec_f) per instance (mesh.cpp:1373-1403)
     const int ni = int(sc->inst.size());
     std::vector<double> ef(N), eh(N);
-    bool cached = true;
-    {
-        std::lock_guard<std::mutex> lk(ecache_mu);
-        for (int k = 0; k < ni && cached; ++k) {
-            const auto it = ecache.find(sc->inst[k].mesh->stamp);
-            const size_t n_k = size_t(sc->inst_patch_ptr[k + 1] - sc->inst_patch_ptr[k]);
-            cached = it != ecache.end() && it->second.ef.size() == n_k;
-            if (cached) {
-                std::copy(it->second.ef.begin(), it->second.ef.end(), ef.begin() + sc->inst_patch_ptr[k]);
-                std::copy(it->second.eh.begin(), it->second.eh.end(), eh.begin() + sc->inst_patch_ptr[k]);
-            }
-        }
-    }
-    if (!cached) {
-        check_hip(ctx, beep_hip_energy_coefficients(ctx, ni, sc->inst_patch_ptr.data(),
-                                                    sc->inst_charge_ptr.data(),
-                                                    nch ? sc->charges.data() : nullptr, ef.data(),
-                                                    eh.data()), "energy coefficients");
-        std::lock_guard<std::mutex> lk(ecache_mu);
-        if (ecache.size() + size_t(ni) > ECACHE_MAX) ecache.clear();
-        for (int k = 0; k < ni; ++k) {
-            const int p0 = sc->inst_patch_ptr[k], p1 = sc->inst_patch_ptr[k + 1];
-            ecache.emplace(sc->inst[k].mesh->stamp,
-                           EnergyCoeffs{std::vector<double>(ef.begin() + p0, ef.begin() + p1),
-                                        std::vector<double>(eh.begin() + p0, eh.begin() + p1)});
-        }
-    }
+    // computed per request, in the request's own placement: coefficients
+    // kept from another placement of the same mesh agree only to rounding,
+    // and a result must not depend on which requests ran before it
+    check_hip(ctx, beep_hip_energy_coefficients(ctx, ni, sc->inst_patch_ptr.data(),
+                                                sc->inst_charge_ptr.data(),
+                                                nch ? sc->charges.data() : nullptr, ef.data(),
+                                                eh.data()), "energy coefficients");
     double E = 0;
     for (int k = 0; k < ni; ++k) {
         const int p0 = sc->inst_patch_ptr[k], p1 = sc->inst_patch_ptr[k + 1];

@@ -69,6 +69,40 @@ def test_alone_and_in_mix_are_identical(far):
         assert np.array_equal(f, fa) and np.array_equal(h, ha), (k, rep)
 
 
+def test_poses_of_one_mesh_alone_and_in_mix():
+    """Poses of one shared mesh (two instances, FMM far field): every result,
+    energy included, has the same bits alone and in a mix whatever request
+    of the same mesh ran before it."""
+    mesh = configs.pose_mesh(5)
+    rng = np.random.default_rng(11)
+    poses = [configs.random_pose(rng) for _ in range(5)]
+
+    def scene(k):
+        q, off = poses[k]
+        sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 1500)
+        return sc.add_instance(mesh, 2.0, offset=[0.0, 0.0, 0.0]).add_instance(
+            mesh, 2.0, quat=q, offset=off)
+
+    opts = dict(keep_fh=True, far_field="fmm", fmm_order=5, fmm_leaf=40, budget_gib=32)
+    alone = {}
+    for k in range(len(poses)):
+        srv = Server(slots=1, **opts)  # a fresh server: nothing ran before
+        alone.update(_collect(srv, {srv.submit(scene(k)): k}))
+        srv.close()
+    srv = Server(slots=3, **opts)
+    ids = {}
+    for rep in range(2):
+        for k in reversed(range(len(poses))):
+            ids[srv.submit(scene(k))] = (k, rep)
+    mix = _collect(srv, ids)
+    srv.close()
+    for (k, rep), (r, f, h) in mix.items():
+        ra, fa, ha = alone[k]
+        assert r["converged"] == 1
+        assert r["energy"] == ra["energy"], (k, rep, r["energy"], ra["energy"])
+        assert np.array_equal(f, fa) and np.array_equal(h, ha), (k, rep)
+
+
 def test_born_ion_through_the_server():
     m = Mesh.icosphere(4, 2.0).set_charges([[0.0, 0.0, 0.0, 1.0]])
     srv = Server(slots=2, keep_fh=False, mode="mixed")
