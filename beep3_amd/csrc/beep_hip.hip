@@ -324,6 +324,14 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
         hip_check(hipEventRecord(c->ev_fork, s), "hipEventRecord");
         hip_check(hipStreamWaitEvent(sf, c->ev_fork, 0), "hipStreamWaitEvent");
     }
+    // near field (explicit pairs) + diagonal into y, launched first: the
+    // SpMV is the longer of the two, and the far field's dozen short kernels
+    // take the host a while to launch
+    const int mode = spmv_mode(precision);
+    launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
+                     c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
+                     c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
+    if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     // far field (1-point interaction of all pairs) into yfar
     launch_fill(sf, n2, 0.0, c->yfar.p);
     if (c->use_fmm) {
@@ -338,12 +346,6 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     }
     if (timed) hip_check(hipEventRecord(ev[3], sf), "hipEventRecord");
     if (c->overlap) hip_check(hipEventRecord(c->ev_join, sf), "hipEventRecord");
-    // near field (explicit pairs) + diagonal into y
-    const int mode = spmv_mode(precision);
-    launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
-                     c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
-                     c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
-    if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     if (c->use_fmm) {
         // the P2P part (VALU) follows the SpMV on this stream, beside the
         // MFMA-bound M2L of the far stream; it needs the packed sources
