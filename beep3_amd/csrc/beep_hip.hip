@@ -266,10 +266,10 @@ void timing_pop(beep_hip_ctx* c) {
     hipEvent_t* e = c->ev[slot];
     float a = 0, b = 0, t = 0;
     hip_check(hipEventSynchronize(e[2]), "hipEventSynchronize");
-    // e0 start, e3 far done (far stream), e1 near done, e2 end; serially
-    // (one stream) the far part runs first and the near part after e3
-    hip_check(hipEventElapsedTime(&b, e[0], e[3]), "hipEventElapsedTime");
-    hip_check(hipEventElapsedTime(&a, c->overlap ? e[0] : e[3], e[1]), "hipEventElapsedTime");
+    // e0 start, e1 near done, e3 far done (far stream), e2 end; serially
+    // (one stream) the near part runs first and the far part after e1
+    hip_check(hipEventElapsedTime(&a, e[0], e[1]), "hipEventElapsedTime");
+    hip_check(hipEventElapsedTime(&b, c->overlap ? e[0] : e[1], e[3]), "hipEventElapsedTime");
     hip_check(hipEventElapsedTime(&t, e[0], e[2]), "hipEventElapsedTime");
     float m2l = 0;
     if (c->slot_fmm[slot]) hip_check(hipEventElapsedTime(&m2l, e[4], e[5]), "hipEventElapsedTime");

@@ -135,7 +135,9 @@ typedef struct {
 /* Device time of the phases of a mat-vec.  With overlap on (the default) the
  * far field runs on a second stream while the near SpMV runs on the context
  * stream: near_ms and far_ms are then both measured from the start of the
- * mat-vec and total_ms is the whole mat-vec (max of the two + the final add). */
+ * mat-vec and total_ms is the whole mat-vec (max of the two + the final add).
+ * With overlap off both run on the context stream, the near part first:
+ * far_ms is then measured from the end of the near part. */
 typedef struct {
     double near_ms;     /* last mat-vec: near-field SpMV + diagonal */
     double far_ms;      /* last mat-vec: far field */
