@@ -195,9 +195,12 @@ __global__ __launch_bounds__(256) void k_m2m(const FmmDev d, int first, T* __res
 // the pairs of a level by canonical offset into chunks of 64; a block stages
 // the source expansions of its chunk through LDS with each pair's node
 // permutation applied (B operands), multiplies by the one canonical table
-// (A operands, 16 tables per level: they stay in L2) with 16x16x4 MFMA tiles,
-// and writes each pair's result, permuted back, to that pair's slot; the
-// slots of a target are summed by k_m2l_sum (deterministic, no atomics).
+// (A operands, 16 tables per level: they stay in L2) with 16x16x4 MFMA tiles.
+// Pairs of a class are listed target by target, so a chunk holds runs of one
+// target's pairs (segments, 3.3 pairs on average): the block sums each
+// segment's results, permuted back to the target's node order, through LDS
+// and writes one slot per segment; k_m2l_sum adds a target's segment slots
+// (deterministic, no atomics).
 typedef double mf64x4 __attribute__((ext_vector_type(4)));
 typedef float mf32x4 __attribute__((ext_vector_type(4)));
 
@@ -211,9 +214,6 @@ struct Mfma<double> {
     }
     // accumulator element r of lane l holds row l/16 + 4r (column l % 16)
     static __device__ __forceinline__ int row(int l, int r) { return l / 16 + 4 * r; }
-    // lane group (l / 16) and element of row rr
-    static __device__ __forceinline__ int lane_group(int rr) { return rr & 3; }
-    static __device__ __forceinline__ int elem(int rr) { return rr >> 2; }
 };
 template <>
 struct Mfma<float> {
@@ -223,8 +223,6 @@ struct Mfma<float> {
     }
     // accumulator element r of lane l holds row 4 (l/16) + r (column l % 16)
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l / 16) + r; }
-    static __device__ __forceinline__ int lane_group(int rr) { return rr >> 2; }
-    static __device__ __forceinline__ int elem(int rr) { return rr & 3; }
 };
 
 // actual node index of canonical node index mc for a pair with symmetry code
@@ -268,11 +266,16 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     static_assert(SPS % RV == 0 && NSTEP % RV == 0, "slab / vector shape");
     using acc_t = typename Mfma<T>::acc_t;
     typedef T vec_t __attribute__((ext_vector_type(RV)));
-    // two B slab buffers [g][position of m]: slab s + 1 is stored while slab
-    // s feeds the matrix cores
-    __shared__ alignas(16) T sb[2][G * MSP];
+    // two B slab buffers [g][position of m] (slab s + 1 is stored while slab
+    // s feeds the matrix cores), then the results Y[g][canonical node] of
+    // the block's pairs, rows padded by one (conflict-free column writes)
+    constexpr int SLAB_BUF = G * MSP, YS = NPP + 1;
+    constexpr int SMEM = 2 * SLAB_BUF > G * YS ? 2 * SLAB_BUF : G * YS;
+    __shared__ alignas(16) T smem[SMEM];
     __shared__ int psrc[G];
     __shared__ const unsigned short* pperm[G];
+    __shared__ const unsigned short* pphi[G];
+    __shared__ int sstart[G + 1];  // first pair of each segment of the chunk (+ end)
     // block = (chunk, potential): potential 0 uses the Laplace table,
     // potentials 1 and 2 the Yukawa table of the level
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -280,9 +283,17 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     const int4 ch = chunks[cid];  // canonical offset, pairs, level
     const int off = ch.x, p0 = ch.y, np = ch.z - ch.y, level = ch.w;
     const int t = threadIdx.x;
+    const int seg0 = d.chunk_seg[cid], nseg = d.chunk_seg[cid + 1] - seg0;
     if (t < G) {
+        const size_t sym = size_t(t < np ? d.pair_sym[p0 + t] : 0) * NP;
         psrc[t] = t < np ? d.pair_src[p0 + t] : 0;
-        pperm[t] = d.psi + size_t(t < np ? d.pair_sym[p0 + t] : 0) * NP;
+        pperm[t] = d.psi + sym;
+        pphi[t] = d.phi + sym;
+        if (t < np) {
+            const int sg = d.pair_seg[p0 + t];
+            if (t == 0 || d.pair_seg[p0 + t - 1] != sg) sstart[sg - seg0] = t;
+        }
+        if (t == 0) sstart[nseg] = np;
     }
     const int w = t >> 6, l = t & 63;
     const int kl16 = l / 16, il = l % 16;
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
     fetch(0, nxt);
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
-        if (qmm[i] >= 0) sb[0][qmm[i]] = nxt[i];
+        if (qmm[i] >= 0) smem[qmm[i]] = nxt[i];
     if (NSLAB > 1) fetch(MS, nxt);  // in flight during the first slab
     // A operands (kernel table values) one k-step ahead: the loads of step
     // k+1 are in flight while the MFMAs of step k run
@@ -345,11 +356,11 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
             if (slab + 1 < NSLAB) {
 #pragma unroll
                 for (int i = 0; i < NQ; ++i)
-                    if (qmm[i] >= 0) sb[(slab + 1) & 1][qmm[i]] = nxt[i];
+                    if (qmm[i] >= 0) smem[((slab + 1) & 1) * SLAB_BUF + qmm[i]] = nxt[i];
                 if (slab + 2 < NSLAB) fetch((slab + 2) * MS, nxt);
             }
         }
-        const T* sbuf = sb[slab & 1];
+        const T* sbuf = smem + (slab & 1) * SLAB_BUF;
         vec_t bv[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c)
@@ -369,80 +380,60 @@ __global__ __launch_bounds__(512) void k_m2l(const FmmDev d, const int4* __restr
             }
         }
     }
-    // each pair's result goes to its own slot of 3 x NPP values, the rows of
-    // a 16-row tile kept in the order the accumulators hold them (position
-    // 4 (l / 16) + element), so every lane stores one contiguous run of 4
-    // values and k_m2l_sum reads each slot as one contiguous block; the sum
-    // maps a node to its position through the pair's node map and the MFMA
-    // layout
+    // the results through LDS: Y[g][n] of pair g at canonical target node n
+    // (accumulator element r of lane l holds row Mfma<T>::row(l, r) of the
+    // tile, column l % 16)
     const T scale = e == 0 ? T(1.0 / d.box_h[level]) : T(1);  // Laplace tables are built at h = 1
+    __syncthreads();  // every read of the last slab done: the slab buffers become Y
 #pragma unroll
     for (int k = 0; k < NTW; ++k) {
         const int nt = w + M2L_WAVES * k;
         if (nt >= NTILE) continue;
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            acc_t v = acc[k][c];
-            if (e == 0) v = v * scale;
-            const size_t pair = size_t(cid) * G + c * 16 + il;
-            *reinterpret_cast<acc_t*>(Cout + (pair * 3 + e) * NPP + nt * 16 + kl16 * 4) = v;
+            const acc_t v = acc[k][c];
+            T* col = smem + (c * 16 + il) * YS + nt * 16;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) col[Mfma<T>::row(l, r)] = v[r] * scale;
         }
+    }
+    __syncthreads();
+    // one slot per segment: its pairs' results at each actual target node m
+    // (canonical node phi_g(m) of pair g), added in pair order
+    for (int o = t; o < nseg * NP; o += 512) {
+        const int sl = o / NP, m = o - sl * NP;
+        T sum = 0;
+        for (int g = sstart[sl]; g < sstart[sl + 1]; ++g) sum += smem[g * YS + pphi[g][m]];
+        Cout[(size_t(seg0 + sl) * 3 + e) * NP + m] = sum;
     }
 }
 
-// Sum the M2L slots of every target box into its local expansion (one block
-// per box at levels >= 2; pairs listed per target in tgt_ptr / tgt_pair, at
-// most 189 per box).  The slot codes and node maps of the box's pairs are
-// staged in LDS; each thread owns outputs (e, n) and adds the pairs in list
-// order, so the sum is deterministic.
-constexpr int M2L_SUM_K = 192;  // pairs staged per pass
-
+// Sum the M2L segment slots of every target box into its local expansion
+// (one block per box at levels >= 2; a box's segments listed in increasing
+// order in tgt_ptr / tgt_seg).  Each thread owns outputs (e, n) and adds the
+// slots in list order, so the sum is deterministic.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_m2l_sum(const FmmDev d, int first_box,
                                                  const T* __restrict__ Cin, T* __restrict__ L) {
     constexpr int NP = P * P * P;
-    constexpr int NPP = (NP + 15) / 16 * 16;
     constexpr int NQ = (3 * NP + 255) / 256;
-    __shared__ int s_code[M2L_SUM_K];
-    __shared__ int s_sym[M2L_SUM_K];
     const int box = first_box + blockIdx.x;
     const int k0 = d.tgt_ptr[box], k1 = d.tgt_ptr[box + 1];
     if (k0 == k1) return;
     const int t = threadIdx.x;
     T acc[NQ];
-    int qoff[NQ], qn[NQ];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        const int q = t + 256 * i;
-        const int e = q / NP, n = q - e * NP;
-        acc[i] = T(0);
-        qoff[i] = q < 3 * NP ? e * NPP : -1;
-        qn[i] = n;
-    }
-    for (int kb = k0; kb < k1; kb += M2L_SUM_K) {
-        const int nk = min(M2L_SUM_K, k1 - kb);
-        __syncthreads();  // previous pass consumed
-        if (t < nk) {
-            const int pr = d.tgt_pair[kb + t];
-            s_code[t] = d.pair_slot[pr];
-            s_sym[t] = d.pair_sym[pr] * NP;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-            if (qoff[i] < 0) continue;
+    for (int i = 0; i < NQ; ++i) acc[i] = T(0);
 #pragma unroll 4
-            for (int k = 0; k < nk; ++k) {
-                const int nc = d.phi[s_sym[k] + qn[i]];  // canonical node
-                const int rr = nc & 15;
-                const int pos = (nc & ~15) + Mfma<T>::lane_group(rr) * 4 + Mfma<T>::elem(rr);
-                acc[i] += Cin[size_t(s_code[k]) * 3 * NPP + qoff[i] + pos];
-            }
-        }
+    for (int k = k0; k < k1; ++k) {
+        const T* c = Cin + size_t(d.tgt_seg[k]) * 3 * NP;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+            if (t + 256 * i < 3 * NP) acc[i] += c[t + 256 * i];
     }
 #pragma unroll
     for (int i = 0; i < NQ; ++i)
-        if (qoff[i] >= 0) L[size_t(box) * 3 * NP + t + 256 * i] += acc[i];
+        if (t + 256 * i < 3 * NP) L[size_t(box) * 3 * NP + t + 256 * i] += acc[i];
 }
 
 // L2L: one block per child box at level l+1 (boxes [first, first+count)).
@@ -972,8 +963,9 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     // interaction lists per level (l >= 2), grouped by offset into chunks
     constexpr int G = FmmPlan::CHUNK;
     static_assert(FmmPlan::CHUNK == M2L_G, "plan chunks match the M2L block");
-    std::vector<int> pair_src, pair_tgt, pair_sym, pair_slot;
+    std::vector<int> pair_src, pair_tgt, pair_sym, pair_seg, chunk_seg, seg_tgt;
     std::vector<int4> chunks;
+    int n_segs = 0;
     pl->lvl_chunk.assign(D + 2, 0);
     for (int l = 0; l <= D; ++l) {
         pl->lvl_chunk[l] = int(chunks.size());
@@ -1007,25 +999,36 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
             auto& v = byoff[off];
             for (size_t k = 0; k < v.size(); k += G) {
                 const int b0 = int(pair_src.size());
+                chunk_seg.push_back(n_segs);
                 for (size_t j = k; j < std::min(v.size(), k + G); ++j) {
+                    // a segment: a run of one target's pairs inside the chunk
+                    // (pairs of a class are listed target by target)
+                    if (j == k || v[j][0] != v[j - 1][0]) {
+                        seg_tgt.push_back(v[j][0]);
+                        ++n_segs;
+                    }
                     pair_tgt.push_back(v[j][0]);
                     pair_src.push_back(v[j][1]);
                     pair_sym.push_back(v[j][2]);
-                    pair_slot.push_back(int(chunks.size()) * G + int(j - k));
+                    pair_seg.push_back(n_segs - 1);
                 }
                 chunks.push_back(make_int4(off, b0, int(pair_src.size()), l));
             }
         }
     }
     pl->lvl_chunk[D + 1] = int(chunks.size());
+    chunk_seg.push_back(n_segs);
     if (D >= 2 && pl->lvl_chunk[2] != 0) throw std::runtime_error("M2L chunks must start at level 2");
     pl->n_pairs = int(pair_src.size());
-    std::vector<int> tgt_ptr(nbox + 1, 0), tgt_pair(pair_tgt.size());
-    for (int tg : pair_tgt) tgt_ptr[tg + 1]++;
+    pl->n_segs = n_segs;
+    // per target box its segments, in increasing order (the order k_m2l_sum
+    // adds them in)
+    std::vector<int> tgt_ptr(nbox + 1, 0), tgt_seg(static_cast<size_t>(n_segs));
+    for (int tg : seg_tgt) tgt_ptr[tg + 1]++;
     for (int g = 0; g < nbox; ++g) tgt_ptr[g + 1] += tgt_ptr[g];
     {
         std::vector<int> fill(tgt_ptr.begin(), tgt_ptr.end() - 1);
-        for (size_t k = 0; k < pair_tgt.size(); ++k) tgt_pair[fill[pair_tgt[k]]++] = int(k);
+        for (int k = 0; k < n_segs; ++k) tgt_seg[fill[seg_tgt[k]]++] = k;
     }
     // 1-D Chebyshev tables
     const int P = order;
@@ -1075,7 +1078,8 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
     upload(arena, "pair_tgt", pl->dv.pair_tgt, pair_tgt, s);
     upload(arena, "pair_sym", pl->dv.pair_sym, pair_sym, s);
-    upload(arena, "pair_slot", pl->dv.pair_slot, pair_slot, s);
+    upload(arena, "pair_seg", pl->dv.pair_seg, pair_seg, s);
+    upload(arena, "chunk_seg", pl->dv.chunk_seg, chunk_seg, s);
     // node maps of the 48 symmetries: psi (canonical -> actual) and phi
     // (actual -> canonical), order^3 entries each (host copies live until the
     // stream is synchronised below)
@@ -1106,7 +1110,7 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
         upload(arena, "canon", pl->d_canon, canon, s);
     }
     upload(arena, "tgt_ptr", pl->dv.tgt_ptr, tgt_ptr, s);
-    upload(arena, "tgt_pair", pl->dv.tgt_pair, tgt_pair, s);
+    upload(arena, "tgt_seg", pl->dv.tgt_seg, tgt_seg, s);
     upload(arena, "tn", pl->dv.tn, tn, s);
     upload(arena, "t_oct", pl->dv.t_oct, t_oct, s);
     upload(arena, "chunks", pl->d_chunks, chunks, s);
@@ -1157,12 +1161,10 @@ static void make_tables(FmmPlan* pl, double kappa, std::vector<T*>& tabs, T**& y
               "H2D ytab");
 }
 
-// M2L result slots: 3 x NPP values for each of the 64 pair positions of
-// every chunk
+// M2L result slots: 3 x P^3 values per target segment
 static size_t slot_count(const FmmPlan* pl) {
-    const int NP = pl->P * pl->P * pl->P, NPP = (NP + 15) / 16 * 16;
-    const size_t nch = size_t(pl->lvl_chunk.empty() ? 0 : pl->lvl_chunk.back());
-    return std::max<size_t>(1, nch * 3 * NPP * M2L_G);
+    const int NP = pl->P * pl->P * pl->P;
+    return std::max<size_t>(1, size_t(pl->n_segs) * 3 * NP);
 }
 
 void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s) {

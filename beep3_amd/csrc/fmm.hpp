@@ -27,11 +27,12 @@ struct FmmDev {
     int* pair_src = nullptr;    // M2L pairs grouped by offset
     int* pair_tgt = nullptr;
     int* pair_sym = nullptr;    // symmetry (0..47) taking each pair to its canonical offset
-    int* pair_slot = nullptr;   // chunk * 64 + position of each pair in its M2L chunk
+    int* pair_seg = nullptr;    // target segment of each pair (runs of one target in a chunk)
+    int* chunk_seg = nullptr;   // per chunk: its first segment (+ end)
     unsigned short* psi = nullptr;  // [48][P^3] canonical node -> actual node
     unsigned short* phi = nullptr;  // [48][P^3] actual node -> canonical node
-    int* tgt_ptr = nullptr;     // per box: its M2L pairs (indices into pair_*)
-    int* tgt_pair = nullptr;
+    int* tgt_ptr = nullptr;     // per box: its M2L segments, increasing
+    int* tgt_seg = nullptr;
     double* tn = nullptr;       // T_j(xi_k), P x P
     double* t_oct = nullptr;    // child-node interpolation factors, 2 x P x P
     // leaf-pass pairs that are not explicit near pairs (the others are folded
@@ -86,7 +87,7 @@ struct FmmPlan {
     static constexpr int CHUNK = 64;  // M2L pairs per block
     static constexpr int NCANON = 16; // canonical M2L offsets
     unsigned long long uid = 0;       // unique per plan built in the process
-    int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0;
+    int N = 0, P = 4, D = 0, nbox = 0, nleaf = 0, n_pairs = 0, n_segs = 0;
     long long n_p2p = 0;              // target-source pairs of the leaf pass (self included)
     unsigned* cellkey = nullptr;      // per patch (original index); null when D > 10
     int* d_iperm = nullptr;           // patch -> FMM order
@@ -108,7 +109,7 @@ struct FmmPlan {
     float* exp32 = nullptr;
     double** ytab64 = nullptr;        // device copies of the table pointers
     float** ytab32 = nullptr;
-    double* slots64 = nullptr;        // M2L results per pair (n_pairs x 3 x P^3)
+    double* slots64 = nullptr;        // M2L results per target segment (n_segs x 3 x P^3)
     float* slots32 = nullptr;
     Src64* src = nullptr;
     bool have64 = false, have32 = false;
