@@ -963,7 +963,7 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     // interaction lists per level (l >= 2), grouped by offset into chunks
     constexpr int G = FmmPlan::CHUNK;
     static_assert(FmmPlan::CHUNK == M2L_G, "plan chunks match the M2L block");
-    std::vector<int> pair_src, pair_tgt, pair_sym, pair_seg, chunk_seg, seg_tgt;
+    std::vector<int> pair_src, pair_sym, pair_seg, chunk_seg, seg_tgt;
     std::vector<int4> chunks;
     int n_segs = 0;
     pl->lvl_chunk.assign(D + 2, 0);
@@ -1007,7 +1007,6 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
                         seg_tgt.push_back(v[j][0]);
                         ++n_segs;
                     }
-                    pair_tgt.push_back(v[j][0]);
                     pair_src.push_back(v[j][1]);
                     pair_sym.push_back(v[j][2]);
                     pair_seg.push_back(n_segs - 1);
@@ -1076,7 +1075,6 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     upload(arena, "nb_ptr", pl->dv.nb_ptr, nb_ptr, s);
     upload(arena, "nb_leaf", pl->dv.nb_leaf, nb_leaf, s);
     upload(arena, "pair_src", pl->dv.pair_src, pair_src, s);
-    upload(arena, "pair_tgt", pl->dv.pair_tgt, pair_tgt, s);
     upload(arena, "pair_sym", pl->dv.pair_sym, pair_sym, s);
     upload(arena, "pair_seg", pl->dv.pair_seg, pair_seg, s);
     upload(arena, "chunk_seg", pl->dv.chunk_seg, chunk_seg, s);

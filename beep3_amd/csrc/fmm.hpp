@@ -25,7 +25,6 @@ struct FmmDev {
     int* nb_ptr = nullptr;      // per leaf: neighbour leaf list
     int* nb_leaf = nullptr;
     int* pair_src = nullptr;    // M2L pairs grouped by offset
-    int* pair_tgt = nullptr;
     int* pair_sym = nullptr;    // symmetry (0..47) taking each pair to its canonical offset
     int* pair_seg = nullptr;    // target segment of each pair (runs of one target in a chunk)
     int* chunk_seg = nullptr;   // per chunk: its first segment (+ end)
