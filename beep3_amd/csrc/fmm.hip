@@ -1219,8 +1219,8 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
         if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, M);
     }
     {
-        // chunks exist from level 2 on and are numbered from 0 (the slot
-        // index of a pair is its chunk number)
+        // chunks exist from level 2 on and are numbered from 0; their
+        // segments (slots) are numbered in chunk order
         const int nch = D >= 2 ? pl->lvl_chunk[D + 1] : 0;
         if (m2l_ev) hip_check(hipEventRecord(m2l_ev[0], s), "hipEventRecord");
         if (nch > 0) {
