@@ -291,9 +291,9 @@ def main():
     ap.add_argument("--pose-seq", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: every box in this setup has one MI355X, and the "
-              "multi-GPU path (octree-subtree sharding, all-gather x / reduce-scatter y per "
-              "mat-vec) is a design, not a build.  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the multi-GPU path (octree-subtree sharding, "
+              "one all-gather of x per mat-vec, each GPU writing its own rows of y) is a "
+              "design, not a build.  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     t_setup = time.perf_counter()
