@@ -142,46 +142,58 @@ __global__ __launch_bounds__(256) void k_p2m(const FmmDev d, const Src64* __rest
     }
 }
 
-// M2M: one block per parent box at level l-1 (boxes [first, first+count));
-// children listed in child_ptr/child_box with octant bits.
+// M2M: one block per parent box (boxes [first, first+count)).  The child
+// -> parent interpolation is a tensor product of one P x P factor per axis,
+// applied one axis at a time (3 P^4 instead of P^6 products per potential),
+// all children side by side; the children are then added in list order.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_m2m(const FmmDev d, int first, T* __restrict__ M) {
-    constexpr int NP = P * P * P;
-    __shared__ T cm[3 * NP];
+    constexpr int NP = P * P * P, NV = 3 * NP;
     __shared__ T tt[2][P][P];
+    __shared__ T buf[2][8][NV];
+    __shared__ int soct[8];
     const int box = first + blockIdx.x;
     const int t = threadIdx.x;
+    const int k0 = d.child_ptr[box], nch = d.child_ptr[box + 1] - k0;  // at most 8
     for (int k = t; k < 2 * P * P; k += blockDim.x) (&tt[0][0][0])[k] = T(d.t_oct[k]);
-    const int a = t / (P * P), b = (t / P) % P, c = t % P;
-    T acc[3] = {0, 0, 0};
-    for (int k = d.child_ptr[box]; k < d.child_ptr[box + 1]; ++k) {
-        const int ch = d.child_box[k];
-        const int oct = d.child_oct[k];
-        __syncthreads();
-        for (int q = t; q < 3 * NP; q += blockDim.x) cm[q] = M[size_t(ch) * 3 * NP + q];
-        __syncthreads();
-        if (t < NP) {
-            const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-            for (int a2 = 0; a2 < P; ++a2) {
-                const T fx = tt[ox][a][a2];
-                for (int b2 = 0; b2 < P; ++b2) {
-                    const T fxy = fx * tt[oy][b][b2];
-                    for (int c2 = 0; c2 < P; ++c2) {
-                        const T f = fxy * tt[oz][c][c2];
-                        const int m = (a2 * P + b2) * P + c2;
-                        acc[0] += f * cm[m];
-                        acc[1] += f * cm[NP + m];
-                        acc[2] += f * cm[2 * NP + m];
-                    }
-                }
-            }
-        }
+    if (t < nch) soct[t] = d.child_oct[k0 + t];
+    for (int q = t; q < nch * NV; q += blockDim.x) {
+        const int k = q / NV, r = q - k * NV;
+        buf[0][k][r] = M[size_t(d.child_box[k0 + k]) * NV + r];
     }
-    if (t < NP) {
-        T* out = M + size_t(box) * 3 * NP;
-        out[t] = acc[0];
-        out[NP + t] = acc[1];
-        out[2 * NP + t] = acc[2];
+    __syncthreads();
+    // z: buf1[k][e][a2][b2][c] = sum_c2 tt[oz][c][c2] buf0[k][e][a2][b2][c2]
+    for (int q = t; q < nch * NV; q += blockDim.x) {
+        const int k = q / NV, r = q - k * NV;
+        const int c = r % P, base = r - c;
+        const int oz = (soct[k] >> 2) & 1;
+        T v = 0;
+#pragma unroll
+        for (int c2 = 0; c2 < P; ++c2) v += tt[oz][c][c2] * buf[0][k][base + c2];
+        buf[1][k][r] = v;
+    }
+    __syncthreads();
+    // y: buf0[k][e][a2][b][c] = sum_b2 tt[oy][b][b2] buf1[k][e][a2][b2][c]
+    for (int q = t; q < nch * NV; q += blockDim.x) {
+        const int k = q / NV, r = q - k * NV;
+        const int b = (r / P) % P, base = r - b * P;
+        const int oy = (soct[k] >> 1) & 1;
+        T v = 0;
+#pragma unroll
+        for (int b2 = 0; b2 < P; ++b2) v += tt[oy][b][b2] * buf[1][k][base + b2 * P];
+        buf[0][k][r] = v;
+    }
+    __syncthreads();
+    // x, and the sum over children: M[e][a][b][c] = sum_k sum_a2 tt[ox][a][a2] buf0[k][e][a2][b][c]
+    for (int r = t; r < NV; r += blockDim.x) {
+        const int a = (r / (P * P)) % P, base = r - a * P * P;
+        T v = 0;
+        for (int k = 0; k < nch; ++k) {
+            const int ox = soct[k] & 1;
+#pragma unroll
+            for (int a2 = 0; a2 < P; ++a2) v += tt[ox][a][a2] * buf[0][k][base + a2 * P * P];
+        }
+        M[size_t(box) * NV + r] = v;
     }
 }
 
@@ -437,39 +449,45 @@ __global__ __launch_bounds__(256) void k_m2l_sum(const FmmDev d, int first_box,
 }
 
 // L2L: one block per child box at level l+1 (boxes [first, first+count)).
+// The parent -> child interpolation, one axis at a time as in k_m2m.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void k_l2l(const FmmDev d, int first, T* __restrict__ L) {
-    constexpr int NP = P * P * P;
-    __shared__ T pl[3 * NP];
+    constexpr int NP = P * P * P, NV = 3 * NP;
+    __shared__ T buf[2][NV];
     __shared__ T tt[2][P][P];
     const int box = first + blockIdx.x;
     const int par = d.box_parent[box];
     const int oct = d.box_oct[box];
+    const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
     const int t = threadIdx.x;
     for (int k = t; k < 2 * P * P; k += blockDim.x) (&tt[0][0][0])[k] = T(d.t_oct[k]);
-    for (int q = t; q < 3 * NP; q += blockDim.x) pl[q] = L[size_t(par) * 3 * NP + q];
+    for (int q = t; q < NV; q += blockDim.x) buf[0][q] = L[size_t(par) * NV + q];
     __syncthreads();
-    if (t < NP) {
-        const int a2 = t / (P * P), b2 = (t / P) % P, c2 = t % P;
-        const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1;
-        T acc[3] = {0, 0, 0};
-        for (int a = 0; a < P; ++a) {
-            const T fx = tt[ox][a][a2];
-            for (int b = 0; b < P; ++b) {
-                const T fxy = fx * tt[oy][b][b2];
-                for (int c = 0; c < P; ++c) {
-                    const T f = fxy * tt[oz][c][c2];
-                    const int m = (a * P + b) * P + c;
-                    acc[0] += f * pl[m];
-                    acc[1] += f * pl[NP + m];
-                    acc[2] += f * pl[2 * NP + m];
-                }
-            }
-        }
-        T* out = L + size_t(box) * 3 * NP;
-        out[t] += acc[0];
-        out[NP + t] += acc[1];
-        out[2 * NP + t] += acc[2];
+    // z: buf1[e][a][b][c2] = sum_c tt[oz][c][c2] buf0[e][a][b][c]
+    for (int r = t; r < NV; r += blockDim.x) {
+        const int c2 = r % P, base = r - c2;
+        T v = 0;
+#pragma unroll
+        for (int c = 0; c < P; ++c) v += tt[oz][c][c2] * buf[0][base + c];
+        buf[1][r] = v;
+    }
+    __syncthreads();
+    // y: buf0[e][a][b2][c2] = sum_b tt[oy][b][b2] buf1[e][a][b][c2]
+    for (int r = t; r < NV; r += blockDim.x) {
+        const int b2 = (r / P) % P, base = r - b2 * P;
+        T v = 0;
+#pragma unroll
+        for (int b = 0; b < P; ++b) v += tt[oy][b][b2] * buf[1][base + b * P];
+        buf[0][r] = v;
+    }
+    __syncthreads();
+    // x: L_child[e][a2][b2][c2] += sum_a tt[ox][a][a2] buf0[e][a][b2][c2]
+    for (int r = t; r < NV; r += blockDim.x) {
+        const int a2 = (r / (P * P)) % P, base = r - a2 * P * P;
+        T v = 0;
+#pragma unroll
+        for (int a = 0; a < P; ++a) v += tt[ox][a][a2] * buf[0][base + a * P * P];
+        L[size_t(box) * NV + r] += v;
     }
 }
 
@@ -1216,7 +1234,7 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     // parents at levels D-1 .. 2 (level-1 multipoles are never translated)
     for (int l = D; l >= 3; --l) {
         const int first = pl->lvl_first[l - 1], cnt = pl->lvl_first[l] - first;
-        if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, M);
+        if (cnt > 0) hipLaunchKernelGGL((k_m2m<T, P>), dim3(cnt), dim3(256), 0, s, d, first, M);
     }
     {
         // chunks exist from level 2 on and are numbered from 0; their
@@ -1233,7 +1251,7 @@ static void evaluate(FmmPlan* pl, hipStream_t s, const Patch64* patches, const d
     }
     for (int l = 2; l < D; ++l) {
         const int first = pl->lvl_first[l + 1], cnt = pl->lvl_first[l + 2] - first;
-        if (cnt > 0) hipLaunchKernelGGL((k_l2l<T, P>), dim3(cnt), dim3(NT), 0, s, d, first, L);
+        if (cnt > 0) hipLaunchKernelGGL((k_l2l<T, P>), dim3(cnt), dim3(256), 0, s, d, first, L);
     }
     if (D >= 2)
         hipLaunchKernelGGL((k_l2p<T, P>), dim3(pl->nleaf), dim3(64), 0, s, d, patches, L, y,
