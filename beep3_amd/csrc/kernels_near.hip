@@ -162,11 +162,12 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
     const T inv_eps = T(pj.inv_eps);
     const T ek = Math<T>::ek_of(kappa);
     T A = 0, B = 0, C = 0, D = 0;
+    constexpr int UNROLL = sizeof(T) == 4 ? 4 : 2;  // source points per unrolled step
     for (int a = 0; a < qn; ++a) {
         const Pt<T> q = qual[qb + a];
         T a_ = 0, b_ = 0, c_ = 0, d_ = 0;
         const T ox = dcx - q.x, oy = dcy - q.y, oz = dcz - q.z;
-#pragma unroll(sizeof(T) == 4 ? 4 : 2)
+#pragma unroll UNROLL
         for (int k = 0; k < pn; ++k) {
             const Pt<T> p = quad[pb + k];
             integrand<T, KAPPA>(kappa, ek, inv_eps, ox + p.x, oy + p.y, oz + p.z, q.nx, q.ny,
