@@ -12,6 +12,7 @@
 #include <cmath>
 #include <map>
 #include <string>
+#include <atomic>
 #include <vector>
 
 namespace beep {
@@ -170,10 +171,19 @@ struct Octree {
 }  // namespace beep
 
 // Opaque handle behind beep_mesh* of include/beep_host.h.
+// A fresh value from a process-wide counter: a mesh takes one when it is
+// made and again whenever its charges change, so equal stamps mean equal
+// surface and charges (a copy keeps its original's stamp).
+inline uint64_t next_mesh_stamp() {
+    static std::atomic<uint64_t> next{1};
+    return next++;
+}
+
 struct beep_mesh {
     beep::Mesh mesh;
     bool has_shape = false;
     beep::StarShape shape;
+    uint64_t stamp = next_mesh_stamp();
 };
 
 // Sets the message returned by beep_host_last_error() on this thread.

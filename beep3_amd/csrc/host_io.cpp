@@ -151,6 +151,7 @@ extern "C" int beep_mesh_read_charges(beep_mesh* m, const char* path) {
             m->mesh.charges.push_back({x, y, z, q});
         }
         m->mesh.finalize_charges();
+        m->stamp = next_mesh_stamp();
         return 0;
     } catch (const std::exception& e) {
         beep_host_set_error(e.what());

@@ -70,6 +70,7 @@ extern "C" int beep_mesh_set_charges(beep_mesh* m, int n, const double* xyzq) {
         for (int i = 0; i < n; ++i)
             for (int k = 0; k < 4; ++k) m->mesh.charges[i][k] = xyzq[4 * i + k];
         m->mesh.finalize_charges();
+        m->stamp = next_mesh_stamp();
     });
 }
 
@@ -79,6 +80,7 @@ extern "C" int beep_mesh_charges_ball(beep_mesh* m, int n, double rmax, double q
         if (!m || n < 0 || rmax <= 0) throw std::runtime_error("bad ball charges");
         charges_uniform_ball(m->mesh, n, rmax, qmin, qmax, seed);
         m->mesh.finalize_charges();
+        m->stamp = next_mesh_stamp();
     });
 }
 
@@ -89,6 +91,7 @@ extern "C" int beep_mesh_charges_walk(beep_mesh* m, int n, double step, double c
         if (n < 0 || step <= 0) throw std::runtime_error("bad walk charges");
         charges_random_walk(m->mesh, m->shape, n, step, clearance, qsigma, seed);
         m->mesh.finalize_charges();
+        m->stamp = next_mesh_stamp();
     });
 }
 

@@ -67,6 +67,46 @@ double estimate_bytes(const beep_scene* s, const beep_server_opts& o) {
     return b;
 }
 
+// The built scene's surface to the context (placed vertices, triangles,
+// triangles around each vertex, patch arrays); the device generates the
+// point sets.
+void upload_surface(beep_hip_ctx* ctx, beep_scene* sc) {
+    const int N = int(sc->centroid.size());
+    beep_scene_sizes sz{};
+    if (beep_scene_get_sizes(sc, &sz) != 0)
+        throw std::runtime_error(std::string("scene sizes: ") + beep_host_last_error());
+    std::vector<double> verts(size_t(3) * N);
+    std::vector<int> tris(size_t(3) * sz.n_triangles), vt_ptr(size_t(N) + 1),
+        vt_tri(size_t(sz.n_vertex_triangles)), vt_corner(size_t(sz.n_vertex_triangles));
+    if (beep_scene_surface(sc, verts.data(), tris.data(), vt_ptr.data(), vt_tri.data(),
+                           vt_corner.data()) != 0)
+        throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
+    beep_hip_surface g{};
+    g.n_patches = N;
+    g.centroid = &sc->centroid[0].x;
+    g.normal = &sc->normal[0].x;
+    g.area = sc->area.data();
+    g.eps_ratio = sc->eps.data();
+    g.gc = sc->gc.data();
+    g.verts = verts.data();
+    g.n_tris = sz.n_triangles;
+    g.tris = tris.data();
+    g.vt_ptr = vt_ptr.data();
+    g.vt_tri = vt_tri.data();
+    g.vt_corner = vt_corner.data();
+    g.qual_rule = sc->qual;
+    g.quad_rule = sc->quad;
+    check_hip(ctx, beep_hip_set_surface(ctx, &g), "geometry");
+}
+
+// An instance at the canonical placement of its mesh: no rotation and the
+// centre left where it is (beep_scene_add_instance with no pose).
+bool canonical_placement(const Instance& in) {
+    return in.rot.w == 1.0 && in.rot.x == 0.0 && in.rot.y == 0.0 && in.rot.z == 0.0 &&
+           in.offset.x == in.mesh->mesh.centre.x && in.offset.y == in.mesh->mesh.centre.y &&
+           in.offset.z == in.mesh->mesh.centre.z;
+}
+
 }  // namespace
 
 struct beep_server {
@@ -80,6 +120,17 @@ struct beep_server {
     std::vector<std::thread> workers;
     std::vector<double> held;  // bytes charged to each slot
     double used = 0;
+    // raw energy coefficients of a mesh at its canonical placement, per mesh
+    // stamp (surface and charges): Mesh::init_energy_precalcs runs once per
+    // mesh, in the mesh's own frame (mesh.cpp:730-887).  An instance placed
+    // otherwise uses them, so its energy does not depend on the requests
+    // that ran before it.
+    struct EnergyCoeffs {
+        std::vector<double> ef, eh;
+    };
+    std::mutex ecache_mu;
+    std::unordered_map<uint64_t, EnergyCoeffs> ecache;
+    static constexpr size_t ECACHE_MAX = 64;  // meshes kept
     int waiting_for_memory = 0;
     long long next_id = 1;
     int pending = 0;  // submitted, not yet polled
@@ -91,6 +142,8 @@ struct beep_server {
     }
     void worker(int slot);
     void solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d);
+    void energy_coefficients(beep_hip_ctx* ctx, beep_scene* sc, std::vector<double>& ef,
+                             std::vector<double>& eh);
     void release_slot(int slot, beep_hip_ctx*& ctx);  // caller holds mu
 };
 
@@ -100,6 +153,70 @@ void beep_server::release_slot(int slot, beep_hip_ctx*& ctx) {
     used -= held[slot];
     held[slot] = 0;
     cv_mem.notify_all();
+}
+
+// Raw energy coefficients of every instance of a built scene whose geometry
+// is on the context.  Instances at the canonical placement of their mesh are
+// computed from the context; the others take their mesh's coefficients at
+// the canonical placement from the cache, computed on a miss from a
+// one-instance scene (which replaces the context's geometry).
+void beep_server::energy_coefficients(beep_hip_ctx* ctx, beep_scene* sc, std::vector<double>& ef,
+                                      std::vector<double>& eh) {
+    const int ni = int(sc->inst.size());
+    const int nch = int(sc->charge_inst.size());
+    bool any_canonical = false;
+    for (int k = 0; k < ni; ++k) any_canonical |= canonical_placement(sc->inst[k]);
+    if (any_canonical)
+        check_hip(ctx, beep_hip_energy_coefficients(ctx, ni, sc->inst_patch_ptr.data(),
+                                                    sc->inst_charge_ptr.data(),
+                                                    nch ? sc->charges.data() : nullptr, ef.data(),
+                                                    eh.data()), "energy coefficients");
+    for (int k = 0; k < ni; ++k) {
+        const Instance& in = sc->inst[k];
+        if (canonical_placement(in)) continue;
+        const int p0 = sc->inst_patch_ptr[k], n_k = sc->inst_patch_ptr[k + 1] - p0;
+        const uint64_t key = (in.mesh->stamp << 10) | (uint64_t(sc->quad) << 5) | uint64_t(sc->qual);
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> lk(ecache_mu);
+            const auto it = ecache.find(key);
+            if (it != ecache.end() && it->second.ef.size() == size_t(n_k)) {
+                std::copy(it->second.ef.begin(), it->second.ef.end(), ef.begin() + p0);
+                std::copy(it->second.eh.begin(), it->second.eh.end(), eh.begin() + p0);
+                hit = true;
+            }
+        }
+        if (hit) continue;
+        beep_scene* one = nullptr;
+        if (beep_scene_create(sc->Dsolvent, sc->kappa, sc->quad, sc->qual, sc->nbsize, &one) != 0)
+            throw std::runtime_error(std::string("scene: ") + beep_host_last_error());
+        try {
+            if (beep_scene_add_instance(one, in.mesh, nullptr, nullptr, in.Dprotein) != 0)
+                throw std::runtime_error(std::string("scene: ") + beep_host_last_error());
+            one->host_points = false;
+            if (beep_scene_build(one) != 0)
+                throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
+            if (int(one->centroid.size()) != n_k)
+                throw std::runtime_error("canonical instance differs in size");
+            upload_surface(ctx, one);
+            EnergyCoeffs c{std::vector<double>(size_t(n_k)), std::vector<double>(size_t(n_k))};
+            const int nc1 = int(one->charge_inst.size());
+            const int ipp[2] = {0, n_k}, icp[2] = {0, nc1};
+            check_hip(ctx, beep_hip_energy_coefficients(ctx, 1, ipp, icp,
+                                                        nc1 ? one->charges.data() : nullptr,
+                                                        c.ef.data(), c.eh.data()),
+                      "energy coefficients");
+            std::copy(c.ef.begin(), c.ef.end(), ef.begin() + p0);
+            std::copy(c.eh.begin(), c.eh.end(), eh.begin() + p0);
+            std::lock_guard<std::mutex> lk(ecache_mu);
+            if (ecache.size() >= ECACHE_MAX) ecache.clear();
+            ecache.emplace(key, std::move(c));
+        } catch (...) {
+            beep_scene_destroy(one);
+            throw;
+        }
+        beep_scene_destroy(one);
+    }
 }
 
 void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
@@ -139,32 +256,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
             throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
     }
     check_hip(ctx, beep_hip_set_far_field(ctx, o.far_mode, o.fmm_order, o.fmm_leaf), "far field");
-    // the surface: placed vertices, triangles, triangles around each vertex
-    beep_scene_sizes sz{};
-    if (beep_scene_get_sizes(sc, &sz) != 0)
-        throw std::runtime_error(std::string("scene sizes: ") + beep_host_last_error());
-    std::vector<double> verts(size_t(3) * N);
-    std::vector<int> tris(size_t(3) * sz.n_triangles), vt_ptr(size_t(N) + 1),
-        vt_tri(size_t(sz.n_vertex_triangles)), vt_corner(size_t(sz.n_vertex_triangles));
-    if (beep_scene_surface(sc, verts.data(), tris.data(), vt_ptr.data(), vt_tri.data(),
-                           vt_corner.data()) != 0)
-        throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
-    beep_hip_surface g{};
-    g.n_patches = N;
-    g.centroid = &sc->centroid[0].x;
-    g.normal = &sc->normal[0].x;
-    g.area = sc->area.data();
-    g.eps_ratio = sc->eps.data();
-    g.gc = sc->gc.data();
-    g.verts = verts.data();
-    g.n_tris = sz.n_triangles;
-    g.tris = tris.data();
-    g.vt_ptr = vt_ptr.data();
-    g.vt_tri = vt_tri.data();
-    g.vt_corner = vt_corner.data();
-    g.qual_rule = sc->qual;
-    g.quad_rule = sc->quad;
-    check_hip(ctx, beep_hip_set_surface(ctx, &g), "geometry");
+    upload_surface(ctx, sc);
     beep_hip_tree t{};
     t.n_leaves = int(sc->leaf_ptr.size()) - 1;
     t.leaf_ptr = sc->leaf_ptr.data();
@@ -191,13 +283,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     // 1/2 sum (h eps ec_h - f ec_f) per instance (mesh.cpp:1373-1403)
     const int ni = int(sc->inst.size());
     std::vector<double> ef(N), eh(N);
-    // computed per request, in the request's own placement: coefficients
-    // kept from another placement of the same mesh agree only to rounding,
-    // and a result must not depend on which requests ran before it
-    check_hip(ctx, beep_hip_energy_coefficients(ctx, ni, sc->inst_patch_ptr.data(),
-                                                sc->inst_charge_ptr.data(),
-                                                nch ? sc->charges.data() : nullptr, ef.data(),
-                                                eh.data()), "energy coefficients");
+    energy_coefficients(ctx, sc, ef, eh);
     double E = 0;
     for (int k = 0; k < ni; ++k) {
         const int p0 = sc->inst_patch_ptr[k], p1 = sc->inst_patch_ptr[k + 1];
