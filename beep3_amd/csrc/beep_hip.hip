@@ -462,8 +462,9 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
 // GMRES without restart (BEEP::gmres, beep.cpp:1343-1432) on device vectors:
 // r = K x - b, classical Gram-Schmidt, Givens on the host, stop when
 // |y_j| <= eps |b|, x -= V y.  Returns inner iterations; adds mat-vecs.
+// bnorm: |b| when the caller has it (the same dot product), else < 0.
 int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double eps, int m,
-          int* matvecs, bool* converged, double* est_resid, bool x_is_zero) {
+          int* matvecs, bool* converged, double* est_resid, bool x_is_zero, double bnorm) {
     const int n = 2 * c->N;
     hipStream_t s = c->stream;
     c->V.alloc(size_t(n) * (m + 1));
@@ -482,7 +483,7 @@ int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double
         c->hh_cap = size_t(m + 2);
     }
     auto NEAR0 = [](double v) { return v == 0 ? 2.2250738585072014e-308 : v; };
-    const double nrm2b = std::sqrt(dev_dot(c, d_b, d_b, n));
+    const double nrm2b = bnorm >= 0 ? bnorm : std::sqrt(dev_dot(c, d_b, d_b, n));
     double* v0 = c->V.p;
     if (x_is_zero) {
         // K 0 - b = -b without a mat-vec
@@ -492,7 +493,8 @@ int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double
         ++*matvecs;
         launch_axpby(s, n, -1.0, d_b, 1.0, v0);
     }
-    const double beta = std::sqrt(dev_dot(c, v0, v0, n));
+    // |-b| is |b| to the bit (the same squares summed in the same order)
+    const double beta = x_is_zero ? nrm2b : std::sqrt(dev_dot(c, v0, v0, n));
     launch_scale_copy(s, n, v0, nullptr, 1.0 / NEAR0(beta), v0);
     std::vector<double> U(size_t(m) * (m + 1) / 2 + 1, 0.0), y(m + 2, 0.0), cs(m + 1), sn(m + 1),
         hh(m + 2);
@@ -1218,7 +1220,7 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
         bool conv = false;
         double est = 0;
         stats.iterations = gmres(ctx, prec, d_b, d_x, o->tol, o->max_iter, &matvecs, &conv, &est,
-                                 x_zero);
+                                 x_zero, bnorm);
         stats.converged = conv;
         stats.residual = est;
         stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
@@ -1239,7 +1241,8 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
                 ++matvecs;
                 launch_axpby(s, n, 1.0, d_b, -1.0, ctx->rvec.p);
             }
-            rn = std::sqrt(dev_dot(ctx, ctx->rvec.p, ctx->rvec.p, n));
+            // r = b on a cold start: its norm is |b|
+            rn = (k == 0 && x_zero) ? bnorm : std::sqrt(dev_dot(ctx, ctx->rvec.p, ctx->rvec.p, n));
             if (o->verbose)
                 fprintf(stderr, "refine %d: |r|/|b| = %.3e\n", k, bnorm > 0 ? rn / bnorm : 0.0);
             if (rn <= o->tol * bnorm) {
@@ -1251,7 +1254,7 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
             bool conv = false;
             double est = 0;
             const int it = gmres(ctx, BEEP_FP32, ctx->rvec.p, ctx->dvec.p, inner_tol,
-                                 o->max_iter, &matvecs, &conv, &est, true);
+                                 o->max_iter, &matvecs, &conv, &est, true, rn);
             if (o->verbose)
                 fprintf(stderr, "inner %d: %d iterations, estimate |r - K d|/|r| = %.3e%s\n", k,
                         it, rn > 0 ? est / rn : 0.0, conv ? "" : " (not converged)");
