@@ -192,6 +192,7 @@ void beep_host_set_error(const char* msg);
 // A rigid placement of a reference mesh (MeshInstance, mesh_instance.h:23-87).
 struct Instance {
     const beep_mesh* mesh;
+    uint64_t stamp = 0;  // the mesh's stamp when the instance was added
     beep::Quat rot;
     beep::Vec3 offset;
     double Dprotein;

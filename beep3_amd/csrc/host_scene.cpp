@@ -157,6 +157,7 @@ extern "C" int beep_scene_add_instance(beep_scene* s, const beep_mesh* m, const 
         if (!s || !m || Dprotein <= 0) throw std::runtime_error("bad instance");
         Instance in;
         in.mesh = m;
+        in.stamp = m->stamp;
         if (q4) {
             double n = std::sqrt(q4[0] * q4[0] + q4[1] * q4[1] + q4[2] * q4[2] + q4[3] * q4[3]);
             if (!(n > 0)) throw std::runtime_error("zero quaternion");

@@ -175,7 +175,7 @@ void beep_server::energy_coefficients(beep_hip_ctx* ctx, beep_scene* sc, std::ve
         const Instance& in = sc->inst[k];
         if (canonical_placement(in)) continue;
         const int p0 = sc->inst_patch_ptr[k], n_k = sc->inst_patch_ptr[k + 1] - p0;
-        const uint64_t key = (in.mesh->stamp << 10) | (uint64_t(sc->quad) << 5) | uint64_t(sc->qual);
+        const uint64_t key = (in.stamp << 10) | (uint64_t(sc->quad) << 5) | uint64_t(sc->qual);
         bool hit = false;
         {
             std::lock_guard<std::mutex> lk(ecache_mu);

@@ -74,7 +74,8 @@ typedef struct {
 
 int beep_server_create(const beep_server_opts* opts, beep_server** out);
 /* Takes ownership of the scene (destroyed by the server after the solve);
- * the meshes it refers to must outlive the request.  Non-blocking. */
+ * the meshes it refers to must outlive the request and must not change
+ * (charges included) while it is pending.  Non-blocking. */
 int beep_server_submit(beep_server* s, beep_scene* scene, long long* id);
 /* Next finished request: 0 when one was returned, 1 when none finished
  * within timeout_s (<= 0: do not wait). */

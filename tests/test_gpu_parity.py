@@ -36,7 +36,9 @@ def test_diag_and_near_only_fp64(dev_born):
     assert rel(y_dev, y_orc) < 1e-12
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 2e-5), ("fp32_resid", 2e-5)])
+# measured 1.2e-7 (FP32 coefficients and sums) and 7.3e-9 (FP32 coefficients,
+# FP64 sums): the bounds are a few times that
+@pytest.mark.parametrize("prec,tol", [("fp32", 5e-7), ("fp32_resid", 3e-8)])
 def test_matvec_fp32(dev_born, prec, tol):
     d, a, orc = dev_born
     rng = np.random.default_rng(12)
@@ -51,10 +53,25 @@ def test_rhs_and_energy_coeffs(dev_born):
     b_dev = d.rhs(a["charges"], 80.0)
     assert rel(b_dev, orc.rhs()) < 1e-13
     ef, eh = d.energy_coefficients(a["inst_patch_ptr"], a["inst_charge_ptr"], a["charges"])
-    # oracle coefficients are renormalised; compare the raw h coefficients and
-    # the renormalised f coefficients after the same host renormalisation
+    # the oracle's f coefficients are Gauss-renormalised (mesh.cpp:891-925);
+    # apply the same per-instance renormalisation to the device's raw ones
     ecf, ech = orc.energy_coeffs()
+    ef = renormalise(ef, a)
+    print("energy coefficients rel err: f", rel(ef, ecf), "h", rel(eh, ech))
     assert rel(eh, ech) < 1e-13
+    assert rel(ef, ecf) < 1e-13
+
+
+def renormalise(ef, a):
+    """Mesh::renormalise_energy_precalcs: spread the gap between the net
+    charge and -sum(ef) over the instance's patches by area."""
+    ef = ef.copy()
+    pp, cp = a["inst_patch_ptr"], a["inst_charge_ptr"]
+    for k in range(len(pp) - 1):
+        s = slice(int(pp[k]), int(pp[k + 1]))
+        net = float(np.sum(a["charges"][int(cp[k]):int(cp[k + 1]), 3]))
+        ef[s] -= (a["area"][s] / np.sum(a["area"][s])) * (net + float(np.sum(ef[s])))
+    return ef
 
 
 def test_solve_born_ion():
