@@ -12,6 +12,7 @@
 #include "../../include/beep_hip.h"
 #include "device.hpp"
 #include "fmm.hpp"
+#include "pw.hpp"
 #include "host.hpp"
 
 using namespace beepdev;
@@ -192,7 +193,8 @@ void far_setup(beep_hip_ctx* c) {
         fmm_plan_destroy(c->fmm);
         c->fmm = nullptr;
     }
-    c->use_fmm = c->far_mode == BEEP_FAR_FMM || (c->far_mode == BEEP_FAR_AUTO && c->N > 20000);
+    c->use_fmm = c->far_mode == BEEP_FAR_FMM || c->far_mode == BEEP_FAR_PW ||
+                 (c->far_mode == BEEP_FAR_AUTO && c->N > 20000);
     // the plan itself is built on first use (ensure_far), so its host work
     // can overlap the coefficient precompute already running on the device
 }
@@ -205,7 +207,7 @@ void begin_plan(beep_hip_ctx* c) {
     hip_check(hipStreamSynchronize(c->far_stream), "hipStreamSynchronize");
     const auto t0 = std::chrono::steady_clock::now();
     c->fmm = fmm_plan_begin(c->N, c->h_centroid.data(), c->fmm_order, c->fmm_leaf, c->fmm_arena,
-                            c->far_stream);
+                            c->far_stream, c->far_mode == BEEP_FAR_PW ? 1 : 0);
     c->plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     c->gen++;
 }
@@ -893,9 +895,13 @@ int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision) {
 
 int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target) {
     return guarded(ctx, [&] {
-        require(mode == BEEP_FAR_AUTO || mode == BEEP_FAR_DIRECT || mode == BEEP_FAR_FMM,
+        require(mode == BEEP_FAR_AUTO || mode == BEEP_FAR_DIRECT || mode == BEEP_FAR_FMM ||
+                    mode == BEEP_FAR_PW,
                 "unknown far-field mode");
-        require(order >= 3 && order <= 6, "FMM order must be 3..6");
+        if (mode == BEEP_FAR_PW)
+            require(pw_order_supported(order), "plane-wave FMM order must be 9, 12 or 18");
+        else
+            require(order >= 3 && order <= 6, "FMM order must be 3..6");
         require(leaf_target >= 1, "leaf_target must be positive");
         set_device(ctx);
         ctx->far_mode = mode;
@@ -928,10 +934,11 @@ int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
     return guarded(ctx, [&] {
         require(out != nullptr, "null output");
         *out = beep_far_info{};
-        out->mode = ctx->use_fmm ? BEEP_FAR_FMM : BEEP_FAR_DIRECT;
+        out->mode = ctx->use_fmm ? (ctx->far_mode == BEEP_FAR_PW ? BEEP_FAR_PW : BEEP_FAR_FMM)
+                                 : BEEP_FAR_DIRECT;
         ensure_far(ctx);
         if (ctx->use_fmm && ctx->fmm) {
-            out->order = ctx->fmm->P;
+            out->order = ctx->fmm->engine == 1 ? ctx->fmm->pw_p : ctx->fmm->P;
             out->leaf_level = ctx->fmm->D;
             out->n_leaf_boxes = ctx->fmm->nleaf;
             out->n_boxes = ctx->fmm->nbox;
@@ -1325,6 +1332,20 @@ int beep_hip_solve_device(beep_hip_ctx* ctx, const double* d_b, double* d_x,
         const bool x_zero = dev_dot(ctx, d_x, d_x, n) == 0.0;
         solve_impl(ctx, d_b, d_x, x_zero, o, st);
     });
+}
+
+int beep_hip_pw_selftest(int order, int nodes, double beta, double* err5) {
+    try {
+        if (!err5) throw std::runtime_error("null output");
+        if (order < 1 || order > PW_MAXP) throw std::runtime_error("plane-wave order out of range");
+        if (nodes != 9 && nodes != 18) throw std::runtime_error("quadrature nodes must be 9 or 18");
+        if (!(beta >= 0)) throw std::runtime_error("beta must be >= 0");
+        pw_selftest(order, nodes, beta, err5);
+        return 0;
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return -1;
+    }
 }
 
 int beep_hip_fmm_symmetry(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map) {

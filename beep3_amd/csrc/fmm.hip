@@ -33,6 +33,7 @@
 
 #include "device.hpp"
 #include "fmm.hpp"
+#include "pw.hpp"
 
 namespace beepdev {
 
@@ -806,12 +807,22 @@ struct FmmPlanStage {
 };
 
 FmmPlan* fmm_plan_begin(int N, const double* centroid, int order, int leaf_target,
-                        DevArena& arena, hipStream_t s) {
-    if (order < 3 || order > 6) throw std::runtime_error("FMM order must be 3..6");
+                        DevArena& arena, hipStream_t s, int engine) {
+    if (engine == 0 && (order < 3 || order > 6)) throw std::runtime_error("FMM order must be 3..6");
+    if (engine == 1 && !pw_order_supported(order))
+        throw std::runtime_error("plane-wave FMM order must be 9, 12 or 18");
+    if (engine != 0 && engine != 1) throw std::runtime_error("unknown FMM engine");
     auto* pl = new FmmPlan();
     pl->arena = &arena;
     pl->N = N;
-    pl->P = order;
+    pl->engine = engine;
+    // the Chebyshev order; the plane-wave engine keeps its own
+    pl->P = engine == 0 ? order : 3;
+    if (engine == 1) {
+        pl->pw_p = order;
+        pl->pw_nlam = order <= 9 ? 9 : 18;
+        pl->pw_jp = ((order + 1) * (order + 1) + 15) / 16 * 16;
+    }
     pl->stage = new FmmPlanStage();
     // bounding cube
     double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -1047,6 +1058,86 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
         std::vector<int> fill(tgt_ptr.begin(), tgt_ptr.end() - 1);
         for (int k = 0; k < n_segs; ++k) tgt_seg[fill[seg_tgt[k]]++] = k;
     }
+    if (pl->engine == 1) {
+        // children of every box (by octant), the X2X lists of every box at
+        // levels >= 2 (per direction, sources in the order met) and the L2L
+        // rows of every level grouped by child octant
+        std::vector<int> child8(size_t(nbox) * 8, -1);
+        for (int g = 0; g < nbox; ++g)
+            for (auto& kc : kids[g]) child8[size_t(g) * 8 + kc.second] = kc.first;
+        const int base = D >= 2 ? pl->lvl_first[2] : nbox;
+        std::vector<int> xl_ptr(size_t(nbox - base) * 6 + 1, 0), xl_src, xl_code;
+        double rot[6][3][3];
+        pw_direction_frames(rot);
+        pl->pw_maxlev = 0;
+        for (int l = 2; l <= D; ++l) pl->pw_maxlev = std::max(pl->pw_maxlev, int(lev[l].size()));
+        for (int l = 2; l <= D; ++l) {
+            const long pcells = 1L << (l - 1);
+            for (size_t q = 0; q < lev[l].size(); ++q) {
+                const Box& b = lev[l][q];
+                std::vector<std::pair<int, int>> byd[6];
+                const long px = b.x >> 1, py = b.y >> 1, pz = b.z >> 1;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const long X = px + dx, Y = py + dy, Z = pz + dz;
+                            if (X < 0 || Y < 0 || Z < 0 || X >= pcells || Y >= pcells || Z >= pcells) continue;
+                            const int pq = idx[l - 1].find(uint32_t(X), uint32_t(Y), uint32_t(Z));
+                            if (pq < 0) continue;
+                            for (auto& kc : kids[pl->lvl_first[l - 1] + pq]) {
+                                const Box& sb = lev[l][kc.first - pl->lvl_first[l]];
+                                // target centre - source centre, box units
+                                const int tx = int(long(b.x) - long(sb.x)), ty = int(long(b.y) - long(sb.y)),
+                                          tz = int(long(b.z) - long(sb.z));
+                                const int dir = pw_direction(tx, ty, tz);
+                                if (dir < 0) continue;
+                                int tl[3];
+                                for (int a = 0; a < 3; ++a)
+                                    tl[a] = int(std::lround(rot[dir][0][a] * tx + rot[dir][1][a] * ty +
+                                                            rot[dir][2][a] * tz));
+                                if (tl[2] < 2 || tl[2] > 3 || std::abs(tl[0]) > 3 || std::abs(tl[1]) > 3)
+                                    throw std::runtime_error("plane-wave list: offset out of range");
+                                byd[dir].push_back({kc.first, (tl[2] - 2) * 49 + (tl[1] + 3) * 7 + (tl[0] + 3)});
+                            }
+                        }
+                const size_t row = size_t(pl->lvl_first[l] + int(q) - base) * 6;
+                for (int dir = 0; dir < 6; ++dir) {
+                    for (auto& e : byd[dir]) {
+                        xl_src.push_back(e.first);
+                        xl_code.push_back(e.second);
+                    }
+                    xl_ptr[row + dir + 1] = int(xl_src.size());
+                }
+            }
+        }
+        pl->pw_n_x2x = (long long)xl_src.size();
+        std::vector<int> l2l_rows, l2l_gptr(size_t(D + 1) * 9, 0);
+        pl->pw_l2l_maxgroup.assign(D + 1, 0);
+        for (int l = 0; l <= D; ++l) {
+            for (int o = 0; o < 8; ++o) {
+                l2l_gptr[size_t(l) * 9 + o] = int(l2l_rows.size());
+                if (l >= 3)
+                    for (int g = pl->lvl_first[l]; g < pl->lvl_first[l + 1]; ++g)
+                        if (box_oct[g] == o) l2l_rows.push_back(g);
+                pl->pw_l2l_maxgroup[l] =
+                    std::max(pl->pw_l2l_maxgroup[l], int(l2l_rows.size()) - l2l_gptr[size_t(l) * 9 + o]);
+            }
+            l2l_gptr[size_t(l) * 9 + 8] = int(l2l_rows.size());
+        }
+        if (l2l_rows.empty()) l2l_rows.push_back(0);
+        if (xl_src.empty()) {
+            xl_src.push_back(0);
+            xl_code.push_back(0);
+        }
+        upload(arena, "pw_child8", pl->pw_child8, child8, s);
+        upload(arena, "pw_xl_ptr", pl->pw_xl_ptr, xl_ptr, s);
+        upload(arena, "pw_xl_src", pl->pw_xl_src, xl_src, s);
+        upload(arena, "pw_xl_code", pl->pw_xl_code, xl_code, s);
+        upload(arena, "pw_l2l_rows", pl->pw_l2l_rows, l2l_rows, s);
+        upload(arena, "pw_l2l_gptr", pl->pw_l2l_gptr, l2l_gptr, s);
+        // the host vectors must live until the uploads complete
+        hip_check(hipStreamSynchronize(s), "sync plane-wave lists");
+    }
     // 1-D Chebyshev tables
     const int P = order;
     std::vector<double> xi(P), tn(P * P), t_oct(2 * P * P);
@@ -1101,7 +1192,7 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     // stream is synchronised below)
     const int NPn = order * order * order;
     std::vector<unsigned short> psi(size_t(48) * NPn), phi(size_t(48) * NPn);
-    {
+    if (pl->engine == 0) {
         for (int k = 0; k < 48; ++k)
             for (int m = 0; m < NPn; ++m) {
                 int a = 0;
@@ -1142,8 +1233,8 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
 }
 
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
-                         DevArena& arena, hipStream_t s) {
-    FmmPlan* pl = fmm_plan_begin(N, centroid, order, leaf_target, arena, s);
+                         DevArena& arena, hipStream_t s, int engine) {
+    FmmPlan* pl = fmm_plan_begin(N, centroid, order, leaf_target, arena, s, engine);
     fmm_plan_finish(pl, s);
     return pl;
 }
@@ -1188,6 +1279,11 @@ void fmm_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s)
     if (pl->tables_kappa != kappa) {
         pl->have64 = pl->have32 = false;
         pl->tables_kappa = kappa;
+    }
+    if (pl->engine == 1) {
+        pw_prepare(pl, kappa, fp64, fp32, s);
+        hip_check(hipStreamSynchronize(s), "sync plane-wave tables");
+        return;
     }
     if (fp64 && !pl->have64) {
         switch (pl->P) {
@@ -1295,6 +1391,12 @@ void fmm_p2p(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, doubl
 
 void fmm_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, const double* x,
                   double kappa, double* y, hipEvent_t* m2l_ev, hipEvent_t pack_ev) {
+    if (pl->engine == 1) {
+        launch_far_pack(s, pl->N, patches, pl->dv.perm, x, pl->src);
+        if (pack_ev) hip_check(hipEventRecord(pack_ev, s), "hipEventRecord");
+        pw_evaluate(pl, s, fp32, patches, kappa, y, m2l_ev);
+        return;
+    }
     if (fp32) {
         if (!pl->have32) throw std::runtime_error("FMM FP32 tables not prepared");
         float* M = pl->exp32;

@@ -82,6 +82,21 @@ __device__ __forceinline__ void p2p_pair(T dx, T dy, T dz, T ie, T sdx, T sdy, T
 
 struct FmmPlanStage;
 
+// Device tables of the plane-wave engine, per level: [0] Laplace, [1] Yukawa
+template <typename T>
+struct PwDevLevel {
+    T* m2mT[2] = {nullptr, nullptr};  // M2M of this level's children ([8 jp][jp])
+    T* l2lT[2] = {nullptr, nullptr};  // L2L into this level's children ([8][jp][jp])
+    T* m2x[2] = {nullptr, nullptr};   // [jp][nwp]
+    T* x2l[2] = {nullptr, nullptr};   // [nwp][jp]
+    T* shift[2] = {nullptr, nullptr}; // [98][nw][2]
+    int nw[2] = {0, 0};
+};
+template <typename T>
+struct PwDevTables {
+    std::vector<PwDevLevel<T>> lev;
+};
+
 struct FmmPlan {
     static constexpr int CHUNK = 64;  // M2L pairs per block
     static constexpr int NCANON = 16; // canonical M2L offsets
@@ -114,18 +129,46 @@ struct FmmPlan {
     bool have64 = false, have32 = false;
     double tables_kappa = -1;
     DevArena* arena = nullptr;        // owner of every device buffer above
+    // plane-wave engine (engine 1): expansion order, quadrature nodes, padded
+    // coefficient and wave widths, the largest level's box count, the
+    // children of every box (8 per box, -1 absent), the X2X lists (per box
+    // at levels >= 2 and direction: source boxes and local offset codes),
+    // the L2L rows of every level grouped by child octant (9 offsets per
+    // level into pw_l2l_rows, device) with the largest group per level
+    int engine = 0;                   // 0: Chebyshev interpolation, 1: plane waves
+    int pw_p = 0, pw_nlam = 0, pw_jp = 0, pw_nwp = 0, pw_maxlev = 0;
+    long long pw_n_x2x = 0;           // X2X translations per mat-vec (all levels)
+    int* pw_child8 = nullptr;
+    int* pw_xl_ptr = nullptr;
+    int* pw_xl_src = nullptr;
+    int* pw_xl_code = nullptr;
+    int* pw_l2l_rows = nullptr;
+    int* pw_l2l_gptr = nullptr;
+    std::vector<int> pw_l2l_maxgroup;
+    PwDevTables<double> pw64;
+    PwDevTables<float> pw32;
+    double *pw_M64 = nullptr, *pw_L64 = nullptr, *pw_W64 = nullptr, *pw_Wacc64 = nullptr;
+    float *pw_M32 = nullptr, *pw_L32 = nullptr, *pw_W32 = nullptr, *pw_Wacc32 = nullptr;
 };
 
 // The plan's device buffers come from the arena (owned by the caller, reused
-// by the next plan built on it); one live plan per arena.
+// by the next plan built on it); one live plan per arena.  engine 0:
+// Chebyshev interpolation of the given order (3..6); engine 1: plane waves
+// of expansion order 9, 12 or 18 (9 quadrature nodes for order 9, else 18).
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
-                         DevArena& arena, hipStream_t s);
+                         DevArena& arena, hipStream_t s, int engine = 0);
 // The same in two halves: begin chooses the leaf level and uploads the cell
 // keys (stream-ordered on s; uid, D, cellkey valid), finish builds the rest
 // and synchronises s.
 FmmPlan* fmm_plan_begin(int N, const double* centroid, int order, int leaf_target,
-                        DevArena& arena, hipStream_t s);
+                        DevArena& arena, hipStream_t s, int engine = 0);
 void fmm_plan_finish(FmmPlan* p, hipStream_t s);
+// plane-wave engine (fmm_pw.hip): tables of the plan's levels for kappa and
+// the expansion buffers; the far field without the leaf pass into y
+bool pw_order_supported(int p);
+void pw_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
+void pw_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, double kappa, double* y,
+                 hipEvent_t* m2l_ev);
 void fmm_plan_destroy(FmmPlan* p);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 inline bool fmm_tables_ready(const FmmPlan* p, double kappa, bool fp64, bool fp32) {

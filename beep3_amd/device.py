@@ -83,7 +83,7 @@ class Device:
         t.nb_ptr, t.nb_idx = keep["nb_ptr"].ctypes.data, keep["nb_idx"].ctypes.data
         self._check(lib().beep_hip_set_tree(self._h, C.byref(t)))
 
-    FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1}
+    FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
 
     def set_far_field(self, mode="auto", order=4, leaf_target=48):
         m = self.FAR_MODES[mode] if isinstance(mode, str) else int(mode)

@@ -18,7 +18,7 @@ from ._lib import BeepError, ServerOpts, ServerResult, lib, vp
 from .scene import Scene
 
 SOLVE_MODES = {"fp64": 0, "mixed": 1, "fp32": 2}
-FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1}
+FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
 
 
 class Server:

@@ -166,6 +166,10 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* surface);
  *   BEEP_FAR_DIRECT  exact O(N^2) sum,
  *   BEEP_FAR_FMM     Chebyshev-interpolation FMM of the given order (3..6),
  *                    leaf boxes holding about leaf_target patches,
+ *   BEEP_FAR_PW      plane-wave (Yukawa / Laplace) FMM of expansion order 9
+ *                    (9-node plane-wave quadrature, about 3 digits, the
+ *                    reference's <9,9,67>), 12 or 18 (18 nodes, about 6
+ *                    digits), leaf boxes as for BEEP_FAR_FMM,
  *   BEEP_FAR_AUTO    FMM above 20,000 patches, direct below.
  * Takes effect immediately (kernel tables are built by the first mat-vec or
  * precompute that needs them); default BEEP_FAR_AUTO, order 4, leaf_target 48.
@@ -175,6 +179,7 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* surface);
 #define BEEP_FAR_AUTO (-1)
 #define BEEP_FAR_DIRECT 0
 #define BEEP_FAR_FMM 1
+#define BEEP_FAR_PW 2
 int beep_hip_set_far_field(beep_hip_ctx* ctx, int mode, int order, int leaf_target);
 
 /* Far field on a second HIP stream, concurrent with the near-field SpMV
@@ -303,6 +308,17 @@ int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const d
  * K_t[psi(m)][psi(n)] = K_canonical[m][n]; -1 on bad input (message from
  * beep_hip_last_error(NULL)). */
 int beep_hip_fmm_symmetry(int tx, int ty, int tz, int order, int* canon_xyz, int* node_map);
+
+/* Diagnostic (host only): the plane-wave tables of expansion order `order`
+ * with the `nodes`-node quadrature (9 or 18) at scaled frequency beta (box
+ * units), against direct sums of seeded sources and targets.  err5 receives
+ * errors relative to the largest value of each quantity: [0] the kernel's
+ * plane-wave quadrature over 1 <= z <= 4, [1] P2M -> M2X -> X2X -> X2L ->
+ * L2P potential over the six directions, [2] the same through M2M and L2L
+ * one level down, [3] the gradient of [1], [4] the orthogonality defect of
+ * the direction rotations.  The tables play the role of the reference's
+ * Level_Dependent_FMM_Globals (fmm_globals.h:372-493). */
+int beep_hip_pw_selftest(int order, int nodes, double beta, double* err5);
 
 /* device memory helpers for callers that keep vectors resident */
 int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out);
