@@ -147,6 +147,10 @@ struct PwQuad {
     int nw = 0;                    // waves kept (sum of nhalf)
 };
 
+// psi_j(v), j < (p+1)^2, at frequency beta (box units), and the gradients
+// (grad may be null): the host evaluation of the basis above.
+void pw_basis(int p, double beta, const double v[3], double* psi, double (*grad)[3]);
+
 // The six directions of the plane-wave lists: 0 up (+z), 1 down (-z),
 // 2 north (+y), 3 south (-y), 4 east (+x), 5 west (-x).  rot[d] maps local to
 // global coordinates (columns: local x, y, z axes), a proper rotation.

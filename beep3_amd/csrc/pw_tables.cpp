@@ -211,6 +211,11 @@ std::vector<double> rotation_matrices(int p) {
 
 }  // namespace
 
+void pw_basis(int p, double beta, const double v[3], double* psi, double (*grad)[3]) {
+    if (p < 0 || p > PW_MAXP) throw std::runtime_error("plane-wave order out of range");
+    basis(p, beta, v, psi, grad);
+}
+
 void pw_direction_frames(double rot[6][3][3]) {
     // columns: local x, y, z axes (global components); rot[d][row][col]
     const double ax[6][3][3] = {

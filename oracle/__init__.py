@@ -35,7 +35,7 @@ CONVERT_ENERGY_TO_KJ_PER_MOL = AVOGADRO * ELEMENTARY_CHARGE**2 / (EPSILON0 * ANG
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
 
 
 def _load():
@@ -64,6 +64,13 @@ def lib():
         _lib.orc_energy_coeffs.argtypes = [g, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp]
         _lib.orc_gmres_dense.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_int, dp]
         _lib.orc_gmres_dense.restype = C.c_int
+        _lib.cref_create.argtypes = [g, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        _lib.cref_create.restype = C.c_void_p
+        _lib.cref_matvec.argtypes = [C.c_void_p, dp, dp]
+        _lib.cref_matvec.restype = C.c_double
+        _lib.cref_gmres.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int, dp]
+        _lib.cref_gmres.restype = C.c_int
+        _lib.cref_destroy.argtypes = [C.c_void_p]
     return _lib
 
 
@@ -192,6 +199,51 @@ class Oracle:
         hist = np.zeros(max_iter)
         its = lib().orc_gmres_dense(n, _dp(K), _dp(b), _dp(x), tol, max_iter, _dp(hist))
         return x, its, hist[: its - 1]
+
+
+class CpuReference:
+    """The reference's CPU mat-vec path restated in C++/OpenMP
+    (oracle/cpu_ref.cpp): FP32 24-byte LocalIntegrations built once and
+    applied by scatter (beep.cpp:768-956, local_integrations.h:123-130), an
+    FP64 plane-wave FMM of order 9 with the 9-node quadrature
+    (fmm_bem_hybrid.h:184-642, the reference's <9,9,67>), and the reference
+    GMRES (beep.cpp:1262-1483).  Timed as the CPU baseline."""
+
+    def __init__(self, orc: "Oracle", order=9, nodes=9, leaf_target=40, stride=1):
+        """stride > 1: coefficients of every stride-th near-list leaf only
+        (a timing sample; the other records keep their indices, zero values)."""
+        self.orc = orc
+        self.n = orc.n
+        info = np.zeros(9)
+        self._h = lib().cref_create(C.byref(orc.g), orc.kappa, int(order), int(nodes), int(leaf_target),
+                                    int(stride), _dp(info))
+        self.info = {"setup_ms": info[0], "near_ms": info[1], "records": int(info[2]),
+                     "residual_pairs": int(info[3]), "leaf_level": int(info[4]), "boxes": int(info[5]),
+                     "tables_ms": info[6], "threads": int(info[7]), "records_computed": int(info[8])}
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(2 * self.n)
+        ms = lib().cref_matvec(self._h, _dp(x), _dp(y))
+        return y, ms
+
+    def gmres(self, b, tol=1e-6, max_iter=100, x0=None):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(2 * self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        ms = C.c_double()
+        mv = lib().cref_gmres(self._h, _dp(b), _dp(x), float(tol), int(max_iter), C.byref(ms))
+        return x, mv, ms.value
+
+    def close(self):
+        if self._h:
+            lib().cref_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def born_energy(q, a, eps_in, eps_out, kappa=0.0):

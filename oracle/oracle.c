@@ -32,20 +32,9 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define ONE_OVER_4PI 0.07957747154594767
+#include "oracle.h"
 
-typedef struct {
-    int n;
-    const double *centroid, *normal, *area, *eps, *gc;
-    const int* qual_ptr;
-    const double* qual; /* 7 per point: x y z nx ny nz w */
-    const int* quad_ptr;
-    const double* quad;
-    const int* gal_ptr;
-    const double* gal;
-    int n_leaves;
-    const int *leaf_ptr, *leaf_idx, *nb_ptr, *nb_idx;
-} orc_geom;
+#define ONE_OVER_4PI 0.07957747154594767
 
 /* Integrand accumulated over qualocation points (src, with weights w_q) and
  * quadrature points (targ).  Follows bem_kernels.h:161-248 term by term. */
