@@ -77,6 +77,7 @@ class SolveOpts(C.Structure):
     _fields_ = [
         ("tol", C.c_double), ("max_iter", C.c_int), ("mode", C.c_int),
         ("max_refine", C.c_int), ("inner_tol", C.c_double), ("verbose", C.c_int),
+        ("resid_fp64", C.c_int),
     ]
 
 
@@ -121,7 +122,7 @@ class ServerResult(C.Structure):
         ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
         ("host_ms", C.c_double), ("device_ms", C.c_double), ("upload_ms", C.c_double),
         ("precompute_ms", C.c_double), ("solve_ms", C.c_double), ("bytes", C.c_double),
-        ("slot", C.c_int), ("message", C.c_char * 128),
+        ("device_free", C.c_double), ("slot", C.c_int), ("message", C.c_char * 128),
     ]
 
 
@@ -181,6 +182,13 @@ def _declare_hip(L):
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.beep_hip_set_overlap.argtypes = [vp, C.c_int]
     L.beep_hip_set_graphs.argtypes = [vp, C.c_int]
+    L.beep_hip_set_near_singular_fp64.argtypes = [vp, C.c_int]
+    L.beep_hip_set_guard.argtypes = [vp, C.c_double]
+    L.beep_hip_set_gram_schmidt.argtypes = [vp, C.c_int]
+    L.beep_hip_guard_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.beep_hip_sanity_check.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.beep_hip_debug_poke.argtypes = [vp, C.c_int, C.c_longlong, dp]
+    L.beep_hip_near_singular_count.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.beep_hip_autotune_leaf.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
     L.beep_hip_autotune_spmv.argtypes = [vp, C.c_int, ip, C.c_int, C.c_int, dp, ip]
     L.beep_hip_set_spmv_residency.argtypes = [vp, C.c_int]
@@ -198,6 +206,7 @@ def _declare_hip(L):
     L.beep_hip_solve_device.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.beep_hip_rhs_device.argtypes = [vp, C.c_int, dp, C.c_double, vp]
     L.beep_hip_selftest_mfma.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.beep_hip_selftest_exp.argtypes = [vp, C.c_int, dp, dp]
     L.beep_hip_fmm_symmetry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip]
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.beep_hip_device_free.argtypes = [vp, vp]
@@ -205,6 +214,7 @@ def _declare_hip(L):
     L.beep_hip_memory_held.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.beep_hip_synchronize.argtypes = [vp]
     L.beep_hip_memory_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    L.beep_hip_device_memory.argtypes = [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
 
 
 def check_host(rc):

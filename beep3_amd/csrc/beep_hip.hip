@@ -106,6 +106,21 @@ struct beep_hip_ctx {
     DevBuf<float4> coef32;
     DevBuf<double2> coef64;
     bool have32 = false, have64 = false, have_diag = false;
+    // near-singular entries of the FP32 set, computed in FP64 (on by
+    // default): the list the FP32 kernel fills and its device count
+    bool ns_fp64 = true;
+    DevBuf<NsRec> ns_rec;
+    DevBuf<unsigned> ns_count;
+    bool ns_listed = false;  // the current FP32 set was built with the FP64 pass
+    unsigned ns_cap = 0;     // the list capacity it was built with
+    // coefficient guard (|v| > guard_max or not finite): the list of bad
+    // entries and counters [0] list length, entries repaired by the last
+    // precompute of [1] the FP64 set, [2] the FP32 set, [3] the diagonal,
+    // [4] by the last sanity check
+    static constexpr unsigned GUARD_CAP = 1u << 16;
+    double guard_max = 1e6;
+    DevBuf<NsRec> guard_rec;
+    DevBuf<unsigned> guard_cnt;
     // far-field plan each coefficient set was built for (0: none, the direct
     // far field): pairs of the FMM leaf pass keep the full rule
     unsigned long long uid32 = 0, uid64 = 0;
@@ -132,6 +147,8 @@ struct beep_hip_ctx {
     DevBuf<double> V, hvec, partial, wtmp, bvec, rvec, dvec;
     DevBuf<double> xin, wvec;   // fixed mat-vec input / output of an Arnoldi step
     DevBuf<int> jstep;          // Arnoldi step index on the device
+    int gs_passes = 1;          // Gram-Schmidt passes per Arnoldi step (1: CGS, 2: CGS2)
+    DevBuf<double> hvec2;       // the second pass's coefficients
     double* hh_pinned = nullptr;  // step results read by the host (pinned)
     size_t hh_cap = 0;
     int part_blocks = 0;
@@ -385,7 +402,7 @@ void arnoldi_launches(beep_hip_ctx* c, int precision, int m) {
     launch_copy_col(s, n, c->V.p, c->jstep.p, c->xin.p);
     matvec_dev(c, c->xin.p, c->wvec.p, precision);
     launch_arnoldi_step(s, n, m, c->part_blocks, c->V.p, c->jstep.p, c->partial.p, c->hvec.p,
-                        c->wvec.p);
+                        c->wvec.p, c->gs_passes == 2 ? c->hvec2.p : nullptr);
     hip_check(hipMemcpyAsync(c->hh_pinned, c->hvec.p, sizeof(double) * (m + 2),
                              hipMemcpyDeviceToHost, s), "D2H arnoldi");
     launch_inc_int(s, c->jstep.p);
@@ -417,7 +434,8 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         reinterpret_cast<const void*>(intptr_t(m)), reinterpret_cast<const void*>(intptr_t(c->N)),
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
         c->yfar.p, c->src.p, c->far_part.p, c->coef32.p, c->coef64.p, c->diag.p,
-        c->spmv_part.p, c->fmm,
+        c->spmv_part.p, c->fmm, c->hvec2.p,
+        reinterpret_cast<const void*>(intptr_t(c->gs_passes)),
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
         reinterpret_cast<const void*>(intptr_t(c->spmv_cap[spmv_mode(precision)])),
         reinterpret_cast<const void*>(intptr_t(c->spmv_depth[spmv_mode(precision)]))};
@@ -471,6 +489,7 @@ int gmres(beep_hip_ctx* c, int precision, const double* d_b, double* d_x, double
     hipStream_t s = c->stream;
     c->V.alloc(size_t(n) * (m + 1));
     c->hvec.alloc(size_t(m + 2));
+    c->hvec2.alloc(size_t(m + 2));
     c->partial.alloc(size_t(m + 2) * c->part_blocks);
     c->wtmp.alloc(size_t(m + 1));
     c->xin.alloc(size_t(n));
@@ -834,8 +853,16 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
     hipStream_t s = ctx->stream;
     ctx->kappa = kappa;
     ctx->diag.alloc(size_t(4) * ctx->N);
-    launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p, kappa,
-                    ctx->diag.p);
+    ctx->guard_rec.alloc(beep_hip_ctx::GUARD_CAP);
+    if (!ctx->guard_cnt.p) {
+        ctx->guard_cnt.alloc(5);
+        for (int k = 0; k < 5; ++k) launch_set_int(s, reinterpret_cast<int*>(ctx->guard_cnt.p + k), 0);
+    }
+    unsigned* gcnt = ctx->guard_cnt.p;
+    const NsList guard{ctx->guard_rec.p, gcnt, beep_hip_ctx::GUARD_CAP};
+    launch_set_int(s, reinterpret_cast<int*>(gcnt + 3), 0);
+    launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p, kappa, 0,
+                    ctx->guard_max, gcnt + 3, ctx->diag.p);
     ctx->have_diag = true;
     const unsigned* cellkey = nullptr;
     unsigned long long uid = 0;
@@ -856,10 +883,15 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
         require(need <= have || need + (size_t(1) << 30) < freeb + have,
                 "not enough device memory for FP64 coefficients");
         ctx->coef64.alloc(size_t(ctx->nnz) * 2);
+        launch_set_int(s, reinterpret_cast<int*>(gcnt + 1), 0);
         launch_near_coeffs_fp64(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
                                 ctx->leaf_idx.p, ctx->nb_ptr.p, ctx->nb_idx.p, ctx->coef_off.p,
                                 ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
-                                ctx->quad64.p, kappa, 1, cellkey, ctx->coef64.p);
+                                ctx->quad64.p, kappa, 1, cellkey, guard, ctx->guard_max,
+                                ctx->coef64.p);
+        launch_coef_repair(s, 0, guard, ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p,
+                           ctx->quad_ptr.p, ctx->quad64.p, kappa, ctx->guard_max, gcnt + 1,
+                           ctx->coef64.p);
         ctx->have64 = true;
         ctx->uid64 = uid;
     } else {
@@ -868,10 +900,28 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
         require(need <= have || need + (size_t(1) << 30) < freeb + have,
                 "not enough device memory for FP32 coefficients");
         ctx->coef32.alloc(size_t(ctx->nnz));
+        NsList ns{};
+        if (ctx->ns_fp64) {
+            // room for 24 near-singular partners per row (about 12 on a
+            // regular surface mesh); entries past it stay FP32
+            const size_t cap = std::min<size_t>(size_t(24) * ctx->N, size_t(ctx->nnz) + 1);
+            ctx->ns_rec.alloc(cap);
+            ctx->ns_count.alloc(1);
+            ns = NsList{ctx->ns_rec.p, ctx->ns_count.p, unsigned(cap)};
+            ctx->ns_cap = unsigned(cap);
+        }
+        launch_set_int(s, reinterpret_cast<int*>(gcnt + 2), 0);
         launch_near_coeffs_fp32(s, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
                                 ctx->leaf_idx.p, ctx->nb_ptr.p, ctx->nb_idx.p, ctx->coef_off.p,
                                 ctx->patches.p, ctx->qual_ptr.p, ctx->qual32.p, ctx->quad_ptr.p,
-                                ctx->quad32.p, float(kappa), 1, cellkey, ctx->coef32.p);
+                                ctx->quad32.p, float(kappa), 1, cellkey, ns, guard,
+                                ctx->guard_max, ctx->coef32.p);
+        launch_near_fix64(s, ns, ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p, ctx->quad_ptr.p,
+                          ctx->quad64.p, kappa, ctx->guard_max, gcnt + 2, ctx->coef32.p);
+        launch_coef_repair(s, 1, guard, ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p,
+                           ctx->quad_ptr.p, ctx->quad64.p, kappa, ctx->guard_max, gcnt + 2,
+                           ctx->coef32.p);
+        ctx->ns_listed = ctx->ns_fp64;
         ctx->have32 = true;
         ctx->uid32 = uid;
     }
@@ -921,6 +971,119 @@ int beep_hip_set_overlap(beep_hip_ctx* ctx, int on) {
     });
 }
 
+int beep_hip_set_near_singular_fp64(beep_hip_ctx* ctx, int on) {
+    return guarded(ctx, [&] { ctx->ns_fp64 = on != 0; });
+}
+
+int beep_hip_near_singular_count(beep_hip_ctx* ctx, long long* listed, long long* overflow) {
+    return guarded(ctx, [&] {
+        require(listed && overflow, "null output");
+        set_device(ctx);
+        *listed = *overflow = 0;
+        if (!ctx->have32 || !ctx->ns_listed) return;
+        unsigned cnt = 0;
+        hip_check(hipMemcpyAsync(&cnt, ctx->ns_count.p, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                 ctx->stream), "D2H near-singular count");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync near-singular count");
+        const unsigned cap = ctx->ns_cap;
+        *listed = std::min(cnt, cap);
+        *overflow = cnt > cap ? cnt - cap : 0;
+    });
+}
+
+int beep_hip_set_gram_schmidt(beep_hip_ctx* ctx, int passes) {
+    return guarded(ctx, [&] {
+        require(passes == 1 || passes == 2, "Gram-Schmidt passes must be 1 or 2");
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->gs_passes = passes;
+    });
+}
+
+int beep_hip_set_guard(beep_hip_ctx* ctx, double max_value) {
+    return guarded(ctx, [&] {
+        require(max_value > 0, "the guard bound must be positive");
+        ctx->guard_max = max_value;
+    });
+}
+
+int beep_hip_guard_stats(beep_hip_ctx* ctx, long long* repaired4) {
+    return guarded(ctx, [&] {
+        require(repaired4 != nullptr, "null output");
+        set_device(ctx);
+        for (int k = 0; k < 4; ++k) repaired4[k] = 0;
+        if (!ctx->guard_cnt.p) return;
+        unsigned h[5] = {0, 0, 0, 0, 0};
+        hip_check(hipMemcpyAsync(h, ctx->guard_cnt.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream),
+                  "D2H guard counters");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync guard counters");
+        for (int k = 0; k < 4; ++k) repaired4[k] = h[k + 1];
+    });
+}
+
+int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired) {
+    return guarded(ctx, [&] {
+        require(repaired != nullptr, "null output");
+        require(ctx->have_diag, "precompute before the sanity check");
+        set_device(ctx);
+        sync_streams(ctx);
+        hipStream_t s = ctx->stream;
+        unsigned* gcnt = ctx->guard_cnt.p;
+        const NsList guard{ctx->guard_rec.p, gcnt, beep_hip_ctx::GUARD_CAP};
+        launch_set_int(s, reinterpret_cast<int*>(gcnt + 4), 0);
+        launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p,
+                        ctx->kappa, 1, ctx->guard_max, gcnt + 4, ctx->diag.p);
+        for (int fp32 = 0; fp32 < 2; ++fp32) {
+            if (!(fp32 ? ctx->have32 : ctx->have64)) continue;
+            const unsigned long long built = fp32 ? ctx->uid32 : ctx->uid64;
+            require(ctx->use_fmm ? (ctx->fmm && ctx->fmm->uid == built) : built == 0,
+                    "near coefficients were precomputed for another far field: call "
+                    "beep_hip_precompute again");
+            const unsigned* cellkey = ctx->use_fmm ? ctx->fmm->cellkey : nullptr;
+            void* coef = fp32 ? static_cast<void*>(ctx->coef32.p) : static_cast<void*>(ctx->coef64.p);
+            launch_coef_scan(s, fp32, ctx->n_coef_blocks, ctx->L, ctx->coef_blk.p, ctx->leaf_ptr.p,
+                             ctx->leaf_idx.p, ctx->nb_ptr.p, ctx->nb_idx.p, ctx->coef_off.p, cellkey,
+                             1, coef, ctx->guard_max, guard);
+            launch_coef_repair(s, fp32, guard, ctx->patches.p, ctx->qual_ptr.p, ctx->qual64.p,
+                               ctx->quad_ptr.p, ctx->quad64.p, ctx->kappa, ctx->guard_max, gcnt + 4,
+                               coef);
+        }
+        unsigned h = 0;
+        hip_check(hipMemcpyAsync(&h, gcnt + 4, sizeof(h), hipMemcpyDeviceToHost, s), "D2H guard count");
+        hip_check(hipStreamSynchronize(s), "sync sanity check");
+        *repaired = h;
+        ctx->gen++;
+    });
+}
+
+int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const double* v4) {
+    return guarded(ctx, [&] {
+        require(v4 != nullptr, "null values");
+        set_device(ctx);
+        sync_streams(ctx);
+        hipStream_t s = ctx->stream;
+        if (which == 0) {
+            require(ctx->have64 && entry >= 0 && entry < ctx->nnz, "no such FP64 coefficient");
+            const double2 h[2] = {make_double2(v4[0], v4[1]), make_double2(v4[2], v4[3])};
+            hip_check(hipMemcpyAsync(ctx->coef64.p + 2 * entry, h, sizeof(h), hipMemcpyHostToDevice, s),
+                      "H2D poke");
+        } else if (which == 1) {
+            require(ctx->have32 && entry >= 0 && entry < ctx->nnz, "no such FP32 coefficient");
+            const float4 h = make_float4(float(v4[0]), float(v4[1]), float(v4[2]), float(v4[3]));
+            hip_check(hipMemcpyAsync(ctx->coef32.p + entry, &h, sizeof(h), hipMemcpyHostToDevice, s),
+                      "H2D poke");
+        } else if (which == 2) {
+            require(ctx->have_diag && entry >= 0 && entry < ctx->N, "no such patch");
+            hip_check(hipMemcpyAsync(ctx->diag.p + 4 * entry, v4, 4 * sizeof(double), hipMemcpyHostToDevice, s),
+                      "H2D poke");
+        } else {
+            throw std::runtime_error("which must be 0 (FP64 set), 1 (FP32 set) or 2 (diagonal)");
+        }
+        hip_check(hipStreamSynchronize(s), "sync poke");
+        ctx->gen++;
+    });
+}
+
 int beep_hip_set_graphs(beep_hip_ctx* ctx, int on) {
     return guarded(ctx, [&] {
         set_device(ctx);
@@ -965,7 +1128,8 @@ int beep_hip_autotune_leaf(beep_hip_ctx* ctx, int precision, const int* leaf_tar
         require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
         set_device(ctx);
         const int saved_mode = ctx->far_mode;
-        ctx->far_mode = BEEP_FAR_FMM;
+        // an FMM engine: the plane-wave one when selected, else Chebyshev
+        ctx->far_mode = saved_mode == BEEP_FAR_PW ? BEEP_FAR_PW : BEEP_FAR_FMM;
         const size_t n2 = size_t(2) * ctx->N;
         ctx->xbuf.alloc(n2);
         ctx->ybuf.alloc(n2);
@@ -1233,6 +1397,8 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
         stats.rel_residual = bnorm > 0 ? est / bnorm : 0;
     } else if (o->mode == BEEP_SOLVE_MIXED) {
         require(o->max_refine > 0, "bad refinement options");
+        const int resid_prec = o->resid_fp64 ? BEEP_FP64 : BEEP_FP32_RESID;
+        require(!o->resid_fp64 || ctx->have64, "an FP64 residual needs the FP64 precompute");
         // inner FP32 GMRES tolerance; <= 0: half the outer one (one
         // refinement step then usually suffices)
         const double inner_tol = o->inner_tol > 0 ? o->inner_tol : 0.5 * o->tol;
@@ -1244,7 +1410,7 @@ void solve_impl(beep_hip_ctx* ctx, const double* d_b, double* d_x, bool x_zero,
                 hip_check(hipMemcpyAsync(ctx->rvec.p, d_b, sizeof(double) * n,
                                          hipMemcpyDeviceToDevice, s), "D2D r");
             } else {
-                matvec_dev(ctx, d_x, ctx->rvec.p, BEEP_FP32_RESID);
+                matvec_dev(ctx, d_x, ctx->rvec.p, resid_prec);
                 ++matvecs;
                 launch_axpby(s, n, 1.0, d_b, -1.0, ctx->rvec.p);
             }
@@ -1374,6 +1540,33 @@ int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const d
         hip_check(hipMemcpyAsync(d, dd.p, 256 * sizeof(double), hipMemcpyDeviceToHost,
                                  ctx->stream), "D2H selftest");
         hip_check(hipStreamSynchronize(ctx->stream), "sync selftest");
+    });
+}
+
+int beep_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+    try {
+        if (!free_bytes || !total_bytes) throw std::runtime_error("null output");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+        return 0;
+    } catch (const std::exception& e) {
+        g_create_err = e.what();
+        return -1;
+    }
+}
+
+int beep_hip_selftest_exp(beep_hip_ctx* ctx, int n, const double* x, double* y) {
+    return guarded(ctx, [&] {
+        require(x && y && n >= 0, "bad arguments");
+        for (int i = 0; i < n; ++i) require(x[i] <= 0.0, "arguments must be <= 0");
+        set_device(ctx);
+        DevBuf<double> dx, dy;
+        dx.upload(x, size_t(n), ctx->stream);
+        dy.alloc(size_t(n));
+        launch_exp_selftest(ctx->stream, n, dx.p, dy.p);
+        if (n) hip_check(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost,
+                                        ctx->stream), "D2H exp selftest");
+        hip_check(hipStreamSynchronize(ctx->stream), "sync exp selftest");
     });
 }
 

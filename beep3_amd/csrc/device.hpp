@@ -58,6 +58,41 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
     return fma(y, fma(-h * y, y, 0.5), y);
 }
 
+// e^x for x <= 0 in FP64 (x < -708, below the normal range: 0).  Cody-Waite
+// argument reduction (W. J. Cody and W. Waite, Software Manual for the
+// Elementary Functions, 1980): x = k ln2 + r with k = rint(x / ln2), |r| <=
+// ln2 / 2, ln2 split into LN2_HI (its leading 32 significant bits, so k LN2_HI
+// is exact for |k| < 2^21) and LN2_LO = ln2 - LN2_HI.  Then e^r = 1 + s with
+// s = r + r^2 (1/2! + r/3! + ... + r^11/13!), the Taylor series to degree 13
+// (truncation below 4.2e-18 on |r| <= ln2/2) in Horner form, and 2^k put
+// into the exponent field.  About 20 FP64 operations, no branch.
+__device__ __forceinline__ double exp_nonpos(double x) {
+    constexpr double LOG2E = 1.4426950408889634;
+    constexpr double LN2_HI = 0x1.62e42fee00000p-1;
+    constexpr double LN2_LO = 0x1.a39ef35793c76p-33;
+    const double xc = x < -708.0 ? -708.0 : x;
+    const double k = __builtin_rint(xc * LOG2E);
+    double r = fma(-k, LN2_HI, xc);
+    r = fma(-k, LN2_LO, r);
+    // 1/n! for n = 13 .. 2
+    double q = 1.0 / 6227020800.0;
+    q = fma(q, r, 1.0 / 479001600.0);
+    q = fma(q, r, 1.0 / 39916800.0);
+    q = fma(q, r, 1.0 / 3628800.0);
+    q = fma(q, r, 1.0 / 362880.0);
+    q = fma(q, r, 1.0 / 40320.0);
+    q = fma(q, r, 1.0 / 5040.0);
+    q = fma(q, r, 1.0 / 720.0);
+    q = fma(q, r, 1.0 / 120.0);
+    q = fma(q, r, 1.0 / 24.0);
+    q = fma(q, r, 1.0 / 6.0);
+    q = fma(q, r, 0.5);
+    const double sv = fma(r * r, q, r);
+    const double p = 1.0 + sv;
+    const double two_k = __longlong_as_double((long long)(int(k) + 1023) << 52);
+    return x < -708.0 ? 0.0 : p * two_k;
+}
+
 // FMM leaf cell of a patch packed as x | y << 10 | z << 20 (leaf level
 // <= 10): two patches meet in the FMM leaf pass iff every field differs by
 // at most 1.
@@ -122,8 +157,25 @@ struct DevArena {
 };
 
 // ---- launchers (defined in the .hip files) ----
+// singular self terms; a bad one (|v| > maxv or not finite) is zeroed and
+// counted in *repaired; guard_only: check the stored terms and recompute only
+// the bad ones
 void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,
-                     const int* gal_ptr, const Pt64* gal, double kappa, double* diag4);
+                     const int* gal_ptr, const Pt64* gal, double kappa, int guard_only,
+                     double maxv, unsigned* repaired, double* diag4);
+
+// Near-singular entries of an FP32 coefficient set, listed by the FP32
+// coefficient kernel and computed in FP64 afterwards: global entry index,
+// row patch, 2 * column patch + (1 if the entry stores full - 1 point).
+struct NsRec {
+    int64_t idx;
+    int i, j2;
+};
+struct NsList {
+    NsRec* rec = nullptr;      // null: no FP64 pass (every entry in FP32)
+    unsigned* count = nullptr; // records listed (may exceed cap: the rest stay FP32)
+    unsigned cap = 0;
+};
 
 // near-field coefficients; coef points to float4 (FP32) or double2 pairs (FP64)
 void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
@@ -131,13 +183,30 @@ void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
                              const Pt32* quad, float kappa, int subtract_1pt,
-                             const unsigned* cellkey, float4* coef);
+                             const unsigned* cellkey, const NsList& ns, const NsList& guard,
+                             double maxv, float4* coef);
+// the FP64 pass over the near-singular entries the FP32 kernel listed (a bad
+// result is zeroed and counted in *repaired)
+void launch_near_fix64(hipStream_t s, const NsList& ns, const Patch64* patches, const int* qual_ptr,
+                       const Pt64* qual, const int* quad_ptr, const Pt64* quad, double kappa,
+                       double maxv, unsigned* repaired, float4* coef);
+// coefficient guard: list the entries of a stored set with a value |v| >
+// maxv or not finite (the coefficient kernels list theirs as they go), then
+// recompute the listed ones in FP64 (zero if still bad, or a self pair)
+void launch_coef_scan(hipStream_t s, int fp32, int n_blocks, int n_leaves, const int* blk_ptr,
+                      const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
+                      const int64_t* coef_off, const unsigned* cellkey, int subtract_1pt,
+                      const void* coef, double maxv, const NsList& out);
+void launch_coef_repair(hipStream_t s, int fp32, const NsList& list, const Patch64* patches,
+                        const int* qual_ptr, const Pt64* qual, const int* quad_ptr, const Pt64* quad,
+                        double kappa, double maxv, unsigned* repaired, void* coef);
 void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
                              const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr,
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
                              const Pt64* quad, double kappa, int subtract_1pt,
-                             const unsigned* cellkey, double2* coef);
+                             const unsigned* cellkey, const NsList& guard, double maxv,
+                             double2* coef);
 
 // Work decomposition of the near-field SpMV (built on the host from the
 // tree): work items (leaf, first row, first column, columns) of <= 256 rows x
@@ -192,8 +261,9 @@ void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, dou
 void launch_fill(hipStream_t s, int n, double v, double* y);
 // Arnoldi step (classical Gram-Schmidt against V[0..j], norm, V[j+1]) on
 // the mat-vec output w, with j read from *jp; h receives the j + 2 values.
+// h2 non-null: a second Gram-Schmidt pass (its coefficients in h2, added to h)
 void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const int* jp,
-                         double* partial, double* h, double* w);
+                         double* partial, double* h, double* w, double* h2);
 void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out);
 void launch_set_int(hipStream_t s, int* p, int v);
 void launch_inc_int(hipStream_t s, int* p);
@@ -212,6 +282,7 @@ void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tri
 void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
                      const Patch64* patches, Pt64* out64, Pt32* out32);
 void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out);
+void launch_exp_selftest(hipStream_t s, int n, const double* x, double* y);
 void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
                        double scale, double* y);
 

@@ -52,7 +52,8 @@ __device__ __forceinline__ double rsq_or_zero(double r2) {
     return r2 > 0.0 ? ir : 0.0;
 }
 __device__ __forceinline__ float exp_fast(float v) { return __expf(v); }
-__device__ __forceinline__ double exp_fast(double v) { return exp(v); }
+// v = -kappa r <= 0
+__device__ __forceinline__ double exp_fast(double v) { return exp_nonpos(v); }
 
 // 1-point interaction of one packed source (relative position d = source -
 // target, 1/eps ie, dipole a f n = (sdx, sdy, sdz), monopole a h = sh) with

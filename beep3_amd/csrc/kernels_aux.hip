@@ -385,12 +385,26 @@ __global__ void k_scale_store_j(int n, const double* __restrict__ w, const doubl
     V[size_t(j + 1) * n + k] = w[k] * sc;
 }
 
+// h[q] += h2[q], q <= j (the second Gram-Schmidt pass's coefficients)
+__global__ void k_add_j(const double* __restrict__ h2, const int* __restrict__ jp, double* __restrict__ h) {
+    const int q = threadIdx.x + blockIdx.x * blockDim.x;
+    if (q <= *jp) h[q] += h2[q];
+}
+
 void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const int* jp,
-                         double* partial, double* h, double* w) {
+                         double* partial, double* h, double* w, double* h2) {
     const dim3 rows((n + 255) / 256);
     hipLaunchKernelGGL(k_multi_dot_j, dim3(nb, m + 1), dim3(256), 0, s, n, V, jp, w, partial);
     hipLaunchKernelGGL(k_reduce_j, dim3(m + 1), dim3(256), 0, s, partial, nb, jp, 0, h);
     hipLaunchKernelGGL(k_gemv_sub_j, rows, dim3(256), 0, s, n, V, jp, h, w);
+    if (h2) {
+        // classical Gram-Schmidt twice ("CGS2": the second pass restores the
+        // orthogonality the first loses when w is nearly in span V)
+        hipLaunchKernelGGL(k_multi_dot_j, dim3(nb, m + 1), dim3(256), 0, s, n, V, jp, w, partial);
+        hipLaunchKernelGGL(k_reduce_j, dim3(m + 1), dim3(256), 0, s, partial, nb, jp, 0, h2);
+        hipLaunchKernelGGL(k_gemv_sub_j, rows, dim3(256), 0, s, n, V, jp, h2, w);
+        hipLaunchKernelGGL(k_add_j, dim3((m + 1 + 255) / 256), dim3(256), 0, s, h2, jp, h);
+    }
     hipLaunchKernelGGL(k_dot_self_j, dim3(nb), dim3(256), 0, s, n, w, jp, partial);
     hipLaunchKernelGGL(k_reduce_j, dim3(1), dim3(256), 0, s, partial, nb, jp, 1, h);
     hipLaunchKernelGGL(k_scale_store_j, rows, dim3(256), 0, s, n, w, h, jp, V);
@@ -552,6 +566,17 @@ __global__ __launch_bounds__(64) void k_mfma_selftest(const double* __restrict__
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i * 4 + k], b[k * 16 + i], acc, 0, 0, 0);
         for (int r = 0; r < 4; ++r) out[4 * l + r] = acc[r];
     }
+}
+
+__global__ void k_exp_selftest(int n, const double* __restrict__ x, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = exp_nonpos(x[i]);
+}
+
+void launch_exp_selftest(hipStream_t s, int n, const double* x, double* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_exp_selftest, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y);
+    hip_check(hipGetLastError(), "k_exp_selftest launch");
 }
 
 void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out) {

@@ -59,7 +59,8 @@ struct FarMath<double> {
         y = fma(y, fma(-h * y, y, 0.5), y);
         return fma(y, fma(-h * y, y, 0.5), y);
     }
-    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
+    // v = -kappa r <= 0
+    static __device__ __forceinline__ double exp_(double v) { return exp_nonpos(v); }
 };
 
 template <typename T>

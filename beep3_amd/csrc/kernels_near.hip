@@ -36,7 +36,8 @@ struct Math<double> {
     // hardware estimate refined by two Newton steps y <- y + y (1/2 - (v/2) y^2)
     static __device__ __forceinline__ double rsqrt_(double v) { return rsqrt_nr(v); }
     static __host__ __device__ __forceinline__ double ek_of(double kappa) { return -kappa; }
-    static __device__ __forceinline__ double expk(double v) { return exp(v); }
+    // v = -kappa r <= 0
+    static __device__ __forceinline__ double expk(double v) { return exp_nonpos(v); }
 };
 
 // Accumulate A..D of one (qual point, quad point) pair into the per-qual-point
@@ -74,18 +75,35 @@ __device__ __forceinline__ void integrand(T kappa, T ek, T inv_eps, T dx, T dy, 
 }
 
 // ---------------------------------------------------------------------------
+// Coefficient guard (BEEP::sanity_check, beep.cpp:1066-1122, with
+// LocalIntegrations::insane, local_integrations.h:111-121): a value is bad if
+// |v| > maxv, or not finite (the reference's comparison lets NaN through).
+__device__ __forceinline__ bool coef_bad(double v, double maxv) { return !(fabs(v) <= maxv); }
+__device__ __forceinline__ bool coef4_bad(const double v[4], double maxv) {
+    return coef_bad(v[0], maxv) || coef_bad(v[1], maxv) || coef_bad(v[2], maxv) || coef_bad(v[3], maxv);
+}
+
 // Singular self terms (singular_BEM_kernels, bem_kernels.h:18-132) in FP64
 // over the galerkin points of each patch, normalised by the summed weights,
 // plus the geometric correction (bem_kernels.cpp:123-136; beep.cpp:836-841).
-// One wave per patch; lane = source point.
+// One wave per patch; lane = source point.  A bad self term is zeroed (the
+// reference's rule for an ill-conditioned patch, beep.cpp:1097-1100) and
+// counted in *repaired.  guard_only: the wave first checks the stored terms
+// and recomputes only bad ones (the guard pass over an existing diagonal).
 template <bool KAPPA>
 __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restrict__ patches,
                                                  const double* __restrict__ gc,
                                                  const int* __restrict__ gal_ptr,
                                                  const Pt64* __restrict__ gal, double kappa,
+                                                 int guard_only, double maxv,
+                                                 unsigned* __restrict__ repaired,
                                                  double* __restrict__ diag4) {
     const int i = blockIdx.x;
     if (i >= n) return;
+    if (guard_only) {
+        const double v[4] = {diag4[4 * i], diag4[4 * i + 1], diag4[4 * i + 2], diag4[4 * i + 3]};
+        if (!coef4_bad(v, maxv)) return;  // the same for every lane of the wave
+    }
     const int p0 = gal_ptr[i], np = gal_ptr[i + 1] - p0;
     const double inv_eps = patches[i].inv_eps;
     double A = 0, B = 0, C = 0, D = 0, wsum = 0;
@@ -113,22 +131,28 @@ __global__ __launch_bounds__(64) void k_singular(int n, const Patch64* __restric
     if (threadIdx.x == 0) {
         const double e = 1.0 / inv_eps, g = gc[i];
         const double k4 = FOUR_PI_INV / wsum;
-        diag4[4 * i + 0] = A * k4;
-        diag4[4 * i + 1] = B * k4 + ((1.0 - g) + g / e);
-        diag4[4 * i + 2] = C * k4 - (g + (1.0 - g) / e);
-        diag4[4 * i + 3] = D * inv_eps * k4;
+        double v[4] = {A * k4, B * k4 + ((1.0 - g) + g / e), C * k4 - (g + (1.0 - g) / e),
+                       D * inv_eps * k4};
+        if (coef4_bad(v, maxv)) {
+            v[0] = v[1] = v[2] = v[3] = 0.0;
+            atomicAdd(repaired, 1u);
+        } else if (guard_only) {
+            atomicAdd(repaired, 1u);
+        }
+        for (int k = 0; k < 4; ++k) diag4[4 * i + k] = v[k];
     }
 }
 
 void launch_singular(hipStream_t s, int n, const Patch64* patches, const double* gc,
-                     const int* gal_ptr, const Pt64* gal, double kappa, double* diag4) {
+                     const int* gal_ptr, const Pt64* gal, double kappa, int guard_only,
+                     double maxv, unsigned* repaired, double* diag4) {
     if (n <= 0) return;
     if (kappa != 0.0)
         hipLaunchKernelGGL(k_singular<true>, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr, gal,
-                           kappa, diag4);
+                           kappa, guard_only, maxv, repaired, diag4);
     else
         hipLaunchKernelGGL(k_singular<false>, dim3(n), dim3(64), 0, s, n, patches, gc, gal_ptr,
-                           gal, kappa, diag4);
+                           gal, kappa, guard_only, maxv, repaired, diag4);
     hip_check(hipGetLastError(), "k_singular launch");
 }
 
@@ -143,6 +167,16 @@ void launch_singular(hipStream_t s, int n, const Patch64* patches, const double*
 // every pair, except for pairs whose packed FMM leaf cells (cellkey, when
 // given) are adjacent: the FMM leaf pass leaves those out, so they keep the
 // full rule.
+// Near-singular pairs: centroids closer than about two mesh edges,
+// |c_i - c_j|^2 < 2 (a_i + a_j) (the first ring of neighbours has
+// |d|^2 ~ 1.15 a, the second 3.5 a - 4.6 a).  There the kernel differences
+// G0 - Gk and their derivatives cancel (D's by a factor 6 / (kappa r)^2),
+// so the FP32 coefficient set computes these pairs in FP64 and rounds them.
+__device__ __forceinline__ bool near_singular(const Patch64& pi, const Patch64& pj) {
+    const double dx = pj.cx - pi.cx, dy = pj.cy - pi.cy, dz = pj.cz - pi.cz;
+    return dx * dx + dy * dy + dz * dz < 2.0 * (pi.area + pj.area);
+}
+
 __device__ __forceinline__ int find_leaf(const int* __restrict__ blk_ptr, int n_leaves, int b) {
     int lo = 0, hi = n_leaves;  // blk_ptr[lo] <= b < blk_ptr[hi]
     while (hi - lo > 1) {
@@ -202,7 +236,8 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
     const Patch64* __restrict__ patches, const int* __restrict__ qual_ptr,
     const Pt<T>* __restrict__ qual, const int* __restrict__ quad_ptr,
     const Pt<T>* __restrict__ quad, T kappa, int subtract_1pt,
-    const unsigned* __restrict__ cellkey, void* __restrict__ coef) {
+    const unsigned* __restrict__ cellkey, NsList ns, NsList guard, double maxv,
+    void* __restrict__ coef) {
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int L = find_leaf(blk_ptr, n_leaves, b);
     const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
@@ -222,7 +257,23 @@ __global__ __launch_bounds__(256) void k_near_coeffs(
         // pairs the FMM leaf pass covers keep the full rule (that pass then
         // skips them)
         const bool sub = subtract_1pt && !(cellkey && fmm_cells_adjacent(cellkey[i], cellkey[j]));
+        if (sizeof(T) == 4 && ns.rec && near_singular(pi, pj)) {
+            // FP64 pass (k_near_fix64) for this entry; past the list's
+            // capacity the entry stays in FP32
+            const unsigned slot = atomicAdd(ns.count, 1u);
+            if (slot < ns.cap) {
+                ns.rec[slot] = NsRec{coef_off[L] + e, i, 2 * j + (sub ? 1 : 0)};
+                return;
+            }
+        }
         pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, sub, out);
+        const double v[4] = {double(out[0]), double(out[1]), double(out[2]), double(out[3])};
+        if (coef4_bad(v, maxv)) {
+            // stored as zero; k_coef_repair recomputes it in FP64
+            const unsigned slot = atomicAdd(guard.count, 1u);
+            if (slot < guard.cap) guard.rec[slot] = NsRec{coef_off[L] + e, i, 2 * j + (sub ? 1 : 0)};
+            out[0] = out[1] = out[2] = out[3] = T(0);
+        }
     }
     const int64_t idx = coef_off[L] + e;
     if constexpr (sizeof(T) == 4) {
@@ -239,13 +290,160 @@ void launch_near_coeffs_fp32(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt32* qual, const int* quad_ptr,
                              const Pt32* quad, float kappa, int subtract_1pt,
-                             const unsigned* cellkey, float4* coef) {
+                             const unsigned* cellkey, const NsList& ns, const NsList& guard,
+                             double maxv, float4* coef) {
     if (n_blocks <= 0) return;
+    if (ns.rec) launch_set_int(s, reinterpret_cast<int*>(ns.count), 0);
+    launch_set_int(s, reinterpret_cast<int*>(guard.count), 0);
     auto k = kappa != 0.0f ? k_near_coeffs<float, true> : k_near_coeffs<float, false>;
     hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
                        nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
-                       subtract_1pt, cellkey, (void*)coef);
+                       subtract_1pt, cellkey, ns, guard, maxv, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<float> launch");
+}
+
+// The near-singular entries of an FP32 coefficient set, in FP64 on the FP64
+// point sets, rounded to FP32.  The record count is read on the device, so
+// the grid is fixed and walks the list with a stride.
+template <bool KAPPA>
+__global__ __launch_bounds__(256) void k_near_fix64(NsList ns, const Patch64* __restrict__ patches,
+                                                   const int* __restrict__ qual_ptr,
+                                                   const Pt64* __restrict__ qual,
+                                                   const int* __restrict__ quad_ptr,
+                                                   const Pt64* __restrict__ quad, double kappa,
+                                                   double maxv, unsigned* __restrict__ repaired,
+                                                   float4* __restrict__ coef) {
+    const unsigned n = min(*ns.count, ns.cap);
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const NsRec r = ns.rec[k];
+        const int i = r.i, j = r.j2 >> 1;
+        const Patch64 pi = patches[i], pj = patches[j];
+        const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+        const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
+        double out[4];
+        pair_coeffs<double, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, r.j2 & 1, out);
+        const float4 o = make_float4(float(out[0]), float(out[1]), float(out[2]), float(out[3]));
+        const double v[4] = {o.x, o.y, o.z, o.w};
+        if (coef4_bad(v, maxv)) {
+            coef[r.idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+            atomicAdd(repaired, 1u);
+        } else {
+            coef[r.idx] = o;
+        }
+    }
+}
+
+void launch_near_fix64(hipStream_t s, const NsList& ns, const Patch64* patches, const int* qual_ptr,
+                       const Pt64* qual, const int* quad_ptr, const Pt64* quad, double kappa,
+                       double maxv, unsigned* repaired, float4* coef) {
+    if (!ns.rec) return;
+    // enough blocks for every CU several times over; each walks the list
+    const unsigned blocks = std::max(1u, std::min(2048u, (ns.cap + 255) / 256));
+    auto k = kappa != 0.0 ? k_near_fix64<true> : k_near_fix64<false>;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, ns, patches, qual_ptr, qual, quad_ptr, quad,
+                       kappa, maxv, repaired, coef);
+    hip_check(hipGetLastError(), "k_near_fix64 launch");
+}
+
+// Guard pass over a stored coefficient set (the sanity check on demand):
+// every entry is read once and the bad ones are listed for k_coef_repair.
+// Indexing as k_near_coeffs (one thread per entry).
+template <typename TC>
+__global__ __launch_bounds__(256) void k_coef_scan(
+    int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
+    const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
+    const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,
+    const unsigned* __restrict__ cellkey, int subtract_1pt, const void* __restrict__ coef,
+    double maxv, NsList out) {
+    const int b = blockIdx.x;
+    const int L = find_leaf(blk_ptr, n_leaves, b);
+    const int lp = leaf_ptr[L], R = leaf_ptr[L + 1] - lp;
+    const int np = nb_ptr[L], Cn = nb_ptr[L + 1] - np;
+    const int64_t e = int64_t(b - blk_ptr[L]) * 256 + threadIdx.x;
+    if (e >= int64_t(R) * Cn) return;
+    const int64_t idx = coef_off[L] + e;
+    double v[4];
+    if constexpr (sizeof(TC) == 4) {
+        const float4 c = reinterpret_cast<const float4*>(coef)[idx];
+        v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w;
+    } else {
+        const double2* c2 = reinterpret_cast<const double2*>(coef);
+        const double2 a = c2[2 * idx], c = c2[2 * idx + 1];
+        v[0] = a.x; v[1] = a.y; v[2] = c.x; v[3] = c.y;
+    }
+    if (!coef4_bad(v, maxv)) return;
+    const int c = int(e / R);
+    const int r = int(e - int64_t(c) * R);
+    const int i = leaf_idx[lp + r], j = nb_idx[np + c];
+    const bool sub = subtract_1pt && !(cellkey && fmm_cells_adjacent(cellkey[i], cellkey[j]));
+    const unsigned slot = atomicAdd(out.count, 1u);
+    if (slot < out.cap) out.rec[slot] = NsRec{idx, i, 2 * j + (sub ? 1 : 0)};
+}
+
+// Recompute the listed entries in FP64 (a self pair, i == j, is zeroed, as
+// the reference does for an ill-conditioned patch); an entry still bad is
+// zeroed.  Each counts one in *repaired.
+template <typename TC, bool KAPPA>
+__global__ __launch_bounds__(256) void k_coef_repair(NsList list, const Patch64* __restrict__ patches,
+                                                     const int* __restrict__ qual_ptr,
+                                                     const Pt64* __restrict__ qual,
+                                                     const int* __restrict__ quad_ptr,
+                                                     const Pt64* __restrict__ quad, double kappa,
+                                                     double maxv, unsigned* __restrict__ repaired,
+                                                     void* __restrict__ coef) {
+    const unsigned n = min(*list.count, list.cap);
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const NsRec r = list.rec[k];
+        const int i = r.i, j = r.j2 >> 1;
+        double out[4] = {0, 0, 0, 0};
+        if (i != j) {
+            const Patch64 pi = patches[i], pj = patches[j];
+            const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+            const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
+            pair_coeffs<double, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, kappa, r.j2 & 1, out);
+            double v[4] = {out[0], out[1], out[2], out[3]};
+            if constexpr (sizeof(TC) == 4)
+                for (int q = 0; q < 4; ++q) v[q] = double(float(v[q]));
+            if (coef4_bad(v, maxv)) out[0] = out[1] = out[2] = out[3] = 0.0;
+        }
+        if constexpr (sizeof(TC) == 4) {
+            reinterpret_cast<float4*>(coef)[r.idx] =
+                make_float4(float(out[0]), float(out[1]), float(out[2]), float(out[3]));
+        } else {
+            double2* c2 = reinterpret_cast<double2*>(coef);
+            c2[2 * r.idx] = make_double2(out[0], out[1]);
+            c2[2 * r.idx + 1] = make_double2(out[2], out[3]);
+        }
+        atomicAdd(repaired, 1u);
+    }
+}
+
+void launch_coef_scan(hipStream_t s, int fp32, int n_blocks, int n_leaves, const int* blk_ptr,
+                      const int* leaf_ptr, const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
+                      const int64_t* coef_off, const unsigned* cellkey, int subtract_1pt,
+                      const void* coef, double maxv, const NsList& out) {
+    launch_set_int(s, reinterpret_cast<int*>(out.count), 0);
+    if (n_blocks <= 0) return;
+    auto k = fp32 ? k_coef_scan<float> : k_coef_scan<double>;
+    hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
+                       nb_ptr, nb_idx, coef_off, cellkey, subtract_1pt, coef, maxv, out);
+    hip_check(hipGetLastError(), "k_coef_scan launch");
+}
+
+void launch_coef_repair(hipStream_t s, int fp32, const NsList& list, const Patch64* patches,
+                        const int* qual_ptr, const Pt64* qual, const int* quad_ptr, const Pt64* quad,
+                        double kappa, double maxv, unsigned* repaired, void* coef) {
+    const unsigned blocks = std::max(1u, std::min(256u, (list.cap + 255) / 256));
+    if (fp32) {
+        auto k = kappa != 0.0 ? k_coef_repair<float, true> : k_coef_repair<float, false>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, list, patches, qual_ptr, qual, quad_ptr,
+                           quad, kappa, maxv, repaired, coef);
+    } else {
+        auto k = kappa != 0.0 ? k_coef_repair<double, true> : k_coef_repair<double, false>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, s, list, patches, qual_ptr, qual, quad_ptr,
+                           quad, kappa, maxv, repaired, coef);
+    }
+    hip_check(hipGetLastError(), "k_coef_repair launch");
 }
 
 void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const int* blk_ptr,
@@ -253,12 +451,14 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
                              const int* nb_idx, const int64_t* coef_off, const Patch64* patches,
                              const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
                              const Pt64* quad, double kappa, int subtract_1pt,
-                             const unsigned* cellkey, double2* coef) {
+                             const unsigned* cellkey, const NsList& guard, double maxv,
+                             double2* coef) {
     if (n_blocks <= 0) return;
+    launch_set_int(s, reinterpret_cast<int*>(guard.count), 0);
     auto k = kappa != 0.0 ? k_near_coeffs<double, true> : k_near_coeffs<double, false>;
     hipLaunchKernelGGL(k, dim3(n_blocks), dim3(256), 0, s, n_leaves, blk_ptr, leaf_ptr, leaf_idx,
                        nb_ptr, nb_idx, coef_off, patches, qual_ptr, qual, quad_ptr, quad, kappa,
-                       subtract_1pt, cellkey, (void*)coef);
+                       subtract_1pt, cellkey, NsList{}, guard, maxv, (void*)coef);
     hip_check(hipGetLastError(), "k_near_coeffs<double> launch");
 }
 

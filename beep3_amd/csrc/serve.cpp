@@ -120,6 +120,16 @@ struct beep_server {
     std::vector<std::thread> workers;
     std::vector<double> held;  // bytes charged to each slot
     double used = 0;
+    // device bytes each slot's context really holds (measured after its last
+    // request; 0 when released): with the device's free bytes, what this
+    // server's slots could hold together right now
+    std::vector<double> actual;
+    static constexpr double RESERVE = 1073741824.0;  // left free on the device
+    double device_total = 0;
+    // the bytes the slots may hold together: the measured free device memory
+    // plus what the slots hold, less a reserve, capped by the configured
+    // budget (0: no cap); *free_out receives the free bytes read
+    double budget_now(double* free_out) const;
     // raw energy coefficients of a mesh at its canonical placement, per mesh
     // stamp (surface and charges): Mesh::init_energy_precalcs runs once per
     // mesh, in the mesh's own frame (mesh.cpp:730-887).  An instance placed
@@ -152,7 +162,19 @@ void beep_server::release_slot(int slot, beep_hip_ctx*& ctx) {
     ctx = nullptr;
     used -= held[slot];
     held[slot] = 0;
+    actual[slot] = 0;
     cv_mem.notify_all();
+}
+
+double beep_server::budget_now(double* free_out) const {
+    size_t fr = 0, tot = 0;
+    if (beep_hip_device_memory(o.device, &fr, &tot) != 0)
+        throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
+    double ours = 0;
+    for (double a : actual) ours += a;
+    if (free_out) *free_out = double(fr);
+    const double real = double(fr) + ours - RESERVE;
+    return o.hbm_budget > 0 ? std::min(o.hbm_budget, real) : real;
 }
 
 // Raw energy coefficients of every instance of a built scene whose geometry
@@ -230,13 +252,20 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     const int N = int(sc->centroid.size());
     r.n_patches = N;
     // admission: a free slot (this thread) and room in the memory budget
+    // admission against the device's measured free memory (OpenCL_Handler
+    // compares a blob's device_mem_reqd with its own count of the device's
+    // memory, opencl_handler.cpp:451-463)
     const double need_total = estimate_bytes(sc, o);
     {
         std::unique_lock<std::mutex> lk(mu);
-        if (need_total > o.hbm_budget)
-            throw std::runtime_error("request needs more device memory than the budget");
+        double cap = o.hbm_budget > 0 ? o.hbm_budget : 1e300;
+        if (device_total > 0) cap = std::min(cap, device_total - RESERVE);
+        if (need_total > cap)
+            throw std::runtime_error("request needs more device memory than the budget and the "
+                                     "device allow");
+        double budget = budget_now(&r.device_free);
         double grow = std::max(0.0, need_total - held[slot]);
-        if (used + grow > o.hbm_budget && ctx) {
+        if (used + grow > budget && ctx) {
             // give back what this slot holds before waiting, so that waiting
             // slots never hold memory the others need
             release_slot(slot, ctx);
@@ -244,7 +273,12 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
         }
         ++waiting_for_memory;
         cv_job.notify_all();  // idle slots release their contexts under pressure
-        cv_mem.wait(lk, [&] { return used + grow <= o.hbm_budget || used == 0; });
+        // woken by every release and reconciliation; a timed wait also
+        // notices memory freed by other processes
+        while (!(used + grow <= budget || used == 0)) {
+            cv_mem.wait_for(lk, std::chrono::milliseconds(20));
+            budget = budget_now(&r.device_free);
+        }
         --waiting_for_memory;
         used += grow;
         held[slot] += grow;
@@ -275,7 +309,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     r.precompute_ms = (ts - tp) * 1e3;
     std::vector<double> x(size_t(2) * N, 0.0);
     const int nch = int(sc->charge_inst.size());
-    beep_solve_opts so{o.tol, o.max_iter, o.solve_mode, o.max_refine, o.inner_tol, 0};
+    beep_solve_opts so{o.tol, o.max_iter, o.solve_mode, o.max_refine, o.inner_tol, 0, 0};
     beep_solve_stats st{};
     check_hip(ctx, beep_hip_solve_from_charges(ctx, nch, nch ? sc->charges.data() : nullptr,
                                                sc->Dsolvent, &so, x.data(), &st), "solve");
@@ -322,6 +356,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
         std::lock_guard<std::mutex> lk(mu);
         used += double(hb) - held[slot];
         held[slot] = double(hb);
+        actual[slot] = double(hb);
         cv_mem.notify_all();
     }
 }
@@ -376,7 +411,7 @@ void beep_server_default_opts(beep_server_opts* o) {
     *o = beep_server_opts{};
     o->device = 0;
     o->slots = 4;
-    o->hbm_budget = 64.0 * 1024 * 1024 * 1024;
+    o->hbm_budget = 0.0;  // the device's free memory
     o->solve_mode = BEEP_SOLVE_MIXED;
     o->tol = 1e-6;
     o->max_iter = 100;
@@ -395,9 +430,14 @@ int beep_server_create(const beep_server_opts* opts, beep_server** out) {
         if (opts) s->o = *opts;
         else beep_server_default_opts(&s->o);
         if (s->o.slots < 1 || s->o.slots > 64) throw std::runtime_error("slots must be 1..64");
-        if (!(s->o.hbm_budget > 0)) throw std::runtime_error("budget must be positive");
+        if (!(s->o.hbm_budget >= 0)) throw std::runtime_error("budget must be >= 0");
+        // the device's size (0 when it cannot be read: requests then fail
+        // at admission with the error)
+        size_t fr = 0, tot = 0;
+        if (beep_hip_device_memory(s->o.device, &fr, &tot) == 0) s->device_total = double(tot);
         s->t0 = std::chrono::steady_clock::now();
         s->held.assign(size_t(s->o.slots), 0.0);
+        s->actual.assign(size_t(s->o.slots), 0.0);
         for (int k = 0; k < s->o.slots; ++k) s->workers.emplace_back([p = s.get(), k] { p->worker(k); });
         *out = s.release();
         return 0;

@@ -98,6 +98,46 @@ class Device:
         by kernel."""
         self._check(lib().beep_hip_set_graphs(self._h, 1 if on else 0))
 
+    def set_near_singular_fp64(self, on: bool = True):
+        """FP64 integration of the near-singular pairs of the FP32 set
+        (default), from the next FP32 precompute."""
+        self._check(lib().beep_hip_set_near_singular_fp64(self._h, 1 if on else 0))
+
+    def set_gram_schmidt(self, passes: int = 1):
+        """Gram-Schmidt passes per Arnoldi step: 1 (classical, the
+        reference's) or 2 (CGS2)."""
+        self._check(lib().beep_hip_set_gram_schmidt(self._h, int(passes)))
+
+    def set_guard(self, max_value: float = 1e6):
+        self._check(lib().beep_hip_set_guard(self._h, float(max_value)))
+
+    def guard_stats(self) -> dict:
+        """Entries repaired by the last precompute of each set and by the
+        last sanity check."""
+        a = (C.c_longlong * 4)()
+        self._check(lib().beep_hip_guard_stats(self._h, a))
+        return {"fp64": a[0], "fp32": a[1], "diag": a[2], "sanity_check": a[3]}
+
+    def sanity_check(self) -> int:
+        """BEEP::sanity_check: repair stored coefficients that are out of
+        range or not finite; returns how many were repaired."""
+        n = C.c_longlong()
+        self._check(lib().beep_hip_sanity_check(self._h, C.byref(n)))
+        return n.value
+
+    def debug_poke(self, which: str, entry: int, values):
+        """Test hook: overwrite one stored entry ('fp64', 'fp32' or 'diag')."""
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(4)
+        w = {"fp64": 0, "fp32": 1, "diag": 2}[which]
+        self._check(lib().beep_hip_debug_poke(self._h, w, int(entry), _dp(v)))
+
+    def near_singular_count(self):
+        """(entries computed in FP64, entries left in FP32 for lack of room)
+        of the current FP32 set."""
+        a, b = C.c_longlong(), C.c_longlong()
+        self._check(lib().beep_hip_near_singular_count(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def autotune_leaf(self, precision="fp64", candidates=(24, 32, 40, 64, 96, 160), reps=3):
         """Time mat-vecs for each FMM leaf target, keep the fastest; returns
         (best, {leaf: ms per mat-vec})."""
@@ -180,13 +220,14 @@ class Device:
         return ef, eh
 
     def solve(self, b, x0=None, tol=1e-6, max_iter=100, mode="fp64", max_refine=10,
-              inner_tol=0.0, verbose=False):
+              inner_tol=0.0, verbose=False, resid_fp64=False):
         """inner_tol: tolerance of the inner FP32 solves of the mixed mode
-        (0: half of tol)."""
+        (0: half of tol); resid_fp64: the mixed mode's residual with the FP64
+        coefficient set (which must be precomputed) instead of the FP32 one."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
         o = SolveOpts(tol, max_iter, SOLVE_MODES[mode] if isinstance(mode, str) else int(mode),
-                      max_refine, inner_tol, int(verbose))
+                      max_refine, inner_tol, int(verbose), int(resid_fp64))
         st = SolveStats()
         self._check(lib().beep_hip_solve(self._h, _dp(b), _dp(x), C.byref(o), C.byref(st)))
         return x, {f: getattr(st, f) for f, _ in SolveStats._fields_}
@@ -198,10 +239,10 @@ class Device:
                                               vp(d_b)))
 
     def solve_device(self, d_b: int, d_x: int, tol=1e-6, max_iter=100, mode="fp64",
-                     max_refine=10, inner_tol=0.0, verbose=False):
+                     max_refine=10, inner_tol=0.0, verbose=False, resid_fp64=False):
         """solve on device vectors (d_x: initial guess in, solution out)."""
         o = SolveOpts(tol, max_iter, SOLVE_MODES[mode] if isinstance(mode, str) else int(mode),
-                      max_refine, inner_tol, int(verbose))
+                      max_refine, inner_tol, int(verbose), int(resid_fp64))
         st = SolveStats()
         self._check(lib().beep_hip_solve_device(self._h, vp(d_b), vp(d_x), C.byref(o),
                                                 C.byref(st)))

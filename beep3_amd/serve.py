@@ -1,6 +1,6 @@
 """Solve server with continuous batching (Python view of include/beep_serve.h).
 
-    srv = Server(slots=8, budget_gib=64)
+    srv = Server(slots=8)               # budget_gib=0: the device's free memory
     rid = srv.submit(scene)          # a Scene with its instances, not built
     r = srv.poll(timeout=1.0)        # next finished request (dict) or None
     srv.close()
@@ -22,7 +22,7 @@ FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
 
 
 class Server:
-    def __init__(self, slots=4, budget_gib=64.0, mode="mixed", tol=1e-6, max_iter=100,
+    def __init__(self, slots=4, budget_gib=0.0, mode="mixed", tol=1e-6, max_iter=100,
                  max_refine=4, inner_tol=0.0, far_field="auto", fmm_order=5, fmm_leaf=40,
                  keep_fh=False, device=0):
         o = ServerOpts()

@@ -97,32 +97,60 @@ def near_bytes(a, coef_bytes):
 MEASURED_TRAFFIC = {("globular", "fp64"): 8.236e9}
 
 
-def cpu_baseline(b, seconds=15.0):
-    """Oracle (oracle/oracle.c, FP64, OpenMP) on a sample of rows of the same
-    workload; rate scaled to whole mat-vecs."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(b, seconds=15.0, solve_matvecs=None):
+    """The reference's CPU path restated in C++/OpenMP (oracle/cpu_ref.cpp)
+    on the host's cores: FP32 24-byte LocalIntegrations applied by scatter
+    with FP64 sums (beep.cpp:768-956, local_integrations.h:123-130) and the
+    FP64 plane-wave FMM of order 9 with the 9-node quadrature
+    (fmm_bem_hybrid.h:184-642, the reference's <9,9,67>).  The value is whole
+    mat-vecs per second (every record is walked).  The precompute is sampled
+    (every 16th near-list leaf) and scaled to all records; the solve time is
+    the tree + precompute + the mat-vecs the GPU's FP64 solve needed."""
     import oracle
     a = b.host_scene().arrays()
     o = oracle.Oracle(a, b.kappa, b.Dsolvent)
     N = o.n
-    rng = np.random.default_rng(5)
-    x = rng.normal(size=2 * N)
-    o.diag()
-    rows_done, t_used, k = 0, 0.0, 16
-    perm = rng.permutation(N)
-    while t_used < seconds and rows_done < N:
-        rows = perm[rows_done:rows_done + k].astype(np.int32)
-        t0 = time.perf_counter()
-        o.matvec(x, rows=rows)
-        dt = time.perf_counter() - t0
-        t_used += dt
-        rows_done += len(rows)
-        k = min(N - rows_done, max(16, int(len(rows) * min(4.0, (seconds - t_used) / max(dt, 1e-3)))))
-        if k <= 0:
-            break
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": rows_done / t_used / N, "unit": "mat-vecs/s", "cores": cores, "kind": "port",
-            "sample": f"{rows_done} of {N} rows of one FP64 mat-vec (near quadrature + exact "
-                      f"1-point far sum, diagonal), {t_used:.1f} s"}
+    stride = 16
+    t0 = time.perf_counter()
+    c = oracle.CpuReference(o, 9, 9, 40, stride=stride)
+    create_s = time.perf_counter() - t0
+    info = c.info
+    x = np.random.default_rng(5).normal(size=2 * N)
+    times = []
+    t_used = 0.0
+    while len(times) < 3 or (t_used < seconds and len(times) < 20):
+        _, ms = c.matvec(x)
+        times.append(ms)
+        t_used += ms * 1e-3
+    c.close()
+    ms = float(np.median(times))
+    scale = info["records"] / max(info["records_computed"], 1)
+    precompute_ms = info["near_ms"] * scale
+    setup_ms = info["setup_ms"] - info["near_ms"] + precompute_ms
+    out = {"value": 1e3 / ms, "unit": "mat-vecs/s", "cores": int(info["threads"]),
+           "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "sample": (f"{len(times)} whole FP64 mat-vecs at {N} patches (median {ms:.1f} ms); "
+                      f"precompute of {info['records_computed']} of {info['records']} 24-byte "
+                      f"records ({info['near_ms']:.0f} ms, x{scale:.1f} to all records); "
+                      f"{create_s + t_used:.1f} s of CPU work"),
+           "matvec_ms": ms, "precompute_ms_scaled": precompute_ms,
+           "setup_ms_scaled": setup_ms, "records": info["records"],
+           "residual_pairs": info["residual_pairs"], "fmm": "plane-wave order 9, 9 nodes, leaf 40"}
+    if solve_matvecs:
+        out["solve_ms_estimated"] = setup_ms + solve_matvecs * ms
+        out["solve_matvecs"] = solve_matvecs
+    return out
 
 
 def serving_bench(seconds, slots, seq_requests, rate=50.0):
@@ -276,7 +304,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--far", default="fmm", choices=["fmm", "direct", "auto"])
+    ap.add_argument("--far", default="fmm", choices=["fmm", "pw", "direct", "auto"])
     ap.add_argument("--order", type=int, default=5)
     ap.add_argument("--leaf", type=int, default=40)
     ap.add_argument("--no-serve", action="store_true")
@@ -329,7 +357,7 @@ def main():
     dev.free(c_dst)
     copy_GBs = 2 * copy_bytes / min(copy_s) / 1e9
     tune = None
-    if args.far == "fmm" and not args.no_tune:
+    if args.far in ("fmm", "pw") and not args.no_tune:
         # leaf-size autotuner on the FP64 mat-vec at this order
         best, table = dev.autotune_leaf("fp64", TUNE_LEAVES, reps=3)
         tune = {"leaf_ms_fp64": table, "chosen_leaf": best}
@@ -485,8 +513,12 @@ def main():
         fh = {}
         # mixed: inner FP32 GMRES to half the outer tolerance (default), and
         # the looser inner tolerances of a classic refinement
+        # mixed_resid64: the refinement residual with the FP64 coefficient set
+        # (needs both precomputes; the default residual uses the FP32 set with
+        # FP64 sums and far field, as the reference's own FP32 LocalIntegrations)
         for mode, kw in (("fp64", {}), ("mixed", {}), ("mixed_inner1e-3", {"inner_tol": 1e-3}),
-                         ("mixed_inner1e-4", {"inner_tol": 1e-4})):
+                         ("mixed_inner1e-4", {"inner_tol": 1e-4}),
+                         ("mixed_resid64", {"resid_fp64": True})):
             rhs_ms, gm_ms, st, x = device_solve(mode.split("_")[0], **kw)
             b.f, b.h = x[:N].copy(), x[N:].copy()
             E = b.calculate_energies()
@@ -542,7 +574,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(b, args.cpu_seconds)
+        mv = extra.get("solve", {}).get("fp64", {}).get("matvecs")
+        cpu = cpu_baseline(b, args.cpu_seconds, mv)
         print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
 
     out = {
@@ -553,7 +586,9 @@ def main():
         "config": {"workload": workload, "patches": N, "quad_rule": 6, "qual_rule": 0,
                    "nbsize": b.nbsize, "kappa": b.kappa, "eps_in": 2.0, "eps_out": 80.0,
                    "far_field": (f"Chebyshev FMM order {args.order}, leaf target {args.leaf}"
-                                 if args.far == "fmm" else args.far)},
+                                 if args.far == "fmm" else
+                                 f"plane-wave FMM order {args.order}, leaf target {args.leaf}"
+                                 if args.far == "pw" else args.far)},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out))

@@ -119,6 +119,9 @@ typedef struct {
     int max_refine;    /* refinement steps (mixed mode) */
     double inner_tol;  /* relative tolerance of each inner solve (mixed mode); <= 0: tol / 2 */
     int verbose;
+    int resid_fp64;    /* mixed mode: 0 residual with the FP32 coefficient set (FP64 sums and
+                          far field, BEEP_FP32_RESID), 1 with the FP64 set (BEEP_FP64; needs its
+                          precompute) */
 } beep_solve_opts;
 
 typedef struct {
@@ -203,7 +206,8 @@ int beep_hip_set_graphs(beep_hip_ctx* ctx, int on);
  * patches per leaf box) at the current order, writes the mean time per
  * mat-vec of each to ms_out (n values, may be NULL), keeps the fastest and
  * returns it in *best.  Needs the geometry, tree and coefficients of that
- * precision; switches the far field to BEEP_FAR_FMM.  Each candidate plan
+ * precision; switches the far field to BEEP_FAR_FMM unless it is BEEP_FAR_PW
+ * (which then stays).  Each candidate plan
  * gets its own coefficients (precomputed per candidate), and every
  * coefficient set the context held is precomputed again for the chosen one.
  * No counterpart in the reference, whose leaf size follows from nbsize
@@ -255,8 +259,48 @@ int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out);
 /* Singular self terms (FP64 always) and near-field coefficients of the given
  * precision (BEEP_FP64 or BEEP_FP32; BEEP_FP32_RESID uses the FP32 set).
  * With the FMM far field, builds its plan first (the coefficients follow
- * it). */
+ * it).  The FP32 set computes its near-singular pairs (centroids closer than
+ * about two mesh edges, |c_i - c_j|^2 < 2 (a_i + a_j)) in FP64 and rounds
+ * them, unless switched off below.  Replaces precalc_neighbour_interactions
+ * (beep.cpp:768-956), whose coefficients are all FP32 (bem_kernels.h:134-251
+ * on float QuadPoints). */
 int beep_hip_precompute(beep_hip_ctx* ctx, double kappa, int precision);
+
+/* Gram-Schmidt passes of every Arnoldi step: 1 (default) is the reference's
+ * classical Gram-Schmidt, two dgemv calls (beep.cpp:1395-1396); 2
+ * orthogonalises twice (CGS2), for ill-conditioned systems whose Krylov
+ * basis loses orthogonality over many iterations. */
+int beep_hip_set_gram_schmidt(beep_hip_ctx* ctx, int passes);
+
+/* Coefficient guard, after BEEP::sanity_check (beep.cpp:1066-1122) and
+ * LocalIntegrations::insane (local_integrations.h:111-121): a coefficient is
+ * bad when |v| > max_value (default 1e6, the reference's) or not finite (the
+ * reference's comparison lets NaN through).  Every precompute checks what it
+ * computes: a bad pair is recomputed in FP64 and zeroed if still bad; a bad
+ * self term is zeroed (the reference's rule for an ill-conditioned patch). */
+int beep_hip_set_guard(beep_hip_ctx* ctx, double max_value);
+/* Entries repaired (recomputed or zeroed) by the last precompute of the FP64
+ * set [0], the FP32 set [1], the diagonal [2], and by the last
+ * beep_hip_sanity_check [3].  Waits for the device. */
+int beep_hip_guard_stats(beep_hip_ctx* ctx, long long* repaired4);
+/* The guard over the stored coefficients (every set held and the diagonal),
+ * repairing bad entries as above; *repaired receives their count.  The
+ * reference runs its check once, before the first mat-vec; here it is on
+ * demand, for memory that may have changed since the precompute. */
+int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired);
+/* Test hook: overwrite one stored entry (which 0: FP64 set, 1: FP32 set, by
+ * global coefficient index; 2: the diagonal of a patch) with v4 = A, B, C,
+ * D, so a test can force the guard's condition from the host. */
+int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const double* v4);
+
+/* FP64 integration of the near-singular pairs of the FP32 coefficient set
+ * (on = 1, the default) or FP32 for every pair (on = 0); applies from the
+ * next FP32 precompute.  No counterpart in the reference (all FP32). */
+int beep_hip_set_near_singular_fp64(beep_hip_ctx* ctx, int on);
+/* Near-singular entries of the current FP32 set computed in FP64 (listed),
+ * and the entries the list had no room for (computed in FP32): both 0 when
+ * the set was built with the FP64 pass off.  Waits for the precompute. */
+int beep_hip_near_singular_count(beep_hip_ctx* ctx, long long* listed, long long* overflow);
 
 /* y = K x on host arrays (do_bem_fmm_iteration); blocks until y is written. */
 int beep_hip_matvec(beep_hip_ctx* ctx, const double* x, double* y, int precision);
@@ -301,6 +345,11 @@ int beep_hip_solve_device(beep_hip_ctx* ctx, const double* d_b, double* d_x,
 int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
                            double* d);
 
+/* Diagnostic: y[i] = e^{x[i]} (every x[i] <= 0) by the FP64 exponential of
+ * the integrand and the far-field kernels (Cody-Waite reduction, degree-13
+ * Taylor polynomial), for comparison with a host exp. */
+int beep_hip_selftest_exp(beep_hip_ctx* ctx, int n, const double* x, double* y);
+
 /* Diagnostic (host only): the M2L symmetry used by the FMM.  For a source
  * box at offset t = (tx, ty, tz) (box units, 2 <= max |t_k| <= 3) returns
  * the index of its canonical offset (components in canon_xyz) and the node
@@ -330,6 +379,9 @@ int beep_hip_memory_info(beep_hip_ctx* ctx, size_t* free_bytes, size_t* total_by
 /* device bytes held by the context (its buffers only grow: a later geometry
  * of no larger size reuses them) */
 int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes);
+/* Free and total bytes of a device, without a context (the server's admission
+ * reads it; beep_hip_last_error(NULL) gives the error text). */
+int beep_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,9 @@ typedef struct beep_server beep_server;
 typedef struct {
     int device;
     int slots;              /* concurrent requests (worker threads, one context each) */
-    double hbm_budget;      /* bytes the slots' contexts may hold together */
+    double hbm_budget;      /* cap on the bytes the slots' contexts hold together (0: none);
+                               admission also needs the device's measured free memory plus
+                               what the slots hold, less 1 GiB, to cover the request */
     int solve_mode;         /* BEEP_SOLVE_FP64 / MIXED / FP32 (beep_hip.h) */
     double tol;             /* GMRES tolerance (relative to |b|) */
     int max_iter;
@@ -46,7 +48,7 @@ typedef struct {
     int keep_fh;            /* keep f and h of each request for beep_server_take_fh */
 } beep_server_opts;
 
-/* fills the defaults: device 0, 4 slots, 64 GiB, mixed, tol 1e-6, 100
+/* fills the defaults: device 0, 4 slots, no budget cap, mixed, tol 1e-6, 100
  * iterations, 4 refinements, inner tol / 2, far field auto, FMM order 5, leaf
  * 40, f and h not kept */
 void beep_server_default_opts(beep_server_opts* o);
@@ -68,6 +70,7 @@ typedef struct {
     double precompute_ms;   /*           singular terms and near coefficients */
     double solve_ms;        /*           RHS, GMRES, energy */
     double bytes;           /* device bytes held by the slot after the request */
+    double device_free;     /* free device bytes read at admission */
     int slot;
     char message[128];
 } beep_result;

@@ -48,6 +48,78 @@ def test_matvec_fp32(dev_born, prec, tol):
     assert e < tol
 
 
+def test_near_singular_pairs_in_fp64(born_scene):
+    """The FP32 coefficient set integrates its near-singular pairs (centroids
+    within about two mesh edges) in FP64 and rounds them.  Column j of the
+    FP32-coefficient / FP64-sum operator (the mat-vec of e_j) against the
+    oracle's column, on the rows near-singular to j: the kernel differences
+    G0 - Gk of A and D cancel there (D's by about 6 / (kappa r)^2), which
+    FP32 integration does not survive and FP64 does."""
+    sc, a, orc = born_scene
+    n = orc.n
+    c, ar = a["centroid"], a["area"]
+    cols = np.random.default_rng(13).choice(n, 8, replace=False)
+    out = {}
+    for on in (False, True):
+        d = Device(0)
+        d.set_near_singular_fp64(on)
+        d.set_geometry(a)
+        d.set_tree(a)
+        d.precompute(0.125, "fp32")
+        listed, over = d.near_singular_count()
+        worst = 0.0
+        for j in cols:
+            for comp in (0, 1):  # f_j = 1 (B, D), then h_j = 1 (A, C)
+                x = np.zeros(2 * n)
+                x[comp * n + j] = 1.0
+                y = d.matvec(x, "fp32_resid")
+                ref = orc.matvec(x)
+                d2 = np.sum((c - c[j]) ** 2, axis=1)
+                ns = np.nonzero((d2 < 2.0 * (ar + ar[j])) & (np.arange(n) != j))[0]
+                for rows in (ns, n + ns):
+                    worst = max(worst, rel(y[rows], ref[rows]))
+        d.close()
+        out[on] = (worst, listed, over)
+    print("near-singular entries: worst rel err FP32 pass", out[False][0], "FP64 pass", out[True][0],
+          "; listed", out[True][1], "for", n, "rows, overflow", out[True][2])
+    assert out[False][1] == 0 and out[False][2] == 0
+    assert out[True][2] == 0 and 6 * n <= out[True][1] <= 24 * n
+    assert out[True][0] < 1e-6
+    assert out[False][0] > 10 * out[True][0]
+
+
+def test_coefficient_guard_repairs_bad_entries(born_scene):
+    """BEEP::sanity_check on the device, with the bad values forced from the
+    host: a NaN in the FP64 set, an out-of-range value in the FP32 set and an
+    infinite self term are found and recomputed; the mat-vecs return to
+    their values before the damage."""
+    sc, a, orc = born_scene
+    nnz = int(np.sum(np.diff(a["leaf_ptr"]) * np.diff(a["nb_ptr"])))
+    d = Device(0)
+    d.set_geometry(a)
+    d.set_tree(a)
+    d.precompute(0.125, "fp64")
+    d.precompute(0.125, "fp32")
+    st = d.guard_stats()
+    assert st["fp64"] == st["fp32"] == st["diag"] == 0, st
+    x = np.random.default_rng(14).normal(size=2 * orc.n)
+    y64, y32 = d.matvec(x, "fp64"), d.matvec(x, "fp32")
+    d.debug_poke("fp64", nnz // 3, [np.nan, 0.0, 0.0, 0.0])
+    d.debug_poke("fp32", (2 * nnz) // 3, [0.0, 1e30, 0.0, 0.0])
+    d.debug_poke("diag", 5, [0.0, 0.0, np.inf, 0.0])
+    assert not np.all(np.isfinite(d.matvec(x, "fp64")))
+    assert rel(d.matvec(x, "fp32"), y32) > 1.0
+    assert d.sanity_check() == 3
+    assert d.guard_stats()["sanity_check"] == 3
+    e64, e32 = rel(d.matvec(x, "fp64"), y64), rel(d.matvec(x, "fp32"), y32)
+    assert d.sanity_check() == 0
+    d.close()
+    print("after repair: fp64 rel change", e64, "fp32", e32)
+    # the FP64 entry and the self term are recomputed by the same formulas;
+    # the FP32 entry is recomputed in FP64 and rounded
+    assert e64 < 1e-14 and e32 < 1e-6
+
+
 def test_rhs_and_energy_coeffs(dev_born):
     d, a, orc = dev_born
     b_dev = d.rhs(a["charges"], 80.0)
@@ -93,6 +165,25 @@ def test_solve_born_ion():
         print(mode, "vs oracle LU", abs(E - E_ref) / abs(E_ref))
         assert abs(E - E_ref) / abs(E_ref) < tol_rel
     b.close()
+
+
+def test_cgs2_solves(dev_born, born_scene):
+    """Two Gram-Schmidt passes per Arnoldi step (CGS2): the solution agrees
+    with the single-pass (reference) solve and with the oracle's LU solve,
+    and the Krylov basis is orthonormal to rounding."""
+    d, a, orc = dev_born
+    b = orc.rhs()
+    x_lu = np.linalg.solve(orc.assemble(), b)
+    out = {}
+    for passes in (1, 2):
+        d.set_gram_schmidt(passes)
+        x, st = d.solve(b, tol=1e-10, max_iter=100, mode="fp64")
+        out[passes] = (x, st["iterations"])
+    d.set_gram_schmidt(1)
+    e1, e2 = rel(out[1][0], x_lu), rel(out[2][0], x_lu)
+    print("CGS vs LU", e1, "CGS2 vs LU", e2, "iterations", out[1][1], out[2][1])
+    assert e1 < 1e-8 and e2 < 1e-8
+    assert out[2][1] <= out[1][1] + 1
 
 
 @pytest.fixture(scope="module")

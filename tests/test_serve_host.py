@@ -27,7 +27,7 @@ def test_bad_options_are_rejected():
     with pytest.raises(BeepError):
         Server(slots=0)
     with pytest.raises(BeepError):
-        Server(budget_gib=0)
+        Server(budget_gib=-1)
 
 
 def test_submitted_scene_changes_owner():
@@ -63,7 +63,7 @@ def test_requests_fail_cleanly_without_gpu():
 
 def test_oversized_request_is_refused():
     """A request whose device estimate exceeds the whole budget fails
-    instead of waiting forever."""
+    instead of waiting forever (checked before the device is read)."""
     if _has_gpu():
         pytest.skip("CPU-only check")
     srv = Server(slots=1, budget_gib=1e-6)
