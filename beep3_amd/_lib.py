@@ -102,6 +102,7 @@ class FarInfo(C.Structure):
         ("mode", C.c_int), ("order", C.c_int), ("leaf_level", C.c_int), ("n_leaf_boxes", C.c_int),
         ("n_boxes", C.c_int), ("m2l_pairs", C.c_longlong), ("p2p_pairs", C.c_longlong),
         ("p2p_residual_pairs", C.c_longlong), ("plan_ms", C.c_double),
+        ("bytes_fp64", C.c_double), ("bytes_fp32", C.c_double),
     ]
 
 
@@ -206,7 +207,6 @@ def _declare_hip(L):
     L.beep_hip_solve_device.argtypes = [vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]
     L.beep_hip_rhs_device.argtypes = [vp, C.c_int, dp, C.c_double, vp]
     L.beep_hip_selftest_mfma.argtypes = [vp, C.c_int, dp, dp, dp]
-    L.beep_hip_selftest_exp.argtypes = [vp, C.c_int, dp, dp]
     L.beep_hip_fmm_symmetry.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, ip, ip]
     L.beep_hip_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
     L.beep_hip_device_free.argtypes = [vp, vp]

@@ -1109,6 +1109,14 @@ int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out) {
             out->p2p_pairs = ctx->fmm->n_p2p;
             out->p2p_residual_pairs = ctx->fmm->n_res;
             out->plan_ms = ctx->plan_ms;
+            out->bytes_fp64 = fmm_algorithmic_bytes(ctx->fmm, 0);
+            out->bytes_fp32 = fmm_algorithmic_bytes(ctx->fmm, 1);
+        } else if (!ctx->use_fmm) {
+            // direct sum: x and patches in, sources out, then every target
+            // reads the source records once per tile pass (counted once)
+            // and writes f and h
+            const double N = ctx->N;
+            out->bytes_fp64 = out->bytes_fp32 = N * (16.0 + 64.0 + 64.0) + N * 64.0 + 4.0 * N * 8.0;
         }
     });
 }
@@ -1553,21 +1561,6 @@ int beep_hip_device_memory(int device, size_t* free_bytes, size_t* total_bytes) 
         g_create_err = e.what();
         return -1;
     }
-}
-
-int beep_hip_selftest_exp(beep_hip_ctx* ctx, int n, const double* x, double* y) {
-    return guarded(ctx, [&] {
-        require(x && y && n >= 0, "bad arguments");
-        for (int i = 0; i < n; ++i) require(x[i] <= 0.0, "arguments must be <= 0");
-        set_device(ctx);
-        DevBuf<double> dx, dy;
-        dx.upload(x, size_t(n), ctx->stream);
-        dy.alloc(size_t(n));
-        launch_exp_selftest(ctx->stream, n, dx.p, dy.p);
-        if (n) hip_check(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost,
-                                        ctx->stream), "D2H exp selftest");
-        hip_check(hipStreamSynchronize(ctx->stream), "sync exp selftest");
-    });
 }
 
 int beep_hip_device_alloc(beep_hip_ctx* ctx, size_t bytes, void** out) {

@@ -58,41 +58,6 @@ __device__ __forceinline__ double rsqrt_nr(double v) {
     return fma(y, fma(-h * y, y, 0.5), y);
 }
 
-// e^x for x <= 0 in FP64 (x < -708, below the normal range: 0).  Cody-Waite
-// argument reduction (W. J. Cody and W. Waite, Software Manual for the
-// Elementary Functions, 1980): x = k ln2 + r with k = rint(x / ln2), |r| <=
-// ln2 / 2, ln2 split into LN2_HI (its leading 32 significant bits, so k LN2_HI
-// is exact for |k| < 2^21) and LN2_LO = ln2 - LN2_HI.  Then e^r = 1 + s with
-// s = r + r^2 (1/2! + r/3! + ... + r^11/13!), the Taylor series to degree 13
-// (truncation below 4.2e-18 on |r| <= ln2/2) in Horner form, and 2^k put
-// into the exponent field.  About 20 FP64 operations, no branch.
-__device__ __forceinline__ double exp_nonpos(double x) {
-    constexpr double LOG2E = 1.4426950408889634;
-    constexpr double LN2_HI = 0x1.62e42fee00000p-1;
-    constexpr double LN2_LO = 0x1.a39ef35793c76p-33;
-    const double xc = x < -708.0 ? -708.0 : x;
-    const double k = __builtin_rint(xc * LOG2E);
-    double r = fma(-k, LN2_HI, xc);
-    r = fma(-k, LN2_LO, r);
-    // 1/n! for n = 13 .. 2
-    double q = 1.0 / 6227020800.0;
-    q = fma(q, r, 1.0 / 479001600.0);
-    q = fma(q, r, 1.0 / 39916800.0);
-    q = fma(q, r, 1.0 / 3628800.0);
-    q = fma(q, r, 1.0 / 362880.0);
-    q = fma(q, r, 1.0 / 40320.0);
-    q = fma(q, r, 1.0 / 5040.0);
-    q = fma(q, r, 1.0 / 720.0);
-    q = fma(q, r, 1.0 / 120.0);
-    q = fma(q, r, 1.0 / 24.0);
-    q = fma(q, r, 1.0 / 6.0);
-    q = fma(q, r, 0.5);
-    const double sv = fma(r * r, q, r);
-    const double p = 1.0 + sv;
-    const double two_k = __longlong_as_double((long long)(int(k) + 1023) << 52);
-    return x < -708.0 ? 0.0 : p * two_k;
-}
-
 // FMM leaf cell of a patch packed as x | y << 10 | z << 20 (leaf level
 // <= 10): two patches meet in the FMM leaf pass iff every field differs by
 // at most 1.
@@ -282,7 +247,6 @@ void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tri
 void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
                      const Patch64* patches, Pt64* out64, Pt32* out32);
 void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out);
-void launch_exp_selftest(hipStream_t s, int n, const double* x, double* y);
 void launch_scale_copy(hipStream_t s, int n, const double* x, const double* scale_ptr,
                        double scale, double* y);
 

@@ -1232,6 +1232,30 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
     pl->stage = nullptr;
 }
 
+double fmm_algorithmic_bytes(const FmmPlan* pl, int fp32) {
+    // every far-field array read once and written once per mat-vec
+    const double T = fp32 ? 4.0 : 8.0;
+    const double N = pl->N;
+    const double ncoef = pl->engine == 1 ? double(pl->pw_jp) : double(pl->P) * pl->P * pl->P;
+    const double E = 3.0 * ncoef * T;  // one box's three expansions
+    const double nbox = pl->nbox, nleaf = pl->nleaf;
+    double b = N * (16.0 + 64.0 + 64.0);  // pack: x and patches in, sources out
+    b += N * 64.0 + nleaf * E;            // P2M
+    b += 2.0 * (nbox - 1.0) * E;          // M2M: children read, parents written
+    b += 2.0 * (nbox - 1.0) * E;          // translations: source moments read, locals written
+    b += 3.0 * (nbox - nleaf) * E;        // L2L: parents read, children read and written
+    b += nleaf * E + N * 64.0 + 4.0 * N * 8.0;  // L2P: locals, patches, far vector in and out
+    if (pl->engine == 1 && pl->D >= 2) {
+        // plane waves: M2X writes W, X2X reads it and writes Wacc, X2L reads it
+        const double Wb = 3.0 * pl->pw_nwp * T;
+        b += 4.0 * (nbox - pl->lvl_first[2]) * Wb;
+    }
+    // leaf pass: its pair list (one index per pair), sources, y in and out
+    const double pairs = pl->n_res >= 0 ? double(pl->n_res) : double(pl->n_p2p);
+    b += pairs * 4.0 + N * 64.0 + 4.0 * N * 8.0;
+    return b;
+}
+
 FmmPlan* fmm_plan_create(int N, const double* centroid, int order, int leaf_target,
                          DevArena& arena, hipStream_t s, int engine) {
     FmmPlan* pl = fmm_plan_begin(N, centroid, order, leaf_target, arena, s, engine);

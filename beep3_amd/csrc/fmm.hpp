@@ -52,8 +52,7 @@ __device__ __forceinline__ double rsq_or_zero(double r2) {
     return r2 > 0.0 ? ir : 0.0;
 }
 __device__ __forceinline__ float exp_fast(float v) { return __expf(v); }
-// v = -kappa r <= 0
-__device__ __forceinline__ double exp_fast(double v) { return exp_nonpos(v); }
+__device__ __forceinline__ double exp_fast(double v) { return exp(v); }
 
 // 1-point interaction of one packed source (relative position d = source -
 // target, 1/eps ie, dipole a f n = (sdx, sdy, sdz), monopole a h = sh) with
@@ -171,6 +170,10 @@ void pw_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 void pw_evaluate(FmmPlan* p, hipStream_t s, int fp32, const Patch64* patches, double kappa, double* y,
                  hipEvent_t* m2l_ev);
 void fmm_plan_destroy(FmmPlan* p);
+// algorithmic HBM bytes of one far-field evaluation: every far-field array
+// (sources, expansions, plane waves, the leaf-pass list, the far vector)
+// read once and written once
+double fmm_algorithmic_bytes(const FmmPlan* p, int fp32);
 void fmm_prepare(FmmPlan* p, double kappa, bool fp64, bool fp32, hipStream_t s);
 inline bool fmm_tables_ready(const FmmPlan* p, double kappa, bool fp64, bool fp32) {
     return p->tables_kappa == kappa && (!fp64 || p->have64) && (!fp32 || p->have32);

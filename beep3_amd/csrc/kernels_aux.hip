@@ -568,17 +568,6 @@ __global__ __launch_bounds__(64) void k_mfma_selftest(const double* __restrict__
     }
 }
 
-__global__ void k_exp_selftest(int n, const double* __restrict__ x, double* __restrict__ y) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = exp_nonpos(x[i]);
-}
-
-void launch_exp_selftest(hipStream_t s, int n, const double* x, double* y) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_exp_selftest, dim3((n + 255) / 256), dim3(256), 0, s, n, x, y);
-    hip_check(hipGetLastError(), "k_exp_selftest launch");
-}
-
 void launch_mfma_selftest(hipStream_t s, const double* a, const double* b, int fp32, double* out) {
     hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, s, a, b, fp32, out);
     hip_check(hipGetLastError(), "k_mfma_selftest launch");

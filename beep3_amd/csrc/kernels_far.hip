@@ -59,8 +59,7 @@ struct FarMath<double> {
         y = fma(y, fma(-h * y, y, 0.5), y);
         return fma(y, fma(-h * y, y, 0.5), y);
     }
-    // v = -kappa r <= 0
-    static __device__ __forceinline__ double exp_(double v) { return exp_nonpos(v); }
+    static __device__ __forceinline__ double exp_(double v) { return exp(v); }
 };
 
 template <typename T>

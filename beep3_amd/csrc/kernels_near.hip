@@ -36,8 +36,7 @@ struct Math<double> {
     // hardware estimate refined by two Newton steps y <- y + y (1/2 - (v/2) y^2)
     static __device__ __forceinline__ double rsqrt_(double v) { return rsqrt_nr(v); }
     static __host__ __device__ __forceinline__ double ek_of(double kappa) { return -kappa; }
-    // v = -kappa r <= 0
-    static __device__ __forceinline__ double expk(double v) { return exp_nonpos(v); }
+    static __device__ __forceinline__ double expk(double v) { return exp(v); }
 };
 
 // Accumulate A..D of one (qual point, quad point) pair into the per-qual-point
@@ -228,8 +227,10 @@ __device__ __forceinline__ void pair_coeffs(const Patch64& pi, const Patch64& pj
     out[3] = D * inv_eps * k4;
 }
 
+// At most 128 VGPRs (FP64) or 102 (FP32): 4 or 5 waves per SIMD.  The FP64
+// kernel otherwise takes 136 and runs 3 waves per SIMD, 16 % slower.
 template <typename T, bool KAPPA>
-__global__ __launch_bounds__(256) void k_near_coeffs(
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_coeffs(
     int n_leaves, const int* __restrict__ blk_ptr, const int* __restrict__ leaf_ptr,
     const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr,
     const int* __restrict__ nb_idx, const int64_t* __restrict__ coef_off,

@@ -237,8 +237,8 @@ int beep_hip_set_spmv_depth(beep_hip_ctx* ctx, int depth);
 int beep_hip_autotune_spmv_depth(beep_hip_ctx* ctx, int precision, const int* depths, int n,
                                  int reps, double* ms_out, int* best);
 
-/* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT or
- * BEEP_FAR_FMM), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs,
+/* Shape of the current far-field plan: mode in use (BEEP_FAR_DIRECT,
+ * BEEP_FAR_FMM or BEEP_FAR_PW), FMM order, leaf level, leaf boxes, all boxes, M2L box pairs,
  * P2P target-source pairs of the leaf pass, the pairs among them that are not
  * explicit near pairs (the only ones the leaf pass evaluates once the near
  * list is set; -1 before), and the host milliseconds spent building the plan
@@ -253,6 +253,8 @@ typedef struct {
     long long p2p_pairs;
     long long p2p_residual_pairs;
     double plan_ms;
+    double bytes_fp64;   /* algorithmic HBM bytes of one far-field evaluation (every far-field */
+    double bytes_fp32;   /* array read once and written once), FP64 and FP32 expansions */
 } beep_far_info;
 int beep_hip_far_info(beep_hip_ctx* ctx, beep_far_info* out);
 
@@ -344,11 +346,6 @@ int beep_hip_solve_device(beep_hip_ctx* ctx, const double* d_b, double* d_x,
  * result layout the M2L kernels rely on. */
 int beep_hip_selftest_mfma(beep_hip_ctx* ctx, int fp32, const double* a, const double* b,
                            double* d);
-
-/* Diagnostic: y[i] = e^{x[i]} (every x[i] <= 0) by the FP64 exponential of
- * the integrand and the far-field kernels (Cody-Waite reduction, degree-13
- * Taylor polynomial), for comparison with a host exp. */
-int beep_hip_selftest_exp(beep_hip_ctx* ctx, int n, const double* x, double* y);
 
 /* Diagnostic (host only): the M2L symmetry used by the FMM.  For a source
  * box at offset t = (tx, ty, tz) (box units, 2 <= max |t_k| <= 3) returns
