@@ -198,6 +198,7 @@ def _declare_hip(L):
     L.beep_hip_far_info.argtypes = [vp, C.POINTER(FarInfo)]
     L.beep_hip_matvec.argtypes = [vp, dp, dp, C.c_int]
     L.beep_hip_matvec_device.argtypes = [vp, vp, vp, C.c_int]
+    L.beep_hip_matvec_device_noalloc.argtypes = [vp, vp, vp, C.c_int]
     L.beep_hip_pw_selftest.argtypes = [C.c_int, C.c_int, C.c_double, dp]
     L.beep_hip_matvec_timing.argtypes = [vp, C.POINTER(MatvecTiming)]
     L.beep_hip_timing_reset.argtypes = [vp]

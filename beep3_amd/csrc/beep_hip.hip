@@ -2,6 +2,7 @@
 // upload, precompute, mat-vec (do_bem_fmm_iteration), RHS, energy
 // coefficients and the device GMRES / iterative-refinement driver.
 #include <algorithm>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -56,10 +57,36 @@ double now_ms() {
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
+// Scratch of the allocation-free mat-vec (beep_hip_matvec_device_noalloc):
+// one array per device, held from the library's load to the end of the
+// process.  Host code never passes `scratch` itself to a kernel (in host code
+// the name is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per
+// device when the first context is created on it.
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// the device address of `scratch` on `device` (current), looked up once
+void* scratch_address(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 struct beep_hip_ctx {
     int device = 0;
+    void* scratch = nullptr;  // device address of this device's scratch array
     hipStream_t stream = nullptr;
     // far field runs on its own stream, concurrently with the near SpMV, into
     // yfar; the context stream adds it to y (fork / join through events)
@@ -597,6 +624,7 @@ int beep_hip_create(int device, beep_hip_ctx** out) {
         hip_check(hipStreamCreate(&c->far_stream), "hipStreamCreate");
         hip_check(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device),
                   "hipDeviceGetAttribute");
+        c->scratch = scratch_address(device);
         hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&c->ev_pack, hipEventDisableTiming), "hipEventCreate");
@@ -1126,6 +1154,54 @@ int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* dx, double* dy, int 
         require(dx && dy && dx != dy, "mat-vec needs distinct device vectors");
         set_device(ctx);
         matvec_dev(ctx, dx, dy, precision);
+    });
+}
+
+int beep_hip_matvec_device_noalloc(beep_hip_ctx* ctx, const double* dx, double* dy, int precision) {
+    return guarded(ctx, [&] {
+        require(dx && dy && dx != dy, "mat-vec needs distinct device vectors");
+        require(ctx->have_geom && ctx->have_tree && ctx->have_diag,
+                "geometry, tree and precompute must come first");
+        require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
+                "unknown precision");
+        require(precision == BEEP_FP64 ? ctx->have64 : ctx->have32, "coefficients not precomputed");
+        require(!ctx->use_fmm, "the allocation-free mat-vec runs with the direct far field only");
+        require((precision == BEEP_FP64 ? ctx->uid64 : ctx->uid32) == 0,
+                "near coefficients were precomputed for another far field: call "
+                "beep_hip_precompute again");
+        require(ctx->scratch != nullptr, "no scratch address for this device");
+        // carve the scratch array: sources, far vector, far partial sums,
+        // SpMV row partials (256-byte aligned pieces)
+        const int N = ctx->N, n2 = 2 * N;
+        const int ns = far_direct_splits(N, ctx->n_cu);
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t b_src = up(sizeof(Src64) * size_t(N));
+        const size_t b_yfar = up(sizeof(double) * size_t(n2));
+        const size_t b_part = ns > 1 ? up(sizeof(double) * size_t(n2) * ns) : 0;
+        const size_t b_spmv = up(sizeof(double) * size_t(ctx->spmv.n_work) * 256 * 2);
+        const size_t need = b_src + b_yfar + b_part + b_spmv;
+        if (need > SCRATCH_BYTES)
+            throw std::runtime_error("the allocation-free mat-vec needs " + std::to_string(need) +
+                                     " bytes of scratch for " + std::to_string(N) +
+                                     " patches; its limit is " + std::to_string(SCRATCH_BYTES));
+        set_device(ctx);
+        unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+        Src64* src = reinterpret_cast<Src64*>(base);
+        double* yfar = reinterpret_cast<double*>(base + b_src);
+        double* part = ns > 1 ? reinterpret_cast<double*>(base + b_src + b_yfar) : nullptr;
+        double* spart = reinterpret_cast<double*>(base + b_src + b_yfar + b_part);
+        hipStream_t s = ctx->stream;
+        // the kernels of matvec_dev with the direct far field, in its order,
+        // all on the context stream
+        const int mode = spmv_mode(precision);
+        launch_near_spmv(s, mode, N, ctx->spmv, ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
+                         ctx->nb_idx.p, ctx->coef_off.p, ctx->coef32.p, ctx->coef64.p, ctx->diag.p,
+                         dx, spart, dy, ctx->spmv_cap[mode] * ctx->n_cu, ctx->spmv_depth[mode]);
+        launch_fill(s, n2, 0.0, yfar);
+        launch_far_pack(s, N, ctx->patches.p, ctx->leaf_idx.p, dx, src);
+        launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, N, ns, ctx->leaf_idx.p, ctx->patches.p,
+                          src, ctx->kappa, yfar, part);
+        launch_axpby(s, n2, 1.0, yfar, 1.0, dy);
     });
 }
 

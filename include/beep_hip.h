@@ -308,6 +308,18 @@ int beep_hip_near_singular_count(beep_hip_ctx* ctx, long long* listed, long long
 int beep_hip_matvec(beep_hip_ctx* ctx, const double* x, double* y, int precision);
 /* y = K x on device arrays; stream-ordered on the context stream. */
 int beep_hip_matvec_device(beep_hip_ctx* ctx, const double* d_x, double* d_y, int precision);
+/* As beep_hip_matvec_device, but allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Its scratch (source records, the far
+ * vector, partial sums) is a 64 MiB array of the library, one per device,
+ * held until the process ends; calls whose scratch would exceed it (with the
+ * direct far field, about 100,000 patches) fail with a message, and so do
+ * contexts with an FMM far field.  Every call on a device shares that
+ * device's array: calls that overlap in time on one device (two contexts or
+ * streams at once) race and must be serialised by the caller.  Stream-ordered
+ * on the context stream; y is bit-identical to beep_hip_matvec_device's. */
+int beep_hip_matvec_device_noalloc(beep_hip_ctx* ctx, const double* d_x, double* d_y,
+                                   int precision);
 /* per-phase device time of mat-vecs (HIP events on the context stream):
  * the last one and the sums since beep_hip_timing_reset */
 int beep_hip_matvec_timing(beep_hip_ctx* ctx, beep_matvec_timing* out);
