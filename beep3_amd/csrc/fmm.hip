@@ -1069,8 +1069,6 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
         std::vector<int> xl_ptr(size_t(nbox - base) * 6 + 1, 0), xl_src, xl_code;
         double rot[6][3][3];
         pw_direction_frames(rot);
-        pl->pw_maxlev = 0;
-        for (int l = 2; l <= D; ++l) pl->pw_maxlev = std::max(pl->pw_maxlev, int(lev[l].size()));
         for (int l = 2; l <= D; ++l) {
             const long pcells = 1L << (l - 1);
             for (size_t q = 0; q < lev[l].size(); ++q) {
@@ -1112,15 +1110,12 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
         }
         pl->pw_n_x2x = (long long)xl_src.size();
         std::vector<int> l2l_rows, l2l_gptr(size_t(D + 1) * 9, 0);
-        pl->pw_l2l_maxgroup.assign(D + 1, 0);
         for (int l = 0; l <= D; ++l) {
             for (int o = 0; o < 8; ++o) {
                 l2l_gptr[size_t(l) * 9 + o] = int(l2l_rows.size());
                 if (l >= 3)
                     for (int g = pl->lvl_first[l]; g < pl->lvl_first[l + 1]; ++g)
                         if (box_oct[g] == o) l2l_rows.push_back(g);
-                pl->pw_l2l_maxgroup[l] =
-                    std::max(pl->pw_l2l_maxgroup[l], int(l2l_rows.size()) - l2l_gptr[size_t(l) * 9 + o]);
             }
             l2l_gptr[size_t(l) * 9 + 8] = int(l2l_rows.size());
         }
@@ -1135,6 +1130,11 @@ void fmm_plan_finish(FmmPlan* pl, hipStream_t s) {
         upload(arena, "pw_xl_code", pl->pw_xl_code, xl_code, s);
         upload(arena, "pw_l2l_rows", pl->pw_l2l_rows, l2l_rows, s);
         upload(arena, "pw_l2l_gptr", pl->pw_l2l_gptr, l2l_gptr, s);
+        pl->pw_l2l_gptr_h = l2l_gptr;
+        std::vector<int> box_level(nbox, 0);
+        for (int l = 0; l <= D; ++l)
+            for (int g = pl->lvl_first[l]; g < pl->lvl_first[l + 1]; ++g) box_level[g] = l;
+        upload(arena, "pw_box_level", pl->pw_box_level, box_level, s);
         // the host vectors must live until the uploads complete
         hip_check(hipStreamSynchronize(s), "sync plane-wave lists");
     }

@@ -92,9 +92,35 @@ struct PwDevLevel {
     T* shift[2] = {nullptr, nullptr}; // [98][nw][2]
     int nw[2] = {0, 0};
 };
+// One GEMM of a batched plane-wave launch: C[row] (= or +=) alpha A[row] B
+// over the rows (box, channel) of a set of boxes.  Row r: box = rows ?
+// rows[r / nch] : first + r / nch, channel ch0 + r % nch.  A and C hold
+// [box - box0][3 channels][ld].  A row by mode: 0 the box's own row; 1 K = 8
+// blocks of kblk, block o the row of child o (zero when absent); 2 the row of
+// the box's parent.  B: K rows of N used columns, row stride ldb.  tiles =
+// row tiles x column tiles (64 x 64).
+template <typename T>
+struct PwProb {
+    int mode, first, nrows, nch, ch0, K, N, kblk, lda, ldb, ldc, accumulate, abox0, cbox0, tiles, ctiles;
+    const int* rows;
+    const T* A;
+    const T* B;
+    T* C;
+    T alpha;
+};
 template <typename T>
 struct PwDevTables {
     std::vector<PwDevLevel<T>> lev;
+    // the problems of every batched launch, in device memory, and the
+    // launches: [first problem, problems, blocks]
+    PwProb<T>* probs = nullptr;
+    std::vector<int3> launch_m2m;  // one per level D-1 .. 2
+    int3 launch_m2x{0, 0, 0}, launch_x2l{0, 0, 0};
+    std::vector<int3> launch_l2l;  // one per level 3 .. D
+    // X2X over every level >= 2 in one launch: per level the Laplace and
+    // Yukawa wave counts and shift tables
+    int* x2x_nw = nullptr;         // [2 (D + 1)]
+    T** x2x_sh = nullptr;          // [2 (D + 1)]
 };
 
 struct FmmPlan {
@@ -130,13 +156,12 @@ struct FmmPlan {
     double tables_kappa = -1;
     DevArena* arena = nullptr;        // owner of every device buffer above
     // plane-wave engine (engine 1): expansion order, quadrature nodes, padded
-    // coefficient and wave widths, the largest level's box count, the
-    // children of every box (8 per box, -1 absent), the X2X lists (per box
-    // at levels >= 2 and direction: source boxes and local offset codes),
-    // the L2L rows of every level grouped by child octant (9 offsets per
-    // level into pw_l2l_rows, device) with the largest group per level
+    // coefficient and wave widths, the children of every box (8 per box, -1
+    // absent), the X2X lists (per box at levels >= 2 and direction: source
+    // boxes and local offset codes), the L2L rows of every level grouped by
+    // child octant (9 offsets per level into pw_l2l_rows)
     int engine = 0;                   // 0: Chebyshev interpolation, 1: plane waves
-    int pw_p = 0, pw_nlam = 0, pw_jp = 0, pw_nwp = 0, pw_maxlev = 0;
+    int pw_p = 0, pw_nlam = 0, pw_jp = 0, pw_nwp = 0;
     long long pw_n_x2x = 0;           // X2X translations per mat-vec (all levels)
     int* pw_child8 = nullptr;
     int* pw_xl_ptr = nullptr;
@@ -144,11 +169,12 @@ struct FmmPlan {
     int* pw_xl_code = nullptr;
     int* pw_l2l_rows = nullptr;
     int* pw_l2l_gptr = nullptr;
-    std::vector<int> pw_l2l_maxgroup;
+    std::vector<int> pw_l2l_gptr_h;   // host copy (9 per level)
+    int* pw_box_level = nullptr;      // level of every box
     PwDevTables<double> pw64;
     PwDevTables<float> pw32;
-    double *pw_M64 = nullptr, *pw_L64 = nullptr, *pw_W64 = nullptr, *pw_Wacc64 = nullptr;
-    float *pw_M32 = nullptr, *pw_L32 = nullptr, *pw_W32 = nullptr, *pw_Wacc32 = nullptr;
+    double *pw_M64 = nullptr, *pw_L64 = nullptr, *pw_W64 = nullptr, *pw_Wacc64 = nullptr, *pw_T64 = nullptr;
+    float *pw_M32 = nullptr, *pw_L32 = nullptr, *pw_W32 = nullptr, *pw_Wacc32 = nullptr, *pw_T32 = nullptr;
 };
 
 // The plan's device buffers come from the arena (owned by the caller, reused

@@ -7,12 +7,12 @@
 // source).  Expansions are real solid-harmonic coefficient vectors (pw.hpp);
 // per level, all boxes at once:
 //   P2M   leaf moments                            VALU   k_pw_p2m
-//   M2M   parent = [8 children] x stacked A_o     MFMA   k_pw_gemm (gathered K)
-//   M2X   six-direction plane waves = M x m2x     MFMA   k_pw_gemm
-//   X2X   diagonal translations, per target and   HBM    k_pw_x2x
+//   M2M   child x A_o by octant, summed per parent MFMA   k_pw_gemm + k_pw_m2m_sum, per level
+//   M2X   six-direction plane waves = M x m2x     MFMA   k_pw_gemm, every level at once
+//   X2X   diagonal translations, per target and   HBM    k_pw_x2x, every level at once
 //         direction in list order
-//   X2L   local += waves x x2l                    MFMA   k_pw_gemm
-//   L2L   child local += parent local x A_o       MFMA   k_pw_gemm (grouped by octant)
+//   X2L   local = waves x x2l                     MFMA   k_pw_gemm, every level at once
+//   L2L   child local += parent local x A_o       MFMA   k_pw_gemm (grouped by octant), per level
 //   L2P   potential and gradient at the targets   VALU   k_pw_l2p
 // and the leaf pass (P2P) of fmm.hip / fmm_residual.hip.
 #include <algorithm>
@@ -306,93 +306,96 @@ __global__ __launch_bounds__(64) void k_pw_l2p(const FmmDev d, const Patch64* __
     }
 }
 
-// Batched GEMM on MFMA: C[row] (=|+=) alpha A[row] B for the rows (box,
-// channel) of a level.  Row r of group g (blockIdx.z): box = rows ?
-// rows[gptr[g] + r / nch] : first + r / nch, channel ch0 + r % nch.  Every
-// array holds [box - box0][3 channels][ld] (box0 = abox0 for A, cbox0 for C:
-// 0 for the per-box expansions, the level's first box for the per-level
-// wave buffers).  A row by mode: 0 the box's own row; 1 K = 8 blocks of kblk,
-// block o the row of child o (zero when absent); 2 the row of the box's
-// parent.  B of group g: B + g bgs, [K][N] row-major.  Tile 64 x 64, four
-// waves of 16 rows x 64 columns, K staged 16 at a time; each output is
-// written by one lane, so the result does not depend on the launch shape.
-template <typename T>
-struct PwGemm {
-    int mode, first, nrows, nch, ch0, K, N, kblk, lda, ldc, accumulate, abox0, cbox0;
-    const int* rows;
-    const int* gptr;
-    const int* child8;
-    const int* parent;
-    const T* A;
-    const T* B;
-    size_t bgs;
-    T* C;
-    T alpha;
-};
+// Batched GEMMs on MFMA (the PwProb problems of one launch, fmm.hpp).  A
+// block computes one 64 x 64 tile of one problem: four waves of 16 rows x 64
+// columns, K staged through LDS 32 at a time with the next slab's loads
+// issued before the current slab's products (register prefetch), so the
+// K loop is not one memory latency per step.  Each output is written by one
+// lane, so the result does not depend on the launch shape.
+constexpr int PW_BM = 64, PW_BN = 64, PW_BK = 32;
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_pw_gemm(const PwGemm<T> g) {
-    constexpr int BM = 64, BN = 64, BK = 16;
+__device__ __forceinline__ T pw_load_a(const PwProb<T>& g, const int* __restrict__ child8,
+                                       const int* __restrict__ parent, int box, int ch, int k) {
+    if (box < 0 || k >= g.K) return T(0);
+    int src = box, kb = k;
+    if (g.mode == 1) {
+        const int o = k / g.kblk;
+        kb = k - o * g.kblk;
+        src = child8[size_t(box) * 8 + o];
+    } else if (g.mode == 2) {
+        src = parent[box];
+    }
+    return src >= 0 ? g.A[(size_t(src - g.abox0) * 3 + ch) * g.lda + kb] : T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pw_gemm(const PwProb<T>* __restrict__ probs, int nprob,
+                                                const int* __restrict__ child8,
+                                                const int* __restrict__ parent) {
     using acc_t = typename PwMfma<T>::acc_t;
-    __shared__ T As[BM][BK + 1];
-    __shared__ T Bs[BK][BN + 4];
-    __shared__ int rbox[BM], rch[BM];
-    const int grp = blockIdx.z;
-    const int gb = g.gptr ? g.gptr[grp] : 0;
-    const int nr = (g.gptr ? g.gptr[grp + 1] - gb : g.nrows) * g.nch;
-    const int r0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-    if (r0 >= nr) return;
-    const T* B = g.B + size_t(grp) * g.bgs;
+    constexpr int AE = PW_BM * PW_BK / 256, BE = PW_BK * PW_BN / 256;  // loads per thread
+    __shared__ T As[PW_BM][PW_BK + 1];
+    __shared__ T Bs[PW_BK][PW_BN + 4];
+    __shared__ int rbox[PW_BM], rch[PW_BM];
+    // this block's problem and tile
+    int b = blockIdx.x, pi = 0;
+    while (pi < nprob - 1 && b >= probs[pi].tiles) {
+        b -= probs[pi].tiles;
+        ++pi;
+    }
+    const PwProb<T>& g = probs[pi];
+    const int r0 = (b / g.ctiles) * PW_BM, n0 = (b % g.ctiles) * PW_BN;
+    const int nr = g.nrows * g.nch;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    if (t < BM) {
+    if (t < PW_BM) {
         const int r = r0 + t;
         rbox[t] = -1;
         rch[t] = 0;
         if (r < nr) {
             const int bi = r / g.nch;
-            rbox[t] = g.rows ? g.rows[gb + bi] : g.first + bi;
+            rbox[t] = g.rows ? g.rows[bi] : g.first + bi;
             rch[t] = g.ch0 + r % g.nch;
         }
     }
+    __syncthreads();
+    T ra[AE], rb[BE];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < AE; ++i) {
+            const int e = t + 256 * i, rr = e / PW_BK, kk = e % PW_BK;
+            ra[i] = pw_load_a(g, child8, parent, rbox[rr], rch[rr], k0 + kk);
+        }
+#pragma unroll
+        for (int i = 0; i < BE; ++i) {
+            const int e = t + 256 * i, kk = e / PW_BN, nn = e % PW_BN;
+            const int k = k0 + kk, n = n0 + nn;
+            rb[i] = (k < g.K && n < g.N) ? g.B[size_t(k) * g.ldb + n] : T(0);
+        }
+    };
     acc_t acc[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] = acc_t{0, 0, 0, 0};
-    __syncthreads();
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        // A tile: 64 rows x 16 K
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += PW_BK) {
 #pragma unroll
-        for (int i = 0; i < (BM * BK) / 256; ++i) {
-            const int e = t + 256 * i, rr = e / BK, kk = e % BK;
-            const int box = rbox[rr], ch = rch[rr];
-            T v = T(0);
-            if (box >= 0) {
-                const int k = k0 + kk;
-                int src = box, kb = k;
-                if (g.mode == 1) {
-                    const int o = k / g.kblk;
-                    kb = k - o * g.kblk;
-                    src = g.child8[size_t(box) * 8 + o];
-                } else if (g.mode == 2) {
-                    src = g.parent[box];
-                }
-                if (src >= 0) v = g.A[(size_t(src - g.abox0) * 3 + ch) * g.lda + kb];
-            }
-            As[rr][kk] = v;
+        for (int i = 0; i < AE; ++i) {
+            const int e = t + 256 * i;
+            As[e / PW_BK][e % PW_BK] = ra[i];
         }
-        // B tile: 16 K x 64 columns
 #pragma unroll
-        for (int i = 0; i < (BK * BN) / 256; ++i) {
-            const int e = t + 256 * i, kk = e / BN, nn = e % BN;
-            const int n = n0 + nn;
-            Bs[kk][nn] = n < g.N ? B[size_t(k0 + kk) * g.N + n] : T(0);
+        for (int i = 0; i < BE; ++i) {
+            const int e = t + 256 * i;
+            Bs[e / PW_BN][e % PW_BN] = rb[i];
         }
         __syncthreads();
+        if (k0 + PW_BK < g.K) fetch(k0 + PW_BK);  // in flight during the products
 #pragma unroll
-        for (int ks = 0; ks < BK / 4; ++ks) {
-            const T a = As[16 * w + (l % 16)][4 * ks + l / 16];
+        for (int ks = 0; ks < PW_BK / 4; ++ks) {
+            const T av = As[16 * w + (l % 16)][4 * ks + l / 16];
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                acc[c] = PwMfma<T>::op(a, Bs[4 * ks + l / 16][16 * c + (l % 16)], acc[c]);
+                acc[c] = PwMfma<T>::op(av, Bs[4 * ks + l / 16][16 * c + (l % 16)], acc[c]);
         }
         __syncthreads();
     }
@@ -412,23 +415,28 @@ __global__ __launch_bounds__(256) void k_pw_gemm(const PwGemm<T> g) {
     }
 }
 
-// X2X: one block per (target box of a level, direction).  For each channel
-// and wave: acc = sum over the direction's list (in list order) of W_src x
-// shift[code]; written to Wacc (every entry of the direction, zero for an
-// empty list; direction 0 also clears the padding).  W and Wacc are indexed
-// by the box's position in its level.  Channel 0 uses the
-// Laplace waves (nwL) and shifts, channels 1 and 2 the level's Yukawa ones.
+// X2X: one block per (target box at a level >= 2, direction), every level
+// in one launch.  For each channel and wave: acc = sum over the direction's
+// list (in list order) of W_src x shift[code]; written to Wacc (every entry
+// of the direction, zero for an empty list; direction 0 also clears the
+// padding).  W and Wacc are indexed by box - base (base = the first box of
+// level 2).  Channel 0 uses the Laplace waves and shifts of the level,
+// channels 1 and 2 the Yukawa ones.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pw_x2x(int first, int nwp, const int* __restrict__ xl_ptr,
+__global__ __launch_bounds__(256) void k_pw_x2x(int base, int nwp, const int* __restrict__ box_level,
+                                               const int* __restrict__ xl_ptr,
                                                const int* __restrict__ xl_src,
-                                               const int* __restrict__ xl_code, int base,
-                                               int nwL, const T* __restrict__ shL, int nwY,
-                                               const T* __restrict__ shY, const T* __restrict__ W,
-                                               T* __restrict__ Wacc) {
+                                               const int* __restrict__ xl_code,
+                                               const int* __restrict__ lv_nw, T* const* __restrict__ lv_sh,
+                                               const T* __restrict__ W, T* __restrict__ Wacc) {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int tb = bid / 6, dir = bid - 6 * tb;
-    const int box = first + tb;
-    const int e0 = xl_ptr[(box - base) * 6 + dir], e1 = xl_ptr[(box - base) * 6 + dir + 1];
+    const int box = base + tb;
+    const int lv = box_level[box];
+    const int nwL = lv_nw[2 * lv], nwY = lv_nw[2 * lv + 1];
+    const T* shL = lv_sh[2 * lv];
+    const T* shY = lv_sh[2 * lv + 1];
+    const int e0 = xl_ptr[tb * 6 + dir], e1 = xl_ptr[tb * 6 + dir + 1];
     const int nitem = nwL + 2 * nwY;
     for (int it = threadIdx.x; it < nitem; it += 256) {
         const int ch = it < nwL ? 0 : 1 + (it - nwL) / nwY;
@@ -438,7 +446,7 @@ __global__ __launch_bounds__(256) void k_pw_x2x(int first, int nwp, const int* _
         const int col = (dir * nw + wv) * 2;
         T ar = 0, ai = 0;
         for (int e = e0; e < e1; ++e) {
-            const int sb = xl_src[e] - first, code = xl_code[e];
+            const int sb = xl_src[e] - base, code = xl_code[e];
             const T* wp = W + (size_t(sb) * 3 + ch) * nwp + col;
             const T wr = wp[0], wi = wp[1];
             const T sr = sh[(size_t(code) * nw + wv) * 2], si = sh[(size_t(code) * nw + wv) * 2 + 1];
@@ -463,13 +471,22 @@ __global__ void k_pw_zero(size_t n, T* __restrict__ p) {
         p[i] = T(0);
 }
 
+// M2M, second half: the moments of every parent box at a level are the sum
+// of its children's translated moments T (written by the octant GEMMs), in
+// octant order.  One thread per (parent, channel, coefficient).
 template <typename T>
-void gemm(hipStream_t s, const PwGemm<T>& g, int groups, int max_rows) {
-    const int rows = max_rows * g.nch;
-    if (rows <= 0) return;
-    dim3 grid((rows + 63) / 64, (g.N + 63) / 64, groups);
-    hipLaunchKernelGGL(k_pw_gemm<T>, grid, dim3(256), 0, s, g);
-    hip_check(hipGetLastError(), "k_pw_gemm launch");
+__global__ void k_pw_m2m_sum(int first, int count, int jp, const int* __restrict__ child8,
+                             const T* __restrict__ Tc, T* __restrict__ M) {
+    const size_t q = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+    if (q >= size_t(count) * 3 * jp) return;
+    const int box = first + int(q / (size_t(3) * jp));
+    const size_t cj = q % (size_t(3) * jp);
+    T sum = T(0);
+    for (int o = 0; o < 8; ++o) {
+        const int c = child8[size_t(box) * 8 + o];
+        if (c >= 0) sum += Tc[size_t(c) * 3 * jp + cj];
+    }
+    M[size_t(box) * 3 * jp + cj] = sum;
 }
 
 template <typename T>
@@ -522,9 +539,139 @@ void make_pw_tables(FmmPlan* pl, double kappa, PwDevTables<T>& out, hipStream_t 
     }
 }
 
+// The problems of every batched GEMM launch of a plan (expansion buffers M,
+// L, plane waves W, accumulated waves Wacc), uploaded once per precision,
+// and the per-level parameters of the all-level X2X launch.
+template <typename T>
+void make_pw_launches(FmmPlan* pl, PwDevTables<T>& tb, T* M, T* L, T* W, T* Wacc, T* Tc, hipStream_t s) {
+    const int D = pl->D, jp = pl->pw_jp, nwp = pl->pw_nwp;
+    const int base = D >= 2 ? pl->lvl_first[2] : pl->nbox;
+    std::vector<PwProb<T>> probs;
+    auto add = [&](PwProb<T> g) {
+        const int rt = (g.nrows * g.nch + PW_BM - 1) / PW_BM;
+        g.ctiles = (g.N + PW_BN - 1) / PW_BN;
+        g.tiles = rt * g.ctiles;
+        if (g.tiles > 0) probs.push_back(g);
+    };
+    auto launch = [&](size_t first) {
+        int blocks = 0;
+        for (size_t q = first; q < probs.size(); ++q) blocks += probs[q].tiles;
+        return int3{int(first), int(probs.size() - first), blocks};
+    };
+    // B's row stride is N, except for M2X (the table's padded width)
+    auto prob = [&](int mode, int k, int K, int N, int lda, int ldc, int acc, const T* A, const T* B, T* C,
+                    T alpha, int abox0, int cbox0) {
+        PwProb<T> g{};
+        g.mode = mode;
+        g.nch = k == 0 ? 1 : 2;
+        g.ch0 = k == 0 ? 0 : 1;
+        g.K = K;
+        g.N = N;
+        g.kblk = jp;
+        g.lda = lda;
+        g.ldb = N;
+        g.ldc = ldc;
+        g.accumulate = acc;
+        g.A = A;
+        g.B = B;
+        g.C = C;
+        g.alpha = alpha;
+        g.abox0 = abox0;
+        g.cbox0 = cbox0;
+        return g;
+    };
+    // M2M of the parents at level l: every child (level l + 1, grouped by
+    // its octant o) translated by its octant's matrix into Tc, then summed
+    // per parent (k_pw_m2m_sum): K = jp per product
+    tb.launch_m2m.clear();
+    for (int l = D - 1; l >= 2; --l) {
+        const size_t f = probs.size();
+        const int* gp = pl->pw_l2l_gptr_h.data() + size_t(l + 1) * 9;
+        for (int k = 0; k < 2; ++k)
+            for (int o = 0; o < 8; ++o) {
+                PwProb<T> g = prob(0, k, jp, jp, jp, jp, 0, M, tb.lev[l].m2mT[k] + size_t(o) * jp * jp, Tc,
+                                   T(1), 0, 0);
+                g.rows = pl->pw_l2l_rows + gp[o];
+                g.nrows = gp[o + 1] - gp[o];
+                add(g);
+            }
+        tb.launch_m2m.push_back(launch(f));
+    }
+    {
+        const size_t f = probs.size();
+        // only the 12 nw real columns of each kernel and level are formed
+        // (X2X reads no others)
+        for (int l = 2; l <= D; ++l)
+            for (int k = 0; k < 2; ++k) {
+                PwProb<T> g = prob(0, k, jp, 12 * tb.lev[l].nw[k], jp, nwp, 0, M, tb.lev[l].m2x[k], W,
+                                   T(1.0 / pl->box_h[l]), 0, base);
+                g.ldb = nwp;
+                g.first = pl->lvl_first[l];
+                g.nrows = pl->lvl_first[l + 1] - pl->lvl_first[l];
+                add(g);
+            }
+        tb.launch_m2x = launch(f);
+    }
+    {
+        // into zeroed locals: L = X2L, then L2L adds the parent's part
+        const size_t f = probs.size();
+        for (int l = 2; l <= D; ++l)
+            for (int k = 0; k < 2; ++k) {
+                PwProb<T> g = prob(0, k, 12 * tb.lev[l].nw[k], jp, nwp, jp, 1, Wacc, tb.lev[l].x2l[k], L, T(1),
+                                   base, 0);
+                g.first = pl->lvl_first[l];
+                g.nrows = pl->lvl_first[l + 1] - pl->lvl_first[l];
+                add(g);
+            }
+        tb.launch_x2l = launch(f);
+    }
+    tb.launch_l2l.clear();
+    for (int l = 3; l <= D; ++l) {
+        const size_t f = probs.size();
+        const int* gp = pl->pw_l2l_gptr_h.data() + size_t(l) * 9;
+        for (int k = 0; k < 2; ++k)
+            for (int o = 0; o < 8; ++o) {
+                PwProb<T> g = prob(2, k, jp, jp, jp, jp, 1, L, tb.lev[l - 1].l2lT[k] + size_t(o) * jp * jp, L,
+                                   T(1), 0, 0);
+                g.rows = pl->pw_l2l_rows + gp[o];
+                g.nrows = gp[o + 1] - gp[o];
+                add(g);
+            }
+        tb.launch_l2l.push_back(launch(f));
+    }
+    const std::string tag = sizeof(T) == 8 ? "64" : "32";
+    if (!probs.empty()) {
+        tb.probs = pl->arena->get<PwProb<T>>("pw_probs" + tag, probs.size());
+        hip_check(hipMemcpyAsync(tb.probs, probs.data(), probs.size() * sizeof(PwProb<T>),
+                                 hipMemcpyHostToDevice, s), "H2D pw problems");
+    }
+    std::vector<int> nw(size_t(2) * (D + 1), 0);
+    std::vector<T*> sh(size_t(2) * (D + 1), nullptr);
+    for (int l = 2; l <= D; ++l)
+        for (int k = 0; k < 2; ++k) {
+            nw[2 * l + k] = tb.lev[l].nw[k];
+            sh[2 * l + k] = tb.lev[l].shift[k];
+        }
+    tb.x2x_nw = pl->arena->get<int>("pw_x2x_nw" + tag, nw.size());
+    tb.x2x_sh = pl->arena->get<T*>("pw_x2x_sh" + tag, sh.size());
+    hip_check(hipMemcpyAsync(tb.x2x_nw, nw.data(), nw.size() * sizeof(int), hipMemcpyHostToDevice, s),
+              "H2D pw x2x");
+    hip_check(hipMemcpyAsync(tb.x2x_sh, sh.data(), sh.size() * sizeof(T*), hipMemcpyHostToDevice, s),
+              "H2D pw x2x");
+    hip_check(hipStreamSynchronize(s), "sync pw launches");  // the host copies die here
+}
+
+template <typename T>
+void gemm_launch(hipStream_t s, const FmmPlan* pl, const PwDevTables<T>& tb, int3 lc) {
+    if (lc.z <= 0) return;
+    hipLaunchKernelGGL(k_pw_gemm<T>, dim3(lc.z), dim3(256), 0, s, tb.probs + lc.x, lc.y, pl->pw_child8,
+                       pl->dv.box_parent);
+    hip_check(hipGetLastError(), "k_pw_gemm launch");
+}
+
 template <typename T, int P>
 void pw_run(FmmPlan* pl, hipStream_t s, const Patch64* patches, double kappa, const PwDevTables<T>& tb,
-            T* M, T* L, T* W, T* Wacc, double* y, hipEvent_t* m2l_ev) {
+            T* M, T* L, T* W, T* Wacc, T* Tc, double* y, hipEvent_t* m2l_ev) {
     const FmmDev& d = pl->dv;
     const int D = pl->D, jp = pl->pw_jp, nwp = pl->pw_nwp;
     {
@@ -535,59 +682,27 @@ void pw_run(FmmPlan* pl, hipStream_t s, const Patch64* patches, double kappa, co
     hipLaunchKernelGGL((k_pw_p2m<T, P>), dim3(pl->nleaf), dim3(256), size_t(G) * 2 * jp * sizeof(T), s, d,
                        pl->src, kappa, jp, M);
     hip_check(hipGetLastError(), "k_pw_p2m launch");
-    // M2M: parents at levels D-1 .. 2
-    for (int l = D - 1; l >= 2; --l) {
+    // M2M: parents at levels D-1 .. 2 (the children's products in one
+    // launch per level, then the sum per parent)
+    for (int l = D - 1, q = 0; l >= 2; --l, ++q) {
+        gemm_launch(s, pl, tb, tb.launch_m2m[q]);
         const int first = pl->lvl_first[l], cnt = pl->lvl_first[l + 1] - first;
-        for (int k = 0; k < 2; ++k) {
-            PwGemm<T> g{};
-            g.mode = 1; g.first = first; g.nrows = cnt; g.nch = k == 0 ? 1 : 2; g.ch0 = k == 0 ? 0 : 1;
-            g.K = 8 * jp; g.N = jp; g.kblk = jp; g.lda = jp; g.ldc = jp; g.accumulate = 0;
-            g.child8 = pl->pw_child8; g.A = M; g.B = tb.lev[l].m2mT[k]; g.C = M; g.alpha = T(1);
-            g.abox0 = 0; g.cbox0 = 0;
-            gemm(s, g, 1, cnt);
-        }
+        const size_t n = size_t(cnt) * 3 * jp;
+        hipLaunchKernelGGL(k_pw_m2m_sum<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, first, cnt, jp,
+                           pl->pw_child8, Tc, M);
+        hip_check(hipGetLastError(), "k_pw_m2m_sum launch");
     }
     if (m2l_ev) hip_check(hipEventRecord(m2l_ev[0], s), "hipEventRecord");
-    for (int l = 2; l <= D; ++l) {
-        const int first = pl->lvl_first[l], cnt = pl->lvl_first[l + 1] - first;
-        if (cnt <= 0) continue;
-        const T ih = T(1.0 / pl->box_h[l]);
-        // L2L from the parents (level l-1 >= 2), grouped by child octant
-        if (l >= 3) {
-            const int* gptr = pl->pw_l2l_gptr + size_t(l) * 9;
-            for (int k = 0; k < 2; ++k) {
-                PwGemm<T> g{};
-                g.mode = 2; g.nch = k == 0 ? 1 : 2; g.ch0 = k == 0 ? 0 : 1;
-                g.K = jp; g.N = jp; g.kblk = jp; g.lda = jp; g.ldc = jp; g.accumulate = 1;
-                g.rows = pl->pw_l2l_rows; g.gptr = gptr; g.parent = d.box_parent;
-                g.A = L; g.B = tb.lev[l - 1].l2lT[k]; g.bgs = size_t(jp) * jp; g.C = L; g.alpha = T(1);
-                g.abox0 = 0; g.cbox0 = 0;
-                gemm(s, g, 8, pl->pw_l2l_maxgroup[l]);
-            }
-        }
-        // M2X: every box of the level
-        for (int k = 0; k < 2; ++k) {
-            PwGemm<T> g{};
-            g.mode = 0; g.first = first; g.nrows = cnt; g.nch = k == 0 ? 1 : 2; g.ch0 = k == 0 ? 0 : 1;
-            g.K = jp; g.N = nwp; g.kblk = jp; g.lda = jp; g.ldc = nwp; g.accumulate = 0;
-            g.A = M; g.B = tb.lev[l].m2x[k]; g.C = W; g.alpha = ih;
-            g.abox0 = 0; g.cbox0 = first;
-            gemm(s, g, 1, cnt);
-        }
-        // X2X
-        hipLaunchKernelGGL(k_pw_x2x<T>, dim3(6 * cnt), dim3(256), 0, s, first, nwp, pl->pw_xl_ptr,
-                           pl->pw_xl_src, pl->pw_xl_code, pl->lvl_first[2], tb.lev[l].nw[0],
-                           tb.lev[l].shift[0], tb.lev[l].nw[1], tb.lev[l].shift[1], W, Wacc);
+    if (D >= 2) {
+        // M2X, X2X and X2L of every level >= 2, one launch each
+        gemm_launch(s, pl, tb, tb.launch_m2x);
+        const int base = pl->lvl_first[2], nb = pl->nbox - base;
+        hipLaunchKernelGGL(k_pw_x2x<T>, dim3(6 * nb), dim3(256), 0, s, base, nwp, pl->pw_box_level,
+                           pl->pw_xl_ptr, pl->pw_xl_src, pl->pw_xl_code, tb.x2x_nw, tb.x2x_sh, W, Wacc);
         hip_check(hipGetLastError(), "k_pw_x2x launch");
-        // X2L
-        for (int k = 0; k < 2; ++k) {
-            PwGemm<T> g{};
-            g.mode = 0; g.first = first; g.nrows = cnt; g.nch = k == 0 ? 1 : 2; g.ch0 = k == 0 ? 0 : 1;
-            g.K = nwp; g.N = jp; g.kblk = nwp; g.lda = nwp; g.ldc = jp; g.accumulate = 1;
-            g.A = Wacc; g.B = tb.lev[l].x2l[k]; g.C = L; g.alpha = T(1);
-            g.abox0 = first; g.cbox0 = 0;
-            gemm(s, g, 1, cnt);
-        }
+        gemm_launch(s, pl, tb, tb.launch_x2l);
+        // L2L from each level's parents, down the tree
+        for (const int3& lc : tb.launch_l2l) gemm_launch(s, pl, tb, lc);
     }
     if (m2l_ev) hip_check(hipEventRecord(m2l_ev[1], s), "hipEventRecord");
     if (D >= 2) {
@@ -599,11 +714,11 @@ void pw_run(FmmPlan* pl, hipStream_t s, const Patch64* patches, double kappa, co
 
 template <typename T>
 void pw_dispatch(FmmPlan* pl, hipStream_t s, const Patch64* patches, double kappa, const PwDevTables<T>& tb,
-                 T* M, T* L, T* W, T* Wacc, double* y, hipEvent_t* m2l_ev) {
+                 T* M, T* L, T* W, T* Wacc, T* Tc, double* y, hipEvent_t* m2l_ev) {
     switch (pl->pw_p) {
-        case 9: pw_run<T, 9>(pl, s, patches, kappa, tb, M, L, W, Wacc, y, m2l_ev); break;
-        case 12: pw_run<T, 12>(pl, s, patches, kappa, tb, M, L, W, Wacc, y, m2l_ev); break;
-        case 18: pw_run<T, 18>(pl, s, patches, kappa, tb, M, L, W, Wacc, y, m2l_ev); break;
+        case 9: pw_run<T, 9>(pl, s, patches, kappa, tb, M, L, W, Wacc, Tc, y, m2l_ev); break;
+        case 12: pw_run<T, 12>(pl, s, patches, kappa, tb, M, L, W, Wacc, Tc, y, m2l_ev); break;
+        case 18: pw_run<T, 18>(pl, s, patches, kappa, tb, M, L, W, Wacc, Tc, y, m2l_ev); break;
         default: throw std::runtime_error("plane-wave order must be 9, 12 or 18");
     }
 }
@@ -614,22 +729,29 @@ bool pw_order_supported(int p) { return p == 9 || p == 12 || p == 18; }
 
 void pw_prepare(FmmPlan* pl, double kappa, bool fp64, bool fp32, hipStream_t s) {
     const size_t nexp = size_t(pl->nbox) * 3 * pl->pw_jp;
+    // plane waves of every box at levels >= 2 (all levels' X2X run at once)
+    const int base = pl->D >= 2 ? pl->lvl_first[2] : pl->nbox;
+    const size_t nwb = size_t(pl->nbox - base);
     if (fp64 && !pl->have64) {
         make_pw_tables<double>(pl, kappa, pl->pw64, s);
-        const size_t nw = size_t(pl->pw_maxlev) * 3 * pl->pw_nwp;
+        const size_t nw = nwb * 3 * pl->pw_nwp;
         pl->pw_M64 = pl->arena->get<double>("pwM64", nexp);
         pl->pw_L64 = pl->arena->get<double>("pwL64", nexp);
         pl->pw_W64 = pl->arena->get<double>("pwW64", std::max<size_t>(nw, 1));
         pl->pw_Wacc64 = pl->arena->get<double>("pwA64", std::max<size_t>(nw, 1));
+        pl->pw_T64 = pl->arena->get<double>("pwT64", nexp);
+        make_pw_launches<double>(pl, pl->pw64, pl->pw_M64, pl->pw_L64, pl->pw_W64, pl->pw_Wacc64, pl->pw_T64, s);
         pl->have64 = true;
     }
     if (fp32 && !pl->have32) {
         make_pw_tables<float>(pl, kappa, pl->pw32, s);
-        const size_t nw = size_t(pl->pw_maxlev) * 3 * pl->pw_nwp;
+        const size_t nw = nwb * 3 * pl->pw_nwp;
         pl->pw_M32 = pl->arena->get<float>("pwM32", nexp);
         pl->pw_L32 = pl->arena->get<float>("pwL32", nexp);
         pl->pw_W32 = pl->arena->get<float>("pwW32", std::max<size_t>(nw, 1));
         pl->pw_Wacc32 = pl->arena->get<float>("pwA32", std::max<size_t>(nw, 1));
+        pl->pw_T32 = pl->arena->get<float>("pwT32", nexp);
+        make_pw_launches<float>(pl, pl->pw32, pl->pw_M32, pl->pw_L32, pl->pw_W32, pl->pw_Wacc32, pl->pw_T32, s);
         pl->have32 = true;
     }
 }
@@ -638,10 +760,12 @@ void pw_evaluate(FmmPlan* pl, hipStream_t s, int fp32, const Patch64* patches, d
                  hipEvent_t* m2l_ev) {
     if (fp32) {
         if (!pl->have32) throw std::runtime_error("plane-wave FP32 tables not prepared");
-        pw_dispatch<float>(pl, s, patches, kappa, pl->pw32, pl->pw_M32, pl->pw_L32, pl->pw_W32, pl->pw_Wacc32, y, m2l_ev);
+        pw_dispatch<float>(pl, s, patches, kappa, pl->pw32, pl->pw_M32, pl->pw_L32, pl->pw_W32, pl->pw_Wacc32,
+                           pl->pw_T32, y, m2l_ev);
     } else {
         if (!pl->have64) throw std::runtime_error("plane-wave FP64 tables not prepared");
-        pw_dispatch<double>(pl, s, patches, kappa, pl->pw64, pl->pw_M64, pl->pw_L64, pl->pw_W64, pl->pw_Wacc64, y, m2l_ev);
+        pw_dispatch<double>(pl, s, patches, kappa, pl->pw64, pl->pw_M64, pl->pw_L64, pl->pw_W64, pl->pw_Wacc64,
+                            pl->pw_T64, y, m2l_ev);
     }
 }
 
