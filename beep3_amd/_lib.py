@@ -154,6 +154,7 @@ def _declare(L):
     L.beep_scene_destroy.argtypes = [vp]
     L.beep_scene_destroy.restype = None
     L.beep_scene_set_host_points.argtypes = [vp, C.c_int]
+    L.beep_scene_set_tree.argtypes = [vp, C.c_int]
     L.beep_scene_surface.argtypes = [vp, dp, ip, ip, ip, ip]
     L.beep_server_default_opts.argtypes = [C.POINTER(ServerOpts)]
     L.beep_server_default_opts.restype = None
@@ -167,6 +168,7 @@ def _declare(L):
     L.beep_server_last_error.argtypes = [vp]
     L.beep_server_last_error.restype = C.c_char_p
     L.beep_server_destroy.argtypes = [vp]
+    _declare_poses(L)
     if hasattr(L, "beep_hip_create"):
         _declare_hip(L)
 
@@ -185,6 +187,7 @@ def _declare_hip(L):
     L.beep_hip_set_graphs.argtypes = [vp, C.c_int]
     L.beep_hip_set_near_singular_fp64.argtypes = [vp, C.c_int]
     L.beep_hip_set_guard.argtypes = [vp, C.c_double]
+    L.beep_hip_share_near.argtypes = [vp, vp, C.c_int, ip, C.c_longlong, ip, ip]
     L.beep_hip_set_gram_schmidt.argtypes = [vp, C.c_int]
     L.beep_hip_guard_stats.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.beep_hip_sanity_check.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -226,3 +229,33 @@ def check_host(rc):
 def check_hip(ctx, rc):
     if rc != 0:
         raise BeepError(lib().beep_hip_last_error(ctx).decode())
+
+
+class PoseOpts(C.Structure):
+    _fields_ = [
+        ("device", C.c_int), ("slots", C.c_int), ("solve_mode", C.c_int), ("tol", C.c_double),
+        ("max_iter", C.c_int), ("max_refine", C.c_int), ("inner_tol", C.c_double),
+        ("far_mode", C.c_int), ("fmm_order", C.c_int), ("fmm_leaf", C.c_int),
+        ("pair_cutoff", C.c_double), ("gram_schmidt", C.c_int), ("keep_fh", C.c_int),
+    ]
+
+
+class PoseResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("energy", C.c_double), ("iterations", C.c_int), ("matvecs", C.c_int),
+        ("refinements", C.c_int), ("converged", C.c_int), ("rel_residual", C.c_double),
+        ("n_pairs", C.c_longlong), ("host_ms", C.c_double), ("device_ms", C.c_double),
+        ("slot", C.c_int), ("message", C.c_char * 128),
+    ]
+
+
+def _declare_poses(L):
+    L.beep_pose_default_opts.argtypes = [C.POINTER(PoseOpts)]
+    L.beep_pose_default_opts.restype = None
+    L.beep_poses_create.argtypes = [C.POINTER(PoseOpts), vp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    C.c_int, C.c_double, C.POINTER(vp)]
+    L.beep_poses_solve.argtypes = [vp, C.c_int, dp, dp, C.POINTER(PoseResult)]
+    L.beep_poses_take_fh.argtypes = [vp, C.c_int, dp, dp]
+    L.beep_poses_info.argtypes = [vp, ip, C.POINTER(C.c_longlong), dp, dp]
+    L.beep_poses_destroy.argtypes = [vp]
+    L.beep_poses_destroy.restype = None

@@ -140,6 +140,18 @@ struct beep_hip_ctx {
     DevBuf<unsigned> ns_count;
     bool ns_listed = false;  // the current FP32 set was built with the FP64 pass
     unsigned ns_cap = 0;     // the list capacity it was built with
+    // near field shared with another context (the instances of a pose): its
+    // coefficient sets, diagonal and tree applied to copies of its patches
+    // at patch offsets share_off, plus this context's own explicit pairs
+    // (CSR over the rows that have any, columns sorted; pair_i / pair_j per
+    // pair for the coefficient kernel)
+    beep_hip_ctx* share = nullptr;
+    std::vector<int> share_off;
+    long long n_pairs = 0;
+    int n_pair_rows = 0;
+    DevBuf<int> pair_rows, pair_ptr, pair_col, pair_i;
+    DevBuf<float4> pcoef32;
+    DevBuf<double2> pcoef64;
     // coefficient guard (|v| > guard_max or not finite): the list of bad
     // entries and counters [0] list length, entries repaired by the last
     // precompute of [1] the FP64 set, [2] the FP32 set, [3] the diagonal,
@@ -298,6 +310,10 @@ void geometry_done(beep_hip_ctx* c, int N, const double* centroid) {
     c->have_geom = true;
     c->have_tree = false;
     c->have32 = c->have64 = c->have_diag = false;
+    c->share = nullptr;
+    c->share_off.clear();
+    c->n_pairs = 0;
+    c->n_pair_rows = 0;
 }
 
 void check_ptr(int N, const int* ptr, const char* what) {
@@ -341,13 +357,16 @@ int spmv_mode(int precision) {
 }
 
 void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
-    require(c->have_geom && c->have_tree, "geometry and tree must be set before a mat-vec");
+    require(c->have_geom && (c->have_tree || c->share), "geometry and tree must be set before a mat-vec");
     require(c->have_diag, "precompute before the mat-vec");
     require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
             "unknown precision");
     if (precision == BEEP_FP64) require(c->have64, "FP64 coefficients not precomputed");
     else require(c->have32, "FP32 coefficients not precomputed");
-    {
+    if (c->share) {
+        require(precision == BEEP_FP64 ? c->share->have64 : c->share->have32,
+                "the shared near field lacks this precision's coefficients");
+    } else {
         const unsigned long long built = precision == BEEP_FP64 ? c->uid64 : c->uid32;
         require(c->use_fmm ? (c->fmm && c->fmm->uid == built) : built == 0,
                 "near coefficients were precomputed for another far field: call "
@@ -374,9 +393,21 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
     // SpMV is the longer of the two, and the far field's dozen short kernels
     // take the host a while to launch
     const int mode = spmv_mode(precision);
-    launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
-                     c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
-                     c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
+    if (c->share) {
+        // every copy of the shared patches (rows and columns at its offset,
+        // f and h halves N apart), then this context's own pairs
+        const beep_hip_ctx* h = c->share;
+        for (const int off : c->share_off)
+            launch_near_spmv(s, mode, c->N, h->spmv, h->leaf_ptr.p, h->leaf_idx.p, h->nb_ptr.p,
+                             h->nb_idx.p, h->coef_off.p, h->coef32.p, h->coef64.p, h->diag.p, dx + off,
+                             c->spmv_part.p, dy + off, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
+        launch_pair_spmv(s, mode, c->N, c->n_pair_rows, c->pair_rows.p, c->pair_ptr.p, c->pair_col.p,
+                         c->pcoef32.p, c->pcoef64.p, dx, dy);
+    } else {
+        launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
+                         c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
+                         c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
+    }
     if (timed) hip_check(hipEventRecord(ev[1], s), "hipEventRecord");
     // far field (1-point interaction of all pairs) into yfar
     launch_fill(sf, n2, 0.0, c->yfar.p);
@@ -461,7 +492,7 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         reinterpret_cast<const void*>(intptr_t(m)), reinterpret_cast<const void*>(intptr_t(c->N)),
         c->V.p, c->xin.p, c->wvec.p, c->hvec.p, c->partial.p, c->jstep.p, c->hh_pinned,
         c->yfar.p, c->src.p, c->far_part.p, c->coef32.p, c->coef64.p, c->diag.p,
-        c->spmv_part.p, c->fmm, c->hvec2.p,
+        c->spmv_part.p, c->fmm, c->hvec2.p, c->share, c->pair_ptr.p, c->pcoef32.p, c->pcoef64.p,
         reinterpret_cast<const void*>(intptr_t(c->gs_passes)),
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
         reinterpret_cast<const void*>(intptr_t(c->spmv_cap[spmv_mode(precision)])),
@@ -795,6 +826,10 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
 int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
     return guarded(ctx, [&] {
         require(ctx->have_geom, "set the geometry before the tree");
+        ctx->share = nullptr;
+        ctx->share_off.clear();
+        ctx->n_pairs = 0;
+        ctx->n_pair_rows = 0;
         require(t && t->n_leaves > 0, "empty tree");
         set_device(ctx);
         sync_streams(ctx);
@@ -866,7 +901,43 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
 // the plan is built first: explicit pairs its leaf pass covers keep the full
 // rule (their packed leaf cells are adjacent) and the leaf pass runs over
 // the residual pairs only.
+// Precompute of a context whose near field is shared (beep_hip_share_near):
+// the listed pairs' coefficients (both precisions) for the shared sets'
+// kappa; the diagonal and the other explicit pairs are the shared context's.
+void precompute_shared(beep_hip_ctx* ctx, double kappa, int precision) {
+    const beep_hip_ctx* h = ctx->share;
+    require(h->have_diag && h->kappa == kappa, "the shared near field has another kappa");
+    require(precision == BEEP_FP64 ? h->have64 : h->have32,
+            "the shared near field lacks this precision's coefficients");
+    sync_streams(ctx);
+    ctx->gen++;
+    hipStream_t s = ctx->stream;
+    ctx->kappa = kappa;
+    if (!ctx->guard_cnt.p) {
+        ctx->guard_cnt.alloc(5);
+        for (int k = 0; k < 5; ++k) launch_set_int(s, reinterpret_cast<int*>(ctx->guard_cnt.p + k), 0);
+    }
+    unsigned* gcnt = ctx->guard_cnt.p;
+    launch_set_int(s, reinterpret_cast<int*>(gcnt + 1), 0);
+    launch_set_int(s, reinterpret_cast<int*>(gcnt + 2), 0);
+    launch_pair_coeffs(s, ctx->n_pairs, ctx->pair_i.p, ctx->pair_col.p, ctx->patches.p, ctx->qual_ptr.p,
+                       ctx->qual64.p, ctx->quad_ptr.p, ctx->quad64.p, kappa, ctx->guard_max, gcnt + 1,
+                       ctx->pcoef32.p, ctx->pcoef64.p);
+    ctx->have_diag = true;
+    ctx->have64 = h->have64;
+    ctx->have32 = h->have32;
+    ctx->uid32 = ctx->uid64 = 0;
+    ensure_far(ctx, precision);
+}
+
 void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
+    if (ctx->share) {
+        require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
+                "unknown precision");
+        require(kappa >= 0, "kappa must be >= 0");
+        precompute_shared(ctx, kappa, precision);
+        return;
+    }
     require(ctx->have_geom && ctx->have_tree, "geometry and tree must be set");
     require(kappa >= 0, "kappa must be >= 0");
     require(precision == BEEP_FP64 || precision == BEEP_FP32 || precision == BEEP_FP32_RESID,
@@ -999,6 +1070,81 @@ int beep_hip_set_overlap(beep_hip_ctx* ctx, int on) {
     });
 }
 
+int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
+                        long long n_pairs, const int* pair_i, const int* pair_j) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "set the geometry first");
+        require(src && src != ctx && !src->share, "the shared near field must come from another plain context");
+        require(src->have_geom && src->have_tree && src->have_diag && (src->have32 || src->have64),
+                "the shared context must have its geometry, tree and coefficients");
+        require(src->uid32 == 0 && src->uid64 == 0,
+                "the shared coefficients must be precomputed with the direct far field");
+        require(n_copies >= 1 && copy_offset, "no copies");
+        require(n_pairs >= 0 && (n_pairs == 0 || (pair_i && pair_j)), "bad pair list");
+        const int N = ctx->N, Ns = src->N;
+        require(size_t(n_copies) * Ns == size_t(N), "the copies must cover every patch");
+        std::vector<char> cover(N, 0);
+        for (int k = 0; k < n_copies; ++k) {
+            require(copy_offset[k] >= 0 && copy_offset[k] + Ns <= N, "copy offset out of range");
+            for (int i = copy_offset[k]; i < copy_offset[k] + Ns; ++i) {
+                require(!cover[i], "copies overlap");
+                cover[i] = 1;
+            }
+        }
+        // pairs sorted by row, then column; CSR over the rows that have any
+        std::vector<std::pair<int, int>> pr(static_cast<size_t>(n_pairs));
+        for (long long q = 0; q < n_pairs; ++q) {
+            require(pair_i[q] >= 0 && pair_i[q] < N && pair_j[q] >= 0 && pair_j[q] < N &&
+                        pair_i[q] != pair_j[q], "pair index out of range");
+            pr[size_t(q)] = {pair_i[q], pair_j[q]};
+        }
+        std::sort(pr.begin(), pr.end());
+        require(std::adjacent_find(pr.begin(), pr.end()) == pr.end(), "repeated pair");
+        std::vector<int> rows, ptr{0}, col(pr.size()), pi(pr.size());
+        for (size_t q = 0; q < pr.size(); ++q) {
+            if (q == 0 || pr[q].first != pr[q - 1].first) {
+                if (q) ptr.push_back(int(q));
+                rows.push_back(pr[q].first);
+            }
+            pi[q] = pr[q].first;
+            col[q] = pr[q].second;
+        }
+        if (!pr.empty()) ptr.push_back(int(pr.size()));
+        set_device(ctx);
+        sync_streams(ctx);
+        hipStream_t s = ctx->stream;
+        ctx->gen++;
+        ctx->have_tree = false;
+        ctx->have32 = ctx->have64 = ctx->have_diag = false;
+        ctx->share = src;
+        ctx->ns_listed = false;
+        ctx->share_off.assign(copy_offset, copy_offset + n_copies);
+        ctx->n_pairs = n_pairs;
+        ctx->n_pair_rows = int(rows.size());
+        // the patch order of the direct far field's source tiles (leaf_idx
+        // of a plain context): each copy in the shared context's leaf order
+        std::vector<int> src_order(Ns), order;
+        order.reserve(N);
+        hip_check(hipMemcpy(src_order.data(), src->leaf_idx.p, sizeof(int) * size_t(Ns), hipMemcpyDeviceToHost),
+                  "D2H shared leaf order");
+        for (int k = 0; k < n_copies; ++k)
+            for (int q = 0; q < Ns; ++q) order.push_back(src_order[q] + copy_offset[k]);
+        ctx->leaf_idx.upload(order.data(), order.size(), s);
+        const size_t np = std::max<size_t>(pr.size(), 1);
+        ctx->pair_rows.upload(rows.data(), rows.size(), s);
+        ctx->pair_ptr.upload(ptr.data(), ptr.size(), s);
+        ctx->pair_col.upload(col.data(), col.size(), s);
+        ctx->pair_i.upload(pi.data(), pi.size(), s);
+        ctx->pcoef32.alloc(np);
+        ctx->pcoef64.alloc(2 * np);
+        ctx->spmv_part.alloc(size_t(src->spmv.n_work) * 256 * 2);
+        ctx->guard_rec.alloc(beep_hip_ctx::GUARD_CAP);
+        hip_check(hipStreamSynchronize(s), "sync share");  // host arrays die here
+        // no residual leaf pass: the explicit pairs store full - 1 point
+        far_setup(ctx);
+    });
+}
+
 int beep_hip_set_near_singular_fp64(beep_hip_ctx* ctx, int on) {
     return guarded(ctx, [&] { ctx->ns_fp64 = on != 0; });
 }
@@ -1053,6 +1199,7 @@ int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired) {
     return guarded(ctx, [&] {
         require(repaired != nullptr, "null output");
         require(ctx->have_diag, "precompute before the sanity check");
+        require(!ctx->share, "the coefficients are shared: check the context that owns them");
         set_device(ctx);
         sync_streams(ctx);
         hipStream_t s = ctx->stream;
@@ -1087,6 +1234,7 @@ int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired) {
 int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const double* v4) {
     return guarded(ctx, [&] {
         require(v4 != nullptr, "null values");
+        require(!ctx->share, "the coefficients are shared: poke the context that owns them");
         set_device(ctx);
         sync_streams(ctx);
         hipStream_t s = ctx->stream;
@@ -1692,6 +1840,8 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         add(ctx->icp_d); add(ctx->V); add(ctx->hvec); add(ctx->partial); add(ctx->wtmp);
         add(ctx->xin); add(ctx->wvec); add(ctx->jstep);
         add(ctx->bvec); add(ctx->rvec); add(ctx->dvec);
+        add(ctx->pair_rows); add(ctx->pair_ptr); add(ctx->pair_col); add(ctx->pair_i);
+        add(ctx->pcoef32); add(ctx->pcoef64); add(ctx->ns_rec); add(ctx->guard_rec); add(ctx->hvec2);
         *bytes = b + ctx->fmm_arena.held;
     });
 }

@@ -196,6 +196,16 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
                       const double* diag4, const double* x, void* part, double* y,
                       int max_blocks, int depth);
 
+// explicit pairs listed one by one: coefficients (FP64, rounded to FP32 as
+// well; bad values zeroed and counted), and y += their products over the
+// rows that have pairs (CSR: rows, ptr, col; mode as launch_near_spmv)
+void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const int* pj,
+                        const Patch64* patches, const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
+                        const Pt64* quad, double kappa, double maxv, unsigned* repaired, float4* c32,
+                        double2* c64);
+void launch_pair_spmv(hipStream_t s, int mode, int n, int n_rows, const int* rows, const int* ptr,
+                      const int* col, const float4* c32, const double2* c64, const double* x, double* y);
+
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,
                      const double* x, Src64* src);

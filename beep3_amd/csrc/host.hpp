@@ -207,6 +207,7 @@ struct beep_scene {
     std::vector<Instance> inst;
     bool built = false;
     bool host_points = true;  // false: point sets are generated on the device
+    bool with_tree = true;    // false: no octree (the near field comes from elsewhere)
     std::vector<beep::Vec3> centroid, normal;
     std::vector<double> area, eps, gc;
     std::vector<int> patch_inst;

@@ -293,8 +293,15 @@ extern "C" int beep_scene_build(beep_scene* s) {
         s->cube_edge = std::max(d.x, std::max(d.y, d.z));
         if (!(s->cube_edge > 0)) throw std::runtime_error("degenerate bounding cube");
         s->cube_centre = (hi + lo) / 2.0;
-        s->tree.build(s->centroid, s->cube_centre, s->cube_edge, s->nbsize);
-        s->tree.leaves_csr(s->leaf_ptr, s->leaf_idx, s->nb_ptr, s->nb_idx);
+        if (s->with_tree) {
+            s->tree.build(s->centroid, s->cube_centre, s->cube_edge, s->nbsize);
+            s->tree.leaves_csr(s->leaf_ptr, s->leaf_idx, s->nb_ptr, s->nb_idx);
+        } else {
+            s->leaf_ptr.assign(1, 0);
+            s->leaf_idx.clear();
+            s->nb_ptr.assign(1, 0);
+            s->nb_idx.clear();
+        }
         s->built = true;
     });
 }
@@ -319,7 +326,7 @@ extern "C" int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* o) {
         for (int l = 0; l < o->n_leaves; ++l)
             pairs += (long long)(s->leaf_ptr[l + 1] - s->leaf_ptr[l]) * (s->nb_ptr[l + 1] - s->nb_ptr[l]);
         o->n_near_pairs = pairs;
-        o->tree_depth = int(s->tree.bottom_level());
+        o->tree_depth = s->with_tree ? int(s->tree.bottom_level()) : 0;
         o->cube_centre[0] = s->cube_centre.x;
         o->cube_centre[1] = s->cube_centre.y;
         o->cube_centre[2] = s->cube_centre.z;
@@ -368,6 +375,14 @@ extern "C" int beep_scene_set_host_points(beep_scene* s, int on) {
     return guarded([&] {
         if (!s) throw std::runtime_error("null scene");
         s->host_points = on != 0;
+        s->built = false;
+    });
+}
+
+extern "C" int beep_scene_set_tree(beep_scene* s, int on) {
+    return guarded([&] {
+        if (!s) throw std::runtime_error("null scene");
+        s->with_tree = on != 0;
         s->built = false;
     });
 }

@@ -638,4 +638,95 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
     hip_check(hipGetLastError(), "k_near_spmv launch");
 }
 
+// ---------------------------------------------------------------------------
+// Explicit pairs listed one by one (the near pairs between instances of a
+// pose, whose near field otherwise comes from a shared per-mesh context):
+// coefficients computed in FP64 (full rule less the 1-point interaction, as
+// every explicit pair with the direct far field), stored in FP64 and
+// rounded to FP32; bad values zeroed and counted, as the guard does.
+template <bool KAPPA>
+__global__ __launch_bounds__(256) void k_pair_coeffs(long long n_pairs, const int* __restrict__ pi_,
+                                                    const int* __restrict__ pj_,
+                                                    const Patch64* __restrict__ patches,
+                                                    const int* __restrict__ qual_ptr,
+                                                    const Pt64* __restrict__ qual,
+                                                    const int* __restrict__ quad_ptr,
+                                                    const Pt64* __restrict__ quad, double kappa,
+                                                    double maxv, unsigned* __restrict__ repaired,
+                                                    float4* __restrict__ c32, double2* __restrict__ c64) {
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    const int i = pi_[k], j = pj_[k];
+    double out[4] = {0, 0, 0, 0};
+    if (i != j) {
+        const Patch64 a = patches[i], b = patches[j];
+        const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+        const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
+        pair_coeffs<double, KAPPA>(a, b, qb, qn, qual, pb, pn, quad, kappa, 1, out);
+        if (coef4_bad(out, maxv)) {
+            out[0] = out[1] = out[2] = out[3] = 0.0;
+            atomicAdd(repaired, 1u);
+        }
+    }
+    c64[2 * k] = make_double2(out[0], out[1]);
+    c64[2 * k + 1] = make_double2(out[2], out[3]);
+    c32[k] = make_float4(float(out[0]), float(out[1]), float(out[2]), float(out[3]));
+}
+
+void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const int* pj,
+                        const Patch64* patches, const int* qual_ptr, const Pt64* qual, const int* quad_ptr,
+                        const Pt64* quad, double kappa, double maxv, unsigned* repaired, float4* c32,
+                        double2* c64) {
+    if (n_pairs <= 0) return;
+    auto k = kappa != 0.0 ? k_pair_coeffs<true> : k_pair_coeffs<false>;
+    hipLaunchKernelGGL(k, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, s, n_pairs, pi, pj, patches,
+                       qual_ptr, qual, quad_ptr, quad, kappa, maxv, repaired, c32, c64);
+    hip_check(hipGetLastError(), "k_pair_coeffs launch");
+}
+
+// y += the listed pairs' products: one wave per row that has pairs, lanes
+// over its pairs (sorted by column), then a fixed-order shuffle sum.
+template <typename TC, typename TA>
+__global__ __launch_bounds__(256) void k_pair_spmv(int n, int n_rows, const int* __restrict__ rows,
+                                                  const int* __restrict__ ptr, const int* __restrict__ col,
+                                                  const void* __restrict__ coef, const double* __restrict__ x,
+                                                  double* __restrict__ y) {
+    const int wv = (blockIdx.x * 256 + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wv >= n_rows) return;
+    const int p0 = ptr[wv], p1 = ptr[wv + 1];
+    TA af = 0, ah = 0;
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const Coef4<TC> v = load_coef<TC>(coef, p);
+        const int j = col[p];
+        const TA xf = TA(x[j]), xh = TA(x[n + j]);
+        af += TA(v.b) * xf - TA(v.a) * xh;
+        ah += TA(v.d) * xf - TA(v.c) * xh;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        af += __shfl_down(af, off, 64);
+        ah += __shfl_down(ah, off, 64);
+    }
+    if (lane == 0) {
+        const int i = rows[wv];
+        y[i] += double(af);
+        y[n + i] += double(ah);
+    }
+}
+
+void launch_pair_spmv(hipStream_t s, int mode, int n, int n_rows, const int* rows, const int* ptr,
+                      const int* col, const float4* c32, const double2* c64, const double* x, double* y) {
+    if (n_rows <= 0) return;
+    const dim3 grid(unsigned((n_rows + 3) / 4));
+    if (mode == 0)
+        hipLaunchKernelGGL((k_pair_spmv<double, double>), grid, dim3(256), 0, s, n, n_rows, rows, ptr, col,
+                           (const void*)c64, x, y);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_pair_spmv<float, float>), grid, dim3(256), 0, s, n, n_rows, rows, ptr, col,
+                           (const void*)c32, x, y);
+    else
+        hipLaunchKernelGGL((k_pair_spmv<float, double>), grid, dim3(256), 0, s, n, n_rows, rows, ptr, col,
+                           (const void*)c32, x, y);
+    hip_check(hipGetLastError(), "k_pair_spmv launch");
+}
+
 }  // namespace beepdev

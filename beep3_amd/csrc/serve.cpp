@@ -5,6 +5,7 @@
 #include "../../include/beep_serve.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -514,6 +515,307 @@ int beep_server_destroy(beep_server* s) {
     for (auto& t : s->workers) t.join();
     delete s;  // results not taken are dropped with it
     return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Pose batches (include/beep_serve.h): one shared near field per mesh, many
+// two-instance poses on worker slots.
+
+struct beep_poses {
+    beep_pose_opts o{};
+    const beep_mesh* mesh = nullptr;
+    double Dsolvent = 80, kappa = 0, Dprotein = 2;
+    int quad = 0, qual = 0, nbsize = 1500;
+    beep_hip_ctx* src = nullptr;     // the mesh's own near field (canonical placement)
+    int N1 = 0;                      // patches of one instance
+    long long intra_pairs = 0;
+    double cutoff = 0, setup_ms = 0;
+    std::vector<double> efr, eh;     // energy coefficients of one instance (f: Gauss-renormalised)
+    std::vector<beep_hip_ctx*> slot_ctx;
+    std::vector<std::vector<double>> fh;  // f then h of every pose of the last solve (keep_fh)
+};
+
+namespace {
+
+// Explicit pairs between instance 0 (patches [0, n1)) and instance 1
+// ([n1, 2 n1)) whose centroids are closer than rc, both orders: a uniform
+// grid of cell rc over instance 1, each instance-0 patch testing its 27
+// cells.
+void pose_pairs(const beep_scene* sc, int n1, double rc, std::vector<int>& pi, std::vector<int>& pj) {
+    pi.clear();
+    pj.clear();
+    const auto& c = sc->centroid;
+    auto cell = [&](double v) { return (long long)std::floor(v / rc); };
+    auto key = [](long long x, long long y, long long z) {
+        return (uint64_t(x & 0x1FFFFF) << 42) | (uint64_t(y & 0x1FFFFF) << 21) | uint64_t(z & 0x1FFFFF);
+    };
+    std::unordered_map<uint64_t, std::vector<int>> grid;
+    grid.reserve(size_t(n1));
+    for (int j = n1; j < 2 * n1; ++j) grid[key(cell(c[j].x), cell(c[j].y), cell(c[j].z))].push_back(j);
+    const double rc2 = rc * rc;
+    for (int i = 0; i < n1; ++i) {
+        const long long cx = cell(c[i].x), cy = cell(c[i].y), cz = cell(c[i].z);
+        for (long long dz = -1; dz <= 1; ++dz)
+            for (long long dy = -1; dy <= 1; ++dy)
+                for (long long dx = -1; dx <= 1; ++dx) {
+                    const auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
+                    if (it == grid.end()) continue;
+                    for (const int j : it->second) {
+                        const double ex = c[j].x - c[i].x, ey = c[j].y - c[i].y, ez = c[j].z - c[i].z;
+                        if (ex * ex + ey * ey + ez * ez < rc2) {
+                            pi.push_back(i);
+                            pj.push_back(j);
+                            pi.push_back(j);
+                            pj.push_back(i);
+                        }
+                    }
+                }
+    }
+}
+
+// Median over (a sample of) the rows of the largest centroid distance to any
+// of the row's explicit columns: how far the mesh's own near lists reach.
+double median_reach(const beep_scene* sc) {
+    std::vector<double> reach;
+    const auto& c = sc->centroid;
+    const int L = int(sc->leaf_ptr.size()) - 1;
+    for (int l = 0; l < L; ++l)
+        for (int k = sc->leaf_ptr[l]; k < sc->leaf_ptr[l + 1]; k += 7) {
+            const int i = sc->leaf_idx[k];
+            double m = 0;
+            for (int q = sc->nb_ptr[l]; q < sc->nb_ptr[l + 1]; ++q) {
+                const int j = sc->nb_idx[q];
+                const double ex = c[j].x - c[i].x, ey = c[j].y - c[i].y, ez = c[j].z - c[i].z;
+                m = std::max(m, ex * ex + ey * ey + ez * ez);
+            }
+            reach.push_back(std::sqrt(m));
+        }
+    if (reach.empty()) throw std::runtime_error("the mesh has no explicit pairs");
+    std::nth_element(reach.begin(), reach.begin() + reach.size() / 2, reach.end());
+    return reach[reach.size() / 2];
+}
+
+void set_tree_from(beep_hip_ctx* ctx, const beep_scene* sc) {
+    beep_hip_tree t{};
+    t.n_leaves = int(sc->leaf_ptr.size()) - 1;
+    t.leaf_ptr = sc->leaf_ptr.data();
+    t.leaf_idx = sc->leaf_idx.data();
+    t.nb_ptr = sc->nb_ptr.data();
+    t.nb_idx = sc->nb_idx.data();
+    check_hip(ctx, beep_hip_set_tree(ctx, &t), "tree");
+}
+
+void pose_one(beep_poses* p, beep_hip_ctx*& ctx, int slot, const double* quat, const double* off,
+              beep_pose_result& r, std::vector<double>* fh) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    beep_scene* sc = nullptr;
+    if (beep_scene_create(p->Dsolvent, p->kappa, p->quad, p->qual, p->nbsize, &sc) != 0)
+        throw std::runtime_error(std::string("scene: ") + beep_host_last_error());
+    std::unique_ptr<beep_scene, void (*)(beep_scene*)> guard(sc, beep_scene_destroy);
+    const double origin[3] = {0.0, 0.0, 0.0};
+    if (beep_scene_add_instance(sc, p->mesh, nullptr, origin, p->Dprotein) != 0 ||
+        beep_scene_add_instance(sc, p->mesh, quat, off, p->Dprotein) != 0 ||
+        beep_scene_set_host_points(sc, 0) != 0 || beep_scene_set_tree(sc, 0) != 0 ||
+        beep_scene_build(sc) != 0)
+        throw std::runtime_error(std::string("pose scene: ") + beep_host_last_error());
+    const int n1 = p->N1, N = 2 * n1;
+    if (int(sc->centroid.size()) != N) throw std::runtime_error("pose scene size");
+    std::vector<int> pi, pj;
+    pose_pairs(sc, n1, p->cutoff, pi, pj);
+    r.n_pairs = (long long)pi.size();
+    const auto t1 = clk::now();
+    r.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (!ctx && beep_hip_create(p->o.device, &ctx) != 0)
+        throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
+    check_hip(ctx, beep_hip_set_far_field(ctx, p->o.far_mode, p->o.fmm_order, p->o.fmm_leaf), "far field");
+    upload_surface(ctx, sc);
+    const int offs[2] = {0, n1};
+    check_hip(ctx, beep_hip_share_near(ctx, p->src, 2, offs, (long long)pi.size(), pi.data(), pj.data()),
+              "shared near field");
+    check_hip(ctx, beep_hip_set_gram_schmidt(ctx, p->o.gram_schmidt), "Gram-Schmidt");
+    check_hip(ctx, beep_hip_precompute(ctx, p->kappa, p->o.solve_mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32),
+              "pair coefficients");
+    std::vector<double> x(size_t(2) * N, 0.0);
+    const int nch = int(sc->charge_inst.size());
+    beep_solve_opts so{p->o.tol, p->o.max_iter, p->o.solve_mode, p->o.max_refine, p->o.inner_tol, 0, 0};
+    beep_solve_stats st{};
+    check_hip(ctx, beep_hip_solve_from_charges(ctx, nch, nch ? sc->charges.data() : nullptr, p->Dsolvent,
+                                               &so, x.data(), &st), "solve");
+    // energy of each instance with the mesh's coefficients (pose-invariant):
+    // 1/2 sum (h eps ec_h - f ec_f) (mesh.cpp:1373-1403)
+    double E = 0;
+    const double eps = p->Dsolvent / p->Dprotein;
+    for (int k = 0; k < 2; ++k) {
+        double s = 0;
+        for (int i = 0; i < n1; ++i)
+            s += x[size_t(N) + k * n1 + i] * eps * p->eh[i] - x[size_t(k) * n1 + i] * p->efr[i];
+        E += 0.5 * s * KJ_PER_MOL;
+    }
+    r.energy = E;
+    r.iterations = st.iterations;
+    r.matvecs = st.matvecs;
+    r.refinements = st.refinements;
+    r.converged = st.converged;
+    r.rel_residual = st.rel_residual;
+    r.slot = slot;
+    r.device_ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+    if (fh) *fh = std::move(x);
+}
+
+}  // namespace
+
+extern "C" {
+
+void beep_pose_default_opts(beep_pose_opts* o) {
+    if (!o) return;
+    *o = beep_pose_opts{};
+    o->device = 0;
+    o->slots = 8;
+    o->solve_mode = BEEP_SOLVE_MIXED;
+    o->tol = 1e-6;
+    o->max_iter = 100;
+    o->max_refine = 4;
+    o->inner_tol = 0.0;
+    o->far_mode = BEEP_FAR_FMM;
+    o->fmm_order = 5;
+    o->fmm_leaf = 40;
+    o->pair_cutoff = 0.0;
+    o->gram_schmidt = 1;
+    o->keep_fh = 0;
+}
+
+int beep_poses_create(const beep_pose_opts* opts, const beep_mesh* mesh, double Dsolvent, double kappa,
+                      int quad_rule, int qual_rule, int nbsize, double Dprotein, beep_poses** out) {
+    if (!out) return -2;
+    *out = nullptr;
+    auto p = std::make_unique<beep_poses>();
+    beep_scene* sc = nullptr;
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        if (!mesh) throw std::runtime_error("null mesh");
+        if (opts) p->o = *opts;
+        else beep_pose_default_opts(&p->o);
+        if (p->o.slots < 1 || p->o.slots > 64) throw std::runtime_error("slots must be 1..64");
+        if (p->o.gram_schmidt != 1 && p->o.gram_schmidt != 2) throw std::runtime_error("gram_schmidt must be 1 or 2");
+        p->mesh = mesh;
+        p->Dsolvent = Dsolvent;
+        p->kappa = kappa;
+        p->quad = quad_rule;
+        p->qual = qual_rule;
+        p->nbsize = nbsize;
+        p->Dprotein = Dprotein;
+        // the mesh at its canonical placement, with its own octree
+        if (beep_scene_create(Dsolvent, kappa, quad_rule, qual_rule, nbsize, &sc) != 0 ||
+            beep_scene_add_instance(sc, mesh, nullptr, nullptr, Dprotein) != 0 ||
+            beep_scene_set_host_points(sc, 0) != 0 || beep_scene_build(sc) != 0)
+            throw std::runtime_error(std::string("mesh scene: ") + beep_host_last_error());
+        p->N1 = int(sc->centroid.size());
+        for (size_t l = 0; l + 1 < sc->leaf_ptr.size(); ++l)
+            p->intra_pairs += (long long)(sc->leaf_ptr[l + 1] - sc->leaf_ptr[l]) * (sc->nb_ptr[l + 1] - sc->nb_ptr[l]);
+        p->cutoff = p->o.pair_cutoff > 0 ? p->o.pair_cutoff : median_reach(sc);
+        if (beep_hip_create(p->o.device, &p->src) != 0)
+            throw std::runtime_error(std::string("device context: ") + beep_hip_last_error(nullptr));
+        // direct far field: every explicit pair stores full - 1 point, which
+        // holds whatever far field a pose then uses
+        check_hip(p->src, beep_hip_set_far_field(p->src, BEEP_FAR_DIRECT, 5, 40), "far field");
+        upload_surface(p->src, sc);
+        set_tree_from(p->src, sc);
+        check_hip(p->src, beep_hip_precompute(p->src, kappa,
+                                              p->o.solve_mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32),
+                  "mesh coefficients");
+        // energy coefficients of one instance, Gauss-renormalised
+        // (mesh.cpp:891-925)
+        const int n1 = p->N1, nch = int(sc->charge_inst.size());
+        std::vector<double> ef(n1);
+        p->eh.assign(n1, 0.0);
+        const int ipp[2] = {0, n1}, icp[2] = {0, nch};
+        check_hip(p->src, beep_hip_energy_coefficients(p->src, 1, ipp, icp, nch ? sc->charges.data() : nullptr,
+                                                       ef.data(), p->eh.data()),
+                  "energy coefficients");
+        double net = 0, total_f = 0, area = 0;
+        for (int c = 0; c < nch; ++c) net += sc->charges[4 * size_t(c) + 3];
+        for (int i = 0; i < n1; ++i) {
+            total_f -= ef[i];
+            area += sc->area[i];
+        }
+        const double e = net - total_f;
+        p->efr.resize(n1);
+        for (int i = 0; i < n1; ++i) p->efr[i] = ef[i] - sc->area[i] / area * e;
+        check_hip(p->src, beep_hip_synchronize(p->src), "set-up");
+        p->slot_ctx.assign(size_t(p->o.slots), nullptr);
+        p->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        beep_scene_destroy(sc);
+        *out = p.release();
+        return 0;
+    } catch (const std::exception& e) {
+        if (sc) beep_scene_destroy(sc);
+        if (p->src) beep_hip_destroy(p->src);
+        beep_host_set_error(e.what());
+        return -1;
+    }
+}
+
+int beep_poses_solve(beep_poses* p, int n, const double* quat4, const double* offset3,
+                     beep_pose_result* results) {
+    if (!p || n < 0 || (n > 0 && (!quat4 || !offset3 || !results))) return -2;
+    p->fh.assign(p->o.keep_fh ? size_t(n) : 0, std::vector<double>());
+    std::atomic<int> next{0};
+    auto work = [&](int slot) {
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n) break;
+            beep_pose_result& r = results[k];
+            r = beep_pose_result{};
+            try {
+                pose_one(p, p->slot_ctx[slot], slot, quat4 + 4 * size_t(k), offset3 + 3 * size_t(k), r,
+                         p->o.keep_fh ? &p->fh[k] : nullptr);
+                r.status = 0;
+                std::snprintf(r.message, sizeof(r.message), "ok");
+            } catch (const std::exception& e) {
+                r.status = -1;
+                r.slot = slot;
+                std::snprintf(r.message, sizeof(r.message), "%s", e.what());
+                if (p->slot_ctx[slot]) beep_hip_destroy(p->slot_ctx[slot]);  // not reused
+                p->slot_ctx[slot] = nullptr;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int s = 0; s < p->o.slots; ++s) th.emplace_back(work, s);
+    for (auto& t : th) t.join();
+    return 0;
+}
+
+int beep_poses_take_fh(beep_poses* p, int k, double* f, double* h) {
+    if (!p || k < 0 || size_t(k) >= p->fh.size() || p->fh[k].empty()) {
+        beep_host_set_error("no f and h kept for that pose");
+        return -1;
+    }
+    const size_t N = p->fh[k].size() / 2;
+    if (f) std::copy(p->fh[k].begin(), p->fh[k].begin() + N, f);
+    if (h) std::copy(p->fh[k].begin() + N, p->fh[k].end(), h);
+    return 0;
+}
+
+int beep_poses_info(const beep_poses* p, int* n_patches, long long* intra_pairs, double* cutoff,
+                    double* setup_ms) {
+    if (!p) return -2;
+    if (n_patches) *n_patches = p->N1;
+    if (intra_pairs) *intra_pairs = p->intra_pairs;
+    if (cutoff) *cutoff = p->cutoff;
+    if (setup_ms) *setup_ms = p->setup_ms;
+    return 0;
+}
+
+void beep_poses_destroy(beep_poses* p) {
+    if (!p) return;
+    for (auto* c : p->slot_ctx)
+        if (c) beep_hip_destroy(c);
+    if (p->src) beep_hip_destroy(p->src);
+    delete p;
 }
 
 }  // extern "C"

@@ -103,6 +103,18 @@ class Device:
         (default), from the next FP32 precompute."""
         self._check(lib().beep_hip_set_near_singular_fp64(self._h, 1 if on else 0))
 
+    def share_near(self, src: "Device", copy_offsets, pair_i=None, pair_j=None):
+        """Near field of src's coefficient sets applied to copies of its
+        patches at the given offsets, plus the listed explicit pairs."""
+        off = np.ascontiguousarray(copy_offsets, dtype=np.int32)
+        pi = np.ascontiguousarray([] if pair_i is None else pair_i, dtype=np.int32)
+        pj = np.ascontiguousarray([] if pair_j is None else pair_j, dtype=np.int32)
+        if pi.shape != pj.shape:
+            raise ValueError("pair_i and pair_j differ in length")
+        self._share_src = src  # src must outlive this context's use of it
+        self._check(lib().beep_hip_share_near(self._h, src._h, len(off), _ip(off), len(pi),
+                                              _ip(pi) if len(pi) else None, _ip(pj) if len(pj) else None))
+
     def set_gram_schmidt(self, passes: int = 1):
         """Gram-Schmidt passes per Arnoldi step: 1 (classical, the
         reference's) or 2 (CGS2)."""

@@ -195,7 +195,7 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
         t_end = max(r["t_done"] for r in res)
         return res, t_end - t_start
 
-    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="auto", fmm_order=5,
+    srv = Server(slots=slots, mode="mixed", far_field="auto", fmm_order=5,
                  fmm_leaf=40)
     res, span = run(srv, sched, paced=True)
     srv.close()
@@ -220,14 +220,14 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
            "unconverged": sum(1 for r in ok if not r["converged"])}
     print(f"[bench] serving: {out}", file=sys.stderr, flush=True)
     # capacity: the same requests submitted at once (closed loop)
-    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="auto", fmm_order=5,
+    srv = Server(slots=slots, mode="mixed", far_field="auto", fmm_order=5,
                  fmm_leaf=40)
     cres, cspan = run(srv, [dict(q, t=0.0) for q in sched], paced=True)
     srv.close()
     out["capacity_solves_per_s"] = sum(1 for r in cres if r["status"] == 0) / cspan
     print(f"[bench] serving capacity: {out['capacity_solves_per_s']:.1f} solves/s",
           file=sys.stderr, flush=True)
-    srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=40)
+    srv = Server(slots=1, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=40)
     sres, sspan = run(srv, sched[:seq_requests], paced=False)
     srv.close()
     sok = [r for r in sres if r["status"] == 0]
@@ -242,13 +242,39 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
 
 def pose_bench(n_poses, slots, seq_poses):
     """Two instances of the 40,962-patch surface (a_lm seed 7) in n_poses
-    seeded rigid poses, all submitted at once to the server (FP32 mat-vecs
-    inside FP64 refinement), then seq_poses of them as sequential plain FP64
-    solves."""
+    seeded rigid poses (FP32 mat-vecs inside FP64 refinement):
+    * the pose batch: the mesh's own near field computed once and shared by
+      both instances of every pose, inter-instance pairs per pose;
+    * the same poses through the server, each with the reference's split
+      (one octree over the pose, every explicit pair integrated per pose);
+    then seq_poses of them as sequential plain FP64 server solves."""
     from beep3_amd import configs
+    from beep3_amd.poses import PoseBatch
     from beep3_amd.serve import Server
 
     mesh = configs.pose_mesh()
+    rng = np.random.default_rng(7)  # the generator of configs.pose_scenes
+    poses = [configs.random_pose(rng) for _ in range(n_poses)]
+    t0 = time.perf_counter()
+    pb = PoseBatch(mesh, slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    setup_s = time.perf_counter() - t0
+    info = pb.info()
+    t0 = time.perf_counter()
+    bres = pb.solve(poses)
+    bspan = time.perf_counter() - t0
+    pb.close()
+    bok = [r for r in bres if r["status"] == 0]
+    batch = {"poses": n_poses, "solved": len(bok), "slots": slots, "span_s": bspan,
+             "poses_per_s": len(bok) / bspan, "setup_s_once": setup_s,
+             "poses_per_s_with_setup": len(bok) / (bspan + setup_s),
+             "mesh_patches": info["n_patches"], "intra_pairs_shared": info["intra_pairs"],
+             "pair_cutoff_A": info["cutoff"],
+             "inter_pairs_median": float(np.median([r["n_pairs"] for r in bok])),
+             "iterations_mean": float(np.mean([r["iterations"] for r in bok])),
+             "phase_ms_median": {f: float(np.median([r[f] for r in bok])) for f in ("host_ms", "device_ms")},
+             "unconverged": sum(1 for r in bok if not r["converged"]),
+             "failures": sorted({r["message"] for r in bres if r["status"] != 0})}
+    print(f"[bench] pose batch: {batch}", file=sys.stderr, flush=True)
     scenes = configs.pose_scenes(mesh, n_poses)
 
     def run(srv, scs, one_at_a_time):
@@ -267,14 +293,20 @@ def pose_bench(n_poses, slots, seq_poses):
                 res.append(r)
         return res, max(r["t_done"] for r in res) - t0
 
-    srv = Server(slots=slots, budget_gib=160.0, mode="mixed", far_field="fmm", fmm_order=5,
-                 fmm_leaf=40)
+    srv = Server(slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=40)
     res, span = run(srv, scenes, False)
     srv.close()
     ok = [r for r in res if r["status"] == 0]
+    # the two splits, pose by pose (the server returns in completion order;
+    # request ids follow submission order from 1)
+    by_id = {r["id"]: r for r in res}
+    dE = [abs(b["energy"] - by_id[k + 1]["energy"]) / abs(by_id[k + 1]["energy"])
+          for k, b in enumerate(bres) if b["status"] == 0 and by_id.get(k + 1, {}).get("status") == 0]
+    batch["energy_rel_diff_vs_reference_split"] = {"median": float(np.median(dE)), "max": float(np.max(dE))}
     lat = np.array([r["latency_s"] for r in ok]) * 1e3
     E = np.array([r["energy"] for r in ok])
-    out = {"poses": n_poses, "solved": len(ok), "patches_per_pose": ok[0]["n_patches"] if ok else 0,
+    out = {"pose_batch": batch,
+           "poses": n_poses, "solved": len(ok), "patches_per_pose": ok[0]["n_patches"] if ok else 0,
            "slots": slots, "span_s": span, "poses_per_s": len(ok) / span,
            "latency_ms_p50": float(np.percentile(lat, 50)),
            "latency_ms_p99": float(np.percentile(lat, 99)),
@@ -284,9 +316,10 @@ def pose_bench(n_poses, slots, seq_poses):
            "unconverged": sum(1 for r in ok if not r["converged"]),
            "energy_kJ_mol_range": [float(E.min()), float(E.max())],
            "failures": sorted({r["message"] for r in res if r["status"] != 0})}
-    print(f"[bench] poses: {out}", file=sys.stderr, flush=True)
+    print(f"[bench] poses through the server (reference split): "
+          f"{ {k: v for k, v in out.items() if k != 'pose_batch'} }", file=sys.stderr, flush=True)
     seq = configs.pose_scenes(mesh, seq_poses)
-    srv = Server(slots=1, budget_gib=160.0, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    srv = Server(slots=1, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=40)
     sres, sspan = run(srv, seq, True)
     srv.close()
     out["sequential_plain_fp64"] = {"poses": seq_poses, "poses_per_s": len(sres) / sspan}

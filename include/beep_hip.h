@@ -295,6 +295,26 @@ int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired);
  * D, so a test can force the guard's condition from the host. */
 int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const double* v4);
 
+/* Near field shared with another context, for the instances of a pose
+ * (several rigid copies of one mesh): the coefficient sets, diagonal and tree
+ * of src (a plain context precomputed with the direct far field, so every
+ * explicit pair stores its full rule less the 1-point interaction) apply to
+ * n_copies copies of src's patches in this context, copy k at patch offset
+ * copy_offset[k] (n_copies x src's patches = this context's patches; the
+ * integrals are invariant under the rigid motion of a copy), plus the
+ * explicit pairs (pair_i[q], pair_j[q]) of this context (patches of
+ * different copies near each other; rows and columns in any order, no
+ * repeats).  Takes the place of beep_hip_set_tree; beep_hip_precompute then
+ * computes only the listed pairs' coefficients (in FP64, kept in both
+ * precisions), for src's kappa.  The FMM leaf pass then covers every
+ * neighbouring-leaf pair.  src must outlive this use and not change
+ * meanwhile; contexts sharing one src may run concurrently.  Replaces the
+ * per-pose LocalIntegrations of the reference's pose loop (tools/
+ * run_pair.py:34-52 -> beep.cpp:223-290, 768-956), which recomputes the
+ * intra-instance pairs for every pose. */
+int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
+                        long long n_pairs, const int* pair_i, const int* pair_j);
+
 /* FP64 integration of the near-singular pairs of the FP32 coefficient set
  * (on = 1, the default) or FP32 for every pair (on = 0); applies from the
  * next FP32 precompute.  No counterpart in the reference (all FP32). */

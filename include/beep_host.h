@@ -92,6 +92,10 @@ int beep_scene_points(const beep_scene* s, int kind, int* ptr, double* pts);
  * beep_scene_points returns only the pointer arrays; the device makes the
  * points from beep_scene_surface (beep_hip_set_surface).  Default on. */
 int beep_scene_set_host_points(beep_scene* s, int on);
+/* Leave out the octree (on = 0): build() then makes the patches only, with
+ * no leaves (for a scene whose near field comes from elsewhere, as the
+ * instances of a pose batch, beep_serve.h).  Default on. */
+int beep_scene_set_tree(beep_scene* s, int on);
 /* The surface in universe coordinates: placed vertices (3 x n_patches: patch
  * p is vertex p), triangles (3 x n_triangles, vertex indices), and for every
  * vertex the triangles around it (vt_ptr: n_patches + 1, vt_tri and the

@@ -93,6 +93,79 @@ const char* beep_server_last_error(const beep_server* s);
 /* waits for every submitted request, then stops the workers */
 int beep_server_destroy(beep_server* s);
 
+/* ---- pose batches: many rigid poses of one mesh pair ----
+ *
+ * Two instances of one mesh per pose: instance 0 unrotated with its centre at
+ * the origin, instance 1 rotated about its centre and moved (the pose loop of
+ * tools/run_pair.py).  The reference solves poses one
+ * after another (tools/run_pair.py:34-52 -> beep.cpp:223-290), recomputing
+ * every explicit pair each time.  Here the mesh's own near field (its
+ * octree's explicit pairs, their coefficients and the singular self terms)
+ * is built once: these integrals do not change under a rigid motion, so one
+ * coefficient set serves both instances of every pose (beep_hip_share_near).
+ * Per pose only the pairs between the instances that are closer than
+ * pair_cutoff are integrated explicitly; every other pair of the pose takes
+ * the 1-point interaction through the far field.  Poses run concurrently on
+ * `slots` device contexts. */
+typedef struct beep_poses beep_poses;
+
+typedef struct {
+    int device;
+    int slots;              /* poses at once (worker threads, one device context each) */
+    int solve_mode;         /* BEEP_SOLVE_* (beep_hip.h) */
+    double tol;
+    int max_iter;
+    int max_refine;
+    double inner_tol;
+    int far_mode;           /* BEEP_FAR_* */
+    int fmm_order;
+    int fmm_leaf;
+    double pair_cutoff;     /* inter-instance explicit pairs closer than this (Angstrom);
+                               <= 0: the median reach of the mesh's own explicit pairs */
+    int gram_schmidt;       /* Gram-Schmidt passes per Arnoldi step (1 or 2) */
+    int keep_fh;            /* keep f and h of each pose for beep_poses_take_fh */
+} beep_pose_opts;
+
+/* device 0, 8 slots, mixed, tol 1e-6, 100 iterations, 4 refinements, inner
+ * tol / 2, FMM order 5 leaf 40, cutoff from the mesh, 1 pass, f and h not
+ * kept */
+void beep_pose_default_opts(beep_pose_opts* o);
+
+typedef struct {
+    int status;             /* 0 = solved, < 0 = failed (see message) */
+    double energy;          /* solvation energy of the pose, kJ/mol (both instances) */
+    int iterations, matvecs, refinements, converged;
+    double rel_residual;
+    long long n_pairs;      /* explicit pairs between the instances */
+    double host_ms;         /* pose scene and its inter-instance pairs */
+    double device_ms;       /* upload, pair coefficients, far-field plan, RHS, solve */
+    int slot;
+    char message[128];
+} beep_pose_result;
+
+/* The mesh's own near field for the given physics (BEEP(Dsolvent, kappa,
+ * quad, qual, nbsize), instance dielectric Dprotein), once: canonical octree,
+ * coefficients of the precisions the solve mode needs, singular terms,
+ * energy coefficients. */
+int beep_poses_create(const beep_pose_opts* opts, const beep_mesh* mesh, double Dsolvent,
+                      double kappa, int quad_rule, int qual_rule, int nbsize, double Dprotein,
+                      beep_poses** out);
+/* Solve n poses (quat4: n x (w, x, y, z) rotation of instance 1 about the
+ * mesh centre; offset3: n x 3, where its centre goes; instance 0's centre is
+ * at the origin) on the slots; blocks
+ * until every pose is done; results: n entries (a failed pose has status <
+ * 0, the others are still solved).  Returns 0 unless the call itself fails. */
+int beep_poses_solve(beep_poses* p, int n, const double* quat4, const double* offset3,
+                     beep_pose_result* results);
+/* f and h (2 N each, N = patches of the pose) of pose k of the last
+ * beep_poses_solve (keep_fh). */
+int beep_poses_take_fh(beep_poses* p, int k, double* f, double* h);
+/* Set-up: patches of one instance, its explicit pairs, the cutoff in use,
+ * and the milliseconds of the one-time set-up (host and device). */
+int beep_poses_info(const beep_poses* p, int* n_patches, long long* intra_pairs, double* cutoff,
+                    double* setup_ms);
+void beep_poses_destroy(beep_poses* p);
+
 #ifdef __cplusplus
 }
 #endif

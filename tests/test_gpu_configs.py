@@ -191,6 +191,37 @@ def test_poses_server_equals_single_solve(pose_case):
         assert abs(r["energy"] - Es) <= 1e-12 * abs(Es)
 
 
+def test_pose_batch_full_size(pose_case):
+    """The first seeded poses through the pose batch (the 40,962-patch
+    mesh's own near field computed once, inter-instance pairs per pose):
+    each pose's result is the same alone and among the others, and its
+    energy is close to the reference split's (one octree over both
+    instances), the difference being the pairs the two splits treat
+    differently."""
+    from beep3_amd.poses import PoseBatch
+
+    mesh, poses = pose_case
+    pb = PoseBatch(mesh, slots=3, keep_fh=True, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    print("pose batch, 2 x 40,962 patches:", pb.info(), flush=True)
+    alone = pb.solve(poses[:1])
+    f0, h0 = pb.take_fh(0)
+    allp = pb.solve(poses)
+    f1, h1 = pb.take_fh(0)
+    pb.close()
+    assert all(r["status"] == 0 and r["converged"] == 1 for r in alone + allp), [r["message"] for r in allp]
+    assert allp[0]["energy"] == alone[0]["energy"]
+    assert np.array_equal(f0, f1) and np.array_equal(h0, h1)
+    for k, (q, off) in enumerate(poses):
+        b = configs.pose_pair(mesh, q, off, fmm_order=5, fmm_leaf=40)
+        b.solve(1e-6, 100, mode="mixed")
+        E = b.calculate_energies()
+        b.close()
+        d = abs(allp[k]["energy"] - E) / abs(E)
+        print(f"pose {k}: batch {allp[k]['energy']:.6f} reference split {E:.6f} rel {d:.2e}, "
+              f"{allp[k]['n_pairs']} inter pairs, {allp[k]['iterations']} iterations")
+        assert d < 2e-4
+
+
 # ---------------- serving mix, 2,562 .. 163,842 patches ----------------
 
 def test_serving_mix_full_sizes_alone_and_in_mix():
