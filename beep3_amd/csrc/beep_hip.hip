@@ -150,6 +150,9 @@ struct beep_hip_ctx {
     long long n_pairs = 0;
     int n_pair_rows = 0;
     DevBuf<int> pair_rows, pair_ptr, pair_col, pair_i;
+    DevBuf<int> cut_off_d, cut_count;  // device pair search: copy offsets, per-row counts
+    DevBuf<double4> cut_sph_d;         // and the copies' bounding spheres
+    DevBuf<char> cut_tmp;              // scan storage
     DevBuf<float4> pcoef32;
     DevBuf<double2> pcoef64;
     // coefficient guard (|v| > guard_max or not finite): the list of bad
@@ -397,10 +400,12 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
         // every copy of the shared patches (rows and columns at its offset,
         // f and h halves N apart), then this context's own pairs
         const beep_hip_ctx* h = c->share;
-        for (const int off : c->share_off)
+        const int nco = int(c->share_off.size());
+        for (int k = 0; k < nco; k += 2)
             launch_near_spmv(s, mode, c->N, h->spmv, h->leaf_ptr.p, h->leaf_idx.p, h->nb_ptr.p,
-                             h->nb_idx.p, h->coef_off.p, h->coef32.p, h->coef64.p, h->diag.p, dx + off,
-                             c->spmv_part.p, dy + off, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode]);
+                             h->nb_idx.p, h->coef_off.p, h->coef32.p, h->coef64.p, h->diag.p, dx,
+                             c->spmv_part.p, dy, c->spmv_cap[mode] * c->n_cu, c->spmv_depth[mode],
+                             std::min(2, nco - k), c->share_off.data() + k);
         launch_pair_spmv(s, mode, c->N, c->n_pair_rows, c->pair_rows.p, c->pair_ptr.p, c->pair_col.p,
                          c->pcoef32.p, c->pcoef64.p, dx, dy);
     } else {
@@ -1070,27 +1075,61 @@ int beep_hip_set_overlap(beep_hip_ctx* ctx, int on) {
     });
 }
 
+namespace {
+
+// Checks and state common to both ways of sharing a near field: src, the
+// copies, the patch order of the direct far field and the SpMV partials.
+void share_common(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset) {
+    require(ctx->have_geom, "set the geometry first");
+    require(src && src != ctx && !src->share, "the shared near field must come from another plain context");
+    require(src->have_geom && src->have_tree && src->have_diag && (src->have32 || src->have64),
+            "the shared context must have its geometry, tree and coefficients");
+    require(src->uid32 == 0 && src->uid64 == 0,
+            "the shared coefficients must be precomputed with the direct far field");
+    require(n_copies >= 1 && copy_offset, "no copies");
+    const int N = ctx->N, Ns = src->N;
+    require(size_t(n_copies) * Ns == size_t(N), "the copies must cover every patch");
+    std::vector<char> cover(N, 0);
+    for (int k = 0; k < n_copies; ++k) {
+        require(copy_offset[k] >= 0 && copy_offset[k] + Ns <= N, "copy offset out of range");
+        for (int i = copy_offset[k]; i < copy_offset[k] + Ns; ++i) {
+            require(!cover[i], "copies overlap");
+            cover[i] = 1;
+        }
+    }
+    set_device(ctx);
+    sync_streams(ctx);
+    hipStream_t s = ctx->stream;
+    ctx->gen++;
+    ctx->have_tree = false;
+    ctx->have32 = ctx->have64 = ctx->have_diag = false;
+    ctx->share = src;
+    ctx->ns_listed = false;
+    ctx->share_off.assign(copy_offset, copy_offset + n_copies);
+    // the patch order of the direct far field's source tiles (leaf_idx of a
+    // plain context): each copy in the shared context's leaf order (copied on
+    // this context's stream: a copy on the legacy stream would break the
+    // graph capture of any other context's stream meanwhile)
+    std::vector<int> src_order(Ns), order;
+    order.reserve(N);
+    hip_check(hipMemcpyAsync(src_order.data(), src->leaf_idx.p, sizeof(int) * size_t(Ns),
+                             hipMemcpyDeviceToHost, s), "D2H shared leaf order");
+    hip_check(hipStreamSynchronize(s), "sync shared leaf order");
+    for (int k = 0; k < n_copies; ++k)
+        for (int q = 0; q < Ns; ++q) order.push_back(src_order[q] + copy_offset[k]);
+    ctx->leaf_idx.upload(order.data(), order.size(), s);
+    ctx->spmv_part.alloc(size_t(2) * src->spmv.n_work * 256 * 2);  // two copies per pass
+    ctx->guard_rec.alloc(beep_hip_ctx::GUARD_CAP);
+    hip_check(hipStreamSynchronize(s), "sync share");  // host arrays die here
+}
+
+}  // namespace
+
 int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
                         long long n_pairs, const int* pair_i, const int* pair_j) {
     return guarded(ctx, [&] {
-        require(ctx->have_geom, "set the geometry first");
-        require(src && src != ctx && !src->share, "the shared near field must come from another plain context");
-        require(src->have_geom && src->have_tree && src->have_diag && (src->have32 || src->have64),
-                "the shared context must have its geometry, tree and coefficients");
-        require(src->uid32 == 0 && src->uid64 == 0,
-                "the shared coefficients must be precomputed with the direct far field");
-        require(n_copies >= 1 && copy_offset, "no copies");
         require(n_pairs >= 0 && (n_pairs == 0 || (pair_i && pair_j)), "bad pair list");
-        const int N = ctx->N, Ns = src->N;
-        require(size_t(n_copies) * Ns == size_t(N), "the copies must cover every patch");
-        std::vector<char> cover(N, 0);
-        for (int k = 0; k < n_copies; ++k) {
-            require(copy_offset[k] >= 0 && copy_offset[k] + Ns <= N, "copy offset out of range");
-            for (int i = copy_offset[k]; i < copy_offset[k] + Ns; ++i) {
-                require(!cover[i], "copies overlap");
-                cover[i] = 1;
-            }
-        }
+        const int N = ctx->N;
         // pairs sorted by row, then column; CSR over the rows that have any
         std::vector<std::pair<int, int>> pr(static_cast<size_t>(n_pairs));
         for (long long q = 0; q < n_pairs; ++q) {
@@ -1110,26 +1149,10 @@ int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, cons
             col[q] = pr[q].second;
         }
         if (!pr.empty()) ptr.push_back(int(pr.size()));
-        set_device(ctx);
-        sync_streams(ctx);
+        share_common(ctx, src, n_copies, copy_offset);
         hipStream_t s = ctx->stream;
-        ctx->gen++;
-        ctx->have_tree = false;
-        ctx->have32 = ctx->have64 = ctx->have_diag = false;
-        ctx->share = src;
-        ctx->ns_listed = false;
-        ctx->share_off.assign(copy_offset, copy_offset + n_copies);
         ctx->n_pairs = n_pairs;
         ctx->n_pair_rows = int(rows.size());
-        // the patch order of the direct far field's source tiles (leaf_idx
-        // of a plain context): each copy in the shared context's leaf order
-        std::vector<int> src_order(Ns), order;
-        order.reserve(N);
-        hip_check(hipMemcpy(src_order.data(), src->leaf_idx.p, sizeof(int) * size_t(Ns), hipMemcpyDeviceToHost),
-                  "D2H shared leaf order");
-        for (int k = 0; k < n_copies; ++k)
-            for (int q = 0; q < Ns; ++q) order.push_back(src_order[q] + copy_offset[k]);
-        ctx->leaf_idx.upload(order.data(), order.size(), s);
         const size_t np = std::max<size_t>(pr.size(), 1);
         ctx->pair_rows.upload(rows.data(), rows.size(), s);
         ctx->pair_ptr.upload(ptr.data(), ptr.size(), s);
@@ -1137,10 +1160,66 @@ int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, cons
         ctx->pair_i.upload(pi.data(), pi.size(), s);
         ctx->pcoef32.alloc(np);
         ctx->pcoef64.alloc(2 * np);
-        ctx->spmv_part.alloc(size_t(src->spmv.n_work) * 256 * 2);
-        ctx->guard_rec.alloc(beep_hip_ctx::GUARD_CAP);
-        hip_check(hipStreamSynchronize(s), "sync share");  // host arrays die here
+        hip_check(hipStreamSynchronize(s), "sync pairs");  // host arrays die here
         // no residual leaf pass: the explicit pairs store full - 1 point
+        far_setup(ctx);
+    });
+}
+
+int beep_hip_share_near_cutoff(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
+                               double cutoff, long long* n_pairs) {
+    return guarded(ctx, [&] {
+        require(cutoff > 0, "the cutoff must be positive");
+        share_common(ctx, src, n_copies, copy_offset);
+        const int N = ctx->N, Ns = src->N;
+        hipStream_t s = ctx->stream;
+        // a bounding sphere per copy (host centroids)
+        std::vector<double4> sph(n_copies);
+        for (int k = 0; k < n_copies; ++k) {
+            double cx = 0, cy = 0, cz = 0;
+            const double* c = ctx->h_centroid.data() + 3 * size_t(copy_offset[k]);
+            for (int i = 0; i < Ns; ++i) {
+                cx += c[3 * i];
+                cy += c[3 * i + 1];
+                cz += c[3 * i + 2];
+            }
+            cx /= Ns;
+            cy /= Ns;
+            cz /= Ns;
+            double r2 = 0;
+            for (int i = 0; i < Ns; ++i) {
+                const double dx = c[3 * i] - cx, dy = c[3 * i + 1] - cy, dz = c[3 * i + 2] - cz;
+                r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+            }
+            sph[k] = make_double4(cx, cy, cz, std::sqrt(r2) * (1.0 + 1e-12));
+        }
+        ctx->cut_off_d.upload(copy_offset, size_t(n_copies), s);
+        ctx->cut_sph_d.upload(sph.data(), sph.size(), s);
+        ctx->cut_count.alloc(size_t(N) + 1);
+        ctx->pair_ptr.alloc(size_t(N) + 1);
+        launch_set_int(s, ctx->cut_count.p + N, 0);
+        launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->patches.p, cutoff, nullptr,
+                         ctx->cut_count.p, nullptr, nullptr);
+        const size_t tb = exclusive_scan_temp_bytes(N + 1);
+        ctx->cut_tmp.alloc(std::max<size_t>(tb, 1));
+        launch_exclusive_scan(s, ctx->cut_tmp.p, tb, ctx->cut_count.p, ctx->pair_ptr.p, N + 1);
+        int total = 0;
+        hip_check(hipMemcpyAsync(&total, ctx->pair_ptr.p + N, sizeof(int), hipMemcpyDeviceToHost, s), "D2H pairs");
+        hip_check(hipStreamSynchronize(s), "sync pair count");
+        require(total >= 0, "too many pairs for 32-bit offsets");
+        const size_t np = std::max<size_t>(size_t(total), 1);
+        ctx->pair_col.alloc(np);
+        ctx->pair_i.alloc(np);
+        ctx->pcoef32.alloc(np);
+        ctx->pcoef64.alloc(2 * np);
+        ctx->pair_rows.alloc(size_t(N));
+        if (total > 0)
+            launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->patches.p, cutoff,
+                             ctx->pair_ptr.p, nullptr, ctx->pair_col.p, ctx->pair_i.p);
+        launch_iota(s, N, ctx->pair_rows.p);
+        ctx->n_pairs = total;
+        ctx->n_pair_rows = total > 0 ? N : 0;
+        if (n_pairs) *n_pairs = total;
         far_setup(ctx);
     });
 }
@@ -1842,6 +1921,7 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         add(ctx->bvec); add(ctx->rvec); add(ctx->dvec);
         add(ctx->pair_rows); add(ctx->pair_ptr); add(ctx->pair_col); add(ctx->pair_i);
         add(ctx->pcoef32); add(ctx->pcoef64); add(ctx->ns_rec); add(ctx->guard_rec); add(ctx->hvec2);
+        add(ctx->cut_off_d); add(ctx->cut_count); add(ctx->cut_sph_d); add(ctx->cut_tmp);
         *bytes = b + ctx->fmm_arena.held;
     });
 }

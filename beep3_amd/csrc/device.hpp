@@ -190,11 +190,14 @@ struct SpmvPlan {
 // n_work * 256 * 2 doubles.  max_blocks > 0 caps the grid (blocks then walk
 // the work items with a grid stride); depth (1, 2, 4) scales the coefficient
 // loads each thread keeps in flight.
+// ncopies (1 or 2) copies of the near field at patch offsets copy_offset
+// (null: 0) share one pass over the coefficients; part then holds ncopies x
+// n_work * 256 * 2 values.
 void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
                       const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
                       const double* diag4, const double* x, void* part, double* y,
-                      int max_blocks, int depth);
+                      int max_blocks, int depth, int ncopies = 1, const int* copy_offset = nullptr);
 
 // explicit pairs listed one by one: coefficients (FP64, rounded to FP32 as
 // well; bad values zeroed and counted), and y += their products over the
@@ -205,6 +208,17 @@ void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const i
                         double2* c64);
 void launch_pair_spmv(hipStream_t s, int mode, int n, int n_rows, const int* rows, const int* ptr,
                       const int* col, const float4* c32, const double2* c64, const double* x, double* y);
+// pairs between copies (contiguous ranges of ns patches at copy_off) closer
+// than rc; sphere: per copy (centre xyz, radius).  ptr null: count[i] =
+// pairs of row i; else their columns at cols[ptr[i]..] (index order) and
+// rows_of[] = i
+void launch_cut_pairs(hipStream_t s, int N, int ncopies, int ns, const int* copy_off, const double4* sphere,
+                      const Patch64* patches, double rc, const int* ptr, int* count, int* cols, int* rows_of);
+void launch_iota(hipStream_t s, int n, int* out);
+// exclusive prefix sum of n ints (temporary storage of the size the first
+// function gives)
+size_t exclusive_scan_temp_bytes(int n);
+void launch_exclusive_scan(hipStream_t s, void* tmp, size_t tmp_bytes, const int* in, int* out, int n);
 
 // far field: pack sources from x, then y += full 1-point sum over all pairs
 void launch_far_pack(hipStream_t s, int n, const Patch64* patches, const int* order,

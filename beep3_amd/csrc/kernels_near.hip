@@ -14,6 +14,8 @@
 // 3x3 loops over gdd - udd.
 #include <algorithm>
 
+#include <hipcub/hipcub.hpp>
+
 #include "device.hpp"
 
 namespace beepdev {
@@ -499,14 +501,22 @@ __device__ __forceinline__ Coef4<TC> load_coef(const void* __restrict__ coef, in
     }
 }
 
-template <typename TC, typename TA, int U>
+// NC copies of the same near field (instances of one mesh in a pose): copy q
+// reads x and writes y at patch offset xoff[q] (f and h halves still n
+// apart), so the coefficients are read once for all copies.  NC = 1 is the
+// plain mat-vec.  Partials of copy q: part + q * n_work * 512.
+struct CopyOffsets {
+    int off[2];
+};
+
+template <typename TC, typename TA, int U, int NC>
 __global__ __launch_bounds__(256) void k_near_spmv_part(
     const int4* __restrict__ work, int n_work, const int* __restrict__ leaf_ptr,
     const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx,
     const int64_t* __restrict__ coef_off, const void* __restrict__ coef,
-    const double* __restrict__ x, int n, TA* __restrict__ part) {
-    __shared__ TA xs[2][SPMV_CW];
-    __shared__ TA red[2][256];
+    const double* __restrict__ x, CopyOffsets co, int n, TA* __restrict__ part) {
+    __shared__ TA xs[NC][2][SPMV_CW];
+    __shared__ TA red[NC][2][256];
     // with fewer blocks than work items (a residency cap that leaves room on
     // every CU for the far-field kernels of the other stream) a block walks
     // the work items with a grid stride
@@ -523,11 +533,16 @@ __global__ __launch_bounds__(256) void k_near_spmv_part(
     const bool active = t < Rb * S;
     for (int k = t; k < nc; k += 256) {
         const int j = nb_idx[np + col0 + k];
-        xs[0][k] = TA(x[j]);
-        xs[1][k] = TA(x[n + j]);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            xs[q][0][k] = TA(x[co.off[q] + j]);
+            xs[q][1][k] = TA(x[co.off[q] + n + j]);
+        }
     }
     __syncthreads();
-    TA af = 0, ah = 0;
+    TA af[NC], ah[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) af[q] = ah[q] = TA(0);
     if (active) {
         const int64_t base = coef_off[L] + row0 + r + int64_t(col0) * R;
         int c = s;
@@ -541,53 +556,70 @@ __global__ __launch_bounds__(256) void k_near_spmv_part(
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const TA xf = xs[0][c + u * S], xh = xs[1][c + u * S];
-                af += TA(v[u].b) * xf - TA(v[u].a) * xh;
-                ah += TA(v[u].d) * xf - TA(v[u].c) * xh;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const TA xf = xs[q][0][c + u * S], xh = xs[q][1][c + u * S];
+                    af[q] += TA(v[u].b) * xf - TA(v[u].a) * xh;
+                    ah[q] += TA(v[u].d) * xf - TA(v[u].c) * xh;
+                }
             }
         }
         for (; c < nc; c += S) {
             const Coef4<TC> v = load_coef<TC>(coef, base + int64_t(c) * R);
-            const TA xf = xs[0][c], xh = xs[1][c];
-            af += TA(v.b) * xf - TA(v.a) * xh;
-            ah += TA(v.d) * xf - TA(v.c) * xh;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const TA xf = xs[q][0][c], xh = xs[q][1][c];
+                af[q] += TA(v.b) * xf - TA(v.a) * xh;
+                ah[q] += TA(v.d) * xf - TA(v.c) * xh;
+            }
         }
     }
-    red[0][t] = af;
-    red[1][t] = ah;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        red[q][0][t] = af[q];
+        red[q][1][t] = ah[q];
+    }
     __syncthreads();
     if (active && s == 0) {
-        for (int k = 1; k < S; ++k) {
-            af += red[0][k * Rb + r];
-            ah += red[1][k * Rb + r];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            TA f = af[q], h = ah[q];
+            for (int k = 1; k < S; ++k) {
+                f += red[q][0][k * Rb + r];
+                h += red[q][1][k * Rb + r];
+            }
+            TA* pq = part + size_t(q) * n_work * 512;
+            pq[2 * (size_t(wi) * 256 + r)] = f;
+            pq[2 * (size_t(wi) * 256 + r) + 1] = h;
         }
-        part[2 * (size_t(wi) * 256 + r)] = af;
-        part[2 * (size_t(wi) * 256 + r) + 1] = ah;
     }
     }
 }
 
+// grid (row groups, copies)
 template <typename TA>
 __global__ __launch_bounds__(256) void k_near_spmv_sum(
     const int4* __restrict__ groups, const int* __restrict__ group_nblk,
     const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_idx,
-    const double* __restrict__ diag4, const double* __restrict__ x, int n,
+    const double* __restrict__ diag4, const double* __restrict__ x, CopyOffsets co, int n, int n_work,
     const TA* __restrict__ part, double* __restrict__ y) {
     const int4 g = groups[blockIdx.x];  // leaf, first row, rows, first work block
     const int nb = group_nblk[blockIdx.x];
     const int r = threadIdx.x;
     if (r >= g.z) return;
+    const int cq = blockIdx.y, off = co.off[cq];
+    const TA* pq = part + size_t(cq) * n_work * 512;
     TA af = 0, ah = 0;
     for (int k = 0; k < nb; ++k) {
         const size_t q = 2 * (size_t(g.w + k) * 256 + r);
-        af += part[q];
-        ah += part[q + 1];
+        af += pq[q];
+        ah += pq[q + 1];
     }
     const int i = leaf_idx[leaf_ptr[g.x] + g.y + r];
     const double* d = diag4 + 4 * size_t(i);
-    const double xf = x[i], xh = x[n + i];
-    y[i] = double(af) + (d[1] * xf - d[0] * xh);
-    y[n + i] = double(ah) + (d[3] * xf - d[2] * xh);
+    const double xf = x[off + i], xh = x[off + n + i];
+    y[off + i] = double(af) + (d[1] * xf - d[0] * xh);
+    y[off + n + i] = double(ah) + (d[3] * xf - d[2] * xh);
 }
 
 // Loads in flight per thread: U = 4 * depth (FP64 coefficients) or 8 * depth
@@ -596,44 +628,53 @@ __global__ __launch_bounds__(256) void k_near_spmv_sum(
 template <typename TC, typename TA, int U>
 static void spmv_part(hipStream_t s, int grid, const SpmvPlan& p, const int* leaf_ptr,
                       const int* nb_ptr, const int* nb_idx, const int64_t* coef_off,
-                      const void* coef, const double* x, int n, TA* part) {
-    hipLaunchKernelGGL((k_near_spmv_part<TC, TA, U>), dim3(grid), dim3(256), 0, s, p.work,
-                       p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off, coef, x, n, part);
+                      const void* coef, const double* x, int ncopies, CopyOffsets co, int n,
+                      TA* part) {
+    if (ncopies == 2)
+        hipLaunchKernelGGL((k_near_spmv_part<TC, TA, U, 2>), dim3(grid), dim3(256), 0, s, p.work,
+                           p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off, coef, x, co, n, part);
+    else
+        hipLaunchKernelGGL((k_near_spmv_part<TC, TA, U, 1>), dim3(grid), dim3(256), 0, s, p.work,
+                           p.n_work, leaf_ptr, nb_ptr, nb_idx, coef_off, coef, x, co, n, part);
 }
 
 void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
                       const int* leaf_idx, const int* nb_ptr, const int* nb_idx,
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
                       const double* diag4, const double* x, void* part, double* y,
-                      int max_blocks, int depth) {
+                      int max_blocks, int depth, int ncopies, const int* copy_offset) {
     if (p.n_work <= 0) return;
     if (depth != 1 && depth != 2 && depth != 4) throw std::runtime_error("SpMV depth must be 1, 2 or 4");
+    if (ncopies != 1 && ncopies != 2) throw std::runtime_error("the SpMV takes 1 or 2 copies");
+    CopyOffsets co{{0, 0}};
+    for (int q = 0; q < ncopies; ++q) co.off[q] = copy_offset ? copy_offset[q] : 0;
     const int grid = max_blocks > 0 ? std::min(p.n_work, max_blocks) : p.n_work;
     const int d = depth == 1 ? 0 : (depth == 2 ? 1 : 2);
+    const dim3 sgrid(unsigned(p.n_groups), unsigned(ncopies));
     if (mode == 0) {
         using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
-                           const int64_t*, const void*, const double*, int, double*);
+                           const int64_t*, const void*, const double*, int, CopyOffsets, int, double*);
         constexpr F f[3] = {spmv_part<double, double, 4>, spmv_part<double, double, 8>,
                             spmv_part<double, double, 16>};
-        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef64, x, n, (double*)part);
-        hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
-                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef64, x, ncopies, co, n, (double*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<double>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const double*)part, y);
     } else if (mode == 1) {
         using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
-                           const int64_t*, const void*, const double*, int, float*);
+                           const int64_t*, const void*, const double*, int, CopyOffsets, int, float*);
         constexpr F f[3] = {spmv_part<float, float, 8>, spmv_part<float, float, 16>,
                             spmv_part<float, float, 32>};
-        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, n, (float*)part);
-        hipLaunchKernelGGL((k_near_spmv_sum<float>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
-                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const float*)part, y);
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, ncopies, co, n, (float*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<float>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const float*)part, y);
     } else {
         using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
-                           const int64_t*, const void*, const double*, int, double*);
+                           const int64_t*, const void*, const double*, int, CopyOffsets, int, double*);
         constexpr F f[3] = {spmv_part<float, double, 8>, spmv_part<float, double, 16>,
                             spmv_part<float, double, 32>};
-        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, n, (double*)part);
-        hipLaunchKernelGGL((k_near_spmv_sum<double>), dim3(p.n_groups), dim3(256), 0, s, p.groups,
-                           p.group_nblk, leaf_ptr, leaf_idx, diag4, x, n, (const double*)part, y);
+        f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, ncopies, co, n, (double*)part);
+        hipLaunchKernelGGL((k_near_spmv_sum<double>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const double*)part, y);
     }
     hip_check(hipGetLastError(), "k_near_spmv launch");
 }
@@ -682,6 +723,77 @@ void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const i
     hipLaunchKernelGGL(k, dim3(unsigned((n_pairs + 255) / 256)), dim3(256), 0, s, n_pairs, pi, pj, patches,
                        qual_ptr, qual, quad_ptr, quad, kappa, maxv, repaired, c32, c64);
     hip_check(hipGetLastError(), "k_pair_coeffs launch");
+}
+
+// Pairs between copies closer than rc (the inter-instance pairs of a pose),
+// found on the device: one thread per row i, testing every patch of every
+// other copy whose bounding sphere (centre, radius) comes within rc of
+// patch i, in index order.  Pass 1 (cols null) counts per row; pass 2 writes
+// each row's columns at ptr[i] and the row of every pair.
+__global__ __launch_bounds__(256) void k_cut_pairs(int N, int ncopies, int ns, const int* __restrict__ copy_off,
+                                                  const double4* __restrict__ sphere,
+                                                  const Patch64* __restrict__ patches, double rc,
+                                                  const int* __restrict__ ptr, int* __restrict__ count,
+                                                  int* __restrict__ cols, int* __restrict__ rows_of) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const Patch64 pi = patches[i];
+    const double rc2 = rc * rc;
+    int own = 0;
+    for (int q = 0; q < ncopies; ++q)
+        if (i >= copy_off[q] && i < copy_off[q] + ns) own = q;
+    int n = 0, o = cols ? ptr[i] : 0;
+    for (int q = 0; q < ncopies; ++q) {
+        if (q == own) continue;
+        const double4 sp = sphere[q];
+        const double sx = pi.cx - sp.x, sy = pi.cy - sp.y, sz = pi.cz - sp.z;
+        const double reach = sp.w + rc;
+        if (sx * sx + sy * sy + sz * sz > reach * reach) continue;
+        const int j0 = copy_off[q];
+        for (int j = j0; j < j0 + ns; ++j) {
+            const Patch64 pj = patches[j];
+            const double dx = pj.cx - pi.cx, dy = pj.cy - pi.cy, dz = pj.cz - pi.cz;
+            if (dx * dx + dy * dy + dz * dz < rc2) {
+                if (cols) {
+                    cols[o + n] = j;
+                    rows_of[o + n] = i;
+                }
+                ++n;
+            }
+        }
+    }
+    if (!cols) count[i] = n;
+}
+
+__global__ void k_iota(int n, int* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
+}
+
+void launch_cut_pairs(hipStream_t s, int N, int ncopies, int ns, const int* copy_off, const double4* sphere,
+                      const Patch64* patches, double rc, const int* ptr, int* count, int* cols, int* rows_of) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_cut_pairs, dim3((N + 255) / 256), dim3(256), 0, s, N, ncopies, ns, copy_off, sphere,
+                       patches, rc, ptr, count, cols, rows_of);
+    hip_check(hipGetLastError(), "k_cut_pairs launch");
+}
+
+size_t exclusive_scan_temp_bytes(int n) {
+    size_t b = 0;
+    hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, b, static_cast<const int*>(nullptr),
+                                               static_cast<int*>(nullptr), n),
+              "hipcub scan size");
+    return b;
+}
+
+void launch_exclusive_scan(hipStream_t s, void* tmp, size_t tmp_bytes, const int* in, int* out, int n) {
+    hip_check(hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, n, s), "hipcub scan");
+}
+
+void launch_iota(hipStream_t s, int n, int* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, n, out);
+    hip_check(hipGetLastError(), "k_iota launch");
 }
 
 // y += the listed pairs' products: one wave per row that has pairs, lanes

@@ -539,42 +539,6 @@ struct beep_poses {
 
 namespace {
 
-// Explicit pairs between instance 0 (patches [0, n1)) and instance 1
-// ([n1, 2 n1)) whose centroids are closer than rc, both orders: a uniform
-// grid of cell rc over instance 1, each instance-0 patch testing its 27
-// cells.
-void pose_pairs(const beep_scene* sc, int n1, double rc, std::vector<int>& pi, std::vector<int>& pj) {
-    pi.clear();
-    pj.clear();
-    const auto& c = sc->centroid;
-    auto cell = [&](double v) { return (long long)std::floor(v / rc); };
-    auto key = [](long long x, long long y, long long z) {
-        return (uint64_t(x & 0x1FFFFF) << 42) | (uint64_t(y & 0x1FFFFF) << 21) | uint64_t(z & 0x1FFFFF);
-    };
-    std::unordered_map<uint64_t, std::vector<int>> grid;
-    grid.reserve(size_t(n1));
-    for (int j = n1; j < 2 * n1; ++j) grid[key(cell(c[j].x), cell(c[j].y), cell(c[j].z))].push_back(j);
-    const double rc2 = rc * rc;
-    for (int i = 0; i < n1; ++i) {
-        const long long cx = cell(c[i].x), cy = cell(c[i].y), cz = cell(c[i].z);
-        for (long long dz = -1; dz <= 1; ++dz)
-            for (long long dy = -1; dy <= 1; ++dy)
-                for (long long dx = -1; dx <= 1; ++dx) {
-                    const auto it = grid.find(key(cx + dx, cy + dy, cz + dz));
-                    if (it == grid.end()) continue;
-                    for (const int j : it->second) {
-                        const double ex = c[j].x - c[i].x, ey = c[j].y - c[i].y, ez = c[j].z - c[i].z;
-                        if (ex * ex + ey * ey + ez * ez < rc2) {
-                            pi.push_back(i);
-                            pj.push_back(j);
-                            pi.push_back(j);
-                            pj.push_back(i);
-                        }
-                    }
-                }
-    }
-}
-
 // Median over (a sample of) the rows of the largest centroid distance to any
 // of the row's explicit columns: how far the mesh's own near lists reach.
 double median_reach(const beep_scene* sc) {
@@ -623,9 +587,6 @@ void pose_one(beep_poses* p, beep_hip_ctx*& ctx, int slot, const double* quat, c
         throw std::runtime_error(std::string("pose scene: ") + beep_host_last_error());
     const int n1 = p->N1, N = 2 * n1;
     if (int(sc->centroid.size()) != N) throw std::runtime_error("pose scene size");
-    std::vector<int> pi, pj;
-    pose_pairs(sc, n1, p->cutoff, pi, pj);
-    r.n_pairs = (long long)pi.size();
     const auto t1 = clk::now();
     r.host_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     if (!ctx && beep_hip_create(p->o.device, &ctx) != 0)
@@ -633,8 +594,7 @@ void pose_one(beep_poses* p, beep_hip_ctx*& ctx, int slot, const double* quat, c
     check_hip(ctx, beep_hip_set_far_field(ctx, p->o.far_mode, p->o.fmm_order, p->o.fmm_leaf), "far field");
     upload_surface(ctx, sc);
     const int offs[2] = {0, n1};
-    check_hip(ctx, beep_hip_share_near(ctx, p->src, 2, offs, (long long)pi.size(), pi.data(), pj.data()),
-              "shared near field");
+    check_hip(ctx, beep_hip_share_near_cutoff(ctx, p->src, 2, offs, p->cutoff, &r.n_pairs), "shared near field");
     check_hip(ctx, beep_hip_set_gram_schmidt(ctx, p->o.gram_schmidt), "Gram-Schmidt");
     check_hip(ctx, beep_hip_precompute(ctx, p->kappa, p->o.solve_mode == BEEP_SOLVE_FP64 ? BEEP_FP64 : BEEP_FP32),
               "pair coefficients");

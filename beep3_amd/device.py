@@ -115,6 +115,16 @@ class Device:
         self._check(lib().beep_hip_share_near(self._h, src._h, len(off), _ip(off), len(pi),
                                               _ip(pi) if len(pi) else None, _ip(pj) if len(pj) else None))
 
+    def share_near_cutoff(self, src: "Device", copy_offsets, cutoff: float) -> int:
+        """As share_near, with the pairs between copies closer than cutoff
+        found on the device; returns their count."""
+        off = np.ascontiguousarray(copy_offsets, dtype=np.int32)
+        n = C.c_longlong()
+        self._share_src = src
+        self._check(lib().beep_hip_share_near_cutoff(self._h, src._h, len(off), _ip(off), float(cutoff),
+                                                     C.byref(n)))
+        return n.value
+
     def set_gram_schmidt(self, passes: int = 1):
         """Gram-Schmidt passes per Arnoldi step: 1 (classical, the
         reference's) or 2 (CGS2)."""

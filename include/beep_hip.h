@@ -314,6 +314,12 @@ int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const dou
  * intra-instance pairs for every pose. */
 int beep_hip_share_near(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
                         long long n_pairs, const int* pair_i, const int* pair_j);
+/* The same, with the explicit pairs between copies found on the device:
+ * every pair of patches of different copies whose centroids are closer than
+ * cutoff (Angstrom), both orders; *n_pairs (may be NULL) receives their
+ * count. */
+int beep_hip_share_near_cutoff(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset,
+                               double cutoff, long long* n_pairs);
 
 /* FP64 integration of the near-singular pairs of the FP32 coefficient set
  * (on = 1, the default) or FP32 for every pair (on = 0); applies from the

@@ -104,8 +104,20 @@ def test_shared_near_field_vs_oracle(small_mesh):
     ref = orc.matvec(x, rows=rows)
     errs = {p: rel(np.vstack([y[rows], y[2 * n1 + rows]]), ref)
             for p, y in ((p, d.matvec(x, p)) for p in ("fp64", "fp32", "fp32_resid"))}
+    # the same pairs found on the device (beep_hip_share_near_cutoff): the
+    # same operator to the bit
+    e = Device(0)
+    e.set_far_field("direct")
+    e.set_geometry(a2)
+    n_dev = e.share_near_cutoff(src, [0, n1], 3.0)
+    e.precompute(0.125, "fp64")
+    e.precompute(0.125, "fp32")
+    same = all(np.array_equal(d.matvec(x, p), e.matvec(x, p)) for p in ("fp64", "fp32", "fp32_resid"))
+    e.close()
     d.close()
     src.close()
+    print("pairs found on the device:", n_dev, "host:", len(pi), "same mat-vecs:", same)
+    assert n_dev == len(pi) and same
     print("shared near field vs oracle (sampled rows):", errs)
     assert errs["fp64"] < 1e-12 and errs["fp32"] < 2e-6 and errs["fp32_resid"] < 2e-7
 
