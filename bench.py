@@ -88,13 +88,15 @@ def near_bytes(a, coef_bytes):
     return algorithmic, compulsory, nnz
 
 
-# HBM bytes per FP64 near-SpMV launch set (k_near_spmv_part<double,double,8>
-# uncapped + k_near_spmv_sum<double>; the depth-1 variant reads the same) on
-# the 163,842-patch globular surface, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (profiles/near_spmv_globular_163842_pmc.md): FETCH_SIZE x 2.0 and
-# WRITE_SIZE x 1.0, the factors that make the counters read 1 GiB for a 1 GiB
-# device copy.
-MEASURED_TRAFFIC = {("globular", "spmv", "fp64"): 8.236e9}
+# HBM bytes per launch on the 163,842-patch globular surface, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (profiles/matvec_globular_163842_fmm5_pmc.md):
+# FETCH_SIZE x 2.0 and WRITE_SIZE x 1.0, the factors that make the counters
+# read 1 GiB for a 1 GiB device copy.  "spmv": the FP64 near-SpMV launch set
+# (k_near_spmv_part<double,double,*> + k_near_spmv_sum<double>; the depth
+# variants read the same); "fmm": one whole FP64 mat-vec with the Chebyshev
+# far field (SpMV, far field and the residual leaf pass).
+MEASURED_TRAFFIC = {("globular", "spmv", "fp64"): 8.309e9,
+                    ("globular", "fmm", "fp64"): 9.390e9}
 
 
 def cpu_model():
