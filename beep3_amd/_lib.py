@@ -186,6 +186,7 @@ def _declare_hip(L):
     L.beep_hip_set_overlap.argtypes = [vp, C.c_int]
     L.beep_hip_set_graphs.argtypes = [vp, C.c_int]
     L.beep_hip_set_near_singular_fp64.argtypes = [vp, C.c_int]
+    L.beep_hip_set_near_mode.argtypes = [vp, C.c_int]
     L.beep_hip_set_guard.argtypes = [vp, C.c_double]
     L.beep_hip_share_near.argtypes = [vp, vp, C.c_int, ip, C.c_longlong, ip, ip]
     L.beep_hip_share_near_cutoff.argtypes = [vp, vp, C.c_int, ip, C.c_double, C.POINTER(C.c_longlong)]

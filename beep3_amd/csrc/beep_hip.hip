@@ -124,6 +124,14 @@ struct beep_hip_ctx {
     DevBuf<int> spmv_group_nblk;
     DevBuf<double> spmv_part;
     SpmvPlan spmv;
+    // the on-the-fly near field: its plan (row groups of <= FLY_RB rows),
+    // the mode, and whether its FP32 pass integrates near-singular pairs in
+    // FP64 (as of the last FP32 precompute)
+    DevBuf<int4> fly_work, fly_groups;
+    DevBuf<int> fly_group_nblk;
+    SpmvPlan fly;
+    int near_mode = BEEP_NEAR_PRECOMPUTE;
+    bool fly_ns64 = true;
     int64_t nnz = 0;
     long long n_nb = 0;  // neighbourhood entries (nb_ptr[L])
     bool have_tree = false;
@@ -408,6 +416,11 @@ void matvec_dev(beep_hip_ctx* c, const double* dx, double* dy, int precision) {
                              std::min(2, nco - k), c->share_off.data() + k);
         launch_pair_spmv(s, mode, c->N, c->n_pair_rows, c->pair_rows.p, c->pair_ptr.p, c->pair_col.p,
                          c->pcoef32.p, c->pcoef64.p, dx, dy);
+    } else if (c->near_mode == BEEP_NEAR_ON_THE_FLY) {
+        launch_near_fly(s, mode, c->N, c->fly, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p, c->nb_idx.p,
+                        c->patches.p, c->qual_ptr.p, c->qual32.p, c->qual64.p, c->quad_ptr.p, c->quad32.p,
+                        c->quad64.p, c->kappa, c->use_fmm ? c->fmm->cellkey : nullptr, c->fly_ns64 ? 1 : 0,
+                        c->guard_max, c->diag.p, dx, c->spmv_part.p, dy);
     } else {
         launch_near_spmv(s, mode, c->N, c->spmv, c->leaf_ptr.p, c->leaf_idx.p, c->nb_ptr.p,
                          c->nb_idx.p, c->coef_off.p, c->coef32.p, c->coef64.p, c->diag.p, dx,
@@ -500,6 +513,7 @@ void arnoldi_step(beep_hip_ctx* c, int precision, int m) {
         c->spmv_part.p, c->fmm, c->hvec2.p, c->share, c->pair_ptr.p, c->pcoef32.p, c->pcoef64.p,
         reinterpret_cast<const void*>(intptr_t(c->gs_passes)),
         reinterpret_cast<const void*>(intptr_t(c->overlap)),
+        reinterpret_cast<const void*>(intptr_t(c->near_mode * 2 + (c->fly_ns64 ? 1 : 0))),
         reinterpret_cast<const void*>(intptr_t(c->spmv_cap[spmv_mode(precision)])),
         reinterpret_cast<const void*>(intptr_t(c->spmv_depth[spmv_mode(precision)]))};
     // graphs of an older generation bake in buffers or plans that changed
@@ -854,8 +868,8 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
             require(t->nb_idx[k] >= 0 && t->nb_idx[k] < N, "neighbourhood index out of range");
         std::vector<int64_t> off(L + 1, 0);
         std::vector<int> cblk(L + 1, 0);
-        std::vector<int4> work, groups;
-        std::vector<int> gnb;
+        std::vector<int4> work, groups, fwork, fgroups;
+        std::vector<int> gnb, fgnb;
         for (int l = 0; l < L; ++l) {
             const int64_t R = t->leaf_ptr[l + 1] - t->leaf_ptr[l];
             const int64_t C = t->nb_ptr[l + 1] - t->nb_ptr[l];
@@ -871,6 +885,13 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
                 groups.push_back(make_int4(l, r0, int(std::min<int64_t>(256, R - r0)), first));
                 gnb.push_back(int(work.size()) - first);
             }
+            for (int r0 = 0; r0 < R; r0 += FLY_RB) {
+                const int first = int(fwork.size());
+                for (int c0 = 0; c0 < C; c0 += SPMV_CW)
+                    fwork.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
+                fgroups.push_back(make_int4(l, r0, int(std::min<int64_t>(FLY_RB, R - r0)), first));
+                fgnb.push_back(int(fwork.size()) - first);
+            }
         }
         hipStream_t s = ctx->stream;
         ctx->leaf_ptr.upload(t->leaf_ptr, L + 1, s);
@@ -882,7 +903,10 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         ctx->spmv_work.upload(work.data(), work.size(), s);
         ctx->spmv_groups.upload(groups.data(), groups.size(), s);
         ctx->spmv_group_nblk.upload(gnb.data(), gnb.size(), s);
-        ctx->spmv_part.alloc(work.size() * 256 * 2);
+        ctx->fly_work.upload(fwork.data(), fwork.size(), s);
+        ctx->fly_groups.upload(fgroups.data(), fgroups.size(), s);
+        ctx->fly_group_nblk.upload(fgnb.data(), fgnb.size(), s);
+        ctx->spmv_part.alloc(std::max(work.size() * 256, fwork.size() * FLY_RB) * 2);
         hip_check(hipStreamSynchronize(s), "sync tree upload");
         ctx->L = L;
         ctx->nnz = off[L];
@@ -892,6 +916,12 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         ctx->spmv.groups = ctx->spmv_groups.p;
         ctx->spmv.group_nblk = ctx->spmv_group_nblk.p;
         ctx->spmv.n_groups = int(groups.size());
+        ctx->fly = SpmvPlan{};
+        ctx->fly.work = ctx->fly_work.p;
+        ctx->fly.n_work = int(fwork.size());
+        ctx->fly.groups = ctx->fly_groups.p;
+        ctx->fly.group_nblk = ctx->fly_group_nblk.p;
+        ctx->fly.n_groups = int(fgroups.size());
         ctx->h_leaf_ptr.assign(t->leaf_ptr, t->leaf_ptr + L + 1);
         ctx->h_nb_ptr.assign(t->nb_ptr, t->nb_ptr + L + 1);
         ctx->n_nb = nnb;
@@ -978,6 +1008,23 @@ void precompute_impl(beep_hip_ctx* ctx, double kappa, int precision) {
         uid = ctx->fmm->uid;
         hip_check(hipEventRecord(ctx->ev_pack, ctx->far_stream), "hipEventRecord");
         hip_check(hipStreamWaitEvent(s, ctx->ev_pack, 0), "hipStreamWaitEvent");
+    }
+    if (ctx->near_mode == BEEP_NEAR_ON_THE_FLY) {
+        // nothing stored: every mat-vec integrates the explicit pairs for
+        // this kappa and far-field plan (the guard and the near-singular
+        // FP64 pass run inside it)
+        if (precision == BEEP_FP64) {
+            ctx->have64 = true;
+            ctx->uid64 = uid;
+        } else {
+            ctx->have32 = true;
+            ctx->uid32 = uid;
+            ctx->fly_ns64 = ctx->ns_fp64;
+        }
+        ensure_far(ctx, precision);
+        if (ctx->use_fmm)
+            require(!cellkey || ctx->fmm->n_res >= 0, "residual leaf-pass list missing");
+        return;
     }
     size_t freeb = 0, totalb = 0;
     hip_check(hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
@@ -1082,6 +1129,8 @@ namespace {
 void share_common(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copies, const int* copy_offset) {
     require(ctx->have_geom, "set the geometry first");
     require(src && src != ctx && !src->share, "the shared near field must come from another plain context");
+    require(ctx->near_mode == BEEP_NEAR_PRECOMPUTE && src->near_mode == BEEP_NEAR_PRECOMPUTE,
+            "a shared near field needs stored coefficients (near mode BEEP_NEAR_PRECOMPUTE)");
     require(src->have_geom && src->have_tree && src->have_diag && (src->have32 || src->have64),
             "the shared context must have its geometry, tree and coefficients");
     require(src->uid32 == 0 && src->uid64 == 0,
@@ -1228,12 +1277,32 @@ int beep_hip_set_near_singular_fp64(beep_hip_ctx* ctx, int on) {
     return guarded(ctx, [&] { ctx->ns_fp64 = on != 0; });
 }
 
+int beep_hip_set_near_mode(beep_hip_ctx* ctx, int mode) {
+    return guarded(ctx, [&] {
+        require(mode == BEEP_NEAR_PRECOMPUTE || mode == BEEP_NEAR_ON_THE_FLY, "unknown near-field mode");
+        require(!ctx->share || mode == BEEP_NEAR_PRECOMPUTE,
+                "a shared near field needs stored coefficients (near mode BEEP_NEAR_PRECOMPUTE)");
+        if (mode == ctx->near_mode) return;
+        set_device(ctx);
+        sync_streams(ctx);
+        ctx->gen++;
+        ctx->near_mode = mode;
+        // the precision sets of the other mode are gone: precompute again
+        ctx->have32 = ctx->have64 = false;
+        ctx->ns_listed = false;
+        if (mode == BEEP_NEAR_ON_THE_FLY) {
+            ctx->coef32.reset();
+            ctx->coef64.reset();
+        }
+    });
+}
+
 int beep_hip_near_singular_count(beep_hip_ctx* ctx, long long* listed, long long* overflow) {
     return guarded(ctx, [&] {
         require(listed && overflow, "null output");
         set_device(ctx);
         *listed = *overflow = 0;
-        if (!ctx->have32 || !ctx->ns_listed) return;
+        if (!ctx->have32 || !ctx->ns_listed || ctx->near_mode != BEEP_NEAR_PRECOMPUTE) return;
         unsigned cnt = 0;
         hip_check(hipMemcpyAsync(&cnt, ctx->ns_count.p, sizeof(unsigned), hipMemcpyDeviceToHost,
                                  ctx->stream), "D2H near-singular count");
@@ -1288,6 +1357,8 @@ int beep_hip_sanity_check(beep_hip_ctx* ctx, long long* repaired) {
         launch_singular(s, ctx->N, ctx->patches.p, ctx->gc.p, ctx->gal_ptr.p, ctx->gal64.p,
                         ctx->kappa, 1, ctx->guard_max, gcnt + 4, ctx->diag.p);
         for (int fp32 = 0; fp32 < 2; ++fp32) {
+            // on the fly, the mat-vec guards every entry it integrates
+            if (ctx->near_mode != BEEP_NEAR_PRECOMPUTE) continue;
             if (!(fp32 ? ctx->have32 : ctx->have64)) continue;
             const unsigned long long built = fp32 ? ctx->uid32 : ctx->uid64;
             require(ctx->use_fmm ? (ctx->fmm && ctx->fmm->uid == built) : built == 0,
@@ -1314,6 +1385,8 @@ int beep_hip_debug_poke(beep_hip_ctx* ctx, int which, long long entry, const dou
     return guarded(ctx, [&] {
         require(v4 != nullptr, "null values");
         require(!ctx->share, "the coefficients are shared: poke the context that owns them");
+        require(which == 2 || ctx->near_mode == BEEP_NEAR_PRECOMPUTE,
+                "no stored coefficients on the fly (near mode BEEP_NEAR_ON_THE_FLY)");
         set_device(ctx);
         sync_streams(ctx);
         hipStream_t s = ctx->stream;
@@ -1405,7 +1478,9 @@ int beep_hip_matvec_device_noalloc(beep_hip_ctx* ctx, const double* dx, double* 
         const size_t b_src = up(sizeof(Src64) * size_t(N));
         const size_t b_yfar = up(sizeof(double) * size_t(n2));
         const size_t b_part = ns > 1 ? up(sizeof(double) * size_t(n2) * ns) : 0;
-        const size_t b_spmv = up(sizeof(double) * size_t(ctx->spmv.n_work) * 256 * 2);
+        const bool fly = ctx->near_mode == BEEP_NEAR_ON_THE_FLY;
+        const size_t b_spmv = up(sizeof(double) * (fly ? size_t(ctx->fly.n_work) * FLY_RB
+                                                        : size_t(ctx->spmv.n_work) * 256) * 2);
         const size_t need = b_src + b_yfar + b_part + b_spmv;
         if (need > SCRATCH_BYTES)
             throw std::runtime_error("the allocation-free mat-vec needs " + std::to_string(need) +
@@ -1421,9 +1496,15 @@ int beep_hip_matvec_device_noalloc(beep_hip_ctx* ctx, const double* dx, double* 
         // the kernels of matvec_dev with the direct far field, in its order,
         // all on the context stream
         const int mode = spmv_mode(precision);
-        launch_near_spmv(s, mode, N, ctx->spmv, ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
-                         ctx->nb_idx.p, ctx->coef_off.p, ctx->coef32.p, ctx->coef64.p, ctx->diag.p,
-                         dx, spart, dy, ctx->spmv_cap[mode] * ctx->n_cu, ctx->spmv_depth[mode]);
+        if (fly)
+            launch_near_fly(s, mode, N, ctx->fly, ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
+                            ctx->nb_idx.p, ctx->patches.p, ctx->qual_ptr.p, ctx->qual32.p, ctx->qual64.p,
+                            ctx->quad_ptr.p, ctx->quad32.p, ctx->quad64.p, ctx->kappa, nullptr,
+                            ctx->fly_ns64 ? 1 : 0, ctx->guard_max, ctx->diag.p, dx, spart, dy);
+        else
+            launch_near_spmv(s, mode, N, ctx->spmv, ctx->leaf_ptr.p, ctx->leaf_idx.p, ctx->nb_ptr.p,
+                             ctx->nb_idx.p, ctx->coef_off.p, ctx->coef32.p, ctx->coef64.p, ctx->diag.p,
+                             dx, spart, dy, ctx->spmv_cap[mode] * ctx->n_cu, ctx->spmv_depth[mode]);
         launch_fill(s, n2, 0.0, yfar);
         launch_far_pack(s, N, ctx->patches.p, ctx->leaf_idx.p, dx, src);
         launch_far_direct(s, precision == BEEP_FP32 ? 1 : 0, N, ns, ctx->leaf_idx.p, ctx->patches.p,
@@ -1521,6 +1602,7 @@ static void tune_spmv_setting(beep_hip_ctx* ctx, int precision, int (beep_hip_ct
     int* field = &(ctx->*table)[spmv_mode(precision)];
     require(values && n > 0 && reps > 0, "bad autotune arguments");
     require(ctx->have_geom && ctx->have_tree && ctx->have_diag, "precompute before tuning");
+    require(ctx->near_mode == BEEP_NEAR_PRECOMPUTE, "the SpMV settings apply to stored coefficients only");
     for (int k = 0; k < n; ++k) require(valid(values[k]), what);
     set_device(ctx);
     const size_t n2 = size_t(2) * ctx->N;

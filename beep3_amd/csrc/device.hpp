@@ -178,6 +178,8 @@ void launch_near_coeffs_fp64(hipStream_t s, int n_blocks, int n_leaves, const in
 // <= SPMV_CW columns, and row groups (leaf, first row, rows, first work item)
 // with their work-item counts.
 constexpr int SPMV_CW = 512;
+// rows per work item of the on-the-fly near field (launch_near_fly)
+constexpr int FLY_RB = 64;
 struct SpmvPlan {
     const int4* work = nullptr;
     int n_work = 0;
@@ -198,6 +200,15 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
                       const int64_t* coef_off, const float4* coef32, const double2* coef64,
                       const double* diag4, const double* x, void* part, double* y,
                       int max_blocks, int depth, int ncopies = 1, const int* copy_offset = nullptr);
+// The same product with every explicit pair integrated on the fly (no
+// coefficients stored), over a plan of row groups of <= FLY_RB rows; part
+// holds 2 * FLY_RB values per work item.  cellkey: the FMM leaf cells (null
+// with the direct far field); ns64: FP32 near-singular pairs in FP64.
+void launch_near_fly(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
+                     const int* leaf_idx, const int* nb_ptr, const int* nb_idx, const Patch64* patches,
+                     const int* qual_ptr, const Pt32* qual32, const Pt64* qual64, const int* quad_ptr,
+                     const Pt32* quad32, const Pt64* quad64, double kappa, const unsigned* cellkey,
+                     int ns64, double maxv, const double* diag4, const double* x, void* part, double* y);
 
 // explicit pairs listed one by one: coefficients (FP64, rounded to FP32 as
 // well; bad values zeroed and counted), and y += their products over the

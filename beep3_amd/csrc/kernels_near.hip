@@ -596,22 +596,23 @@ __global__ __launch_bounds__(256) void k_near_spmv_part(
     }
 }
 
-// grid (row groups, copies)
+// grid (row groups, copies); a work item's partials are pstride rows apart
+// (256 for the SpMV, FLY_RB for the on-the-fly pass)
 template <typename TA>
 __global__ __launch_bounds__(256) void k_near_spmv_sum(
     const int4* __restrict__ groups, const int* __restrict__ group_nblk,
     const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_idx,
     const double* __restrict__ diag4, const double* __restrict__ x, CopyOffsets co, int n, int n_work,
-    const TA* __restrict__ part, double* __restrict__ y) {
+    int pstride, const TA* __restrict__ part, double* __restrict__ y) {
     const int4 g = groups[blockIdx.x];  // leaf, first row, rows, first work block
     const int nb = group_nblk[blockIdx.x];
     const int r = threadIdx.x;
     if (r >= g.z) return;
     const int cq = blockIdx.y, off = co.off[cq];
-    const TA* pq = part + size_t(cq) * n_work * 512;
+    const TA* pq = part + size_t(cq) * n_work * 2 * pstride;
     TA af = 0, ah = 0;
     for (int k = 0; k < nb; ++k) {
-        const size_t q = 2 * (size_t(g.w + k) * 256 + r);
+        const size_t q = 2 * (size_t(g.w + k) * pstride + r);
         af += pq[q];
         ah += pq[q + 1];
     }
@@ -658,7 +659,7 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
                             spmv_part<double, double, 16>};
         f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef64, x, ncopies, co, n, (double*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<double>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
-                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const double*)part, y);
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, 256, (const double*)part, y);
     } else if (mode == 1) {
         using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
                            const int64_t*, const void*, const double*, int, CopyOffsets, int, float*);
@@ -666,7 +667,7 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
                             spmv_part<float, float, 32>};
         f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, ncopies, co, n, (float*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<float>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
-                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const float*)part, y);
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, 256, (const float*)part, y);
     } else {
         using F = void (*)(hipStream_t, int, const SpmvPlan&, const int*, const int*, const int*,
                            const int64_t*, const void*, const double*, int, CopyOffsets, int, double*);
@@ -674,9 +675,159 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
                             spmv_part<float, double, 32>};
         f[d](s, grid, p, leaf_ptr, nb_ptr, nb_idx, coef_off, coef32, x, ncopies, co, n, (double*)part);
         hipLaunchKernelGGL((k_near_spmv_sum<double>), sgrid, dim3(256), 0, s, p.groups, p.group_nblk,
-                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, (const double*)part, y);
+                           leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, 256, (const double*)part, y);
     }
     hip_check(hipGetLastError(), "k_near_spmv launch");
+}
+
+// ---------------------------------------------------------------------------
+// Near field on the fly: the semantics of BEM_kernels_instant
+// (src/bem/opencl_bem_kernels.cl:237-323, driven per mat-vec by
+// BEM_OnDemand_Resources, src/bem/opencl_bem.cpp:346-747): no coefficients are
+// stored; every mat-vec integrates each explicit pair and applies it at once.
+// The coefficient of a pair is the one the precompute stores for it
+// (k_near_coeffs and its FP64 passes): the full rule less the 1-point
+// interaction unless the FMM leaf pass covers the pair; in FP32, a
+// near-singular pair (when ns64) and a bad entry are integrated in FP64 and
+// rounded; an entry still bad is zero.
+//
+// Work items as the SpMV's (leaf, row group, run of <= SPMV_CW columns), but
+// with row groups of <= FLY_RB rows, so that FLY_RB * S of the 256 threads
+// are busy.  Thread (r, s) integrates the pairs of row r in columns s, s+S,
+// ... and sums their products in that order; the block adds its column lanes
+// in order and writes one partial per row (k_near_spmv_sum then adds the
+// partials, the diagonal, and writes y): deterministic.
+// The FP64 integration of an FP32 pass's near-singular or bad entry, rounded
+// to FP32 (zero if still bad), out of line: rare, and inlined it would take
+// registers from the FP32 loop.
+template <bool KAPPA>
+__device__ __attribute__((noinline)) float4 fly_redo64(const Patch64& pi, const Patch64& pj, int qb, int qn,
+                                                       const Pt64* __restrict__ qual64, int pb, int pn,
+                                                       const Pt64* __restrict__ quad64, double kappa, int sub,
+                                                       double maxv) {
+    double o[4];
+    pair_coeffs<double, KAPPA>(pi, pj, qb, qn, qual64, pb, pn, quad64, kappa, sub, o);
+    const float4 v = make_float4(float(o[0]), float(o[1]), float(o[2]), float(o[3]));
+    const double d[4] = {v.x, v.y, v.z, v.w};
+    return coef4_bad(d, maxv) ? make_float4(0.f, 0.f, 0.f, 0.f) : v;
+}
+
+template <typename T, typename TA, bool KAPPA>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
+    const int4* __restrict__ work, int n_work, const int* __restrict__ leaf_ptr,
+    const int* __restrict__ leaf_idx, const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx,
+    const Patch64* __restrict__ patches, const int* __restrict__ qual_ptr,
+    const Pt<T>* __restrict__ qual, const int* __restrict__ quad_ptr, const Pt<T>* __restrict__ quad,
+    const Pt64* __restrict__ qual64, const Pt64* __restrict__ quad64, double kappa,
+    const unsigned* __restrict__ cellkey, int ns64, double maxv, const double* __restrict__ x, int n,
+    TA* __restrict__ part) {
+    __shared__ TA xs[2][SPMV_CW];
+    __shared__ TA red[2][256];
+    const int wi = xcd_remap(blockIdx.x, gridDim.x);
+    const int4 w = work[wi];  // leaf, first row, first column, columns
+    const int L = w.x, row0 = w.y, col0 = w.z, nc = w.w;
+    const int lp = leaf_ptr[L];
+    const int R = leaf_ptr[L + 1] - lp;
+    const int np = nb_ptr[L];
+    const int Rb = min(FLY_RB, R - row0);
+    const int S = 256 / Rb;
+    const int t = threadIdx.x;
+    const int r = t % Rb, s = t / Rb;
+    const bool active = t < Rb * S;
+    for (int k = t; k < nc; k += 256) {
+        const int j = nb_idx[np + col0 + k];
+        xs[0][k] = TA(x[j]);
+        xs[1][k] = TA(x[n + j]);
+    }
+    __syncthreads();
+    TA af = 0, ah = 0;
+    if (active) {
+        const int i = leaf_idx[lp + row0 + r];
+        const Patch64 pi = patches[i];
+        const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+        const unsigned ki = cellkey ? cellkey[i] : 0u;
+        for (int c = s; c < nc; c += S) {
+            const int j = nb_idx[np + col0 + c];
+            if (j == i) continue;  // the self pair is the diagonal's (stored as zero)
+            const Patch64 pj = patches[j];
+            const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
+            const int sub = !(cellkey && fmm_cells_adjacent(ki, cellkey[j]));
+            T v[4];
+            bool redo = false;
+            if constexpr (sizeof(T) == 4) redo = ns64 && near_singular(pi, pj);
+            if (!redo) {
+                pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, T(kappa), sub, v);
+                const double d[4] = {double(v[0]), double(v[1]), double(v[2]), double(v[3])};
+                if (coef4_bad(d, maxv)) {
+                    // FP64 recomputes what FP64 already gave: zero at once
+                    if constexpr (sizeof(T) == 8) v[0] = v[1] = v[2] = v[3] = T(0);
+                    else redo = true;
+                }
+            }
+            if constexpr (sizeof(T) == 4) {
+                if (redo) {
+                    const float4 o = fly_redo64<KAPPA>(pi, pj, qb, qn, qual64, pb, pn, quad64, kappa, sub, maxv);
+                    v[0] = o.x;
+                    v[1] = o.y;
+                    v[2] = o.z;
+                    v[3] = o.w;
+                }
+            }
+            const TA xf = xs[0][c], xh = xs[1][c];
+            af += TA(v[1]) * xf - TA(v[0]) * xh;
+            ah += TA(v[3]) * xf - TA(v[2]) * xh;
+        }
+    }
+    red[0][t] = af;
+    red[1][t] = ah;
+    __syncthreads();
+    if (active && s == 0) {
+        TA f = af, h = ah;
+        for (int k = 1; k < S; ++k) {
+            f += red[0][k * Rb + r];
+            h += red[1][k * Rb + r];
+        }
+        part[2 * (size_t(wi) * FLY_RB + r)] = f;
+        part[2 * (size_t(wi) * FLY_RB + r) + 1] = h;
+    }
+}
+
+template <typename T, typename TA>
+static void fly_launch(hipStream_t s, const SpmvPlan& p, const int* leaf_ptr, const int* leaf_idx,
+                       const int* nb_ptr, const int* nb_idx, const Patch64* patches, const int* qual_ptr,
+                       const Pt<T>* qual, const int* quad_ptr, const Pt<T>* quad, const Pt64* qual64,
+                       const Pt64* quad64, double kappa, const unsigned* cellkey, int ns64, double maxv,
+                       const double* diag4, const double* x, int n, TA* part, double* y) {
+    auto k = kappa != 0.0 ? k_near_fly_part<T, TA, true> : k_near_fly_part<T, TA, false>;
+    hipLaunchKernelGGL(k, dim3(unsigned(p.n_work)), dim3(256), 0, s, p.work, p.n_work, leaf_ptr, leaf_idx,
+                       nb_ptr, nb_idx, patches, qual_ptr, qual, quad_ptr, quad, qual64, quad64, kappa,
+                       cellkey, ns64, maxv, x, n, part);
+    hip_check(hipGetLastError(), "k_near_fly_part launch");
+    const CopyOffsets co{{0, 0}};
+    hipLaunchKernelGGL((k_near_spmv_sum<TA>), dim3(unsigned(p.n_groups), 1u), dim3(256), 0, s, p.groups,
+                       p.group_nblk, leaf_ptr, leaf_idx, diag4, x, co, n, p.n_work, FLY_RB,
+                       (const TA*)part, y);
+    hip_check(hipGetLastError(), "k_near_spmv_sum launch");
+}
+
+void launch_near_fly(hipStream_t s, int mode, int n, const SpmvPlan& p, const int* leaf_ptr,
+                     const int* leaf_idx, const int* nb_ptr, const int* nb_idx, const Patch64* patches,
+                     const int* qual_ptr, const Pt32* qual32, const Pt64* qual64, const int* quad_ptr,
+                     const Pt32* quad32, const Pt64* quad64, double kappa, const unsigned* cellkey,
+                     int ns64, double maxv, const double* diag4, const double* x, void* part, double* y) {
+    if (p.n_work <= 0) return;
+    if (mode == 0)
+        fly_launch<double, double>(s, p, leaf_ptr, leaf_idx, nb_ptr, nb_idx, patches, qual_ptr, qual64,
+                                   quad_ptr, quad64, qual64, quad64, kappa, cellkey, 0, maxv, diag4, x, n,
+                                   (double*)part, y);
+    else if (mode == 1)
+        fly_launch<float, float>(s, p, leaf_ptr, leaf_idx, nb_ptr, nb_idx, patches, qual_ptr, qual32,
+                                 quad_ptr, quad32, qual64, quad64, kappa, cellkey, ns64, maxv, diag4, x,
+                                 n, (float*)part, y);
+    else
+        fly_launch<float, double>(s, p, leaf_ptr, leaf_idx, nb_ptr, nb_idx, patches, qual_ptr, qual32,
+                                  quad_ptr, quad32, qual64, quad64, kappa, cellkey, ns64, maxv, diag4, x,
+                                  n, (double*)part, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -728,41 +879,64 @@ void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const i
 // Pairs between copies closer than rc (the inter-instance pairs of a pose),
 // found on the device: one thread per row i, testing every patch of every
 // other copy whose bounding sphere (centre, radius) comes within rc of
-// patch i, in index order.  Pass 1 (cols null) counts per row; pass 2 writes
-// each row's columns at ptr[i] and the row of every pair.
+// patch i, in index order.  The block stages the other copy's centroids
+// through LDS, 256 at a time, and skips a copy none of its rows can reach.
+// Pass 1 (cols null) counts per row; pass 2 writes each row's columns at
+// ptr[i] and the row of every pair.
 __global__ __launch_bounds__(256) void k_cut_pairs(int N, int ncopies, int ns, const int* __restrict__ copy_off,
                                                   const double4* __restrict__ sphere,
                                                   const Patch64* __restrict__ patches, double rc,
                                                   const int* __restrict__ ptr, int* __restrict__ count,
                                                   int* __restrict__ cols, int* __restrict__ rows_of) {
+    __shared__ double4 tile[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const Patch64 pi = patches[i];
+    const bool row = i < N;
     const double rc2 = rc * rc;
-    int own = 0;
-    for (int q = 0; q < ncopies; ++q)
-        if (i >= copy_off[q] && i < copy_off[q] + ns) own = q;
-    int n = 0, o = cols ? ptr[i] : 0;
+    double cx = 0, cy = 0, cz = 0;
+    int own = -1;
+    if (row) {
+        cx = patches[i].cx;
+        cy = patches[i].cy;
+        cz = patches[i].cz;
+        for (int q = 0; q < ncopies; ++q)
+            if (i >= copy_off[q] && i < copy_off[q] + ns) own = q;
+    }
+    int n = 0;
+    const int o = (row && cols) ? ptr[i] : 0;
     for (int q = 0; q < ncopies; ++q) {
-        if (q == own) continue;
-        const double4 sp = sphere[q];
-        const double sx = pi.cx - sp.x, sy = pi.cy - sp.y, sz = pi.cz - sp.z;
-        const double reach = sp.w + rc;
-        if (sx * sx + sy * sy + sz * sz > reach * reach) continue;
+        bool need = false;
+        if (row && q != own) {
+            const double4 sp = sphere[q];
+            const double sx = cx - sp.x, sy = cy - sp.y, sz = cz - sp.z;
+            const double reach = sp.w + rc;
+            need = sx * sx + sy * sy + sz * sz <= reach * reach;
+        }
+        if (!__syncthreads_or(need)) continue;  // uniform over the block
         const int j0 = copy_off[q];
-        for (int j = j0; j < j0 + ns; ++j) {
-            const Patch64 pj = patches[j];
-            const double dx = pj.cx - pi.cx, dy = pj.cy - pi.cy, dz = pj.cz - pi.cz;
-            if (dx * dx + dy * dy + dz * dz < rc2) {
-                if (cols) {
-                    cols[o + n] = j;
-                    rows_of[o + n] = i;
+        for (int t0 = 0; t0 < ns; t0 += 256) {
+            __syncthreads();  // the previous tile is consumed
+            if (t0 + int(threadIdx.x) < ns) {
+                const Patch64& p = patches[j0 + t0 + threadIdx.x];
+                tile[threadIdx.x] = make_double4(p.cx, p.cy, p.cz, 0.0);
+            }
+            __syncthreads();
+            if (need) {
+                const int m = min(256, ns - t0);
+                for (int k = 0; k < m; ++k) {
+                    const double4 pj = tile[k];
+                    const double dx = pj.x - cx, dy = pj.y - cy, dz = pj.z - cz;
+                    if (dx * dx + dy * dy + dz * dz < rc2) {
+                        if (cols) {
+                            cols[o + n] = j0 + t0 + k;
+                            rows_of[o + n] = i;
+                        }
+                        ++n;
+                    }
                 }
-                ++n;
             }
         }
     }
-    if (!cols) count[i] = n;
+    if (row && !cols) count[i] = n;
 }
 
 __global__ void k_iota(int n, int* __restrict__ out) {

@@ -103,6 +103,15 @@ class Device:
         (default), from the next FP32 precompute."""
         self._check(lib().beep_hip_set_near_singular_fp64(self._h, 1 if on else 0))
 
+    NEAR_MODES = {"precompute": 0, "on_the_fly": 1}
+
+    def set_near_mode(self, mode="precompute"):
+        """"precompute" (stored coefficients, the default) or "on_the_fly"
+        (every mat-vec integrates the explicit pairs; nothing stored).  Drops
+        the precomputed sets: precompute again."""
+        m = self.NEAR_MODES[mode] if isinstance(mode, str) else int(mode)
+        self._check(lib().beep_hip_set_near_mode(self._h, m))
+
     def share_near(self, src: "Device", copy_offsets, pair_i=None, pair_j=None):
         """Near field of src's coefficient sets applied to copies of its
         patches at the given offsets, plus the listed explicit pairs."""

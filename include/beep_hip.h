@@ -50,6 +50,10 @@ extern "C" {
 #define BEEP_FP32_RESID 2 /* FP32 coefficients promoted to FP64, FP64 vectors, sums and far
                              field: the refinement residual of the mixed-precision solve   */
 
+/* near-field modes (beep_hip_set_near_mode) */
+#define BEEP_NEAR_PRECOMPUTE 0 /* coefficients stored by beep_hip_precompute, applied by an SpMV */
+#define BEEP_NEAR_ON_THE_FLY 1 /* every mat-vec integrates the explicit pairs (nothing stored) */
+
 /* solve modes */
 #define BEEP_SOLVE_FP64 0  /* GMRES with BEEP_FP64 mat-vecs (plain FP64 solve)            */
 #define BEEP_SOLVE_MIXED 1 /* FP64 iterative refinement (BEEP_FP32_RESID residuals) around
@@ -325,6 +329,21 @@ int beep_hip_share_near_cutoff(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copie
  * (on = 1, the default) or FP32 for every pair (on = 0); applies from the
  * next FP32 precompute.  No counterpart in the reference (all FP32). */
 int beep_hip_set_near_singular_fp64(beep_hip_ctx* ctx, int on);
+
+/* Near-field mode.  BEEP_NEAR_PRECOMPUTE (the default): beep_hip_precompute
+ * stores the explicit pairs' coefficients (LocalIntegrations, beep.cpp:
+ * 768-956) and each mat-vec applies them.  BEEP_NEAR_ON_THE_FLY: nothing is
+ * stored; beep_hip_precompute computes the self terms only, and every
+ * mat-vec integrates each explicit pair and applies it at once, as the
+ * reference's OpenCL path does (BEM_kernels_instant, opencl_bem_kernels.cl:
+ * 237-323, driven by BEM_OnDemand_Resources, opencl_bem.cpp:346-747): for
+ * memory-bound use (no coefficient memory; a mat-vec costs about one
+ * precompute).  Each pair's values are the ones the precompute would store
+ * (guard and near-singular FP64 pass included), so both modes give the same
+ * operator to rounding.  Changing the mode drops the precomputed sets: call
+ * beep_hip_precompute again.  A context sharing a near field
+ * (beep_hip_share_near) and its source stay in BEEP_NEAR_PRECOMPUTE. */
+int beep_hip_set_near_mode(beep_hip_ctx* ctx, int mode);
 /* Near-singular entries of the current FP32 set computed in FP64 (listed),
  * and the entries the list had no room for (computed in FP32): both 0 when
  * the set was built with the FP64 pass off.  Waits for the precompute. */
