@@ -137,6 +137,8 @@ def _declare(L):
     L.beep_mesh_info.argtypes = [vp, ip, ip, ip, dp, dp]
     L.beep_mesh_vertices.argtypes = [vp, dp, ip]
     L.beep_mesh_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.beep_mesh_read_mtz.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.beep_mesh_initial_fh.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.beep_mesh_write_off.argtypes = [vp, C.c_char_p]
     L.beep_mesh_read_charges.argtypes = [vp, C.c_char_p]
     L.beep_mesh_clone.argtypes = [vp, C.POINTER(vp)]

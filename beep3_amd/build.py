@@ -56,7 +56,7 @@ def build(verbose: bool = False) -> str:
             if verbose and (r.stdout or r.stderr):
                 print(r.stdout, r.stderr, file=sys.stderr)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, OUT)
     return OUT
 

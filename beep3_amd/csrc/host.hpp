@@ -65,6 +65,10 @@ struct Mesh {
     std::vector<Vec3> centroid, normal;
     std::vector<double> area;                      // "bezier_area" of a planar patch
     Vec3 centre;                                   // centroid of charges (mesh.cpp:1654)
+    bool centre_given = false;                     // read from a file (the .mtz centre member,
+                                                   // mesh.cpp:223-233): kept by finalize
+    std::vector<double> init_f, init_h;            // f and h per node patch from a .mtz fh member
+                                                   // (mesh.cpp:630-650), empty if none
     double radius = 0;                             // max |patch centroid - centre|
 
     void finalize();                               // adjacency, patches, centre, radius

@@ -151,7 +151,7 @@ void Mesh::finalize() {
 void Mesh::finalize_charges() {
     Vec3 s;
     for (const auto& q : charges) s += Vec3(q[0], q[1], q[2]);
-    centre = charges.empty() ? Vec3() : s / double(charges.size());
+    if (!centre_given) centre = charges.empty() ? Vec3() : s / double(charges.size());
     radius = 0;
     for (const auto& c : centroid) radius = std::max(radius, (c - centre).norm());
 }

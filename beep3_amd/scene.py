@@ -50,6 +50,24 @@ class Mesh:
             check_host(lib().beep_mesh_read_charges(m._h, str(charges).encode()))
         return m
 
+    @classmethod
+    def read_mtz(cls, path: str) -> "Mesh":
+        """A .mtz mesh tarball: surface, charges and centre from the members
+        its definition.xml names."""
+        h = vp()
+        check_host(lib().beep_mesh_read_mtz(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    def initial_fh(self):
+        """(f, h) per node patch from a .mtz fh member, or None."""
+        n = self.info()["n_vertices"]
+        f, h = np.empty(n), np.empty(n)
+        dp = C.POINTER(C.c_double)
+        rc = lib().beep_mesh_initial_fh(self._h, f.ctypes.data_as(dp), h.ctypes.data_as(dp))
+        if rc < 0:
+            check_host(rc)
+        return None if rc == 1 else (f, h)
+
     def clone(self) -> "Mesh":
         h = vp()
         check_host(lib().beep_mesh_clone(self._h, C.byref(h)))

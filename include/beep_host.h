@@ -49,6 +49,20 @@ int beep_mesh_vertices(const beep_mesh* m, double* xyz, int* tri3);
 int beep_mesh_read(const char* path, beep_mesh** out);
 int beep_mesh_write_off(const beep_mesh* m, const char* path);
 int beep_mesh_read_charges(beep_mesh* m, const char* path);
+/* A .mtz mesh tarball (MeshTarball, mesh_tarball.h:27-140 and
+ * mesh_tarball.cpp:118-270, as Mesh::init(mtz) loads it, mesh.cpp:88-151):
+ * a tar archive, gzip-compressed or plain (bzip2 and compress are refused
+ * with a message), whose definition.xml (root <contents>) names the members
+ * <mesh> (.off or .gts), <xyzqr> (charges), <centre> (x y z: the mesh centre,
+ * kept instead of the charge centroid) and <energies> (mandatory in the
+ * definition; the curved-patch energy precalcs, recomputed here as for the
+ * reference's planar patches), and optionally <fh> (f h per node patch,
+ * beep_mesh_initial_fh). */
+int beep_mesh_read_mtz(const char* path, beep_mesh** out);
+/* f and h per node patch (n_vertices each) from a .mtz fh member
+ * (Mesh::init_fh_vals, mesh.cpp:630-650): returns 0, or 1 if the mesh has
+ * none. */
+int beep_mesh_initial_fh(const beep_mesh* m, double* f, double* h);
 /* independent copy (surface, patches, charges, star shape) */
 int beep_mesh_clone(const beep_mesh* m, beep_mesh** out);
 void beep_mesh_destroy(beep_mesh* m);
