@@ -145,7 +145,17 @@ struct Octree {
     int nbsize = 1500;
     int max_depth = 20;
     std::vector<OctNode> nodes;
-    std::vector<std::map<uint64_t, int>> levels;   // per level: key -> node id
+    std::vector<std::vector<int>> levels;          // per level: node ids, in creation order
+    // per level: key -> node id by open addressing (lookups); keys are
+    // below 2^63, so all ones marks an empty slot
+    struct LevelIndex {
+        std::vector<uint64_t> keys;
+        std::vector<int> ids;
+        size_t count = 0;
+        void insert(uint64_t k, int id);
+        int find(uint64_t k) const;
+    };
+    std::vector<LevelIndex> index;
 
     static uint64_t key(uint32_t x, uint32_t y, uint32_t z) {
         return (uint64_t(x) << 42) | (uint64_t(y) << 21) | uint64_t(z);
@@ -163,6 +173,10 @@ struct Octree {
 
   private:
     std::vector<Vec3> const* pts_ = nullptr;
+    std::vector<unsigned char> scratch_;  // octant of each item of the node being split
+    // per level: node ids in key order, re-sorted when the level has grown
+    std::vector<std::vector<int>> sorted_;
+    const std::vector<int>& ordered(int level);
     int new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& c, double e, int parent);
     void subdivide(int n);
     void subdivide_neighbourhood(int n);

@@ -14,6 +14,48 @@
 
 namespace beep {
 
+namespace {
+inline size_t mix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    return size_t(k);
+}
+}  // namespace
+
+void Octree::LevelIndex::insert(uint64_t k, int id) {
+    if (2 * (count + 1) > keys.size()) {
+        // grow to keep the load factor at most 1/2
+        std::vector<uint64_t> ok;
+        std::vector<int> oi;
+        ok.swap(keys);
+        oi.swap(ids);
+        const size_t cap = std::max<size_t>(64, ok.size() * 2);
+        keys.assign(cap, ~0ull);
+        ids.assign(cap, -1);
+        count = 0;
+        for (size_t s = 0; s < ok.size(); ++s)
+            if (ok[s] != ~0ull) insert(ok[s], oi[s]);
+    }
+    const size_t m = keys.size() - 1;
+    size_t s = mix(k) & m;
+    while (keys[s] != ~0ull && keys[s] != k) s = (s + 1) & m;
+    if (keys[s] == ~0ull) ++count;
+    keys[s] = k;
+    ids[s] = id;
+}
+
+int Octree::LevelIndex::find(uint64_t k) const {
+    if (keys.empty()) return -1;
+    const size_t m = keys.size() - 1;
+    size_t s = mix(k) & m;
+    while (keys[s] != ~0ull) {
+        if (keys[s] == k) return ids[s];
+        s = (s + 1) & m;
+    }
+    return -1;
+}
+
 int Octree::new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& c, double e,
                      int parent) {
     OctNode n;
@@ -24,18 +66,39 @@ int Octree::new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& 
     n.parent = parent;
     nodes.push_back(std::move(n));
     int id = int(nodes.size()) - 1;
-    while (int(levels.size()) <= level) levels.emplace_back();
-    levels[level][key(x, y, z)] = id;
+    while (int(levels.size()) <= level) {
+        levels.emplace_back();
+        index.emplace_back();
+    }
+    levels[level].push_back(id);
+    index[level].insert(key(x, y, z), id);
     return id;
+}
+
+// the walks over a level go in key order (x, y, z bits), the order of the
+// reference's per-level maps: ties among equally fat neighbourhoods and the
+// leaf order depend on it
+const std::vector<int>& Octree::ordered(int level) {
+    if (int(sorted_.size()) < int(levels.size())) sorted_.resize(levels.size());
+    std::vector<int>& v = sorted_[level];
+    if (v.size() != levels[level].size()) {
+        v = levels[level];
+        std::sort(v.begin(), v.end(), [&](int a, int b) {
+            const OctNode& p = nodes[a];
+            const OctNode& q = nodes[b];
+            return key(p.ix, p.iy, p.iz) < key(q.ix, q.iy, q.iz);
+        });
+    }
+    return v;
 }
 
 int Octree::find(int level, long x, long y, long z) const {
     if (level < 0 || level >= int(levels.size())) return -1;
     const long cells = 1L << level;
     if (x < 0 || y < 0 || z < 0 || x >= cells || y >= cells || z >= cells) return -1;
-    auto it = levels[level].find(key(uint32_t(x), uint32_t(y), uint32_t(z)));
-    if (it == levels[level].end()) return -1;
-    return nodes[it->second].removed ? -1 : it->second;
+    const int id = index[level].find(key(uint32_t(x), uint32_t(y), uint32_t(z)));
+    if (id < 0) return -1;
+    return nodes[id].removed ? -1 : id;
 }
 
 // neighbour id nb in [0, 27): dz = nb/9 - 1, dy = (nb%9)/3 - 1, dx = nb%3 - 1
@@ -61,16 +124,25 @@ void Octree::subdivide(int n) {
         int id = new_node(level, p.ix * 2 + (c & 1), p.iy * 2 + ((c >> 1) & 1),
                           p.iz * 2 + ((c >> 2) & 1), cc, h, n);
         nodes[n].children[c] = id;
-        // the child receives the parent's items that fall in its octant
-        // (Node child constructor, octree.h:120-131), in the parent's order
-        const OctNode& pp = nodes[n];
-        const Vec3& pc = pp.centre;
-        for (int item : pp.contents) {
-            const Vec3& q = (*pts_)[item];
-            int cid = (q.x > pc.x ? 1 : 0) + (q.y > pc.y ? 2 : 0) + (q.z > pc.z ? 4 : 0);
-            if (cid == c) nodes[id].contents.push_back(item);
-        }
     }
+    // each child receives the parent's items that fall in its octant (Node
+    // child constructor, octree.h:120-131), in the parent's order: one pass
+    const Vec3 pc = nodes[n].centre;
+    const std::vector<int>& items = nodes[n].contents;
+    std::vector<unsigned char>& oct = scratch_;
+    oct.resize(items.size());
+    size_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t k = 0; k < items.size(); ++k) {
+        const Vec3& q = (*pts_)[items[k]];
+        oct[k] = static_cast<unsigned char>((q.x > pc.x ? 1 : 0) + (q.y > pc.y ? 2 : 0) + (q.z > pc.z ? 4 : 0));
+        ++cnt[oct[k]];
+    }
+    std::vector<int>* dst[8];
+    for (int c = 0; c < 8; ++c) {
+        dst[c] = &nodes[nodes[n].children[c]].contents;
+        dst[c]->reserve(cnt[c]);
+    }
+    for (size_t k = 0; k < items.size(); ++k) dst[oct[k]]->push_back(items[k]);
 }
 
 void Octree::add(int item, int n) {
@@ -127,11 +199,12 @@ long Octree::cached_nb(int n) {
 
 double Octree::interacts_per_item(long total_items) {
     long total = 0;
+    // an integer sum: the walk order does not matter
     for (int level = 0; level < int(levels.size()) && level < max_depth; ++level)
-        for (const auto& kv : levels[level]) {
-            const OctNode& o = nodes[kv.second];
+        for (int id : levels[level]) {
+            const OctNode& o = nodes[id];
             if (o.is_leaf && !o.is_shadow && !o.is_deleted)
-                total += cached_nb(kv.second) * long(o.contents.size());
+                total += cached_nb(id) * long(o.contents.size());
         }
     return double(total) / double(total_items);
 }
@@ -139,11 +212,14 @@ double Octree::interacts_per_item(long total_items) {
 void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb) {
     nodes.clear();
     levels.clear();
+    index.clear();
+    sorted_.clear();
     leaf_nodes.clear();
     centre = c;
     edge = e;
     nbsize = nb;
     pts_ = &pts;
+    nodes.reserve(64 + pts.size() / 4);
     new_node(0, 0, 0, 0, c, e, -1);
     for (int i = 0; i < int(pts.size()); ++i) add(i, 0);
 
@@ -151,9 +227,9 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     // complete set of same-level neighbours with children
     for (int level = 0; level < int(levels.size()); ++level) {
         std::vector<int> with_children;
-        for (const auto& kv : levels[level]) {
-            const OctNode& o = nodes[kv.second];
-            if (o.has_children && !o.is_shadow) with_children.push_back(kv.second);
+        for (int id : ordered(level)) {
+            const OctNode& o = nodes[id];
+            if (o.has_children && !o.is_shadow) with_children.push_back(id);
         }
         for (int n : with_children) subdivide_neighbourhood(n);
     }
@@ -161,18 +237,18 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     // optimize_neighbourhoods
     long total_items = 0;
     for (int level = 0; level < int(levels.size()); ++level)
-        for (const auto& kv : levels[level]) {
-            const OctNode& o = nodes[kv.second];
+        for (int id : ordered(level)) {
+            const OctNode& o = nodes[id];
             if (!o.is_shadow && o.is_leaf && !o.is_deleted) total_items += long(o.contents.size());
         }
     double avg = interacts_per_item(total_items);
     while (std::floor(avg) > nbsize) {
         std::vector<int> fat;
         for (int level = 0; level < int(levels.size()); ++level)
-            for (const auto& kv : levels[level]) {
-                const OctNode& o = nodes[kv.second];
-                if (!o.is_shadow && o.is_leaf && !o.contents.empty() && cached_nb(kv.second) > nbsize)
-                    fat.push_back(kv.second);
+            for (int id : ordered(level)) {
+                const OctNode& o = nodes[id];
+                if (!o.is_shadow && o.is_leaf && !o.contents.empty() && cached_nb(id) > nbsize)
+                    fat.push_back(id);
             }
         // fattest neighbourhoods first; ties keep level/key order
         std::stable_sort(fat.begin(), fat.end(),
@@ -200,21 +276,21 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
 
     // remove_empty_nodes (bottom-up), keeping the root
     for (int level = int(levels.size()) - 1; level >= 0; --level)
-        for (const auto& kv : levels[level]) {
-            OctNode& o = nodes[kv.second];
-            if (o.contents.empty() && kv.second != 0 && !o.removed) {
+        for (int id : ordered(level)) {
+            OctNode& o = nodes[id];
+            if (o.contents.empty() && id != 0 && !o.removed) {
                 o.removed = true;
                 if (o.parent >= 0)
                     for (int c = 0; c < 8; ++c)
-                        if (nodes[o.parent].children[c] == kv.second) nodes[o.parent].children[c] = -1;
+                        if (nodes[o.parent].children[c] == id) nodes[o.parent].children[c] = -1;
             }
         }
 
     for (int level = 0; level < int(levels.size()); ++level)
-        for (const auto& kv : levels[level]) {
-            const OctNode& o = nodes[kv.second];
+        for (int id : ordered(level)) {
+            const OctNode& o = nodes[id];
             if (!o.removed && o.is_leaf && !o.is_shadow && !o.contents.empty())
-                leaf_nodes.push_back(kv.second);
+                leaf_nodes.push_back(id);
         }
     // sanity: every item belongs to exactly one leaf
     std::vector<int> owner(pts.size(), 0);
@@ -230,6 +306,18 @@ void Octree::leaves_csr(std::vector<int>& leaf_ptr, std::vector<int>& leaf_idx,
     nb_ptr.assign(1, 0);
     leaf_idx.clear();
     nb_idx.clear();
+    size_t n_items = 0, n_nb = 0;
+    for (int n : leaf_nodes) {
+        n_items += nodes[n].contents.size();
+        for (int nb = 0; nb < 27; ++nb) {
+            const int m = neighbour(n, nb);
+            if (m >= 0) n_nb += nodes[m].contents.size();
+        }
+    }
+    leaf_ptr.reserve(leaf_nodes.size() + 1);
+    nb_ptr.reserve(leaf_nodes.size() + 1);
+    leaf_idx.reserve(n_items);
+    nb_idx.reserve(n_nb);
     for (int n : leaf_nodes) {
         const OctNode& o = nodes[n];
         leaf_idx.insert(leaf_idx.end(), o.contents.begin(), o.contents.end());

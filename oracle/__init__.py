@@ -71,7 +71,31 @@ def lib():
         _lib.cref_gmres.argtypes = [C.c_void_p, dp, dp, C.c_double, C.c_int, dp]
         _lib.cref_gmres.restype = C.c_int
         _lib.cref_destroy.argtypes = [C.c_void_p]
+        pp = C.POINTER(ip)
+        _lib.orc_octree.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, ip, C.POINTER(C.c_longlong),
+                                    pp, pp, pp, pp]
+        _lib.orc_free.argtypes = [C.c_void_p]
     return _lib
+
+
+def octree_csr(centroid, cube_centre, cube_edge: float, nbsize: int) -> dict:
+    """Leaves and neighbourhood contents of the adaptive octree over the
+    patch centroids (octree_ref.cpp, restating src/fmm/octree.h under the
+    policy of src/beep/beep.cpp:1585-1641), as CSR arrays."""
+    pts = np.ascontiguousarray(centroid, dtype=np.float64).reshape(-1, 3)
+    cc = np.ascontiguousarray(cube_centre, dtype=np.float64).reshape(3)
+    nl, nn = C.c_int(), C.c_longlong()
+    ptrs = [C.POINTER(C.c_int)() for _ in range(4)]
+    rc = lib().orc_octree(pts.shape[0], _dp(pts), _dp(cc), float(cube_edge), int(nbsize), C.byref(nl),
+                          C.byref(nn), *[C.byref(p) for p in ptrs])
+    if rc != 0:
+        raise RuntimeError("oracle octree failed")
+    sizes = [nl.value + 1, pts.shape[0], nl.value + 1, nn.value]
+    out = {}
+    for name, p, k in zip(("leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx"), ptrs, sizes):
+        out[name] = np.ctypeslib.as_array(p, shape=(k,)).copy()
+        lib().orc_free(C.cast(p, C.c_void_p))
+    return out
 
 
 class OrcGeom(C.Structure):
