@@ -138,6 +138,8 @@ def _declare(L):
     L.beep_mesh_vertices.argtypes = [vp, dp, ip]
     L.beep_mesh_read.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.beep_mesh_read_mtz.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.beep_mesh_set_curved.argtypes = [vp, C.c_int]
+    L.beep_mesh_curved_point.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.beep_mesh_initial_fh.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.beep_mesh_write_off.argtypes = [vp, C.c_char_p]
     L.beep_mesh_read_charges.argtypes = [vp, C.c_char_p]

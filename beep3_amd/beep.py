@@ -69,8 +69,11 @@ class BEEP:
         """Build patches and octree (host), upload to the GPU and precompute
         the explicit-pair coefficients (precalc_neighbour_interactions)."""
         t0 = time.perf_counter()
+        # the device generates planar point sets: curved meshes take the host's
+        dev_points = self.device_points and not any(
+            getattr(self.library[lib_id], "curved", False) for lib_id, _, _, _ in self.instances)
         sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize,
-                   host_points=not self.device_points)
+                   host_points=not dev_points)
         for lib_id, off, rot, Dp in self.instances:
             sc.add_instance(self.library[lib_id], Dp, quat=rot, offset=off)
         sc.build()
@@ -79,7 +82,7 @@ class BEEP:
         if self.dev is None:
             self.dev = Device(self.device_id)
         self.dev.set_far_field(self.far_field, self.fmm_order, self.fmm_leaf)
-        if self.device_points:
+        if dev_points:
             self.dev.set_surface(a, sc.surface(), self.qual, self.quad)
         else:
             self.dev.set_geometry(a)

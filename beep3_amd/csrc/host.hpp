@@ -56,6 +56,27 @@ struct Quat {  // unit quaternion (w, x, y, z)
 };
 
 // Reference mesh (the reference's Mesh + BasicNodePatch list).
+// A curved triangle patch with G1 blending across its edges: the PNG1
+// construction (PNG1_Triangle, src/bem/png1.h:86-121 and 135-346), a cubic
+// Bezier-like surface whose edge control points lie in the tangent planes of
+// the corner normals and whose centre control point blends three variants,
+// each built in the averaged tangent plane of one edge.  Barycentric
+// (u, v, w = 1 - u - v): w weights vertex 1, u vertex 2, v vertex 3.
+struct CurvedTri {
+    Vec3 b300, b030, b003;                          // corners
+    Vec3 p1_210, p1_201, p2_120, p2_021, p3_102, p3_012;  // corner tangent-plane points
+    Vec3 p2_201, p2_102, p3_120, p3_210, p1_012, p1_021;  // the same carried to the next corner
+    Vec3 c12_1, c12_2, c23_2, c23_3, c31_3, c31_1;        // centre variants per edge
+    // vertices, vertex normals n1..n3, and the planar normals of the
+    // triangles across edges 1-2, 2-3, 3-1
+    CurvedTri(const Vec3& v1, const Vec3& v2, const Vec3& v3, const Vec3& n1, const Vec3& n2,
+              const Vec3& n3, const Vec3& n12, const Vec3& n23, const Vec3& n31);
+    Vec3 point(double u, double v) const;
+    // d/du and d/dv with the blended control points held fixed (as the
+    // construction's jacobian and normal take them)
+    void tangents(double u, double v, Vec3& du, Vec3& dv) const;
+};
+
 struct Mesh {
     std::vector<Vec3> verts;
     std::vector<std::array<int, 3>> tris;          // outward CCW
@@ -69,6 +90,8 @@ struct Mesh {
                                                    // mesh.cpp:223-233): kept by finalize
     std::vector<double> init_f, init_h;            // f and h per node patch from a .mtz fh member
                                                    // (mesh.cpp:630-650), empty if none
+    bool curved = false;                           // PNG1 curved patches instead of planar ones
+    std::vector<CurvedTri> ctri;                   // per triangle, when curved
     double radius = 0;                             // max |patch centroid - centre|
 
     void finalize();                               // adjacency, patches, centre, radius

@@ -9,6 +9,8 @@
 #include "host.hpp"
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -85,6 +87,184 @@ Vec3 Quat::rotate(const Vec3& v) const {
     return v + t * w + u.cross(t);
 }
 
+// ---- curved patches (PNG1) ----
+namespace {
+
+// pt projected onto the plane through `origin` with normal n, then pulled to a
+// third of its original distance from origin along that direction
+Vec3 tangent_third(const Vec3& pt, const Vec3& origin, const Vec3& n) {
+    const Vec3 proj = pt + n * n.dot(origin - pt);
+    return origin + (proj - origin).unit() * ((origin - pt).norm() / 3.0);
+}
+
+// In-plane frame of triangle (a, b, c): origin a, x along b - a (length
+// `scale`), y = unit(planar normal x x); (dx, dy) are c's coordinates.
+struct Frame {
+    Vec3 o, x, y;
+    double scale, dx, dy;
+    Frame(const Vec3& a, const Vec3& b, const Vec3& c) {
+        o = a;
+        x = b - a;
+        scale = x.norm();
+        x = x / scale;
+        const Vec3 n = (b - a).cross(c - a).unit();
+        y = n.cross(x).unit();
+        dx = (c - a).dot(x);
+        dy = (c - a).dot(y);
+    }
+};
+
+// Third vertex of each triangle carried into the other's frame, scaled so the
+// two base edges match: (first's third vertex in the second, second's third
+// vertex in the first)
+void swap_thirds(const Frame& f1, const Frame& f2, Vec3& one_in_two, Vec3& two_in_one) {
+    one_in_two = f2.o + (f2.x * f1.dx + f2.y * f1.dy) * (f2.scale / f1.scale);
+    two_in_one = f1.o + (f1.x * f2.dx + f1.y * f2.dy) * (f1.scale / f2.scale);
+}
+
+// Third vertex of frame f placed on the base a -> b in the plane with normal n
+Vec3 third_on(const Frame& f, const Vec3& a, const Vec3& b, const Vec3& n) {
+    Vec3 x = b - a;
+    const double scale = x.norm();
+    x = x / scale;
+    const Vec3 y = n.cross(x).unit();
+    return a + (x * f.dx + y * f.dy) * (scale / f.scale);
+}
+
+// rational blend of two control points by a coordinate t: a at t = 0, b at t = 1
+inline Vec3 blend(double t, const Vec3& a, const Vec3& b) {
+    const double s = 1.0 - t, ss = s * s, tt = t * t;
+    return (a * ss + b * tt) / (tt + ss);
+}
+
+}  // namespace
+
+CurvedTri::CurvedTri(const Vec3& v1, const Vec3& v2, const Vec3& v3, const Vec3& n1, const Vec3& n2,
+                     const Vec3& n3, const Vec3& n12, const Vec3& n23, const Vec3& n31)
+    : b300(v1), b030(v2), b003(v3) {
+    p1_210 = tangent_third(v2, v1, n1);
+    p1_201 = tangent_third(v3, v1, n1);
+    p2_120 = tangent_third(v1, v2, n2);
+    p2_021 = tangent_third(v3, v2, n2);
+    p3_102 = tangent_third(v1, v3, n3);
+    p3_012 = tangent_third(v2, v3, n3);
+    swap_thirds(Frame(b300, p1_210, p1_201), Frame(p2_120, b030, p2_021), p1_021, p2_201);
+    swap_thirds(Frame(b030, p2_021, p2_120), Frame(p3_012, b003, p3_102), p2_102, p3_120);
+    swap_thirds(Frame(b003, p3_102, p3_012), Frame(p1_201, b300, p1_210), p3_210, p1_012);
+    const Vec3 nt = (v2 - v1).cross(v3 - v1).unit();
+    const Vec3 a12 = (nt + n12).unit(), a23 = (nt + n23).unit(), a31 = (nt + n31).unit();
+    c12_1 = third_on(Frame(b300, p1_210, p1_201), p1_210, p2_120, a12);
+    c12_2 = third_on(Frame(p2_120, b030, p2_021), p1_210, p2_120, a12);
+    c23_2 = third_on(Frame(b030, p2_021, p2_120), p2_021, p3_012, a23);
+    c23_3 = third_on(Frame(p3_012, b003, p3_102), p2_021, p3_012, a23);
+    c31_3 = third_on(Frame(b003, p3_102, p3_012), p3_102, p1_201, a31);
+    c31_1 = third_on(Frame(p1_201, b300, p1_210), p3_102, p1_201, a31);
+}
+
+namespace {
+struct Ctl {
+    Vec3 b210, b120, b201, b021, b102, b012, b111;
+};
+
+Ctl controls(const CurvedTri& c, double u, double v) {
+    const double w = 1.0 - u - v;
+    Ctl k;
+    k.b201 = blend(u, c.p1_201, c.p2_201);
+    k.b102 = blend(u, c.p3_102, c.p2_102);
+    k.b012 = blend(w, c.p3_012, c.p1_012);
+    k.b021 = blend(w, c.p2_021, c.p1_021);
+    k.b120 = blend(v, c.p2_120, c.p3_120);
+    k.b210 = blend(v, c.p1_210, c.p3_210);
+    // the centre point: the three edge variants blended by weights that pick
+    // the variant of the nearest edge (shifted off the corners, where the
+    // weights are 0/0)
+    const double eps = 1e-6;
+    double a = u + eps, b = v + eps, g = w + eps;
+    const double tot = a + b + g;
+    a /= tot;
+    b /= tot;
+    g /= tot;
+    const double sum = a * b + a * g + b * g;
+    const Vec3 t1 = c.c12_1 * ((1.0 - a) * g) + c.c12_2 * ((1.0 - g) * a);
+    const Vec3 t2 = c.c23_2 * ((1.0 - b) * a) + c.c23_3 * ((1.0 - a) * b);
+    const Vec3 t3 = c.c31_3 * ((1.0 - g) * b) + c.c31_1 * ((1.0 - b) * g);
+    k.b111 = t1 * ((a * g / sum) / (g + a - 2.0 * a * g)) + t2 * ((a * b / sum) / (a + b - 2.0 * a * b)) +
+             t3 * ((b * g / sum) / (g + b - 2.0 * b * g));
+    return k;
+}
+}  // namespace
+
+Vec3 CurvedTri::point(double u, double v) const {
+    const double w = 1.0 - u - v;
+    const Ctl k = controls(*this, u, v);
+    return b300 * (w * w * w) + b030 * (u * u * u) + b003 * (v * v * v) + k.b210 * (3 * w * w * u) +
+           k.b120 * (3 * w * u * u) + k.b201 * (3 * w * w * v) + k.b021 * (3 * u * u * v) +
+           k.b102 * (3 * w * v * v) + k.b012 * (3 * u * v * v) + k.b111 * (6 * w * u * v);
+}
+
+void CurvedTri::tangents(double u, double v, Vec3& du, Vec3& dv) const {
+    const double w = 1.0 - u - v;
+    const Ctl k = controls(*this, u, v);
+    du = b030 * (3 * u * u) - b300 * (3 * w * w) + k.b120 * (6 * u * w - 3 * u * u) + k.b021 * (6 * u * v) -
+         k.b102 * (3 * v * v) + k.b012 * (3 * v * v) + k.b210 * (3 * w * w - 6 * w * u) - k.b201 * (6 * w * v) +
+         k.b111 * (6 * w * v - 6 * u * v);
+    dv = b003 * (3 * v * v) - b300 * (3 * w * w) - k.b120 * (3 * u * u) + k.b021 * (3 * u * u) +
+         k.b102 * (6 * v * w - 3 * v * v) + k.b012 * (6 * v * u) - k.b210 * (6 * w * u) +
+         k.b201 * (3 * w * w - 6 * w * v) + k.b111 * (6 * w * u - 6 * u * v);
+}
+
+// Vertex normals (Vertex::set_normal, src/bem/vertex.cpp:86-141: the
+// area-weighted mean of the planar normals around the vertex, triangles below
+// 1e-12 in area left out) and the planar normals across each edge
+// (Triangle::set_adjacent_normal, src/bem/triangle.h:737-757, over the
+// mesh's edges, src/bem/mesh.cpp:997-1004); then one curved triangle each.
+static void build_curved(Mesh& m) {
+    const size_t nt = m.tris.size();
+    std::vector<Vec3> tn(nt);
+    std::vector<double> ta(nt);
+    for (size_t t = 0; t < nt; ++t) {
+        const auto& tr = m.tris[t];
+        const Vec3 cr = (m.verts[tr[1]] - m.verts[tr[0]]).cross(m.verts[tr[2]] - m.verts[tr[0]]);
+        ta[t] = 0.5 * cr.norm();
+        tn[t] = cr.unit();
+    }
+    std::vector<Vec3> vn(m.verts.size());
+    for (size_t v = 0; v < m.verts.size(); ++v) {
+        Vec3 s;
+        for (int t : m.vert_tris[v])
+            if (ta[t] >= 1e-12) s += tn[t] * ta[t];
+        if (s.dot(s) < 1e-12) throw std::runtime_error("curved patches: a vertex has no usable triangle");
+        vn[v] = s.unit();
+    }
+    // edge (lo, hi) -> the triangles on it
+    std::unordered_map<uint64_t, std::array<int, 2>> edge;
+    edge.reserve(nt * 2);
+    auto ekey = [](int a, int b) {
+        return (uint64_t(std::min(a, b)) << 32) | uint64_t(uint32_t(std::max(a, b)));
+    };
+    for (size_t t = 0; t < nt; ++t)
+        for (int k = 0; k < 3; ++k) {
+            auto it = edge.emplace(ekey(m.tris[t][k], m.tris[t][(k + 1) % 3]), std::array<int, 2>{-1, -1}).first;
+            if (it->second[0] < 0) it->second[0] = int(t);
+            else if (it->second[1] < 0) it->second[1] = int(t);
+            else throw std::runtime_error("curved patches: an edge has more than two triangles");
+        }
+    m.ctri.clear();
+    m.ctri.reserve(nt);
+    for (size_t t = 0; t < nt; ++t) {
+        const auto& tr = m.tris[t];
+        Vec3 across[3];  // edges 1-2, 2-3, 3-1
+        for (int k = 0; k < 3; ++k) {
+            const auto& e = edge.at(ekey(tr[k], tr[(k + 1) % 3]));
+            const int other = e[0] == int(t) ? e[1] : e[0];
+            if (other < 0) throw std::runtime_error("curved patches need a closed surface");
+            across[k] = tn[other];
+        }
+        m.ctri.emplace_back(m.verts[tr[0]], m.verts[tr[1]], m.verts[tr[2]], vn[tr[0]], vn[tr[1]], vn[tr[2]],
+                            across[0], across[1], across[2]);
+    }
+}
+
 void Mesh::patch_points(int vid, const Rule& rule, std::vector<QuadPt>& out) const {
     const double third = 1.0 / 3.0;
     const Vec3 centre_uvw(third, third, third);
@@ -113,6 +293,18 @@ void Mesh::patch_points(int vid, const Rule& rule, std::vector<QuadPt>& out) con
             for (size_t k = 0; k < rule.w.size(); ++k) {
                 const double ru = rule.u[k], rv = rule.v[k];
                 Vec3 uvw = P1 * (1.0 - ru - rv) + P2 * ru + P3 * rv;
+                if (curved) {
+                    // the parametric point mapped onto the curved patch; the
+                    // weight takes the surface jacobian |dS/du x dS/dv|, which
+                    // is twice the area for a flat triangle
+                    const CurvedTri& c = ctri[t];
+                    Vec3 du, dv;
+                    c.tangents(uvw.x, uvw.y, du, dv);
+                    const Vec3 cr2 = du.cross(dv);
+                    const double jac = std::sqrt(du.dot(du) * dv.dot(dv) - du.dot(dv) * du.dot(dv));
+                    out.push_back(QuadPt{c.point(uvw.x, uvw.y), cr2.unit(), rule.w[k] / 6.0 * jac / 2.0});
+                    continue;
+                }
                 // each sub-triangle is one sixth of the triangle
                 const double wt = rule.w[k] * tri_area / 6.0;
                 Vec3 pos = v1 * uvw.z + v2 * uvw.x + v3 * uvw.y;
@@ -126,6 +318,8 @@ void Mesh::finalize() {
     vert_tris.assign(verts.size(), {});
     for (size_t t = 0; t < tris.size(); ++t)
         for (int k = 0; k < 3; ++k) vert_tris[tris[t][k]].push_back(int(t));
+    if (curved) build_curved(*this);
+    else ctri.clear();
     const Rule& g4 = rule_for_order(4);
     centroid.resize(verts.size());
     normal.resize(verts.size());

@@ -74,6 +74,32 @@ extern "C" int beep_mesh_set_charges(beep_mesh* m, int n, const double* xyzq) {
     });
 }
 
+extern "C" int beep_mesh_set_curved(beep_mesh* m, int on) {
+    return guarded([&] {
+        if (!m) throw std::runtime_error("null mesh");
+        if (m->mesh.curved == (on != 0)) return;
+        m->mesh.curved = on != 0;
+        m->mesh.finalize();  // patches (centroid, normal, area) on the new surface
+        m->stamp = next_mesh_stamp();
+    });
+}
+
+extern "C" int beep_mesh_curved_point(const beep_mesh* m, int tri, double u, double v, double* p3,
+                                      double* n3) {
+    return guarded([&] {
+        if (!m || !p3 || !n3) throw std::runtime_error("null argument");
+        if (!m->mesh.curved) throw std::runtime_error("the mesh has planar patches");
+        if (tri < 0 || tri >= int(m->mesh.ctri.size())) throw std::runtime_error("no such triangle");
+        const CurvedTri& c = m->mesh.ctri[size_t(tri)];
+        const Vec3 p = c.point(u, v);
+        Vec3 du, dv;
+        c.tangents(u, v, du, dv);
+        const Vec3 n = du.cross(dv).unit();
+        p3[0] = p.x; p3[1] = p.y; p3[2] = p.z;
+        n3[0] = n.x; n3[1] = n.y; n3[2] = n.z;
+    });
+}
+
 extern "C" int beep_mesh_charges_ball(beep_mesh* m, int n, double rmax, double qmin, double qmax,
                                       uint64_t seed) {
     return guarded([&] {
@@ -237,6 +263,9 @@ extern "C" int beep_scene_build(beep_scene* s) {
         }
         const int N = int(plist.size());
         // point counts: one centroid point for rule 0, else k points on each of
+        for (const Instance& in : s->inst)
+            if (in.mesh->mesh.curved && !s->host_points)
+                throw std::runtime_error("curved patches need host point sets (the device generates planar ones)");
         // the 2 x valence sub-triangles of the patch
         for (int k = 0; k < 3; ++k) {
             s->ptr[k].assign(size_t(N) + 1, 0);

@@ -68,6 +68,18 @@ class Mesh:
             check_host(rc)
         return None if rc == 1 else (f, h)
 
+    def set_curved(self, on: bool = True) -> "Mesh":
+        """Curved (PNG1) node patches instead of planar ones."""
+        check_host(lib().beep_mesh_set_curved(self._h, 1 if on else 0))
+        self.curved = bool(on)
+        return self
+
+    def curved_point(self, tri: int, u: float, v: float):
+        """(point, unit normal) of curved triangle `tri` at barycentric (u, v)."""
+        p, n = np.zeros(3), np.zeros(3)
+        check_host(lib().beep_mesh_curved_point(self._h, int(tri), float(u), float(v), _dp(p), _dp(n)))
+        return p, n
+
     def clone(self) -> "Mesh":
         h = vp()
         check_host(lib().beep_mesh_clone(self._h, C.byref(h)))

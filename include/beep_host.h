@@ -63,6 +63,19 @@ int beep_mesh_read_mtz(const char* path, beep_mesh** out);
  * (Mesh::init_fh_vals, mesh.cpp:630-650): returns 0, or 1 if the mesh has
  * none. */
 int beep_mesh_initial_fh(const beep_mesh* m, double* f, double* h);
+/* Curved node patches (on = 1) or planar ones (0, the default): the
+ * reference's PNG1 triangles (PNG1_Triangle, src/bem/png1.h:86-346; chosen
+ * by Mesh::init unless force_planar, mesh.cpp:65-72), with vertex normals
+ * from the area-weighted planar normals (Vertex::set_normal, vertex.cpp:
+ * 86-141) and the planar normals across each edge (triangle.h:737-757).  The
+ * node patches' points, normals, weights (surface jacobian), centroids and
+ * areas follow the curved surface.  Scenes with a curved mesh need host point
+ * sets (the device point generation is planar). */
+int beep_mesh_set_curved(beep_mesh* m, int on);
+/* Point and unit normal of curved triangle `tri` (the mesh's triangle order)
+ * at barycentric (u, v) (w = 1 - u - v weights its first vertex): the
+ * surface PNG1_Triangle::get_point / get_normal evaluate (png1.h:135-207). */
+int beep_mesh_curved_point(const beep_mesh* m, int tri, double u, double v, double* p3, double* n3);
 /* independent copy (surface, patches, charges, star shape) */
 int beep_mesh_clone(const beep_mesh* m, beep_mesh** out);
 void beep_mesh_destroy(beep_mesh* m);

@@ -1,0 +1,124 @@
+"""Curved node patches (the reference's PNG1 triangles, src/bem/png1.h),
+pinned by what the geometry fixes on a sphere: the curved patches' points,
+area and normals sit closer to the true sphere than the planar ones, and the
+BEM answers that depend on the surface (Born energy of a central charge,
+the double-layer identity) move toward their closed forms.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from beep3_amd._lib import BeepError
+from beep3_amd import Mesh, Scene
+
+
+def scene(sub, curved, radius=2.0, kappa=0.125, charges=((0, 0, 0, 1.0),), qual=0):
+    m = Mesh.icosphere(sub, radius).set_charges(np.array(charges, float))
+    if curved:
+        m.set_curved(True)
+    return Scene(80.0, kappa, 6, qual, 1500).add_instance(m, 2.0).build()
+
+
+@pytest.mark.parametrize("sub", [2, 3])
+def test_curved_points_area_and_normals_follow_the_sphere(sub):
+    r = 2.0
+    out = {}
+    for curved in (False, True):
+        a = scene(sub, curved, r, qual=6).arrays()
+        q = a["quad"]  # x, y, z, nx, ny, nz, w
+        p, n = q[:, 0:3], q[:, 3:6]
+        rad = np.linalg.norm(p, axis=1)
+        dist = np.max(np.abs(rad - r)) / r
+        ndev = np.max(np.arccos(np.clip(np.sum(n * p, axis=1) / rad, -1, 1)))
+        area_err = abs(a["area"].sum() - 4 * math.pi * r * r) / (4 * math.pi * r * r)
+        # the quadrature weights of every patch add up to its area
+        wsum = np.add.reduceat(q[:, 6], a["quad_ptr"][:-1])
+        wdev = np.max(np.abs(wsum - a["area"]) / a["area"])
+        out[curved] = (dist, ndev, area_err, wdev)
+    print("sub", sub, "(max |r - R| / R, max normal angle, area error, weight-sum dev) planar",
+          out[False], "curved", out[True])
+    assert out[True][0] < out[False][0] / 5
+    assert out[True][1] < out[False][1] / 2
+    assert out[True][2] < out[False][2] / 5
+    # planar: exact up to the rounded 4-point weights of the area rule (sum 1 - 1e-12)
+    assert out[True][3] < 1e-3 and out[False][3] < 1e-11
+
+
+def test_curved_double_layer_and_born_convergence():
+    """The double-layer identity (f = 1, kappa = 0: y_f = 1 at every node,
+    the integral of the double layer over a closed surface) holds several
+    times more closely on the curved surface; the curved Born energy (central
+    +1e, eps 2 | 80, kappa 0.125) converges to the closed form under
+    refinement.  (Its error at a given size is not below the planar one: the
+    collocation point, the patch centroid, lies inside the curved surface.)"""
+    born = oracle.born_energy(1.0, 2.0, 2.0, 80.0, 0.125)
+    dl = {}
+    for curved in (False, True):
+        sc0 = scene(3, curved, kappa=0.0)
+        o0 = oracle.Oracle(sc0.arrays(), 0.0, 80.0)
+        y = o0.matvec(np.concatenate([np.ones(o0.n), np.zeros(o0.n)]))
+        dl[curved] = float(np.max(np.abs(y[: o0.n] - 1.0)))
+    errs = []
+    for sub in (2, 3):
+        o = oracle.Oracle(scene(sub, True).arrays(), 0.125, 80.0)
+        x = np.linalg.solve(o.assemble(), o.rhs())
+        errs.append(abs(o.energies(x).sum() - born) / abs(born))
+    print("double layer max dev planar", dl[False], "curved", dl[True], "; curved Born rel err", errs)
+    assert dl[True] < dl[False] / 3
+    assert errs[1] < errs[0] / 1.8 and errs[1] < 0.03
+
+
+def test_curved_triangle_corners_edges_and_vertex_normals():
+    """The curved triangles interpolate their corners, meet along shared
+    edges (an edge curve depends only on its end points and their normals),
+    and have the vertex normal at each corner."""
+    m = Mesh.icosphere(2, 2.0).set_curved(True)
+    xyz, tri = m.vertices()
+    # vertex normals: area-weighted planar normals around each vertex
+    vn = np.zeros_like(xyz)
+    for t in tri:
+        cr = np.cross(xyz[t[1]] - xyz[t[0]], xyz[t[2]] - xyz[t[0]])
+        for k in t:
+            vn[k] += cr / 2.0  # unit normal x area
+    vn /= np.linalg.norm(vn, axis=1)[:, None]
+    corners = ((0.0, 0.0), (1.0, 0.0), (0.0, 1.0))  # w = 1, u = 1, v = 1
+    for k in range(0, len(tri), 7):
+        for c, (u, v) in enumerate(corners):
+            p, n = m.curved_point(k, u, v)
+            assert np.allclose(p, xyz[tri[k][c]], atol=1e-12)
+            assert np.allclose(n, vn[tri[k][c]], atol=1e-9)
+    # shared edges: the same curve from both sides
+    edges = {}
+    for k, t in enumerate(tri):
+        for c in range(3):
+            a, b = int(t[c]), int(t[(c + 1) % 3])
+            edges.setdefault((min(a, b), max(a, b)), []).append((k, c))
+    worst = 0.0
+    for (a, b), sides in list(edges.items())[::11]:
+        assert len(sides) == 2
+        for s in np.linspace(0.1, 0.9, 5):
+            pts = []
+            for k, c in sides:
+                # corner c -> corner c+1 of triangle k, a fraction s of the way
+                # from vertex a
+                lam = np.zeros(3)
+                first = int(tri[k][c]) == a
+                lam[c], lam[(c + 1) % 3] = (1 - s, s) if first else (s, 1 - s)
+                w_, u_, v_ = lam  # corner 0 <-> w, corner 1 <-> u, corner 2 <-> v
+                pts.append(m.curved_point(k, u_, v_)[0])
+            worst = max(worst, float(np.linalg.norm(pts[0] - pts[1])))
+    assert worst < 1e-12
+
+
+def test_curved_needs_host_points_and_round_trips():
+    m = Mesh.icosphere(2, 2.0).set_charges([[0, 0, 0, 1.0]]).set_curved(True)
+    with pytest.raises(BeepError, match="host point"):
+        Scene(80.0, 0.125, 6, 0, 1500, host_points=False).add_instance(m, 2.0).build()
+    a1 = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
+    m.set_curved(False)
+    a0 = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
+    m.set_curved(True)
+    a2 = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
+    assert not np.array_equal(a0["quad"], a1["quad"])
+    assert np.array_equal(a1["quad"], a2["quad"]) and np.array_equal(a1["area"], a2["area"])
