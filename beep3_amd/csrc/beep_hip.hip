@@ -160,6 +160,7 @@ struct beep_hip_ctx {
     DevBuf<int> pair_rows, pair_ptr, pair_col, pair_i;
     DevBuf<int> cut_off_d, cut_count;  // device pair search: copy offsets, per-row counts
     DevBuf<double4> cut_sph_d;         // and the copies' bounding spheres
+    DevBuf<double4> cut_chunk_d;       // and those of each copy's chunks of 256 patches
     DevBuf<char> cut_tmp;              // scan storage
     DevBuf<float4> pcoef32;
     DevBuf<double2> pcoef64;
@@ -1222,33 +1223,41 @@ int beep_hip_share_near_cutoff(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copie
         share_common(ctx, src, n_copies, copy_offset);
         const int N = ctx->N, Ns = src->N;
         hipStream_t s = ctx->stream;
-        // a bounding sphere per copy (host centroids)
-        std::vector<double4> sph(n_copies);
-        for (int k = 0; k < n_copies; ++k) {
+        // bounding spheres (host centroids) of each copy and of each chunk of
+        // 256 consecutive patches of a copy
+        auto bound = [&](const double* c, int n) {
             double cx = 0, cy = 0, cz = 0;
-            const double* c = ctx->h_centroid.data() + 3 * size_t(copy_offset[k]);
-            for (int i = 0; i < Ns; ++i) {
+            for (int i = 0; i < n; ++i) {
                 cx += c[3 * i];
                 cy += c[3 * i + 1];
                 cz += c[3 * i + 2];
             }
-            cx /= Ns;
-            cy /= Ns;
-            cz /= Ns;
+            cx /= n;
+            cy /= n;
+            cz /= n;
             double r2 = 0;
-            for (int i = 0; i < Ns; ++i) {
+            for (int i = 0; i < n; ++i) {
                 const double dx = c[3 * i] - cx, dy = c[3 * i + 1] - cy, dz = c[3 * i + 2] - cz;
                 r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
             }
-            sph[k] = make_double4(cx, cy, cz, std::sqrt(r2) * (1.0 + 1e-12));
+            return make_double4(cx, cy, cz, std::sqrt(r2) * (1.0 + 1e-12));
+        };
+        const int nch = (Ns + 255) / 256;
+        std::vector<double4> sph(n_copies), csph(size_t(n_copies) * nch);
+        for (int k = 0; k < n_copies; ++k) {
+            const double* c = ctx->h_centroid.data() + 3 * size_t(copy_offset[k]);
+            sph[k] = bound(c, Ns);
+            for (int q = 0; q < nch; ++q)
+                csph[size_t(k) * nch + q] = bound(c + 3 * size_t(q) * 256, std::min(256, Ns - 256 * q));
         }
         ctx->cut_off_d.upload(copy_offset, size_t(n_copies), s);
         ctx->cut_sph_d.upload(sph.data(), sph.size(), s);
+        ctx->cut_chunk_d.upload(csph.data(), csph.size(), s);
         ctx->cut_count.alloc(size_t(N) + 1);
         ctx->pair_ptr.alloc(size_t(N) + 1);
         launch_set_int(s, ctx->cut_count.p + N, 0);
-        launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->patches.p, cutoff, nullptr,
-                         ctx->cut_count.p, nullptr, nullptr);
+        launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->cut_chunk_d.p, ctx->patches.p,
+                         cutoff, nullptr, ctx->cut_count.p, nullptr, nullptr);
         const size_t tb = exclusive_scan_temp_bytes(N + 1);
         ctx->cut_tmp.alloc(std::max<size_t>(tb, 1));
         launch_exclusive_scan(s, ctx->cut_tmp.p, tb, ctx->cut_count.p, ctx->pair_ptr.p, N + 1);
@@ -1263,8 +1272,8 @@ int beep_hip_share_near_cutoff(beep_hip_ctx* ctx, beep_hip_ctx* src, int n_copie
         ctx->pcoef64.alloc(2 * np);
         ctx->pair_rows.alloc(size_t(N));
         if (total > 0)
-            launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->patches.p, cutoff,
-                             ctx->pair_ptr.p, nullptr, ctx->pair_col.p, ctx->pair_i.p);
+            launch_cut_pairs(s, N, n_copies, Ns, ctx->cut_off_d.p, ctx->cut_sph_d.p, ctx->cut_chunk_d.p,
+                             ctx->patches.p, cutoff, ctx->pair_ptr.p, nullptr, ctx->pair_col.p, ctx->pair_i.p);
         launch_iota(s, N, ctx->pair_rows.p);
         ctx->n_pairs = total;
         ctx->n_pair_rows = total > 0 ? N : 0;
@@ -2003,7 +2012,7 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         add(ctx->bvec); add(ctx->rvec); add(ctx->dvec);
         add(ctx->pair_rows); add(ctx->pair_ptr); add(ctx->pair_col); add(ctx->pair_i);
         add(ctx->pcoef32); add(ctx->pcoef64); add(ctx->ns_rec); add(ctx->guard_rec); add(ctx->hvec2);
-        add(ctx->cut_off_d); add(ctx->cut_count); add(ctx->cut_sph_d); add(ctx->cut_tmp);
+        add(ctx->cut_off_d); add(ctx->cut_count); add(ctx->cut_sph_d); add(ctx->cut_chunk_d); add(ctx->cut_tmp);
         *bytes = b + ctx->fmm_arena.held;
     });
 }

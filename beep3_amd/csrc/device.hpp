@@ -220,11 +220,13 @@ void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const i
 void launch_pair_spmv(hipStream_t s, int mode, int n, int n_rows, const int* rows, const int* ptr,
                       const int* col, const float4* c32, const double2* c64, const double* x, double* y);
 // pairs between copies (contiguous ranges of ns patches at copy_off) closer
-// than rc; sphere: per copy (centre xyz, radius).  ptr null: count[i] =
+// than rc; sphere: per copy (centre xyz, radius); chunk_sph: per copy, the
+// spheres of its chunks of 256 consecutive patches.  ptr null: count[i] =
 // pairs of row i; else their columns at cols[ptr[i]..] (index order) and
 // rows_of[] = i
 void launch_cut_pairs(hipStream_t s, int N, int ncopies, int ns, const int* copy_off, const double4* sphere,
-                      const Patch64* patches, double rc, const int* ptr, int* count, int* cols, int* rows_of);
+                      const double4* chunk_sph, const Patch64* patches, double rc, const int* ptr, int* count,
+                      int* cols, int* rows_of);
 void launch_iota(hipStream_t s, int n, int* out);
 // exclusive prefix sum of n ints (temporary storage of the size the first
 // function gives)

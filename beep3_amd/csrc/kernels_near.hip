@@ -880,11 +880,14 @@ void launch_pair_coeffs(hipStream_t s, long long n_pairs, const int* pi, const i
 // found on the device: one thread per row i, testing every patch of every
 // other copy whose bounding sphere (centre, radius) comes within rc of
 // patch i, in index order.  The block stages the other copy's centroids
-// through LDS, 256 at a time, and skips a copy none of its rows can reach.
-// Pass 1 (cols null) counts per row; pass 2 writes each row's columns at
-// ptr[i] and the row of every pair.
+// through LDS, 256 at a time (a chunk), and skips a copy or a chunk none of
+// its rows can reach (chunk_sph: the bounding sphere of every chunk of 256
+// consecutive patches of each copy; the test is conservative, so the pairs
+// found are the brute-force ones).  Pass 1 (cols null) counts per row; pass
+// 2 writes each row's columns at ptr[i] and the row of every pair.
 __global__ __launch_bounds__(256) void k_cut_pairs(int N, int ncopies, int ns, const int* __restrict__ copy_off,
                                                   const double4* __restrict__ sphere,
+                                                  const double4* __restrict__ chunk_sph,
                                                   const Patch64* __restrict__ patches, double rc,
                                                   const int* __restrict__ ptr, int* __restrict__ count,
                                                   int* __restrict__ cols, int* __restrict__ rows_of) {
@@ -913,14 +916,23 @@ __global__ __launch_bounds__(256) void k_cut_pairs(int N, int ncopies, int ns, c
         }
         if (!__syncthreads_or(need)) continue;  // uniform over the block
         const int j0 = copy_off[q];
+        const int nch = (ns + 255) / 256;
         for (int t0 = 0; t0 < ns; t0 += 256) {
-            __syncthreads();  // the previous tile is consumed
+            bool need_t = false;
+            if (need) {
+                const double4 cs = chunk_sph[q * nch + t0 / 256];
+                const double sx = cx - cs.x, sy = cy - cs.y, sz = cz - cs.z;
+                const double reach = cs.w + rc;
+                need_t = sx * sx + sy * sy + sz * sz <= reach * reach * (1.0 + 1e-9);
+            }
+            // a barrier too: the previous tile is consumed
+            if (!__syncthreads_or(need_t)) continue;
             if (t0 + int(threadIdx.x) < ns) {
                 const Patch64& p = patches[j0 + t0 + threadIdx.x];
                 tile[threadIdx.x] = make_double4(p.cx, p.cy, p.cz, 0.0);
             }
             __syncthreads();
-            if (need) {
+            if (need_t) {
                 const int m = min(256, ns - t0);
                 for (int k = 0; k < m; ++k) {
                     const double4 pj = tile[k];
@@ -945,10 +957,11 @@ __global__ void k_iota(int n, int* __restrict__ out) {
 }
 
 void launch_cut_pairs(hipStream_t s, int N, int ncopies, int ns, const int* copy_off, const double4* sphere,
-                      const Patch64* patches, double rc, const int* ptr, int* count, int* cols, int* rows_of) {
+                      const double4* chunk_sph, const Patch64* patches, double rc, const int* ptr, int* count,
+                      int* cols, int* rows_of) {
     if (N <= 0) return;
     hipLaunchKernelGGL(k_cut_pairs, dim3((N + 255) / 256), dim3(256), 0, s, N, ncopies, ns, copy_off, sphere,
-                       patches, rc, ptr, count, cols, rows_of);
+                       chunk_sph, patches, rc, ptr, count, cols, rows_of);
     hip_check(hipGetLastError(), "k_cut_pairs launch");
 }
 

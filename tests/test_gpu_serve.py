@@ -136,3 +136,19 @@ def test_pose_through_server_matches_single_solve():
     assert np.array_equal(f, b.f) and np.array_equal(h, b.h)
     assert abs(r["energy"] - E_single) <= 1e-12 * abs(E_single)
     b.close()
+
+
+def test_curved_mesh_through_the_server():
+    """A curved (PNG1) mesh goes through the server on host point sets: its
+    energy matches the oracle's LU solve on the same curved arrays."""
+    m = Mesh.icosphere(4, 2.0).set_charges([[0.3, 0.0, -0.2, 1.0], [-0.5, 0.4, 0.1, -0.5]]).set_curved(True)
+    srv = Server(slots=2, keep_fh=False, mode="mixed")
+    rid = srv.submit(Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0))
+    r = srv.poll(timeout=120.0)
+    srv.close()
+    assert r is not None and r["id"] == rid and r["status"] == 0, r
+    sc = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build()
+    o = oracle.Oracle(sc.arrays(), 0.125, 80.0)
+    E_ref = o.energies(np.linalg.solve(o.assemble(), o.rhs())).sum()
+    print("curved mesh through the server: energy", r["energy"], "oracle LU", E_ref)
+    assert abs(r["energy"] - E_ref) / abs(E_ref) < 1e-5
