@@ -223,6 +223,7 @@ def _declare_hip(L):
     L.beep_hip_memcpy.argtypes = [vp, vp, vp, C.c_size_t, C.c_int]
     L.beep_hip_memory_held.argtypes = [vp, C.POINTER(C.c_size_t)]
     L.beep_hip_synchronize.argtypes = [vp]
+    L.beep_hip_gather_rows64.argtypes = [vp, vp, vp, C.c_size_t, vp]
     L.beep_hip_memory_info.argtypes = [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     L.beep_hip_device_memory.argtypes = [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
 

@@ -1985,6 +1985,15 @@ int beep_hip_memcpy(beep_hip_ctx* ctx, void* dst, const void* src, size_t bytes,
     });
 }
 
+int beep_hip_gather_rows64(beep_hip_ctx* ctx, const void* src, const int* idx, size_t rows, void* dst) {
+    return guarded(ctx, [&] {
+        require(src && idx && dst && rows > 0, "bad gather arguments");
+        require(rows <= (size_t(1) << 33), "too many rows");
+        set_device(ctx);
+        launch_gather_rows64(ctx->stream, src, idx, rows, dst);
+    });
+}
+
 int beep_hip_synchronize(beep_hip_ctx* ctx) {
     return guarded(ctx, [&] {
         set_device(ctx);

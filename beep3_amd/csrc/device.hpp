@@ -267,6 +267,8 @@ void launch_fill(hipStream_t s, int n, double v, double* y);
 void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const int* jp,
                          double* partial, double* h, double* w, double* h2);
 void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out);
+// dst row r = src row idx[r] over 64-byte rows (a scattered read of known size)
+void launch_gather_rows64(hipStream_t s, const void* src, const int* idx, size_t rows, void* dst);
 void launch_set_int(hipStream_t s, int* p, int v);
 void launch_inc_int(hipStream_t s, int* p);
 // A triangle rule for point generation: k points per sub-triangle at

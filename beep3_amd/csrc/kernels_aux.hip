@@ -416,6 +416,25 @@ void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, doubl
     hip_check(hipGetLastError(), "k_copy_col launch");
 }
 
+// Measurement aid: dst row r = src row idx[r], rows of 64 bytes (four float4
+// per row, one per thread), so every row of src is read once in the order idx
+// gives (the counters' view of a scattered read of known size).
+__global__ __launch_bounds__(256) void k_gather_rows64(const float4* __restrict__ src, const int* __restrict__ idx,
+                                                      size_t rows, float4* __restrict__ dst) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= rows * 4) return;
+    const size_t r = t >> 2;
+    dst[t] = src[size_t(idx[r]) * 4 + (t & 3)];
+}
+
+void launch_gather_rows64(hipStream_t s, const void* src, const int* idx, size_t rows, void* dst) {
+    if (rows == 0) return;
+    const size_t threads = rows * 4;
+    hipLaunchKernelGGL(k_gather_rows64, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const float4*>(src), idx, rows, static_cast<float4*>(dst));
+    hip_check(hipGetLastError(), "k_gather_rows64 launch");
+}
+
 void launch_set_int(hipStream_t s, int* p, int v) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, p, v);
     hip_check(hipGetLastError(), "k_set_int launch");

@@ -299,6 +299,10 @@ class Device:
     def copy_device(self, d_dst: int, d_src: int, nbytes: int):
         self._check(lib().beep_hip_memcpy(self._h, vp(d_dst), vp(d_src), int(nbytes), 3))
 
+    def gather_rows64(self, d_src: int, d_idx: int, rows: int, d_dst: int):
+        """dst row r = src row idx[r] over 64-byte rows (a measurement aid)."""
+        self._check(lib().beep_hip_gather_rows64(self._h, vp(d_src), vp(d_idx), int(rows), vp(d_dst)))
+
     def synchronize(self):
         self._check(lib().beep_hip_synchronize(self._h))
 

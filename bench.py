@@ -351,6 +351,10 @@ def main():
     ap.add_argument("--copy-gib", type=float, default=0.0,
                     help="one device-to-device copy of this many GiB first (a plain copy of "
                          "known size, to calibrate hardware byte counters)")
+    ap.add_argument("--gather-gib", type=float, default=0.0,
+                    help="one random gather of this many GiB of 64-byte rows first (every "
+                         "row read once in random order, plus 4-byte indices), to check the "
+                         "counter factors on scattered reads")
     ap.add_argument("--pose-seq", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -375,6 +379,17 @@ def main():
         dev.synchronize()
         dev.free(src)
         dev.free(dst)
+    if args.gather_gib > 0:
+        rows = int(args.gather_gib * 2**30) // 64
+        perm = np.random.default_rng(1).permutation(rows).astype(np.int32)
+        g_src, g_dst, g_idx = dev.alloc(rows * 64), dev.alloc(rows * 64), dev.alloc(rows * 4)
+        dev.upload(g_idx, perm)
+        dev.gather_rows64(g_src, g_idx, rows, g_dst)
+        dev.synchronize()
+        print(f"[bench] gather: {rows} rows of 64 B, reads {rows * 64 + rows * 4} B, writes {rows * 64} B",
+              file=sys.stderr, flush=True)
+        for ptr in (g_src, g_dst, g_idx):
+            dev.free(ptr)
     # what a plain device-to-device copy sustains (read + write bytes over its
     # time, best of 5 copies of 2 GiB): the practical HBM ceiling beside the
     # 8 TB/s of the data sheet

@@ -429,6 +429,12 @@ int beep_hip_device_free(beep_hip_ctx* ctx, void* p);
 /* kind: 1 = host to device, 2 = device to host, 3 = device to device */
 int beep_hip_memcpy(beep_hip_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 int beep_hip_synchronize(beep_hip_ctx* ctx);
+/* Measurement aid, no counterpart in the reference: dst row r = src row
+ * idx[r] over rows of 64 bytes (device pointers; idx a permutation or any
+ * indices below the rows of src), stream-ordered on the context stream.  A
+ * scattered read of known size, to check hardware byte counters on gathers
+ * as a plain copy checks them on streams. */
+int beep_hip_gather_rows64(beep_hip_ctx* ctx, const void* src, const int* idx, size_t rows, void* dst);
 int beep_hip_memory_info(beep_hip_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 /* device bytes held by the context (its buffers only grow: a later geometry
  * of no larger size reuses them) */
