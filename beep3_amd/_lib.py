@@ -112,6 +112,7 @@ class ServerOpts(C.Structure):
         ("solve_mode", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
         ("max_refine", C.c_int), ("inner_tol", C.c_double), ("far_mode", C.c_int),
         ("fmm_order", C.c_int), ("fmm_leaf", C.c_int), ("keep_fh", C.c_int),
+        ("near_mode", C.c_int),
     ]
 
 
@@ -124,6 +125,7 @@ class ServerResult(C.Structure):
         ("host_ms", C.c_double), ("device_ms", C.c_double), ("upload_ms", C.c_double),
         ("precompute_ms", C.c_double), ("solve_ms", C.c_double), ("bytes", C.c_double),
         ("device_free", C.c_double), ("slot", C.c_int), ("message", C.c_char * 128),
+        ("near_on_the_fly", C.c_int),
     ]
 
 
@@ -162,6 +164,7 @@ def _declare(L):
     L.beep_scene_surface.argtypes = [vp, dp, ip, ip, ip, ip]
     L.beep_server_default_opts.argtypes = [C.POINTER(ServerOpts)]
     L.beep_server_default_opts.restype = None
+    L.beep_server_estimate.argtypes = [C.POINTER(ServerOpts), vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.beep_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
     L.beep_server_submit.argtypes = [vp, vp, C.POINTER(C.c_longlong)]
     L.beep_server_poll.argtypes = [vp, C.c_double, C.POINTER(ServerResult)]

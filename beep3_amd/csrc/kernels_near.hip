@@ -700,11 +700,18 @@ void launch_near_spmv(hipStream_t s, int mode, int n, const SpmvPlan& p, const i
 // The FP64 integration of an FP32 pass's near-singular or bad entry, rounded
 // to FP32 (zero if still bad), out of line: rare, and inlined it would take
 // registers from the FP32 loop.
+// It takes indices, not patch records, so the caller keeps nothing on the
+// stack for the call.
 template <bool KAPPA>
-__device__ __attribute__((noinline)) float4 fly_redo64(const Patch64& pi, const Patch64& pj, int qb, int qn,
-                                                       const Pt64* __restrict__ qual64, int pb, int pn,
-                                                       const Pt64* __restrict__ quad64, double kappa, int sub,
+__device__ __attribute__((noinline)) float4 fly_redo64(int i, int j, int sub, const Patch64* __restrict__ patches,
+                                                       const int* __restrict__ qual_ptr,
+                                                       const Pt64* __restrict__ qual64,
+                                                       const int* __restrict__ quad_ptr,
+                                                       const Pt64* __restrict__ quad64, double kappa,
                                                        double maxv) {
+    const Patch64 pi = patches[i], pj = patches[j];
+    const int qb = qual_ptr[i], qn = qual_ptr[i + 1] - qb;
+    const int pb = quad_ptr[j], pn = quad_ptr[j + 1] - pb;
     double o[4];
     pair_coeffs<double, KAPPA>(pi, pj, qb, qn, qual64, pb, pn, quad64, kappa, sub, o);
     const float4 v = make_float4(float(o[0]), float(o[1]), float(o[2]), float(o[3]));
@@ -721,8 +728,15 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
     const Pt64* __restrict__ qual64, const Pt64* __restrict__ quad64, double kappa,
     const unsigned* __restrict__ cellkey, int ns64, double maxv, const double* __restrict__ x, int n,
     TA* __restrict__ part) {
+    // FP32 pass: the near-singular pairs of a thread (up to DEF of them) are
+    // set aside (column and leaf-pass bit) and integrated in FP64 by the
+    // whole block after the loop, so a wave does not run the FP64 path for a
+    // few of its lanes; each thread then adds its own in the order found.
+    constexpr int DEF = sizeof(T) == 4 ? 4 : 1;
     __shared__ TA xs[2][SPMV_CW];
     __shared__ TA red[2][256];
+    __shared__ int dcode[256 * DEF];
+    __shared__ TA dprod[2][256 * DEF];
     const int wi = xcd_remap(blockIdx.x, gridDim.x);
     const int4 w = work[wi];  // leaf, first row, first column, columns
     const int L = w.x, row0 = w.y, col0 = w.z, nc = w.w;
@@ -741,6 +755,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
     }
     __syncthreads();
     TA af = 0, ah = 0;
+    int nd = 0;  // pairs this thread set aside
     if (active) {
         const int i = leaf_idx[lp + row0 + r];
         const Patch64 pi = patches[i];
@@ -754,7 +769,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
             const int sub = !(cellkey && fmm_cells_adjacent(ki, cellkey[j]));
             T v[4];
             bool redo = false;
-            if constexpr (sizeof(T) == 4) redo = ns64 && near_singular(pi, pj);
+            if constexpr (sizeof(T) == 4) {
+                redo = ns64 && near_singular(pi, pj);
+                if (redo && nd < DEF) {
+                    dcode[t * DEF + nd] = 2 * c + sub;
+                    ++nd;
+                    continue;
+                }
+            }
             if (!redo) {
                 pair_coeffs<T, KAPPA>(pi, pj, qb, qn, qual, pb, pn, quad, T(kappa), sub, v);
                 const double d[4] = {double(v[0]), double(v[1]), double(v[2]), double(v[3])};
@@ -766,7 +788,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
             }
             if constexpr (sizeof(T) == 4) {
                 if (redo) {
-                    const float4 o = fly_redo64<KAPPA>(pi, pj, qb, qn, qual64, pb, pn, quad64, kappa, sub, maxv);
+                    const float4 o =
+                        fly_redo64<KAPPA>(i, j, sub, patches, qual_ptr, qual64, quad_ptr, quad64, kappa, maxv);
                     v[0] = o.x;
                     v[1] = o.y;
                     v[2] = o.z;
@@ -776,6 +799,30 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 4 : 5) void k_near_fly_part(
             const TA xf = xs[0][c], xh = xs[1][c];
             af += TA(v[1]) * xf - TA(v[0]) * xh;
             ah += TA(v[3]) * xf - TA(v[2]) * xh;
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        // the set-aside pairs, spread over the block: entry e belongs to
+        // thread e / DEF
+        for (int k = nd; k < DEF; ++k) dcode[t * DEF + k] = -1;
+        __syncthreads();
+        for (int e = t; e < 256 * DEF; e += 256) {
+            const int code = dcode[e];
+            if (code < 0) continue;
+            const int owner = e / DEF;
+            const int c = code >> 1;
+            const int i = leaf_idx[lp + row0 + owner % Rb];
+            const int j = nb_idx[np + col0 + c];
+            const float4 o =
+                fly_redo64<KAPPA>(i, j, code & 1, patches, qual_ptr, qual64, quad_ptr, quad64, kappa, maxv);
+            const TA xf = xs[0][c], xh = xs[1][c];
+            dprod[0][e] = TA(o.y) * xf - TA(o.x) * xh;
+            dprod[1][e] = TA(o.w) * xf - TA(o.z) * xh;
+        }
+        __syncthreads();
+        for (int k = 0; k < nd; ++k) {
+            af += dprod[0][t * DEF + k];
+            ah += dprod[1][t * DEF + k];
         }
     }
     red[0][t] = af;

@@ -53,16 +53,18 @@ void check_hip(beep_hip_ctx* c, int rc, const char* what) {
 // with beep_hip_memory_held after the request): point sets in both
 // precisions, coefficients of the solve's precisions, GMRES basis and
 // per-patch FMM storage (tables, expansions, M2L slots: about 12 KB/patch)
-double estimate_bytes(const beep_scene* s, const beep_server_opts& o) {
+double estimate_bytes(const beep_scene* s, const beep_server_opts& o, bool stored) {
     const double N = double(s->centroid.size());
-    const double nq = double(s->pts[0].size()), nd = double(s->pts[1].size()),
-                 ng = double(s->pts[2].size());
+    // point counts from the offsets: the point sets themselves are empty when
+    // the device generates them
+    auto count = [&](int k) { return s->ptr[k].empty() ? 0.0 : double(s->ptr[k].back()); };
+    const double nq = count(0), nd = count(1), ng = count(2);
     double nnz = 0;
     for (size_t L = 0; L + 1 < s->leaf_ptr.size(); ++L)
         nnz += double(s->leaf_ptr[L + 1] - s->leaf_ptr[L]) * double(s->nb_ptr[L + 1] - s->nb_ptr[L]);
     double b = 96.0 * (nq + nd) + 64.0 * ng + 64.0 * N;
-    if (o.solve_mode != BEEP_SOLVE_FP64) b += 16.0 * nnz;
-    if (o.solve_mode == BEEP_SOLVE_FP64) b += 32.0 * nnz;
+    if (stored && o.solve_mode != BEEP_SOLVE_FP64) b += 16.0 * nnz;
+    if (stored && o.solve_mode == BEEP_SOLVE_FP64) b += 32.0 * nnz;
     b += 16.0 * N * double(o.max_iter + 1) + 16.0 * N * 8.0;
     b += 12288.0 * N;
     return b;
@@ -283,11 +285,20 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     // admission against the device's measured free memory (OpenCL_Handler
     // compares a blob's device_mem_reqd with its own count of the device's
     // memory, opencl_handler.cpp:451-463)
-    const double need_total = estimate_bytes(sc, o);
+    bool fly = o.near_mode == BEEP_NEAR_ON_THE_FLY;
+    double need_total = estimate_bytes(sc, o, !fly);
     {
         std::unique_lock<std::mutex> lk(mu);
         double cap = o.hbm_budget > 0 ? o.hbm_budget : 1e300;
         if (device_total > 0) cap = std::min(cap, device_total - RESERVE);
+        if (need_total > cap && o.near_mode == BEEP_SERVE_NEAR_AUTO) {
+            // the stored coefficients do not fit: integrate them on the fly
+            const double need_fly = estimate_bytes(sc, o, false);
+            if (need_fly <= cap) {
+                fly = true;
+                need_total = need_fly;
+            }
+        }
         if (need_total > cap)
             throw std::runtime_error("request needs more device memory than the budget and the "
                                      "device allow");
@@ -328,6 +339,8 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     check_hip(ctx, beep_hip_set_tree(ctx, &t), "tree");
     const double tp = clock();
     r.upload_ms = (tp - td) * 1e3;
+    check_hip(ctx, beep_hip_set_near_mode(ctx, fly ? BEEP_NEAR_ON_THE_FLY : BEEP_NEAR_PRECOMPUTE), "near mode");
+    r.near_on_the_fly = fly ? 1 : 0;
     if (o.solve_mode != BEEP_SOLVE_FP64)
         check_hip(ctx, beep_hip_precompute(ctx, sc->kappa, BEEP_FP32), "precompute");
     else
@@ -448,6 +461,23 @@ void beep_server_default_opts(beep_server_opts* o) {
     o->far_mode = BEEP_FAR_AUTO;
     o->fmm_order = 5;
     o->fmm_leaf = 40;
+    o->near_mode = BEEP_SERVE_NEAR_AUTO;
+}
+
+int beep_server_estimate(const beep_server_opts* o, beep_scene* sc, double* stored_bytes, double* fly_bytes) {
+    try {
+        if (!o || !sc || !stored_bytes || !fly_bytes) throw std::runtime_error("null argument");
+        if (!sc->built) {
+            if (beep_scene_build(sc) != 0)
+                throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
+        }
+        *stored_bytes = estimate_bytes(sc, *o, true);
+        *fly_bytes = estimate_bytes(sc, *o, false);
+        return 0;
+    } catch (const std::exception& e) {
+        beep_host_set_error(e.what());
+        return -1;
+    }
 }
 
 int beep_server_create(const beep_server_opts* opts, beep_server** out) {
@@ -458,6 +488,9 @@ int beep_server_create(const beep_server_opts* opts, beep_server** out) {
         if (opts) s->o = *opts;
         else beep_server_default_opts(&s->o);
         if (s->o.slots < 1 || s->o.slots > 64) throw std::runtime_error("slots must be 1..64");
+        if (s->o.near_mode != BEEP_SERVE_NEAR_AUTO && s->o.near_mode != BEEP_NEAR_PRECOMPUTE &&
+            s->o.near_mode != BEEP_NEAR_ON_THE_FLY)
+            throw std::runtime_error("unknown near-field mode");
         if (!(s->o.hbm_budget >= 0)) throw std::runtime_error("budget must be >= 0");
         // the device's size (0 when it cannot be read: requests then fail
         // at admission with the error)

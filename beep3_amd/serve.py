@@ -19,12 +19,13 @@ from .scene import Scene
 
 SOLVE_MODES = {"fp64": 0, "mixed": 1, "fp32": 2}
 FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
+NEAR_MODES = {"auto": -1, "precompute": 0, "on_the_fly": 1}
 
 
 class Server:
     def __init__(self, slots=4, budget_gib=0.0, mode="mixed", tol=1e-6, max_iter=100,
                  max_refine=4, inner_tol=0.0, far_field="auto", fmm_order=5, fmm_leaf=40,
-                 keep_fh=False, device=0):
+                 keep_fh=False, device=0, near_mode="auto"):
         o = ServerOpts()
         lib().beep_server_default_opts(C.byref(o))
         o.device, o.slots, o.hbm_budget = int(device), int(slots), float(budget_gib) * 2.0**30
@@ -32,6 +33,8 @@ class Server:
         o.tol, o.max_iter, o.max_refine, o.inner_tol = float(tol), int(max_iter), int(max_refine), float(inner_tol)
         o.far_mode, o.fmm_order, o.fmm_leaf = FAR_MODES[far_field], int(fmm_order), int(fmm_leaf)
         o.keep_fh = 1 if keep_fh else 0
+        o.near_mode = NEAR_MODES[near_mode]
+        self._opts = o
         h = vp()
         if lib().beep_server_create(C.byref(o), C.byref(h)) != 0:
             raise BeepError(lib().beep_host_last_error().decode())
@@ -69,6 +72,14 @@ class Server:
         if lib().beep_server_take_fh(self._h, int(rid), f.ctypes.data_as(dp), h.ctypes.data_as(dp)) != 0:
             raise BeepError(lib().beep_server_last_error(self._h).decode())
         return f, h
+
+    def estimate(self, scene: Scene):
+        """(bytes with stored coefficients, bytes on the fly) that admission
+        charges this scene (built here if it is not; it stays the caller's)."""
+        st, fl = C.c_double(), C.c_double()
+        if lib().beep_server_estimate(C.byref(self._opts), scene._h, C.byref(st), C.byref(fl)) != 0:
+            raise BeepError(lib().beep_host_last_error().decode())
+        return st.value, fl.value
 
     def clock(self) -> float:
         return lib().beep_server_clock(self._h)

@@ -337,6 +337,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="globular")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-fly", action="store_true",
+                    help="skip the on-the-fly near-field timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--far", default="fmm", choices=["fmm", "pw", "direct", "auto"])
@@ -556,6 +558,48 @@ def main():
                                        / (near_alone * 1e-3) / 1e9 / copy_GBs,
         "autotune": tune,
     }
+
+    if not args.no_fly:
+        # the near field on the fly (no stored coefficients; every mat-vec
+        # integrates the explicit pairs): a second context on the same
+        # geometry, tree and far field, timed per mat-vec and checked against
+        # the stored-coefficient mat-vec on the same x
+        from beep3_amd.device import Device
+
+        fd = Device(0)
+        try:
+            fd.set_far_field(b.far_field, b.fmm_order, args.leaf)
+            fd.set_near_mode("on_the_fly")
+            sc_ = b.scene
+            if sc_.host_points:
+                fd.set_geometry(a)
+            else:
+                fd.set_surface(a, sc_.surface(), b.qual, b.quad)
+            fd.set_tree(a)
+            fly = {}
+            fx, fy = fd.alloc(16 * N), fd.alloc(16 * N)
+            fd.upload(fx, x)
+            for p in ("fp64", "fp32"):
+                fd.precompute(b.kappa, p)
+                fd.matvec_device(fx, fy, p)  # plan and tables for this precision
+                fd.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    fd.matvec_device(fx, fy, p)
+                fd.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / 3
+                y_fly = fd.download(np.empty(2 * N), fy)
+                dev.matvec_device(d_x, d_y, p)
+                y_ref = dev.download(np.empty(2 * N), d_y)
+                fly[p] = {"ms_per_matvec": ms,
+                          "integrand_tflops": n_evals * FLOP_PER_EVAL / (ms * 1e-3) / 1e12,
+                          "rel_diff_vs_stored": float(np.linalg.norm(y_fly - y_ref) / np.linalg.norm(y_ref)),
+                          "device_bytes_held": fd.memory_held()}
+            fly["stored_context_bytes_held"] = dev.memory_held()
+            extra["near_on_the_fly"] = fly
+            print(f"[bench] near field on the fly: {fly}", file=sys.stderr, flush=True)
+        finally:
+            fd.close()
 
     if not args.no_solve:
         solves = {}

@@ -46,12 +46,24 @@ typedef struct {
     int fmm_order;
     int fmm_leaf;
     int keep_fh;            /* keep f and h of each request for beep_server_take_fh */
+    int near_mode;          /* BEEP_NEAR_PRECOMPUTE, BEEP_NEAR_ON_THE_FLY (beep_hip.h), or
+                               BEEP_SERVE_NEAR_AUTO: stored coefficients when the request
+                               fits the memory the budget and the device allow with them,
+                               else on the fly (the reference's OpenCL path for blobs that
+                               do not fit, opencl_bem.cpp:346-747) */
 } beep_server_opts;
+
+#define BEEP_SERVE_NEAR_AUTO (-1)
 
 /* fills the defaults: device 0, 4 slots, no budget cap, mixed, tol 1e-6, 100
  * iterations, 4 refinements, inner tol / 2, far field auto, FMM order 5, leaf
- * 40, f and h not kept */
+ * 40, f and h not kept, near field auto */
 void beep_server_default_opts(beep_server_opts* o);
+/* The device bytes the server's admission charges a scene (built here if it
+ * is not): with stored near coefficients and on the fly.  The scene stays
+ * the caller's. */
+int beep_server_estimate(const beep_server_opts* o, beep_scene* scene, double* stored_bytes,
+                         double* fly_bytes);
 
 typedef struct {
     long long id;
@@ -73,6 +85,7 @@ typedef struct {
     double device_free;     /* free device bytes read at admission */
     int slot;
     char message[128];
+    int near_on_the_fly;    /* 1 when the near field ran on the fly (near_mode) */
 } beep_result;
 
 int beep_server_create(const beep_server_opts* opts, beep_server** out);

@@ -152,3 +152,29 @@ def test_curved_mesh_through_the_server():
     E_ref = o.energies(np.linalg.solve(o.assemble(), o.rhs())).sum()
     print("curved mesh through the server: energy", r["energy"], "oracle LU", E_ref)
     assert abs(r["energy"] - E_ref) / abs(E_ref) < 1e-5
+
+
+def test_near_field_on_the_fly_when_coefficients_do_not_fit():
+    """A budget below the stored-coefficient estimate but above the
+    on-the-fly one: the auto mode integrates the near field on the fly and
+    the energy matches the stored-coefficient solve."""
+    m = Mesh.icosphere(4, 2.0).set_charges([[0.3, 0.0, -0.2, 1.0], [-0.5, 0.4, 0.1, -0.5]])
+    probe = Server(slots=1)
+    stored, fly = probe.estimate(Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0))
+    probe.close()
+    res = {}
+    for name, kw in (("stored", dict(near_mode="precompute")),
+                     ("forced", dict(near_mode="on_the_fly")),
+                     ("auto", dict(near_mode="auto", budget_gib=(stored + fly) / 2 / 2.0**30))):
+        srv = Server(slots=1, mode="mixed", tol=1e-10, **kw)
+        rid = srv.submit(Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0))
+        r = srv.poll(timeout=120.0)
+        srv.close()
+        assert r is not None and r["id"] == rid and r["status"] == 0, r
+        res[name] = r
+    print("near field stored / forced / auto on the fly:",
+          [(k, v["energy"], v["near_on_the_fly"]) for k, v in res.items()], "estimates", stored, fly)
+    assert res["stored"]["near_on_the_fly"] == 0
+    assert res["forced"]["near_on_the_fly"] == 1 and res["auto"]["near_on_the_fly"] == 1
+    for k in ("forced", "auto"):
+        assert abs(res[k]["energy"] - res["stored"]["energy"]) <= 1e-8 * abs(res["stored"]["energy"])

@@ -73,3 +73,19 @@ def test_oversized_request_is_refused():
     assert r is not None and r["id"] == rid and r["status"] < 0
     assert "budget" in r["message"]
     srv.close()
+
+
+def test_estimate_counts_stored_coefficients():
+    """The admission estimate with stored coefficients exceeds the on-the-fly
+    one by the coefficient bytes (16 per explicit pair in mixed mode)."""
+    import numpy as np
+
+    srv = Server(slots=1)
+    m = Mesh.icosphere(3, 2.0).set_charges([[0, 0, 0, 1.0]])
+    sc = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0)
+    stored, fly = srv.estimate(sc)
+    sc.build()
+    a = sc.arrays()
+    nnz = int(np.sum(np.diff(a["leaf_ptr"]) * np.diff(a["nb_ptr"])))
+    assert fly > 0 and stored - fly == 16.0 * nnz
+    srv.close()
