@@ -460,7 +460,7 @@ void beep_server_default_opts(beep_server_opts* o) {
     o->inner_tol = 0.0;  // half the outer tolerance
     o->far_mode = BEEP_FAR_AUTO;
     o->fmm_order = 5;
-    o->fmm_leaf = 40;
+    o->fmm_leaf = 64;
     o->near_mode = BEEP_SERVE_NEAR_AUTO;
 }
 
@@ -701,7 +701,7 @@ void beep_pose_default_opts(beep_pose_opts* o) {
     o->inner_tol = 0.0;
     o->far_mode = BEEP_FAR_FMM;
     o->fmm_order = 5;
-    o->fmm_leaf = 40;
+    o->fmm_leaf = 64;
     o->pair_cutoff = 0.0;
     o->gram_schmidt = 1;
     o->keep_fh = 0;

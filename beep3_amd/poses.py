@@ -25,7 +25,7 @@ FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
 class PoseBatch:
     def __init__(self, mesh, Dsolvent=80.0, kappa=0.125, quad=6, qual=0, nbsize=1500, Dprotein=2.0,
                  slots=8, mode="mixed", tol=1e-6, max_iter=100, max_refine=4, inner_tol=0.0,
-                 far_field="fmm", fmm_order=5, fmm_leaf=40, pair_cutoff=0.0, gram_schmidt=1,
+                 far_field="fmm", fmm_order=5, fmm_leaf=64, pair_cutoff=0.0, gram_schmidt=1,
                  keep_fh=False, device=0):
         o = PoseOpts()
         lib().beep_pose_default_opts(C.byref(o))

@@ -24,7 +24,7 @@ NEAR_MODES = {"auto": -1, "precompute": 0, "on_the_fly": 1}
 
 class Server:
     def __init__(self, slots=4, budget_gib=0.0, mode="mixed", tol=1e-6, max_iter=100,
-                 max_refine=4, inner_tol=0.0, far_field="auto", fmm_order=5, fmm_leaf=40,
+                 max_refine=4, inner_tol=0.0, far_field="auto", fmm_order=5, fmm_leaf=64,
                  keep_fh=False, device=0, near_mode="auto"):
         o = ServerOpts()
         lib().beep_server_default_opts(C.byref(o))

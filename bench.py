@@ -198,7 +198,7 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
         return res, t_end - t_start
 
     srv = Server(slots=slots, mode="mixed", far_field="auto", fmm_order=5,
-                 fmm_leaf=40)
+                 fmm_leaf=64)
     res, span = run(srv, sched, paced=True)
     srv.close()
     ok = [r for r in res if r["status"] == 0]
@@ -223,13 +223,13 @@ def serving_bench(seconds, slots, seq_requests, rate=50.0):
     print(f"[bench] serving: {out}", file=sys.stderr, flush=True)
     # capacity: the same requests submitted at once (closed loop)
     srv = Server(slots=slots, mode="mixed", far_field="auto", fmm_order=5,
-                 fmm_leaf=40)
+                 fmm_leaf=64)
     cres, cspan = run(srv, [dict(q, t=0.0) for q in sched], paced=True)
     srv.close()
     out["capacity_solves_per_s"] = sum(1 for r in cres if r["status"] == 0) / cspan
     print(f"[bench] serving capacity: {out['capacity_solves_per_s']:.1f} solves/s",
           file=sys.stderr, flush=True)
-    srv = Server(slots=1, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=40)
+    srv = Server(slots=1, mode="fp64", far_field="auto", fmm_order=5, fmm_leaf=64)
     sres, sspan = run(srv, sched[:seq_requests], paced=False)
     srv.close()
     sok = [r for r in sres if r["status"] == 0]
@@ -258,7 +258,7 @@ def pose_bench(n_poses, slots, seq_poses):
     rng = np.random.default_rng(7)  # the generator of configs.pose_scenes
     poses = [configs.random_pose(rng) for _ in range(n_poses)]
     t0 = time.perf_counter()
-    pb = PoseBatch(mesh, slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    pb = PoseBatch(mesh, slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=64)
     setup_s = time.perf_counter() - t0
     info = pb.info()
     t0 = time.perf_counter()
@@ -295,7 +295,7 @@ def pose_bench(n_poses, slots, seq_poses):
                 res.append(r)
         return res, max(r["t_done"] for r in res) - t0
 
-    srv = Server(slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    srv = Server(slots=slots, mode="mixed", far_field="fmm", fmm_order=5, fmm_leaf=64)
     res, span = run(srv, scenes, False)
     srv.close()
     ok = [r for r in res if r["status"] == 0]
@@ -321,7 +321,7 @@ def pose_bench(n_poses, slots, seq_poses):
     print(f"[bench] poses through the server (reference split): "
           f"{ {k: v for k, v in out.items() if k != 'pose_batch'} }", file=sys.stderr, flush=True)
     seq = configs.pose_scenes(mesh, seq_poses)
-    srv = Server(slots=1, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=40)
+    srv = Server(slots=1, mode="fp64", far_field="fmm", fmm_order=5, fmm_leaf=64)
     sres, sspan = run(srv, seq, True)
     srv.close()
     out["sequential_plain_fp64"] = {"poses": seq_poses, "poses_per_s": len(sres) / sspan}

@@ -57,7 +57,7 @@ typedef struct {
 
 /* fills the defaults: device 0, 4 slots, no budget cap, mixed, tol 1e-6, 100
  * iterations, 4 refinements, inner tol / 2, far field auto, FMM order 5, leaf
- * 40, f and h not kept, near field auto */
+ * 64, f and h not kept, near field auto */
 void beep_server_default_opts(beep_server_opts* o);
 /* The device bytes the server's admission charges a scene (built here if it
  * is not): with stored near coefficients and on the fly.  The scene stays
@@ -140,7 +140,7 @@ typedef struct {
 } beep_pose_opts;
 
 /* device 0, 8 slots, mixed, tol 1e-6, 100 iterations, 4 refinements, inner
- * tol / 2, FMM order 5 leaf 40, cutoff from the mesh, 1 pass, f and h not
+ * tol / 2, FMM order 5 leaf 64, cutoff from the mesh, 1 pass, f and h not
  * kept */
 void beep_pose_default_opts(beep_pose_opts* o);
 
