@@ -63,6 +63,7 @@ class HipSurface(C.Structure):
         ("centroid", vp), ("normal", vp), ("area", vp), ("eps_ratio", vp), ("gc", vp),
         ("verts", vp), ("n_tris", C.c_int), ("tris", vp), ("vt_ptr", vp), ("vt_tri", vp),
         ("vt_corner", vp), ("qual_rule", C.c_int), ("quad_rule", C.c_int),
+        ("n_curved", C.c_int), ("ctl_idx", vp), ("ctl", vp),
     ]
 
 
@@ -162,6 +163,7 @@ def _declare(L):
     L.beep_scene_set_host_points.argtypes = [vp, C.c_int]
     L.beep_scene_set_tree.argtypes = [vp, C.c_int]
     L.beep_scene_surface.argtypes = [vp, dp, ip, ip, ip, ip]
+    L.beep_scene_surface_curved.argtypes = [vp, ip, ip, dp]
     L.beep_server_default_opts.argtypes = [C.POINTER(ServerOpts)]
     L.beep_server_default_opts.restype = None
     L.beep_server_estimate.argtypes = [C.POINTER(ServerOpts), vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]

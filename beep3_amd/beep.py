@@ -69,9 +69,7 @@ class BEEP:
         """Build patches and octree (host), upload to the GPU and precompute
         the explicit-pair coefficients (precalc_neighbour_interactions)."""
         t0 = time.perf_counter()
-        # the device generates planar point sets: curved meshes take the host's
-        dev_points = self.device_points and not any(
-            getattr(self.library[lib_id], "curved", False) for lib_id, _, _, _ in self.instances)
+        dev_points = self.device_points
         sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize,
                    host_points=not dev_points)
         for lib_id, off, rot, Dp in self.instances:

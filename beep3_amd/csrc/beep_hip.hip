@@ -793,6 +793,11 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
                         "vertex-triangle list does not match the triangles");
             }
         }
+        const int NC = g->n_curved;
+        require(NC >= 0 && (NC == 0 || (g->ctl_idx && g->ctl)), "null curved-triangle array");
+        if (g->ctl_idx)
+            for (int t = 0; t < T; ++t)
+                require(g->ctl_idx[t] >= -1 && g->ctl_idx[t] < NC, "curved-triangle index out of range");
         std::vector<Patch64> pr(N);
         for (int i = 0; i < N; ++i)
             pr[i] = Patch64{g->centroid[3 * i], g->centroid[3 * i + 1], g->centroid[3 * i + 2],
@@ -807,6 +812,11 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
         ctx->svt_ptr.upload(g->vt_ptr, N + 1, s);
         ctx->svt_tri.upload(g->vt_tri, nvt, s);
         ctx->svt_corner.upload(g->vt_corner, nvt, s);
+        // control points only while the points are made
+        DevBuf<int> cidx;
+        DevBuf<double> cctl;
+        if (g->ctl_idx) cidx.upload(g->ctl_idx, T, s);
+        if (NC > 0) cctl.upload(g->ctl, size_t(63) * NC, s);
         // one point set per rule: counts (1, or 2 sub-triangles x k points per
         // triangle around the vertex), then the points on the device
         std::vector<int> ptr(size_t(N) + 1);
@@ -832,8 +842,9 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
             o64.alloc(size_t(ptr[N]));
             if (o32) o32->alloc(size_t(ptr[N]));
             launch_gen_points(s, N, ctx->sverts.p, ctx->stris.p, ctx->svt_ptr.p, ctx->svt_tri.p,
-                              ctx->svt_corner.p, ctx->patches.p, dptr.p, rd, per_area, o64.p,
-                              o32 ? o32->p : nullptr);
+                              ctx->svt_corner.p, g->ctl_idx ? cidx.p : nullptr,
+                              NC > 0 ? cctl.p : nullptr, ctx->patches.p, dptr.p, rd, per_area,
+                              o64.p, o32 ? o32->p : nullptr);
             hip_check(hipStreamSynchronize(s), "sync point generation");  // ptr is reused
         };
         make(g->qual_rule, ctx->qual_ptr, ctx->qual64, &ctx->qual32, 1);

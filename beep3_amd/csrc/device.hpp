@@ -280,8 +280,9 @@ struct RuleDev {
 // the point set of every patch from the surface, in the local frame
 void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tris,
                        const int* vt_ptr, const int* vt_tri, const int* vt_corner,
-                       const Patch64* patches, const int* ptr, const RuleDev& rule, int per_area,
-                       Pt64* out64, Pt32* out32);
+                       const int* ctl_idx, const double* ctl, const Patch64* patches,
+                       const int* ptr, const RuleDev& rule, int per_area, Pt64* out64,
+                       Pt32* out32);
 // raw CSR points k0 .. k0+n-1 (7 doubles each) -> local-frame records
 void launch_to_local(hipStream_t s, int n, int k0, const double* raw, const int* ptr, int N,
                      const Patch64* patches, Pt64* out64, Pt32* out32);

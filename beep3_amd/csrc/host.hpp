@@ -63,10 +63,12 @@ struct Quat {  // unit quaternion (w, x, y, z)
 // each built in the averaged tangent plane of one edge.  Barycentric
 // (u, v, w = 1 - u - v): w weights vertex 1, u vertex 2, v vertex 3.
 struct CurvedTri {
-    Vec3 b300, b030, b003;                          // corners
-    Vec3 p1_210, p1_201, p2_120, p2_021, p3_102, p3_012;  // corner tangent-plane points
-    Vec3 p2_201, p2_102, p3_120, p3_210, p1_012, p1_021;  // the same carried to the next corner
-    Vec3 c12_1, c12_2, c23_2, c23_3, c31_3, c31_1;        // centre variants per edge
+    // the 21 control points, 63 doubles, in the order curved.hpp lists:
+    // corners b300 b030 b003; corner tangent-plane points p1_210 p1_201
+    // p2_120 p2_021 p3_102 p3_012; the same carried to the next corner
+    // p2_201 p2_102 p3_120 p3_210 p1_012 p1_021; centre variants per edge
+    // c12_1 c12_2 c23_2 c23_3 c31_3 c31_1
+    double ctl[63];
     // vertices, vertex normals n1..n3, and the planar normals of the
     // triangles across edges 1-2, 2-3, 3-1
     CurvedTri(const Vec3& v1, const Vec3& v2, const Vec3& v3, const Vec3& n1, const Vec3& n2,

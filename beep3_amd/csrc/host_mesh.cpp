@@ -7,6 +7,7 @@
 // Point sets follow BasicTriangle::get_constant_basis_quad_points
 // (src/bem/triangle.h:388-479).
 #include "host.hpp"
+#include "curved.hpp"
 
 #include <algorithm>
 #include <array>
@@ -131,86 +132,50 @@ Vec3 third_on(const Frame& f, const Vec3& a, const Vec3& b, const Vec3& n) {
     return a + (x * f.dx + y * f.dy) * (scale / f.scale);
 }
 
-// rational blend of two control points by a coordinate t: a at t = 0, b at t = 1
-inline Vec3 blend(double t, const Vec3& a, const Vec3& b) {
-    const double s = 1.0 - t, ss = s * s, tt = t * t;
-    return (a * ss + b * tt) / (tt + ss);
-}
-
 }  // namespace
 
 CurvedTri::CurvedTri(const Vec3& v1, const Vec3& v2, const Vec3& v3, const Vec3& n1, const Vec3& n2,
-                     const Vec3& n3, const Vec3& n12, const Vec3& n23, const Vec3& n31)
-    : b300(v1), b030(v2), b003(v3) {
-    p1_210 = tangent_third(v2, v1, n1);
-    p1_201 = tangent_third(v3, v1, n1);
-    p2_120 = tangent_third(v1, v2, n2);
-    p2_021 = tangent_third(v3, v2, n2);
-    p3_102 = tangent_third(v1, v3, n3);
-    p3_012 = tangent_third(v2, v3, n3);
+                     const Vec3& n3, const Vec3& n12, const Vec3& n23, const Vec3& n31) {
+    const Vec3 &b300 = v1, &b030 = v2, &b003 = v3;
+    const Vec3 p1_210 = tangent_third(v2, v1, n1);
+    const Vec3 p1_201 = tangent_third(v3, v1, n1);
+    const Vec3 p2_120 = tangent_third(v1, v2, n2);
+    const Vec3 p2_021 = tangent_third(v3, v2, n2);
+    const Vec3 p3_102 = tangent_third(v1, v3, n3);
+    const Vec3 p3_012 = tangent_third(v2, v3, n3);
+    Vec3 p2_201, p2_102, p3_120, p3_210, p1_012, p1_021;
     swap_thirds(Frame(b300, p1_210, p1_201), Frame(p2_120, b030, p2_021), p1_021, p2_201);
     swap_thirds(Frame(b030, p2_021, p2_120), Frame(p3_012, b003, p3_102), p2_102, p3_120);
     swap_thirds(Frame(b003, p3_102, p3_012), Frame(p1_201, b300, p1_210), p3_210, p1_012);
     const Vec3 nt = (v2 - v1).cross(v3 - v1).unit();
     const Vec3 a12 = (nt + n12).unit(), a23 = (nt + n23).unit(), a31 = (nt + n31).unit();
-    c12_1 = third_on(Frame(b300, p1_210, p1_201), p1_210, p2_120, a12);
-    c12_2 = third_on(Frame(p2_120, b030, p2_021), p1_210, p2_120, a12);
-    c23_2 = third_on(Frame(b030, p2_021, p2_120), p2_021, p3_012, a23);
-    c23_3 = third_on(Frame(p3_012, b003, p3_102), p2_021, p3_012, a23);
-    c31_3 = third_on(Frame(b003, p3_102, p3_012), p3_102, p1_201, a31);
-    c31_1 = third_on(Frame(p1_201, b300, p1_210), p3_102, p1_201, a31);
+    const Vec3 c12_1 = third_on(Frame(b300, p1_210, p1_201), p1_210, p2_120, a12);
+    const Vec3 c12_2 = third_on(Frame(p2_120, b030, p2_021), p1_210, p2_120, a12);
+    const Vec3 c23_2 = third_on(Frame(b030, p2_021, p2_120), p2_021, p3_012, a23);
+    const Vec3 c23_3 = third_on(Frame(p3_012, b003, p3_102), p2_021, p3_012, a23);
+    const Vec3 c31_3 = third_on(Frame(b003, p3_102, p3_012), p3_102, p1_201, a31);
+    const Vec3 c31_1 = third_on(Frame(p1_201, b300, p1_210), p3_102, p1_201, a31);
+    const Vec3* order[21] = {&b300,   &b030,   &b003,   &p1_210, &p1_201, &p2_120, &p2_021,
+                             &p3_102, &p3_012, &p2_201, &p2_102, &p3_120, &p3_210, &p1_012,
+                             &p1_021, &c12_1,  &c12_2,  &c23_2,  &c23_3,  &c31_3,  &c31_1};
+    for (int i = 0; i < 21; ++i) {
+        ctl[3 * i] = order[i]->x;
+        ctl[3 * i + 1] = order[i]->y;
+        ctl[3 * i + 2] = order[i]->z;
+    }
 }
-
-namespace {
-struct Ctl {
-    Vec3 b210, b120, b201, b021, b102, b012, b111;
-};
-
-Ctl controls(const CurvedTri& c, double u, double v) {
-    const double w = 1.0 - u - v;
-    Ctl k;
-    k.b201 = blend(u, c.p1_201, c.p2_201);
-    k.b102 = blend(u, c.p3_102, c.p2_102);
-    k.b012 = blend(w, c.p3_012, c.p1_012);
-    k.b021 = blend(w, c.p2_021, c.p1_021);
-    k.b120 = blend(v, c.p2_120, c.p3_120);
-    k.b210 = blend(v, c.p1_210, c.p3_210);
-    // the centre point: the three edge variants blended by weights that pick
-    // the variant of the nearest edge (shifted off the corners, where the
-    // weights are 0/0)
-    const double eps = 1e-6;
-    double a = u + eps, b = v + eps, g = w + eps;
-    const double tot = a + b + g;
-    a /= tot;
-    b /= tot;
-    g /= tot;
-    const double sum = a * b + a * g + b * g;
-    const Vec3 t1 = c.c12_1 * ((1.0 - a) * g) + c.c12_2 * ((1.0 - g) * a);
-    const Vec3 t2 = c.c23_2 * ((1.0 - b) * a) + c.c23_3 * ((1.0 - a) * b);
-    const Vec3 t3 = c.c31_3 * ((1.0 - g) * b) + c.c31_1 * ((1.0 - b) * g);
-    k.b111 = t1 * ((a * g / sum) / (g + a - 2.0 * a * g)) + t2 * ((a * b / sum) / (a + b - 2.0 * a * b)) +
-             t3 * ((b * g / sum) / (g + b - 2.0 * b * g));
-    return k;
-}
-}  // namespace
 
 Vec3 CurvedTri::point(double u, double v) const {
-    const double w = 1.0 - u - v;
-    const Ctl k = controls(*this, u, v);
-    return b300 * (w * w * w) + b030 * (u * u * u) + b003 * (v * v * v) + k.b210 * (3 * w * w * u) +
-           k.b120 * (3 * w * u * u) + k.b201 * (3 * w * w * v) + k.b021 * (3 * u * u * v) +
-           k.b102 * (3 * w * v * v) + k.b012 * (3 * u * v * v) + k.b111 * (6 * w * u * v);
+    double p[3], du[3], dv[3];
+    beepcurve::eval(ctl, u, v, p, du, dv);
+    return Vec3(p[0], p[1], p[2]);
 }
 
 void CurvedTri::tangents(double u, double v, Vec3& du, Vec3& dv) const {
-    const double w = 1.0 - u - v;
-    const Ctl k = controls(*this, u, v);
-    du = b030 * (3 * u * u) - b300 * (3 * w * w) + k.b120 * (6 * u * w - 3 * u * u) + k.b021 * (6 * u * v) -
-         k.b102 * (3 * v * v) + k.b012 * (3 * v * v) + k.b210 * (3 * w * w - 6 * w * u) - k.b201 * (6 * w * v) +
-         k.b111 * (6 * w * v - 6 * u * v);
-    dv = b003 * (3 * v * v) - b300 * (3 * w * w) - k.b120 * (3 * u * u) + k.b021 * (3 * u * u) +
-         k.b102 * (6 * v * w - 3 * v * v) + k.b012 * (6 * v * u) - k.b210 * (6 * w * u) +
-         k.b201 * (3 * w * w - 6 * w * v) + k.b111 * (6 * w * u - 6 * u * v);
+    double p[3], a[3], b[3];
+    beepcurve::eval(ctl, u, v, p, a, b);
+    du = Vec3(a[0], a[1], a[2]);
+    dv = Vec3(b[0], b[1], b[2]);
 }
 
 // Vertex normals (Vertex::set_normal, src/bem/vertex.cpp:86-141: the
@@ -297,12 +262,12 @@ void Mesh::patch_points(int vid, const Rule& rule, std::vector<QuadPt>& out) con
                     // the parametric point mapped onto the curved patch; the
                     // weight takes the surface jacobian |dS/du x dS/dv|, which
                     // is twice the area for a flat triangle
-                    const CurvedTri& c = ctri[t];
-                    Vec3 du, dv;
-                    c.tangents(uvw.x, uvw.y, du, dv);
+                    double p[3], a[3], b[3];
+                    beepcurve::eval(ctri[t].ctl, uvw.x, uvw.y, p, a, b);
+                    const Vec3 du(a[0], a[1], a[2]), dv(b[0], b[1], b[2]);
                     const Vec3 cr2 = du.cross(dv);
                     const double jac = std::sqrt(du.dot(du) * dv.dot(dv) - du.dot(dv) * du.dot(dv));
-                    out.push_back(QuadPt{c.point(uvw.x, uvw.y), cr2.unit(), rule.w[k] / 6.0 * jac / 2.0});
+                    out.push_back(QuadPt{Vec3(p[0], p[1], p[2]), cr2.unit(), rule.w[k] / 6.0 * jac / 2.0});
                     continue;
                 }
                 // each sub-triangle is one sixth of the triangle

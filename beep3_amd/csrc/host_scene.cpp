@@ -263,9 +263,6 @@ extern "C" int beep_scene_build(beep_scene* s) {
         }
         const int N = int(plist.size());
         // point counts: one centroid point for rule 0, else k points on each of
-        for (const Instance& in : s->inst)
-            if (in.mesh->mesh.curved && !s->host_points)
-                throw std::runtime_error("curved patches need host point sets (the device generates planar ones)");
         // the 2 x valence sub-triangles of the patch
         for (int k = 0; k < 3; ++k) {
             s->ptr[k].assign(size_t(N) + 1, 0);
@@ -446,6 +443,39 @@ extern "C" int beep_scene_surface(const beep_scene* s, double* verts3, int* tris
             vbase += m.verts.size();
             tbase += m.tris.size();
         }
+    });
+}
+
+extern "C" int beep_scene_surface_curved(const beep_scene* s, int* n_curved, int* ctl_idx, double* ctl) {
+    return guarded([&] {
+        need_built(s);
+        // a rigid motion of the control points moves the surface with them
+        // (every surface point is an affine combination of them), so placed
+        // control points give the placed surface
+        size_t tbase = 0;
+        int nc = 0;
+        for (const Instance& in : s->inst) {
+            const Mesh& m = in.mesh->mesh;
+            for (size_t t = 0; t < m.tris.size(); ++t) {
+                if (!m.curved) {
+                    if (ctl_idx) ctl_idx[tbase + t] = -1;
+                    continue;
+                }
+                if (ctl_idx) ctl_idx[tbase + t] = nc;
+                if (ctl)
+                    for (int i = 0; i < 21; ++i) {
+                        const double* c = m.ctri[t].ctl + 3 * i;
+                        const Vec3 q = place(in, Vec3(c[0], c[1], c[2]));
+                        double* d = ctl + 63 * size_t(nc) + 3 * i;
+                        d[0] = q.x;
+                        d[1] = q.y;
+                        d[2] = q.z;
+                    }
+                ++nc;
+            }
+            tbase += m.tris.size();
+        }
+        if (n_curved) *n_curved = nc;
     });
 }
 

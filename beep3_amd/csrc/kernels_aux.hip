@@ -1,6 +1,7 @@
 // Right-hand side, energy coefficients and the FP64 vector kernels of the
 // device GMRES driver.
 #include "device.hpp"
+#include "curved.hpp"
 
 namespace beepdev {
 
@@ -450,12 +451,16 @@ void launch_inc_int(hipStream_t s, int* p) {
 // triangle.h:387-479): patch p is vertex p, and for every triangle around
 // it (in the listed order) the two sub-triangles (centre, vertex, edge
 // midpoint) carry the points of the rule, weight w * (triangle area) / 6.
+// A curved triangle (ctl_idx >= 0) maps the same parametric points onto its
+// surface S(u, v) (curved.hpp): normal unit(dS/du x dS/dv), weight
+// w * |dS/du x dS/dv| / 12, the flat triangle's w * area / 6 when S is flat.
 // One thread per patch writes its run of points in the local frame (offsets
 // from the patch centroid, FP64, and the FP32 copy rounded once).
 __global__ __launch_bounds__(256) void k_gen_points(
     int n, const double* __restrict__ verts, const int* __restrict__ tris,
     const int* __restrict__ vt_ptr, const int* __restrict__ vt_tri,
-    const int* __restrict__ vt_corner, const Patch64* __restrict__ patches,
+    const int* __restrict__ vt_corner, const int* __restrict__ ctl_idx,
+    const double* __restrict__ ctl, const Patch64* __restrict__ patches,
     const int* __restrict__ ptr, const RuleDev rule, int per_area, Pt64* __restrict__ out64,
     Pt32* __restrict__ out32) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -494,6 +499,7 @@ __global__ __launch_bounds__(256) void k_gen_points(
             o[0] = 0; o[1] = 1; o[2] = 0; m1[0] = 0; m1[1] = 0.5; m1[2] = 0.5; m2[0] = 0.5; m2[1] = 0.5; m2[2] = 0;
         }
         const double c3[3] = {third, third, third};
+        const int ci = ctl_idx ? ctl_idx[tri] : -1;
         for (int s = 0; s < 2; ++s) {
             const double* P1 = s == 0 ? c3 : m2;
             const double* P3 = s == 0 ? m1 : c3;
@@ -502,14 +508,36 @@ __global__ __launch_bounds__(256) void k_gen_points(
                 const double bu = P1[0] * r0 + o[0] * ru + P3[0] * rv;
                 const double bv = P1[1] * r0 + o[1] * ru + P3[1] * rv;
                 const double bw = P1[2] * r0 + o[2] * ru + P3[2] * rv;
-                const double px = x1 * bw + x2 * bu + x3 * bv - pa.cx;
-                const double py = y1 * bw + y2 * bu + y3 * bv - pa.cy;
-                const double pz = z1 * bw + z2 * bu + z3 * bv - pa.cz;
-                const double wt = rule.w[j] * tri_area / 6.0 * wscale;
-                if (out64) out64[k] = Pt64{px, py, pz, wt, nx, ny, nz, 0.0};
+                double px, py, pz, wt, qx = nx, qy = ny, qz = nz;
+                if (ci < 0) {
+                    px = x1 * bw + x2 * bu + x3 * bv - pa.cx;
+                    py = y1 * bw + y2 * bu + y3 * bv - pa.cy;
+                    pz = z1 * bw + z2 * bu + z3 * bv - pa.cz;
+                    wt = rule.w[j] * tri_area / 6.0 * wscale;
+                } else {
+                    double q[3], du[3], dv[3];
+                    beepcurve::eval(ctl + 63 * size_t(ci), bu, bv, q, du, dv);
+                    px = q[0] - pa.cx;
+                    py = q[1] - pa.cy;
+                    pz = q[2] - pa.cz;
+                    const double ex = du[1] * dv[2] - du[2] * dv[1];
+                    const double ey = du[2] * dv[0] - du[0] * dv[2];
+                    const double ez = du[0] * dv[1] - du[1] * dv[0];
+                    const double en = sqrt(ex * ex + ey * ey + ez * ez);
+                    qx = ex / en;
+                    qy = ey / en;
+                    qz = ez / en;
+                    const double uu = du[0] * du[0] + du[1] * du[1] + du[2] * du[2];
+                    const double vv = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                    const double uv = du[0] * dv[0] + du[1] * dv[1] + du[2] * dv[2];
+                    const double jac = sqrt(uu * vv - uv * uv);
+                    wt = rule.w[j] / 6.0 * jac / 2.0;
+                    if (per_area) wt /= pa.area;
+                }
+                if (out64) out64[k] = Pt64{px, py, pz, wt, qx, qy, qz, 0.0};
                 if (out32)
-                    out32[k] = Pt32{float(px), float(py), float(pz), float(wt), float(nx), float(ny),
-                                    float(nz), 0.f};
+                    out32[k] = Pt32{float(px), float(py), float(pz), float(wt), float(qx), float(qy),
+                                    float(qz), 0.f};
                 ++k;
             }
         }
@@ -518,11 +546,13 @@ __global__ __launch_bounds__(256) void k_gen_points(
 
 void launch_gen_points(hipStream_t s, int n, const double* verts, const int* tris,
                        const int* vt_ptr, const int* vt_tri, const int* vt_corner,
-                       const Patch64* patches, const int* ptr, const RuleDev& rule, int per_area,
-                       Pt64* out64, Pt32* out32) {
+                       const int* ctl_idx, const double* ctl, const Patch64* patches,
+                       const int* ptr, const RuleDev& rule, int per_area, Pt64* out64,
+                       Pt32* out32) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gen_points, dim3((n + 255) / 256), dim3(256), 0, s, n, verts, tris,
-                       vt_ptr, vt_tri, vt_corner, patches, ptr, rule, per_area, out64, out32);
+                       vt_ptr, vt_tri, vt_corner, ctl_idx, ctl, patches, ptr, rule, per_area,
+                       out64, out32);
     hip_check(hipGetLastError(), "k_gen_points launch");
 }
 

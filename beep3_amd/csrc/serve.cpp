@@ -70,37 +70,10 @@ double estimate_bytes(const beep_scene* s, const beep_server_opts& o, bool store
     return b;
 }
 
-// A scene with a curved mesh needs host point sets: the device generates
-// planar ones.
-bool scene_curved(const beep_scene* sc) {
-    for (const Instance& in : sc->inst)
-        if (in.mesh->mesh.curved) return true;
-    return false;
-}
-
 // The built scene's surface to the context (placed vertices, triangles,
-// triangles around each vertex, patch arrays); the device generates the
-// point sets.  A scene built with host point sets (curved patches) sends
-// those instead.
+// triangles around each vertex, the curved triangles' control points, patch
+// arrays); the device generates the point sets.
 void upload_surface(beep_hip_ctx* ctx, beep_scene* sc) {
-    if (sc->host_points) {
-        static_assert(sizeof(beep::QuadPt) == 7 * sizeof(double), "points are 7 doubles");
-        beep_hip_geometry g{};
-        g.n_patches = int(sc->centroid.size());
-        g.centroid = &sc->centroid[0].x;
-        g.normal = &sc->normal[0].x;
-        g.area = sc->area.data();
-        g.eps_ratio = sc->eps.data();
-        g.gc = sc->gc.data();
-        g.qual_ptr = sc->ptr[0].data();
-        g.qual_pts = &sc->pts[0][0].p.x;
-        g.quad_ptr = sc->ptr[1].data();
-        g.quad_pts = &sc->pts[1][0].p.x;
-        g.gal_ptr = sc->ptr[2].data();
-        g.gal_pts = &sc->pts[2][0].p.x;
-        check_hip(ctx, beep_hip_set_geometry(ctx, &g), "geometry");
-        return;
-    }
     const int N = int(sc->centroid.size());
     beep_scene_sizes sz{};
     if (beep_scene_get_sizes(sc, &sz) != 0)
@@ -110,6 +83,14 @@ void upload_surface(beep_hip_ctx* ctx, beep_scene* sc) {
         vt_tri(size_t(sz.n_vertex_triangles)), vt_corner(size_t(sz.n_vertex_triangles));
     if (beep_scene_surface(sc, verts.data(), tris.data(), vt_ptr.data(), vt_tri.data(),
                            vt_corner.data()) != 0)
+        throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
+    int nc = 0;
+    std::vector<int> ctl_idx(size_t(sz.n_triangles));
+    std::vector<double> ctl;
+    if (beep_scene_surface_curved(sc, &nc, nullptr, nullptr) != 0)
+        throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
+    ctl.resize(size_t(63) * nc);
+    if (nc > 0 && beep_scene_surface_curved(sc, nullptr, ctl_idx.data(), ctl.data()) != 0)
         throw std::runtime_error(std::string("scene surface: ") + beep_host_last_error());
     beep_hip_surface g{};
     g.n_patches = N;
@@ -126,6 +107,11 @@ void upload_surface(beep_hip_ctx* ctx, beep_scene* sc) {
     g.vt_corner = vt_corner.data();
     g.qual_rule = sc->qual;
     g.quad_rule = sc->quad;
+    if (nc > 0) {
+        g.n_curved = nc;
+        g.ctl_idx = ctl_idx.data();
+        g.ctl = ctl.data();
+    }
     check_hip(ctx, beep_hip_set_surface(ctx, &g), "geometry");
 }
 
@@ -245,7 +231,7 @@ void beep_server::energy_coefficients(beep_hip_ctx* ctx, beep_scene* sc, std::ve
         try {
             if (beep_scene_add_instance(one, in.mesh, nullptr, nullptr, in.Dprotein) != 0)
                 throw std::runtime_error(std::string("scene: ") + beep_host_last_error());
-            one->host_points = in.mesh->mesh.curved;
+            one->host_points = false;  // the device generates the point sets
             if (beep_scene_build(one) != 0)
                 throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
             if (int(one->centroid.size()) != n_k)
@@ -275,7 +261,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     beep_result& r = d.r;
     beep_scene* sc = j.scene;
     const double th = clock();
-    sc->host_points = scene_curved(sc);  // else the device generates the point sets
+    sc->host_points = false;  // the device generates the point sets
     if (beep_scene_build(sc) != 0)
         throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
     r.host_ms = (clock() - th) * 1e3;
@@ -642,7 +628,7 @@ void pose_one(beep_poses* p, beep_hip_ctx*& ctx, int slot, const double* quat, c
     const double origin[3] = {0.0, 0.0, 0.0};
     if (beep_scene_add_instance(sc, p->mesh, nullptr, origin, p->Dprotein) != 0 ||
         beep_scene_add_instance(sc, p->mesh, quat, off, p->Dprotein) != 0 ||
-        beep_scene_set_host_points(sc, scene_curved(sc) ? 1 : 0) != 0 || beep_scene_set_tree(sc, 0) != 0 ||
+        beep_scene_set_host_points(sc, 0) != 0 || beep_scene_set_tree(sc, 0) != 0 ||
         beep_scene_build(sc) != 0)
         throw std::runtime_error(std::string("pose scene: ") + beep_host_last_error());
     const int n1 = p->N1, N = 2 * n1;
@@ -730,7 +716,7 @@ int beep_poses_create(const beep_pose_opts* opts, const beep_mesh* mesh, double 
         // the mesh at its canonical placement, with its own octree
         if (beep_scene_create(Dsolvent, kappa, quad_rule, qual_rule, nbsize, &sc) != 0 ||
             beep_scene_add_instance(sc, mesh, nullptr, nullptr, Dprotein) != 0 ||
-            beep_scene_set_host_points(sc, scene_curved(sc) ? 1 : 0) != 0 || beep_scene_build(sc) != 0)
+            beep_scene_set_host_points(sc, 0) != 0 || beep_scene_build(sc) != 0)
             throw std::runtime_error(std::string("mesh scene: ") + beep_host_last_error());
         p->N1 = int(sc->centroid.size());
         for (size_t l = 0; l + 1 < sc->leaf_ptr.size(); ++l)

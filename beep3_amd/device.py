@@ -72,6 +72,9 @@ class Device:
         g.vt_ptr, g.vt_tri, g.vt_corner = (keep["vt_ptr"].ctypes.data, keep["vt_tri"].ctypes.data,
                                            keep["vt_corner"].ctypes.data)
         g.qual_rule, g.quad_rule = int(qual_rule), int(quad_rule)
+        if "ctl" in keep and keep["ctl"].shape[0] > 0:
+            g.n_curved = int(keep["ctl"].shape[0])
+            g.ctl_idx, g.ctl = keep["ctl_idx"].ctypes.data, keep["ctl"].ctypes.data
         self._check(lib().beep_hip_set_surface(self._h, C.byref(g)))
 
     def set_tree(self, a: dict):

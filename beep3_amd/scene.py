@@ -197,7 +197,9 @@ class Scene:
 
     def surface(self) -> dict:
         """Placed vertices, triangles and the triangles around every vertex
-        (with the vertex's corner in each), as the device generates points."""
+        (with the vertex's corner in each), as the device generates points;
+        for curved meshes also each triangle's index among the curved ones
+        (-1: planar) and their placed control points (n_curved x 63)."""
         s = self.sizes()
         out = {"verts": np.zeros((s.n_patches, 3)), "tris": np.zeros((s.n_triangles, 3), np.int32),
                "vt_ptr": np.zeros(s.n_patches + 1, np.int32),
@@ -206,6 +208,11 @@ class Scene:
         check_host(lib().beep_scene_surface(self._h, _dp(out["verts"]), _ip(out["tris"]),
                                             _ip(out["vt_ptr"]), _ip(out["vt_tri"]),
                                             _ip(out["vt_corner"])))
+        nc = C.c_int(0)
+        check_host(lib().beep_scene_surface_curved(self._h, C.byref(nc), None, None))
+        out["ctl_idx"] = np.zeros(s.n_triangles, np.int32)
+        out["ctl"] = np.zeros((nc.value, 63))
+        check_host(lib().beep_scene_surface_curved(self._h, None, _ip(out["ctl_idx"]), _dp(out["ctl"])))
         return out
 
     def __del__(self):

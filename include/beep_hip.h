@@ -103,6 +103,14 @@ typedef struct {
     const int* vt_corner;    /* corner 0, 1, 2 of the vertex in each of them */
     int qual_rule;           /* rule order of the qualocation points */
     int quad_rule;           /* rule order of the quadrature points */
+    /* curved (PNG1) triangles, as beep_scene_surface_curved gives them;
+     * n_curved = 0 with NULL arrays for a planar surface.  A curved
+     * triangle's points lie on its surface S(u, v), with normal
+     * unit(dS/du x dS/dv) and weight w * |dS/du x dS/dv| / 12 (node_patch.cpp
+     * with the curved triangle's jacobian, png1.h:135-346). */
+    int n_curved;
+    const int* ctl_idx;      /* n_tris: index among the curved triangles, or -1 */
+    const double* ctl;       /* 63 n_curved: placed control points */
 } beep_hip_surface;
 
 /* Explicit-pair structure from the octree: every leaf's rows interact

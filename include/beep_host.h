@@ -130,6 +130,16 @@ int beep_scene_set_tree(beep_scene* s, int on);
  * patch's points are generated (node_patch.cpp:55-273). */
 int beep_scene_surface(const beep_scene* s, double* verts3, int* tris3, int* vt_ptr, int* vt_tri,
                        int* vt_corner);
+/* The curved triangles of the surface (meshes set curved, PNG1_Triangle,
+ * src/bem/png1.h:86-121): n_curved, then per triangle (n_triangles, the
+ * order of beep_scene_surface's tris) its index among the curved ones or -1
+ * for a planar triangle, and the 21 placed control points (63 doubles) of
+ * each curved one, in the order b300 b030 b003, p1_210 p1_201 p2_120 p2_021
+ * p3_102 p3_012, p2_201 p2_102 p3_120 p3_210 p1_012 p1_021, c12_1 c12_2
+ * c23_2 c23_3 c31_3 c31_1 (evaluated by beep3_amd/csrc/curved.hpp).  Any
+ * pointer may be NULL; the device generates curved point sets from these
+ * (beep_hip_surface). */
+int beep_scene_surface_curved(const beep_scene* s, int* n_curved, int* ctl_idx, double* ctl);
 /* leaf_ptr: L+1, leaf_idx: N, nb_ptr: L+1, nb_idx: n_near_cols */
 int beep_scene_tree(const beep_scene* s, int* leaf_ptr, int* leaf_idx, int* nb_ptr, int* nb_idx);
 /* posed charges: xyzq n x 4, charge_instance n */

@@ -111,10 +111,29 @@ def test_curved_triangle_corners_edges_and_vertex_normals():
     assert worst < 1e-12
 
 
-def test_curved_needs_host_points_and_round_trips():
+def test_curved_surface_export_and_round_trips():
+    """A scene that leaves its points to the device exports the curved
+    triangles' placed control points: every triangle of a curved instance has
+    an index, a planar instance's have -1, and the three corner control
+    points are the placed vertices of the triangle (the rest are pinned on
+    the device against the host's point sets)."""
     m = Mesh.icosphere(2, 2.0).set_charges([[0, 0, 0, 1.0]]).set_curved(True)
-    with pytest.raises(BeepError, match="host point"):
-        Scene(80.0, 0.125, 6, 0, 1500, host_points=False).add_instance(m, 2.0).build()
+    flat = Mesh.icosphere(2, 2.0).set_charges([[0, 0, 0, 1.0]])
+    sc = Scene(80.0, 0.125, 6, 0, 1500, host_points=False)
+    sc.add_instance(m, 2.0, quat=[0.6, -0.2, 0.7, 0.3], offset=[9.0, -1.0, 2.0])
+    sc.add_instance(flat, 2.0, offset=[-9.0, 0.0, 0.0]).build()
+    surf = sc.surface()
+    nt = m.vertices()[1].shape[0]
+    assert surf["ctl"].shape == (nt, 63)
+    assert np.array_equal(surf["ctl_idx"][:nt], np.arange(nt))
+    assert np.all(surf["ctl_idx"][nt:] == -1)
+    xyz, tri = surf["verts"], surf["tris"]
+    for t in range(0, nt, 5):
+        corners = surf["ctl"][t, :9].reshape(3, 3)
+        assert np.allclose(corners, xyz[tri[t]], atol=1e-12)
+    # a planar scene has none
+    flat_only = Scene(80.0, 0.125, 6, 0, 1500, host_points=False).add_instance(flat, 2.0).build()
+    assert flat_only.surface()["ctl"].shape == (0, 63)
     a1 = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build().arrays()
     m.set_curved(False)
     a0 = Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0).build().arrays()

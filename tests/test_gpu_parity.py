@@ -298,3 +298,42 @@ def test_curved_patches_on_the_device():
     print("curved 2,562 patches: mat-vec rel err fp64", e64, "fp32_resid", e32, "energy", E, "LU", E_ref)
     assert e64 < 1e-12 and e32 < 3e-8
     assert abs(E - E_ref) / abs(E_ref) < 1e-7
+
+
+def test_curved_points_generated_on_the_device():
+    """The device generates curved (PNG1) point sets from the placed control
+    points: a rotated and shifted curved instance beside a planar one gives
+    the mat-vecs and energy coefficients of the host's point sets to
+    rounding (the surface and its tangents are rigid-motion covariant)."""
+    from beep3_amd import Mesh
+    from beep3_amd.scene import Scene
+
+    curved = Mesh.icosphere(4, 2.0).set_charges([[0.3, 0.0, -0.2, 1.0]]).set_curved(True)
+    flat = Mesh.icosphere(3, 1.5).set_charges([[0.0, 0.1, 0.0, -1.0]])
+    sc = Scene(80.0, 0.125, 6, 4, 1500)
+    sc.add_instance(curved, 2.0, quat=[0.6, -0.2, 0.7, 0.3], offset=[1.0, -2.0, 0.5])
+    sc.add_instance(flat, 2.0, offset=[6.0, 0.0, 0.0])
+    sc.build()
+    a, surf = sc.arrays(), sc.surface()
+    assert surf["ctl"].shape[0] > 0 and np.any(surf["ctl_idx"] < 0)
+    devs = []
+    for mode in ("host", "device"):
+        d = Device(0)
+        d.set_far_field("direct")
+        if mode == "host":
+            d.set_geometry(a)
+        else:
+            d.set_surface(a, surf, 4, 6)
+        d.set_tree(a)
+        d.precompute(0.125, "fp64")
+        d.precompute(0.125, "fp32")
+        devs.append(d)
+    x = np.random.default_rng(12).normal(size=2 * a["area"].shape[0])
+    e64 = rel(devs[1].matvec(x, "fp64"), devs[0].matvec(x, "fp64"))
+    e32 = rel(devs[1].matvec(x, "fp32"), devs[0].matvec(x, "fp32"))
+    ef = [d.energy_coefficients(a["inst_patch_ptr"], a["inst_charge_ptr"], a["charges"]) for d in devs]
+    ee = max(rel(ef[1][k], ef[0][k]) for k in range(2))
+    print(f"curved device vs host points: fp64 {e64:.2e}, fp32 {e32:.2e}, energy coefficients {ee:.2e}")
+    assert e64 < 1e-12 and e32 < 1e-6 and ee < 1e-12
+    for d in devs:
+        d.close()

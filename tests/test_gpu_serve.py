@@ -139,8 +139,9 @@ def test_pose_through_server_matches_single_solve():
 
 
 def test_curved_mesh_through_the_server():
-    """A curved (PNG1) mesh goes through the server on host point sets: its
-    energy matches the oracle's LU solve on the same curved arrays."""
+    """A curved (PNG1) mesh goes through the server, which has the device
+    generate its point sets from the placed control points: its energy
+    matches the oracle's LU solve on the host's curved arrays."""
     m = Mesh.icosphere(4, 2.0).set_charges([[0.3, 0.0, -0.2, 1.0], [-0.5, 0.4, 0.1, -0.5]]).set_curved(True)
     srv = Server(slots=2, keep_fh=False, mode="mixed")
     rid = srv.submit(Scene(80.0, 0.125, 6, 0, 1500).add_instance(m, 2.0))
