@@ -189,6 +189,8 @@ def _declare_hip(L):
     L.beep_hip_destroy.argtypes = [vp]
     L.beep_hip_set_geometry.argtypes = [vp, C.POINTER(HipGeometry)]
     L.beep_hip_set_tree.argtypes = [vp, C.POINTER(HipTree)]
+    L.beep_hip_build_tree.argtypes = [vp, dp, C.c_double, C.c_int, ip, C.POINTER(C.c_longlong)]
+    L.beep_hip_get_tree.argtypes = [vp, ip, ip, ip, ip]
     L.beep_hip_set_surface.argtypes = [vp, C.POINTER(HipSurface)]
     L.beep_hip_precompute.argtypes = [vp, C.c_double, C.c_int]
     L.beep_hip_set_far_field.argtypes = [vp, C.c_int, C.c_int, C.c_int]

@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "libbeep3.so")
 HOST_SOURCES = ["host_mesh.cpp", "host_octree.cpp", "host_scene.cpp", "host_io.cpp", "serve.cpp",
                 "pw_tables.cpp"]
 HIP_SOURCES = ["kernels_near.hip", "kernels_far.hip", "kernels_aux.hip", "fmm.hip",
-               "fmm_residual.hip", "fmm_pw.hip", "beep_hip.hip"]
+               "fmm_residual.hip", "fmm_pw.hip", "octree_dev.hip", "beep_hip.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

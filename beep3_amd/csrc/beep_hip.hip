@@ -15,6 +15,7 @@
 #include "fmm.hpp"
 #include "pw.hpp"
 #include "host.hpp"
+#include "tree_key.hpp"
 
 using namespace beepdev;
 
@@ -854,17 +855,89 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
     });
 }
 
+// Forget the previous near-field structure before a new tree is set.
+static void begin_tree(beep_hip_ctx* ctx) {
+    ctx->share = nullptr;
+    ctx->share_off.clear();
+    ctx->n_pairs = 0;
+    ctx->n_pair_rows = 0;
+    set_device(ctx);
+    sync_streams(ctx);
+    ctx->gen++;
+}
+
+// The SpMV and on-the-fly plans of a tree whose leaf_idx and nb_idx are on
+// the device already: per leaf the offset of its coefficient block, and the
+// work items (row group x column slab) of both plans.
+static void install_tree(beep_hip_ctx* ctx, int L, const int* leaf_ptr, const int* nb_ptr) {
+    std::vector<int64_t> off(L + 1, 0);
+    std::vector<int> cblk(L + 1, 0);
+    std::vector<int4> work, groups, fwork, fgroups;
+    std::vector<int> gnb, fgnb;
+    for (int l = 0; l < L; ++l) {
+        const int64_t R = leaf_ptr[l + 1] - leaf_ptr[l];
+        const int64_t C = nb_ptr[l + 1] - nb_ptr[l];
+        require(R > 0 && C > 0, "empty leaf or neighbourhood");
+        off[l + 1] = off[l] + R * C;
+        const int64_t nb = (R * C + 255) / 256;
+        require(int64_t(cblk[l]) + nb < (int64_t(1) << 31), "too many coefficient blocks");
+        cblk[l + 1] = cblk[l] + int(nb);
+        for (int r0 = 0; r0 < R; r0 += 256) {
+            const int first = int(work.size());
+            for (int c0 = 0; c0 < C; c0 += SPMV_CW)
+                work.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
+            groups.push_back(make_int4(l, r0, int(std::min<int64_t>(256, R - r0)), first));
+            gnb.push_back(int(work.size()) - first);
+        }
+        for (int r0 = 0; r0 < R; r0 += FLY_RB) {
+            const int first = int(fwork.size());
+            for (int c0 = 0; c0 < C; c0 += SPMV_CW)
+                fwork.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
+            fgroups.push_back(make_int4(l, r0, int(std::min<int64_t>(FLY_RB, R - r0)), first));
+            fgnb.push_back(int(fwork.size()) - first);
+        }
+    }
+    hipStream_t s = ctx->stream;
+    ctx->leaf_ptr.upload(leaf_ptr, L + 1, s);
+    ctx->nb_ptr.upload(nb_ptr, L + 1, s);
+    ctx->coef_off.upload(off.data(), L + 1, s);
+    ctx->coef_blk.upload(cblk.data(), L + 1, s);
+    ctx->spmv_work.upload(work.data(), work.size(), s);
+    ctx->spmv_groups.upload(groups.data(), groups.size(), s);
+    ctx->spmv_group_nblk.upload(gnb.data(), gnb.size(), s);
+    ctx->fly_work.upload(fwork.data(), fwork.size(), s);
+    ctx->fly_groups.upload(fgroups.data(), fgroups.size(), s);
+    ctx->fly_group_nblk.upload(fgnb.data(), fgnb.size(), s);
+    ctx->spmv_part.alloc(std::max(work.size() * 256, fwork.size() * FLY_RB) * 2);
+    hip_check(hipStreamSynchronize(s), "sync tree upload");
+    ctx->L = L;
+    ctx->nnz = off[L];
+    ctx->n_coef_blocks = cblk[L];
+    ctx->spmv.work = ctx->spmv_work.p;
+    ctx->spmv.n_work = int(work.size());
+    ctx->spmv.groups = ctx->spmv_groups.p;
+    ctx->spmv.group_nblk = ctx->spmv_group_nblk.p;
+    ctx->spmv.n_groups = int(groups.size());
+    ctx->fly = SpmvPlan{};
+    ctx->fly.work = ctx->fly_work.p;
+    ctx->fly.n_work = int(fwork.size());
+    ctx->fly.groups = ctx->fly_groups.p;
+    ctx->fly.group_nblk = ctx->fly_group_nblk.p;
+    ctx->fly.n_groups = int(fgroups.size());
+    ctx->h_leaf_ptr.assign(leaf_ptr, leaf_ptr + L + 1);
+    ctx->h_nb_ptr.assign(nb_ptr, nb_ptr + L + 1);
+    ctx->n_nb = nb_ptr[L];
+    ctx->have_tree = true;
+    ctx->have32 = ctx->have64 = ctx->have_diag = false;
+    if (ctx->fmm) ctx->fmm->n_res = -1;  // the residual leaf-pass list follows the near list
+    // the coefficient buffers stay allocated for the next precompute
+}
+
 int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
     return guarded(ctx, [&] {
         require(ctx->have_geom, "set the geometry before the tree");
-        ctx->share = nullptr;
-        ctx->share_off.clear();
-        ctx->n_pairs = 0;
-        ctx->n_pair_rows = 0;
+        begin_tree(ctx);
         require(t && t->n_leaves > 0, "empty tree");
-        set_device(ctx);
-        sync_streams(ctx);
-        ctx->gen++;
         const int N = ctx->N, L = t->n_leaves;
         check_ptr(L, t->leaf_ptr, "bad leaf pointer array");
         check_ptr(L, t->nb_ptr, "bad neighbourhood pointer array");
@@ -878,69 +951,116 @@ int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* t) {
         const int nnb = t->nb_ptr[L];
         for (int k = 0; k < nnb; ++k)
             require(t->nb_idx[k] >= 0 && t->nb_idx[k] < N, "neighbourhood index out of range");
-        std::vector<int64_t> off(L + 1, 0);
-        std::vector<int> cblk(L + 1, 0);
-        std::vector<int4> work, groups, fwork, fgroups;
-        std::vector<int> gnb, fgnb;
-        for (int l = 0; l < L; ++l) {
-            const int64_t R = t->leaf_ptr[l + 1] - t->leaf_ptr[l];
-            const int64_t C = t->nb_ptr[l + 1] - t->nb_ptr[l];
-            require(R > 0 && C > 0, "empty leaf or neighbourhood");
-            off[l + 1] = off[l] + R * C;
-            const int64_t nb = (R * C + 255) / 256;
-            require(int64_t(cblk[l]) + nb < (int64_t(1) << 31), "too many coefficient blocks");
-            cblk[l + 1] = cblk[l] + int(nb);
-            for (int r0 = 0; r0 < R; r0 += 256) {
-                const int first = int(work.size());
-                for (int c0 = 0; c0 < C; c0 += SPMV_CW)
-                    work.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
-                groups.push_back(make_int4(l, r0, int(std::min<int64_t>(256, R - r0)), first));
-                gnb.push_back(int(work.size()) - first);
-            }
-            for (int r0 = 0; r0 < R; r0 += FLY_RB) {
-                const int first = int(fwork.size());
-                for (int c0 = 0; c0 < C; c0 += SPMV_CW)
-                    fwork.push_back(make_int4(l, r0, c0, int(std::min<int64_t>(SPMV_CW, C - c0))));
-                fgroups.push_back(make_int4(l, r0, int(std::min<int64_t>(FLY_RB, R - r0)), first));
-                fgnb.push_back(int(fwork.size()) - first);
-            }
-        }
         hipStream_t s = ctx->stream;
-        ctx->leaf_ptr.upload(t->leaf_ptr, L + 1, s);
         ctx->leaf_idx.upload(t->leaf_idx, N, s);
-        ctx->nb_ptr.upload(t->nb_ptr, L + 1, s);
         ctx->nb_idx.upload(t->nb_idx, nnb, s);
-        ctx->coef_off.upload(off.data(), L + 1, s);
-        ctx->coef_blk.upload(cblk.data(), L + 1, s);
-        ctx->spmv_work.upload(work.data(), work.size(), s);
-        ctx->spmv_groups.upload(groups.data(), groups.size(), s);
-        ctx->spmv_group_nblk.upload(gnb.data(), gnb.size(), s);
-        ctx->fly_work.upload(fwork.data(), fwork.size(), s);
-        ctx->fly_groups.upload(fgroups.data(), fgroups.size(), s);
-        ctx->fly_group_nblk.upload(fgnb.data(), fgnb.size(), s);
-        ctx->spmv_part.alloc(std::max(work.size() * 256, fwork.size() * FLY_RB) * 2);
-        hip_check(hipStreamSynchronize(s), "sync tree upload");
-        ctx->L = L;
-        ctx->nnz = off[L];
-        ctx->n_coef_blocks = cblk[L];
-        ctx->spmv.work = ctx->spmv_work.p;
-        ctx->spmv.n_work = int(work.size());
-        ctx->spmv.groups = ctx->spmv_groups.p;
-        ctx->spmv.group_nblk = ctx->spmv_group_nblk.p;
-        ctx->spmv.n_groups = int(groups.size());
-        ctx->fly = SpmvPlan{};
-        ctx->fly.work = ctx->fly_work.p;
-        ctx->fly.n_work = int(fwork.size());
-        ctx->fly.groups = ctx->fly_groups.p;
-        ctx->fly.group_nblk = ctx->fly_group_nblk.p;
-        ctx->fly.n_groups = int(fgroups.size());
-        ctx->h_leaf_ptr.assign(t->leaf_ptr, t->leaf_ptr + L + 1);
-        ctx->h_nb_ptr.assign(t->nb_ptr, t->nb_ptr + L + 1);
-        ctx->n_nb = nnb;
-        ctx->have_tree = true;
-        ctx->have32 = ctx->have64 = ctx->have_diag = false;
-        if (ctx->fmm) ctx->fmm->n_res = -1;  // the residual leaf-pass list follows the near list
-        // the coefficient buffers stay allocated for the next precompute
+        install_tree(ctx, L, t->leaf_ptr, t->nb_ptr);
+    });
+}
+
+int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, int nbsize,
+                        int* n_leaves, long long* n_nb) {
+    return guarded(ctx, [&] {
+        require(ctx->have_geom, "set the geometry before the tree");
+        require(centre && edge > 0 && nbsize > 0, "bad bounding cube or neighbourhood size");
+        begin_tree(ctx);
+        const int N = ctx->N;
+        hipStream_t s = ctx->stream;
+        // descent keys of the centroids, sorted with their patches
+        DevBuf<uint64_t> kfull, ka, kb;
+        DevBuf<int> ia, ib;
+        kfull.alloc(N);
+        ka.alloc(N);
+        kb.alloc(N);
+        ia.alloc(N);
+        ib.alloc(N);
+        DevBuf<unsigned char> tmp;
+        const int key_bits = 3 * beeptree::KEY_LEVELS;
+        size_t tb = tree_sort_bytes(N, key_bits);
+        tmp.alloc(std::max<size_t>(tb, 1));
+        launch_tree_keys(s, N, ctx->patches.p, centre, edge, 0, nullptr, kfull.p, ia.p);
+        tree_sort_pairs(s, tmp.p, tb, kfull.p, ka.p, ia.p, ib.p, N, key_bits);
+        std::vector<uint64_t> hk(N);
+        hip_check(hipMemcpyAsync(hk.data(), ka.p, sizeof(uint64_t) * N, hipMemcpyDeviceToHost, s),
+                  "D2H tree keys");
+        hip_check(hipStreamSynchronize(s), "sync tree keys");
+        // the tree's shape from the counts the sorted keys give
+        beep::Octree tree;
+        tree.build_sorted(hk.data(), N, beep::Vec3(centre[0], centre[1], centre[2]), edge, nbsize);
+        std::vector<int> lev, sptr, slo, shi;
+        tree.leaf_segments(lev, sptr, slo, shi);
+        const int L = int(lev.size());
+        require(L > 0, "empty tree");
+        // per level that has leaves, the patches sorted by their box at that
+        // level, each box's in ascending order (stable sort from patch order)
+        std::vector<int> slot(size_t(beeptree::KEY_LEVELS) + 1, -1);
+        int nlev = 0;
+        for (int l : lev)
+            if (slot[l] < 0) slot[l] = nlev++;
+        DevBuf<int> S;
+        S.alloc(size_t(nlev) * N);
+        for (int l = 0; l <= beeptree::KEY_LEVELS; ++l) {
+            if (slot[l] < 0) continue;
+            int* dst = S.p + size_t(slot[l]) * N;
+            if (l == 0) {
+                launch_tree_keys(s, N, nullptr, nullptr, 0.0, 0, kfull.p, ka.p, dst);  // all in one box
+                continue;
+            }
+            const size_t lb = tree_sort_bytes(N, 3 * l);
+            if (lb > tb) {
+                tmp.alloc(lb);
+                tb = lb;
+            }
+            launch_tree_keys(s, N, nullptr, nullptr, 0.0, l, kfull.p, ka.p, ia.p);
+            tree_sort_pairs(s, tmp.p, tb, ka.p, kb.p, ia.p, dst, N, 3 * l);
+        }
+        // the leaves' items and their neighbourhoods' items, gathered
+        std::vector<int> leaf_ptr(size_t(L) + 1, 0), nb_ptr(size_t(L) + 1, 0);
+        std::vector<int4> segl, segn;
+        segl.reserve(L);
+        segn.reserve(sptr[L]);
+        for (int f = 0; f < L; ++f) {
+            const size_t base = size_t(slot[lev[f]]) * N;
+            const int g0 = sptr[f];
+            leaf_ptr[f + 1] = leaf_ptr[f] + (shi[g0] - slo[g0]);
+            segl.push_back(make_int4(int(base) + slo[g0], shi[g0] - slo[g0], leaf_ptr[f], 0));
+            int64_t at = nb_ptr[f];
+            for (int g = g0 + 1; g < sptr[f + 1]; ++g) {
+                segn.push_back(make_int4(int(base) + slo[g], shi[g] - slo[g], int(at), 0));
+                at += shi[g] - slo[g];
+            }
+            require(at < int64_t(0x7fffffff), "neighbourhood lists exceed 32-bit offsets");
+            nb_ptr[f + 1] = int(at);
+        }
+        require(leaf_ptr[L] == N, "octree: leaves do not cover the patches");
+        require(int64_t(nlev) * N < int64_t(0x7fffffff), "too many patches for 32-bit offsets");
+        DevBuf<int4> dsegl, dsegn;
+        dsegl.upload(segl.data(), segl.size(), s);
+        dsegn.upload(segn.data(), segn.size(), s);
+        ctx->leaf_idx.alloc(N);
+        ctx->nb_idx.alloc(std::max(nb_ptr[L], 1));
+        launch_gather_segments(s, int(segl.size()), dsegl.p, S.p, ctx->leaf_idx.p);
+        launch_gather_segments(s, int(segn.size()), dsegn.p, S.p, ctx->nb_idx.p);
+        install_tree(ctx, L, leaf_ptr.data(), nb_ptr.data());  // synchronises the stream
+        if (n_leaves) *n_leaves = L;
+        if (n_nb) *n_nb = nb_ptr[L];
+    });
+}
+
+int beep_hip_get_tree(beep_hip_ctx* ctx, int* leaf_ptr, int* leaf_idx, int* nb_ptr, int* nb_idx) {
+    return guarded(ctx, [&] {
+        require(ctx->have_tree, "no tree set");
+        set_device(ctx);
+        sync_streams(ctx);
+        const int L = ctx->L;
+        if (leaf_ptr) std::copy(ctx->h_leaf_ptr.begin(), ctx->h_leaf_ptr.end(), leaf_ptr);
+        if (nb_ptr) std::copy(ctx->h_nb_ptr.begin(), ctx->h_nb_ptr.end(), nb_ptr);
+        if (leaf_idx)
+            hip_check(hipMemcpy(leaf_idx, ctx->leaf_idx.p, sizeof(int) * size_t(ctx->h_leaf_ptr[L]),
+                                hipMemcpyDeviceToHost), "D2H leaf contents");
+        if (nb_idx && ctx->n_nb > 0)
+            hip_check(hipMemcpy(nb_idx, ctx->nb_idx.p, sizeof(int) * size_t(ctx->n_nb),
+                                hipMemcpyDeviceToHost), "D2H neighbourhoods");
     });
 }
 

@@ -269,6 +269,18 @@ void launch_arnoldi_step(hipStream_t s, int n, int m, int nb, double* V, const i
 void launch_copy_col(hipStream_t s, int n, const double* V, const int* jp, double* out);
 // dst row r = src row idx[r] over 64-byte rows (a scattered read of known size)
 void launch_gather_rows64(hipStream_t s, const void* src, const int* idx, size_t rows, void* dst);
+
+// octree build on the device (octree_dev.hip): descent keys of the patch
+// centroids in the cube (centre, edge) when full is null, else the prefixes
+// at `level` of the full keys; items[i] = i
+void launch_tree_keys(hipStream_t s, int n, const Patch64* patches, const double* centre,
+                      double edge, int level, const uint64_t* full, uint64_t* keys, int* items);
+// stable radix sort of (key, item) pairs over key bits [0, end_bit)
+size_t tree_sort_bytes(int n, int end_bit);
+void tree_sort_pairs(hipStream_t s, void* tmp, size_t tmp_bytes, const uint64_t* kin,
+                     uint64_t* kout, const int* vin, int* vout, int n, int end_bit);
+// dst[seg.z + k] = src[seg.x + k] for k < seg.y, per segment
+void launch_gather_segments(hipStream_t s, int nseg, const int4* seg, const int* src, int* dst);
 void launch_set_int(hipStream_t s, int* p, int v);
 void launch_inc_int(hipStream_t s, int* p);
 // A triangle rule for point generation: k points per sub-triangle at

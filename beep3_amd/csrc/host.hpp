@@ -12,7 +12,12 @@
 #include <cmath>
 #include <map>
 #include <string>
+#include <algorithm>
 #include <atomic>
+#include <cstdlib>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace beep {
@@ -148,6 +153,38 @@ struct Rng {
 };
 
 // ---------------------------------------------------------------------------
+// worker threads for the host build: OMP_NUM_THREADS if set, else the
+// hardware concurrency, at most 16
+inline int host_threads() {
+    int n = int(std::thread::hardware_concurrency());
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) n = std::atoi(e);
+    return std::max(1, std::min(16, n));
+}
+
+template <typename F>
+void parallel_ranges(int n, F&& f) {
+    const int nt = std::min(host_threads(), std::max(1, n / 2048));
+    if (nt <= 1) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::exception_ptr err;
+    std::mutex mu;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            try {
+                f(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt));
+            } catch (...) {
+                std::lock_guard<std::mutex> g(mu);
+                err = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
+// ---------------------------------------------------------------------------
 // Adaptive octree over patch centroids.  Restates src/fmm/octree.h with the
 // BEM-FMM policy of BEEP::init_fmm_octree (beep.cpp:1585-1641): relaxed
 // insertion, build_neighbourhoods, optimize_neighbourhoods, remove_empty_nodes.
@@ -161,7 +198,10 @@ struct OctNode {
     bool has_children = false, is_leaf = true, is_shadow = false, is_deleted = false;
     bool removed = false;
     long cached_nb = -1;
-    std::vector<int> contents;   // patch indices, insertion order
+    // the node's items: positions lo .. hi-1 of the key-sorted items (the
+    // items whose descent keys share the node's prefix)
+    int lo = 0, hi = 0;
+    int count() const { return hi - lo; }
 };
 
 struct Octree {
@@ -187,25 +227,40 @@ struct Octree {
     }
     int find(int level, long x, long y, long z) const;
     int neighbour(int node, int nb) const;         // -1 when absent / outside
+    // the tree over points pts in the cube (c, e)
     void build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb);
+    // the tree from n descent keys in ascending order (the items' own order
+    // among equal keys is not needed: only counts decide the tree)
+    void build_sorted(const uint64_t* sorted_keys, int n, const Vec3& c, double e, int nb);
 
     // flattened result: leaves (non-shadow, non-empty) with their contents and
-    // neighbourhood contents (27 same-level boxes, octree.h:627-648)
+    // neighbourhood contents (27 same-level boxes, octree.h:627-648), each
+    // box's items in ascending order (the reference's insertion order);
+    // after build()
     std::vector<int> leaf_nodes;
     void leaves_csr(std::vector<int>& leaf_ptr, std::vector<int>& leaf_idx,
                     std::vector<int>& nb_ptr, std::vector<int>& nb_idx) const;
+    // the same as ranges of the key-sorted items: per leaf its level, and
+    // segments (lo, hi) of that level's boxes: the leaf's own first, then its
+    // neighbourhood's non-empty boxes in neighbour order (seg_ptr: leaves + 1,
+    // in segments); after either build
+    void leaf_segments(std::vector<int>& leaf_level, std::vector<int>& seg_ptr,
+                       std::vector<int>& seg_lo, std::vector<int>& seg_hi) const;
     long bottom_level() const { return long(levels.size()) - 1; }
 
   private:
+    std::vector<uint64_t> skeys_;  // build_sorted(): the descent keys, ascending
+    std::vector<int> perm_;        // build(): the items, each node's a range
     std::vector<Vec3> const* pts_ = nullptr;
-    std::vector<unsigned char> scratch_;  // octant of each item of the node being split
+    std::vector<unsigned char> oct_;  // octant of each item of the node being split
+    std::vector<int> tmp_;
+    void grow(int n, const Vec3& c, double e, int nb);
     // per level: node ids in key order, re-sorted when the level has grown
     std::vector<std::vector<int>> sorted_;
     const std::vector<int>& ordered(int level);
     int new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& c, double e, int parent);
     void subdivide(int n);
     void subdivide_neighbourhood(int n);
-    void add(int item, int node);
     long count_nb(int n) const;
     long cached_nb(int n);
     double interacts_per_item(long total_items);

@@ -7,10 +7,24 @@
 // The result decides which patch pairs are integrated explicitly: every leaf
 // interacts explicitly with the contents of its 27 same-level neighbour boxes
 // (get_neighbourhood_contents, octree.h:627-648), everything else is far field.
+//
+// The nodes hold no item lists: a node's items are one range of a single
+// array of the items, and a child's range is part of its parent's.  Every
+// decision of the build depends only on node counts.  Two ways to split a
+// node's range into its children's:
+//   build():        the items start in ascending order and a split is a
+//                   stable in-place partition by octant (the reference's
+//                   comparison of the item with the node centre), so a node
+//                   that has no children keeps its items in ascending order
+//                   (the reference's insertion order);
+//   build_sorted(): the items are given as their descent keys (tree_key.hpp)
+//                   in ascending order, and a child's range is found by binary
+//                   search on its key digit (the device builds the tree so).
 #include <algorithm>
 #include <stdexcept>
 
 #include "host.hpp"
+#include "tree_key.hpp"
 
 namespace beep {
 
@@ -112,8 +126,47 @@ int Octree::neighbour(int n, int nb) const {
 // child ids 1..8 of the reference map to (x, y, z) bits: id-1 = x + 2y + 4z
 void Octree::subdivide(int n) {
     if (nodes[n].has_children) return;
-    nodes[n].has_children = true;
     const int level = nodes[n].level + 1;
+    const bool keyed = perm_.empty();
+    if (keyed && level > beeptree::KEY_LEVELS)
+        throw std::runtime_error("octree: deeper than 21 levels (coincident patches?)");
+    nodes[n].has_children = true;
+    const int lo0 = nodes[n].lo, hi = nodes[n].hi;
+    int ends[8];
+    if (keyed) {
+        // a child's items: the part of the parent's range whose key digit at
+        // the child's level is its octant
+        const int shift = 3 * (beeptree::KEY_LEVELS - level);
+        const uint64_t* K = skeys_.data();
+        int lo = lo0;
+        for (int c = 0; c < 7; ++c) {
+            lo = int(std::partition_point(K + lo, K + hi, [&](uint64_t k) { return int((k >> shift) & 7) <= c; }) - K);
+            ends[c] = lo;
+        }
+        ends[7] = hi;
+    } else {
+        // each child receives the parent's items that fall in its octant
+        // (Node child constructor, octree.h:120-131), in the parent's order
+        const Vec3 pc = nodes[n].centre;
+        const int m = hi - lo0;
+        oct_.resize(size_t(m));
+        tmp_.resize(size_t(m));
+        int cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < m; ++k) {
+            const Vec3& q = (*pts_)[perm_[lo0 + k]];
+            oct_[k] = static_cast<unsigned char>((q.x > pc.x ? 1 : 0) + (q.y > pc.y ? 2 : 0) + (q.z > pc.z ? 4 : 0));
+            ++cnt[oct_[k]];
+        }
+        int at[8], run = 0;
+        for (int c = 0; c < 8; ++c) {
+            at[c] = run;
+            run += cnt[c];
+            ends[c] = lo0 + run;
+        }
+        for (int k = 0; k < m; ++k) tmp_[at[oct_[k]]++] = perm_[lo0 + k];
+        std::copy(tmp_.begin(), tmp_.end(), perm_.begin() + lo0);
+    }
+    int lo = lo0;
     for (int c = 0; c < 8; ++c) {
         const OctNode& p = nodes[n];
         const double h = p.edge / 2.0;
@@ -123,42 +176,10 @@ void Octree::subdivide(int n) {
         cc.z += (c & 4) ? h / 2.0 : -h / 2.0;
         int id = new_node(level, p.ix * 2 + (c & 1), p.iy * 2 + ((c >> 1) & 1),
                           p.iz * 2 + ((c >> 2) & 1), cc, h, n);
+        nodes[id].lo = lo;
+        nodes[id].hi = ends[c];
         nodes[n].children[c] = id;
-    }
-    // each child receives the parent's items that fall in its octant (Node
-    // child constructor, octree.h:120-131), in the parent's order: one pass
-    const Vec3 pc = nodes[n].centre;
-    const std::vector<int>& items = nodes[n].contents;
-    std::vector<unsigned char>& oct = scratch_;
-    oct.resize(items.size());
-    size_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (size_t k = 0; k < items.size(); ++k) {
-        const Vec3& q = (*pts_)[items[k]];
-        oct[k] = static_cast<unsigned char>((q.x > pc.x ? 1 : 0) + (q.y > pc.y ? 2 : 0) + (q.z > pc.z ? 4 : 0));
-        ++cnt[oct[k]];
-    }
-    std::vector<int>* dst[8];
-    for (int c = 0; c < 8; ++c) {
-        dst[c] = &nodes[nodes[n].children[c]].contents;
-        dst[c]->reserve(cnt[c]);
-    }
-    for (size_t k = 0; k < items.size(); ++k) dst[oct[k]]->push_back(items[k]);
-}
-
-void Octree::add(int item, int n) {
-    for (;;) {
-        nodes[n].contents.push_back(item);
-        if (!nodes[n].has_children) {
-            if (long(nodes[n].contents.size()) > 4L * nbsize && nodes[n].level < max_depth) {
-                subdivide(n);
-                nodes[n].is_leaf = false;
-            }
-            return;
-        }
-        const Vec3& q = (*pts_)[item];
-        const Vec3& c = nodes[n].centre;
-        int cid = (q.x > c.x ? 1 : 0) + (q.y > c.y ? 2 : 0) + (q.z > c.z ? 4 : 0);
-        n = nodes[n].children[cid];
+        lo = ends[c];
     }
 }
 
@@ -183,11 +204,11 @@ void Octree::subdivide_neighbourhood(int n) {
 }
 
 long Octree::count_nb(int n) const {
-    long r = long(nodes[n].contents.size());
+    long r = long(nodes[n].count());
     for (int nb = 0; nb < 27; ++nb) {
         if (nb == 13) continue;
         int m = neighbour(n, nb);
-        if (m >= 0) r += long(nodes[m].contents.size());
+        if (m >= 0) r += long(nodes[m].count());
     }
     return r;
 }
@@ -204,12 +225,30 @@ double Octree::interacts_per_item(long total_items) {
         for (int id : levels[level]) {
             const OctNode& o = nodes[id];
             if (o.is_leaf && !o.is_shadow && !o.is_deleted)
-                total += cached_nb(id) * long(o.contents.size());
+                total += cached_nb(id) * long(o.count());
         }
     return double(total) / double(total_items);
 }
 
 void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb) {
+    skeys_.clear();
+    perm_.resize(pts.size());
+    for (size_t i = 0; i < pts.size(); ++i) perm_[i] = int(i);
+    pts_ = &pts;
+    grow(int(pts.size()), c, e, nb);
+    pts_ = nullptr;
+}
+
+void Octree::build_sorted(const uint64_t* sorted_keys, int n, const Vec3& c, double e, int nb) {
+    perm_.clear();
+    skeys_.assign(sorted_keys, sorted_keys + n);
+    for (int i = 1; i < n; ++i)
+        if (skeys_[i] < skeys_[i - 1]) throw std::runtime_error("octree: keys not in ascending order");
+    grow(n, c, e, nb);
+}
+
+void Octree::grow(int n, const Vec3& c, double e, int nb) {
+    if (n <= 0) throw std::runtime_error("octree: no items");
     nodes.clear();
     levels.clear();
     index.clear();
@@ -218,10 +257,19 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     centre = c;
     edge = e;
     nbsize = nb;
-    pts_ = &pts;
-    nodes.reserve(64 + pts.size() / 4);
+    nodes.reserve(64 + size_t(n) / 4);
     new_node(0, 0, 0, 0, c, e, -1);
-    for (int i = 0; i < int(pts.size()); ++i) add(i, 0);
+    nodes[0].hi = n;
+    // relaxed insertion: a childless node splits when an item takes it past
+    // 4 * nbsize, and later items go on to its children, so a node ends up
+    // split exactly when it holds more than 4 * nbsize items in all (and is
+    // above max_depth)
+    for (size_t q = 0; q < nodes.size(); ++q) {
+        if (long(nodes[q].count()) > 4L * nbsize && nodes[q].level < max_depth) {
+            subdivide(int(q));
+            nodes[q].is_leaf = false;
+        }
+    }
 
     // build_neighbourhoods: every non-shadow node with children gets a
     // complete set of same-level neighbours with children
@@ -239,7 +287,7 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     for (int level = 0; level < int(levels.size()); ++level)
         for (int id : ordered(level)) {
             const OctNode& o = nodes[id];
-            if (!o.is_shadow && o.is_leaf && !o.is_deleted) total_items += long(o.contents.size());
+            if (!o.is_shadow && o.is_leaf && !o.is_deleted) total_items += long(o.count());
         }
     double avg = interacts_per_item(total_items);
     while (std::floor(avg) > nbsize) {
@@ -247,7 +295,7 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
         for (int level = 0; level < int(levels.size()); ++level)
             for (int id : ordered(level)) {
                 const OctNode& o = nodes[id];
-                if (!o.is_shadow && o.is_leaf && !o.contents.empty() && cached_nb(id) > nbsize)
+                if (!o.is_shadow && o.is_leaf && o.count() > 0 && cached_nb(id) > nbsize)
                     fat.push_back(id);
             }
         // fattest neighbourhoods first; ties keep level/key order
@@ -278,7 +326,7 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     for (int level = int(levels.size()) - 1; level >= 0; --level)
         for (int id : ordered(level)) {
             OctNode& o = nodes[id];
-            if (o.contents.empty() && id != 0 && !o.removed) {
+            if (o.count() == 0 && id != 0 && !o.removed) {
                 o.removed = true;
                 if (o.parent >= 0)
                     for (int c = 0; c < 8; ++c)
@@ -289,46 +337,95 @@ void Octree::build(const std::vector<Vec3>& pts, const Vec3& c, double e, int nb
     for (int level = 0; level < int(levels.size()); ++level)
         for (int id : ordered(level)) {
             const OctNode& o = nodes[id];
-            if (!o.removed && o.is_leaf && !o.is_shadow && !o.contents.empty())
+            if (!o.removed && o.is_leaf && !o.is_shadow && o.count() > 0)
                 leaf_nodes.push_back(id);
         }
-    // sanity: every item belongs to exactly one leaf
-    std::vector<int> owner(pts.size(), 0);
-    for (int n : leaf_nodes)
-        for (int i : nodes[n].contents) owner[i]++;
-    for (int c : owner)
-        if (c != 1) throw std::runtime_error("octree: a patch is not in exactly one leaf");
+    // sanity: the leaves' ranges tile the sorted items (every item in exactly
+    // one leaf)
+    std::vector<std::pair<int, int>> r;
+    r.reserve(leaf_nodes.size());
+    for (int id : leaf_nodes) r.push_back({nodes[id].lo, nodes[id].hi});
+    std::sort(r.begin(), r.end());
+    int at = 0;
+    for (const auto& x : r) {
+        if (x.first != at) throw std::runtime_error("octree: a patch is not in exactly one leaf");
+        at = x.second;
+    }
+    if (at != n) throw std::runtime_error("octree: a patch is not in exactly one leaf");
 }
 
 void Octree::leaves_csr(std::vector<int>& leaf_ptr, std::vector<int>& leaf_idx,
                         std::vector<int>& nb_ptr, std::vector<int>& nb_idx) const {
+    if (!skeys_.empty()) throw std::runtime_error("octree: built from keys, its items are on the device");
+    // a box without children holds its items in ascending order; a box with
+    // children had its range partitioned, so its items are sorted once
+    std::vector<int> sorted_at(nodes.size(), -1);
+    std::vector<int> pool;
+    auto items = [&](int m, const int*& b, const int*& e) {
+        const OctNode& o = nodes[m];
+        if (!o.has_children) {
+            b = perm_.data() + o.lo;
+            e = perm_.data() + o.hi;
+            return;
+        }
+        if (sorted_at[m] < 0) {
+            sorted_at[m] = int(pool.size());
+            pool.insert(pool.end(), perm_.begin() + o.lo, perm_.begin() + o.hi);
+            std::sort(pool.begin() + sorted_at[m], pool.end());
+        }
+        b = pool.data() + sorted_at[m];
+        e = b + o.count();
+    };
+    size_t n_items = 0, n_nb = 0;
+    for (int n : leaf_nodes) {
+        n_items += size_t(nodes[n].count());
+        for (int nb = 0; nb < 27; ++nb) {
+            const int m = neighbour(n, nb);
+            if (m >= 0) n_nb += size_t(nodes[m].count());
+        }
+    }
     leaf_ptr.assign(1, 0);
     nb_ptr.assign(1, 0);
     leaf_idx.clear();
     nb_idx.clear();
-    size_t n_items = 0, n_nb = 0;
-    for (int n : leaf_nodes) {
-        n_items += nodes[n].contents.size();
-        for (int nb = 0; nb < 27; ++nb) {
-            const int m = neighbour(n, nb);
-            if (m >= 0) n_nb += nodes[m].contents.size();
-        }
-    }
     leaf_ptr.reserve(leaf_nodes.size() + 1);
     nb_ptr.reserve(leaf_nodes.size() + 1);
     leaf_idx.reserve(n_items);
     nb_idx.reserve(n_nb);
+    pool.reserve(n_items);
+    const int *b, *e;
     for (int n : leaf_nodes) {
-        const OctNode& o = nodes[n];
-        leaf_idx.insert(leaf_idx.end(), o.contents.begin(), o.contents.end());
+        items(n, b, e);
+        leaf_idx.insert(leaf_idx.end(), b, e);
         leaf_ptr.push_back(int(leaf_idx.size()));
         for (int nb = 0; nb < 27; ++nb) {
-            int m = neighbour(n, nb);
-            if (m < 0) continue;
-            const auto& c = nodes[m].contents;
-            nb_idx.insert(nb_idx.end(), c.begin(), c.end());
+            const int m = neighbour(n, nb);
+            if (m < 0 || nodes[m].count() == 0) continue;
+            items(m, b, e);
+            nb_idx.insert(nb_idx.end(), b, e);
         }
         nb_ptr.push_back(int(nb_idx.size()));
+    }
+}
+
+void Octree::leaf_segments(std::vector<int>& leaf_level, std::vector<int>& seg_ptr,
+                           std::vector<int>& seg_lo, std::vector<int>& seg_hi) const {
+    leaf_level.clear();
+    seg_ptr.assign(1, 0);
+    seg_lo.clear();
+    seg_hi.clear();
+    for (int n : leaf_nodes) {
+        const OctNode& o = nodes[n];
+        leaf_level.push_back(o.level);
+        seg_lo.push_back(o.lo);
+        seg_hi.push_back(o.hi);
+        for (int nb = 0; nb < 27; ++nb) {
+            const int m = neighbour(n, nb);
+            if (m < 0 || nodes[m].count() == 0) continue;
+            seg_lo.push_back(nodes[m].lo);
+            seg_hi.push_back(nodes[m].hi);
+        }
+        seg_ptr.push_back(int(seg_lo.size()));
     }
 }
 

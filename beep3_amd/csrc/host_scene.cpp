@@ -202,36 +202,6 @@ static Vec3 place(const Instance& in, const Vec3& p) {
     return in.rot.rotate(p - in.mesh->mesh.centre) + in.offset;
 }
 
-// worker threads for the host build: OMP_NUM_THREADS if set, else the
-// hardware concurrency, at most 16
-static int host_threads() {
-    int n = int(std::thread::hardware_concurrency());
-    if (const char* e = std::getenv("OMP_NUM_THREADS")) n = std::atoi(e);
-    return std::max(1, std::min(16, n));
-}
-
-template <typename F>
-static void parallel_ranges(int n, F&& f) {
-    const int nt = std::min(host_threads(), std::max(1, n / 2048));
-    if (nt <= 1) {
-        f(0, n);
-        return;
-    }
-    std::vector<std::thread> th;
-    std::exception_ptr err;
-    std::mutex mu;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            try {
-                f(int(int64_t(n) * t / nt), int(int64_t(n) * (t + 1) / nt));
-            } catch (...) {
-                std::lock_guard<std::mutex> g(mu);
-                err = std::current_exception();
-            }
-        });
-    for (auto& x : th) x.join();
-    if (err) std::rethrow_exception(err);
-}
 
 extern "C" int beep_scene_build(beep_scene* s) {
     return guarded([&] {

@@ -86,6 +86,28 @@ class Device:
         t.nb_ptr, t.nb_idx = keep["nb_ptr"].ctypes.data, keep["nb_idx"].ctypes.data
         self._check(lib().beep_hip_set_tree(self._h, C.byref(t)))
 
+    def build_tree(self, centre, edge: float, nbsize: int):
+        """The octree and near list built on the device from the centroids of
+        the geometry set (the host's octree, same leaves and lists), in the
+        bounding cube (centre, edge).  Returns (leaves, neighbourhood entries)."""
+        c = np.ascontiguousarray(centre, dtype=np.float64).reshape(3)
+        nl, nn = C.c_int(0), C.c_longlong(0)
+        self._check(lib().beep_hip_build_tree(self._h, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                              float(edge), int(nbsize), C.byref(nl), C.byref(nn)))
+        self._tree_sizes = (nl.value, nn.value)
+        return self._tree_sizes
+
+    def tree_arrays(self) -> dict:
+        """leaf_ptr, leaf_idx, nb_ptr, nb_idx of the tree last built on the device."""
+        L, nnb = self._tree_sizes
+        out = {"leaf_ptr": np.zeros(L + 1, np.int32), "leaf_idx": np.zeros(self.n, np.int32),
+               "nb_ptr": np.zeros(L + 1, np.int32), "nb_idx": np.zeros(max(nnb, 1), np.int32)}
+        ip_ = C.POINTER(C.c_int)
+        self._check(lib().beep_hip_get_tree(self._h, *(out[k].ctypes.data_as(ip_) for k in (
+            "leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx"))))
+        out["nb_idx"] = out["nb_idx"][:nnb]
+        return out
+
     FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}
 
     def set_far_field(self, mode="auto", order=4, leaf_target=48):

@@ -172,6 +172,23 @@ int beep_hip_destroy(beep_hip_ctx* ctx);
 
 int beep_hip_set_geometry(beep_hip_ctx* ctx, const beep_hip_geometry* geom);
 int beep_hip_set_tree(beep_hip_ctx* ctx, const beep_hip_tree* tree);
+/* The octree and its explicit-pair structure built on the device instead of
+ * received (beep_hip_set_tree with beep_scene_tree's arrays): the same
+ * leaves, leaf contents and neighbourhood contents, in the same order, as the
+ * host's octree (BEEP::init_fmm_octree, beep.cpp:1585-1641: relaxed
+ * insertion at 4 nbsize, build_neighbourhoods, optimize_neighbourhoods,
+ * remove_empty_nodes, octree.h:780-1154; contents octree.h:627-648) over the
+ * patch centroids of the geometry set, in the bounding cube (centre, edge)
+ * (BEEP::calculate_bounding_cube, beep.cpp:1652-1693; beep_scene_get_sizes
+ * gives it).  The device computes every patch's descent key, sorts the keys
+ * and gathers the lists; the host decides the tree's shape from the sorted
+ * keys' box counts.  Returns the number of leaves and of neighbourhood
+ * entries. */
+int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, int nbsize,
+                        int* n_leaves, long long* n_nb);
+/* Copies of the tree set (by beep_hip_set_tree or beep_hip_build_tree):
+ * leaf_ptr L + 1, leaf_idx N, nb_ptr L + 1, nb_idx n_nb; any may be NULL. */
+int beep_hip_get_tree(beep_hip_ctx* ctx, int* leaf_ptr, int* leaf_idx, int* nb_ptr, int* nb_idx);
 /* geometry from the surface (point sets generated on the device); replaces
  * beep_hip_set_geometry, after which the tree is set as usual */
 int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* surface);

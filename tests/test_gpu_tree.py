@@ -1,0 +1,67 @@
+"""The octree and near list built on the device (beep_hip_build_tree) against
+the host's (beep_scene_tree, itself pinned to the oracle's plain restatement
+of octree.h in tests/test_octree.py): the same leaves, leaf contents and
+neighbourhood contents in the same order, so the mat-vec over it has the same
+bits."""
+import time
+
+import numpy as np
+import pytest
+
+from beep3_amd import configs
+from beep3_amd.device import Device
+from beep3_amd.scene import Scene
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx")
+
+
+def _compare(sc: Scene, nbsize: int, matvec: bool):
+    a, s = sc.arrays(), sc.sizes()
+    d = Device(0)
+    d.set_far_field("direct")
+    d.set_geometry(a)
+    t0 = time.perf_counter()
+    nl, nn = d.build_tree(list(s.cube_centre), s.cube_edge, nbsize)
+    ms = (time.perf_counter() - t0) * 1e3
+    got = d.tree_arrays()
+    print(f"{a['area'].shape[0]} patches: {nl} leaves, {nn} neighbourhood entries, device build {ms:.1f} ms")
+    for k in KEYS:
+        assert np.array_equal(got[k], a[k]), k
+    if matvec:
+        x = np.random.default_rng(3).normal(size=2 * a["area"].shape[0])
+        h = Device(0)
+        h.set_far_field("direct")
+        h.set_geometry(a)
+        h.set_tree(a)
+        for dev in (d, h):
+            dev.precompute(sc.kappa, "fp64")
+        assert np.array_equal(d.matvec(x, "fp64"), h.matvec(x, "fp64"))
+        h.close()
+    d.close()
+
+
+@pytest.mark.parametrize("subdiv,nbsize", [(4, 1500), (5, 300), (4, 40)])
+def test_device_tree_single_molecule(subdiv, nbsize):
+    m = configs.globular_mesh(subdiv, 3, 200, 3)
+    sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, nbsize).add_instance(m, 2.0).build()
+    _compare(sc, nbsize, matvec=subdiv == 4)
+
+
+def test_device_tree_poses():
+    m = configs.pose_mesh(5)
+    rng = np.random.default_rng(5)
+    for k in range(2):
+        q, off = configs.random_pose(rng, 30.0, 60.0 if k else 35.0)
+        sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 1500)
+        sc.add_instance(m, 2.0, quat=None, offset=[0.0, 0.0, 0.0])
+        sc.add_instance(m, 2.0, quat=q, offset=off)
+        _compare(sc.build(), 1500, matvec=False)
+
+
+def test_device_tree_globular_full_size():
+    """163,842 patches (the mat-vec benchmark's surface)."""
+    b = configs.globular()
+    sc = Scene(b.Dsolvent, b.kappa, b.quad, b.qual, b.nbsize).add_instance(b.library[0], 2.0).build()
+    _compare(sc, b.nbsize, matvec=False)
