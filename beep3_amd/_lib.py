@@ -113,7 +113,7 @@ class ServerOpts(C.Structure):
         ("solve_mode", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int),
         ("max_refine", C.c_int), ("inner_tol", C.c_double), ("far_mode", C.c_int),
         ("fmm_order", C.c_int), ("fmm_leaf", C.c_int), ("keep_fh", C.c_int),
-        ("near_mode", C.c_int),
+        ("near_mode", C.c_int), ("device_tree", C.c_int),
     ]
 
 

@@ -60,8 +60,15 @@ double estimate_bytes(const beep_scene* s, const beep_server_opts& o, bool store
     auto count = [&](int k) { return s->ptr[k].empty() ? 0.0 : double(s->ptr[k].back()); };
     const double nq = count(0), nd = count(1), ng = count(2);
     double nnz = 0;
-    for (size_t L = 0; L + 1 < s->leaf_ptr.size(); ++L)
-        nnz += double(s->leaf_ptr[L + 1] - s->leaf_ptr[L]) * double(s->nb_ptr[L + 1] - s->nb_ptr[L]);
+    if (s->with_tree) {
+        for (size_t L = 0; L + 1 < s->leaf_ptr.size(); ++L)
+            nnz += double(s->leaf_ptr[L + 1] - s->leaf_ptr[L]) * double(s->nb_ptr[L + 1] - s->nb_ptr[L]);
+    } else {
+        // no tree yet (the device builds it after admission): the octree is
+        // refined until its patches interact with fewer than nbsize + 1
+        // patches each on average (optimize_neighbourhoods, octree.h:867-986)
+        nnz = N * std::min(N, double(s->nbsize) + 1.0);
+    }
     double b = 96.0 * (nq + nd) + 64.0 * ng + 64.0 * N;
     if (stored && o.solve_mode != BEEP_SOLVE_FP64) b += 16.0 * nnz;
     if (stored && o.solve_mode == BEEP_SOLVE_FP64) b += 32.0 * nnz;
@@ -257,11 +264,17 @@ void beep_server::energy_coefficients(beep_hip_ctx* ctx, beep_scene* sc, std::ve
     }
 }
 
+namespace {
+void set_tree_from(beep_hip_ctx* ctx, const beep_scene* sc);
+}  // namespace
+
 void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     beep_result& r = d.r;
     beep_scene* sc = j.scene;
     const double th = clock();
     sc->host_points = false;  // the device generates the point sets
+    sc->with_tree = o.device_tree == 0;  // else the device builds the octree
+    sc->built = false;
     if (beep_scene_build(sc) != 0)
         throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
     r.host_ms = (clock() - th) * 1e3;
@@ -316,13 +329,14 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     }
     check_hip(ctx, beep_hip_set_far_field(ctx, o.far_mode, o.fmm_order, o.fmm_leaf), "far field");
     upload_surface(ctx, sc);
-    beep_hip_tree t{};
-    t.n_leaves = int(sc->leaf_ptr.size()) - 1;
-    t.leaf_ptr = sc->leaf_ptr.data();
-    t.leaf_idx = sc->leaf_idx.data();
-    t.nb_ptr = sc->nb_ptr.data();
-    t.nb_idx = sc->nb_idx.data();
-    check_hip(ctx, beep_hip_set_tree(ctx, &t), "tree");
+    if (o.device_tree) {
+        const double cc[3] = {sc->cube_centre.x, sc->cube_centre.y, sc->cube_centre.z};
+        int nl = 0;
+        long long nn = 0;
+        check_hip(ctx, beep_hip_build_tree(ctx, cc, sc->cube_edge, sc->nbsize, &nl, &nn), "tree");
+    } else {
+        set_tree_from(ctx, sc);
+    }
     const double tp = clock();
     r.upload_ms = (tp - td) * 1e3;
     check_hip(ctx, beep_hip_set_near_mode(ctx, fly ? BEEP_NEAR_ON_THE_FLY : BEEP_NEAR_PRECOMPUTE), "near mode");
@@ -448,6 +462,7 @@ void beep_server_default_opts(beep_server_opts* o) {
     o->fmm_order = 5;
     o->fmm_leaf = 64;
     o->near_mode = BEEP_SERVE_NEAR_AUTO;
+    o->device_tree = 1;
 }
 
 int beep_server_estimate(const beep_server_opts* o, beep_scene* sc, double* stored_bytes, double* fly_bytes) {

@@ -51,6 +51,9 @@ typedef struct {
                                fits the memory the budget and the device allow with them,
                                else on the fly (the reference's OpenCL path for blobs that
                                do not fit, opencl_bem.cpp:346-747) */
+    int device_tree;        /* 1: each request's octree and near list are built on the
+                               device after admission (beep_hip_build_tree), the host
+                               builds the patches only; 0: on the host with the scene */
 } beep_server_opts;
 
 #define BEEP_SERVE_NEAR_AUTO (-1)

@@ -179,3 +179,22 @@ def test_near_field_on_the_fly_when_coefficients_do_not_fit():
     assert res["forced"]["near_on_the_fly"] == 1 and res["auto"]["near_on_the_fly"] == 1
     for k in ("forced", "auto"):
         assert abs(res[k]["energy"] - res["stored"]["energy"]) <= 1e-8 * abs(res["stored"]["energy"])
+
+
+def test_device_tree_and_host_tree_requests_are_identical():
+    """The server's default builds each request's octree on the device after
+    admission; with device_tree=False the host builds it with the scene.  The
+    trees are the same, so f, h and the energy have the same bits."""
+    meshes = _meshes()[:3]
+    res = {}
+    for dev_tree in (True, False):
+        srv = Server(slots=2, keep_fh=True, far_field="fmm", fmm_order=5, device_tree=dev_tree)
+        ids = {srv.submit(_scene(m)): k for k, m in enumerate(meshes)}
+        res[dev_tree] = _collect(srv, ids)
+        srv.close()
+    for k in range(len(meshes)):
+        rd, fd, hd = res[True][k]
+        rh, fh, hh = res[False][k]
+        print(k, "patches", rd["n_patches"], "host ms device tree / host tree", rd["host_ms"], rh["host_ms"])
+        assert rd["energy"] == rh["energy"]
+        assert np.array_equal(fd, fh) and np.array_equal(hd, hh)
