@@ -112,6 +112,14 @@ struct beep_hip_ctx {
     DevBuf<double> stage;  // raw point upload staging (7 doubles per point)
     DevBuf<double> sverts; // surface for point generation
     DevBuf<int> stris, svt_ptr, svt_tri, svt_corner;
+    DevBuf<int> sctl_idx;  // curved triangles: index and control points, while points are made
+    DevBuf<double> sctl;
+    // octree build on the device: keys, sort buffers, per-level sorted
+    // patches and gather segments (grow-only, kept across builds)
+    DevBuf<uint64_t> tk_full, tk_a, tk_b;
+    DevBuf<int> ti_a, ti_b, t_sorted;
+    DevBuf<unsigned char> t_tmp;
+    DevBuf<int4> t_segl, t_segn;
     bool have_geom = false;
 
     // tree
@@ -813,11 +821,8 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
         ctx->svt_ptr.upload(g->vt_ptr, N + 1, s);
         ctx->svt_tri.upload(g->vt_tri, nvt, s);
         ctx->svt_corner.upload(g->vt_corner, nvt, s);
-        // control points only while the points are made
-        DevBuf<int> cidx;
-        DevBuf<double> cctl;
-        if (g->ctl_idx) cidx.upload(g->ctl_idx, T, s);
-        if (NC > 0) cctl.upload(g->ctl, size_t(63) * NC, s);
+        if (g->ctl_idx) ctx->sctl_idx.upload(g->ctl_idx, T, s);
+        if (NC > 0) ctx->sctl.upload(g->ctl, size_t(63) * NC, s);
         // one point set per rule: counts (1, or 2 sub-triangles x k points per
         // triangle around the vertex), then the points on the device
         std::vector<int> ptr(size_t(N) + 1);
@@ -843,8 +848,8 @@ int beep_hip_set_surface(beep_hip_ctx* ctx, const beep_hip_surface* g) {
             o64.alloc(size_t(ptr[N]));
             if (o32) o32->alloc(size_t(ptr[N]));
             launch_gen_points(s, N, ctx->sverts.p, ctx->stris.p, ctx->svt_ptr.p, ctx->svt_tri.p,
-                              ctx->svt_corner.p, g->ctl_idx ? cidx.p : nullptr,
-                              NC > 0 ? cctl.p : nullptr, ctx->patches.p, dptr.p, rd, per_area,
+                              ctx->svt_corner.p, g->ctl_idx ? ctx->sctl_idx.p : nullptr,
+                              NC > 0 ? ctx->sctl.p : nullptr, ctx->patches.p, dptr.p, rd, per_area,
                               o64.p, o32 ? o32->p : nullptr);
             hip_check(hipStreamSynchronize(s), "sync point generation");  // ptr is reused
         };
@@ -967,16 +972,20 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
         const int N = ctx->N;
         hipStream_t s = ctx->stream;
         // descent keys of the centroids, sorted with their patches
-        DevBuf<uint64_t> kfull, ka, kb;
-        DevBuf<int> ia, ib;
+        DevBuf<uint64_t>& kfull = ctx->tk_full;
+        DevBuf<uint64_t>& ka = ctx->tk_a;
+        DevBuf<uint64_t>& kb = ctx->tk_b;
+        DevBuf<int>& ia = ctx->ti_a;
+        DevBuf<int>& ib = ctx->ti_b;
+        DevBuf<unsigned char>& tmp = ctx->t_tmp;
         kfull.alloc(N);
         ka.alloc(N);
         kb.alloc(N);
         ia.alloc(N);
         ib.alloc(N);
-        DevBuf<unsigned char> tmp;
         const int key_bits = 3 * beeptree::KEY_LEVELS;
         size_t tb = tree_sort_bytes(N, key_bits);
+        for (int l = 1; l < beeptree::KEY_LEVELS; ++l) tb = std::max(tb, tree_sort_bytes(N, 3 * l));
         tmp.alloc(std::max<size_t>(tb, 1));
         launch_tree_keys(s, N, ctx->patches.p, centre, edge, 0, nullptr, kfull.p, ia.p);
         tree_sort_pairs(s, tmp.p, tb, kfull.p, ka.p, ia.p, ib.p, N, key_bits);
@@ -997,7 +1006,7 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
         int nlev = 0;
         for (int l : lev)
             if (slot[l] < 0) slot[l] = nlev++;
-        DevBuf<int> S;
+        DevBuf<int>& S = ctx->t_sorted;
         S.alloc(size_t(nlev) * N);
         for (int l = 0; l <= beeptree::KEY_LEVELS; ++l) {
             if (slot[l] < 0) continue;
@@ -1005,11 +1014,6 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
             if (l == 0) {
                 launch_tree_keys(s, N, nullptr, nullptr, 0.0, 0, kfull.p, ka.p, dst);  // all in one box
                 continue;
-            }
-            const size_t lb = tree_sort_bytes(N, 3 * l);
-            if (lb > tb) {
-                tmp.alloc(lb);
-                tb = lb;
             }
             launch_tree_keys(s, N, nullptr, nullptr, 0.0, l, kfull.p, ka.p, ia.p);
             tree_sort_pairs(s, tmp.p, tb, ka.p, kb.p, ia.p, dst, N, 3 * l);
@@ -1034,7 +1038,8 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
         }
         require(leaf_ptr[L] == N, "octree: leaves do not cover the patches");
         require(int64_t(nlev) * N < int64_t(0x7fffffff), "too many patches for 32-bit offsets");
-        DevBuf<int4> dsegl, dsegn;
+        DevBuf<int4>& dsegl = ctx->t_segl;
+        DevBuf<int4>& dsegn = ctx->t_segn;
         dsegl.upload(segl.data(), segl.size(), s);
         dsegn.upload(segn.data(), segn.size(), s);
         ctx->leaf_idx.alloc(N);
@@ -2153,6 +2158,9 @@ int beep_hip_memory_held(beep_hip_ctx* ctx, size_t* bytes) {
         add(ctx->pair_rows); add(ctx->pair_ptr); add(ctx->pair_col); add(ctx->pair_i);
         add(ctx->pcoef32); add(ctx->pcoef64); add(ctx->ns_rec); add(ctx->guard_rec); add(ctx->hvec2);
         add(ctx->cut_off_d); add(ctx->cut_count); add(ctx->cut_sph_d); add(ctx->cut_chunk_d); add(ctx->cut_tmp);
+        add(ctx->sctl_idx); add(ctx->sctl); add(ctx->tk_full); add(ctx->tk_a); add(ctx->tk_b);
+        add(ctx->ti_a); add(ctx->ti_b); add(ctx->t_sorted); add(ctx->t_tmp); add(ctx->t_segl);
+        add(ctx->t_segn);
         *bytes = b + ctx->fmm_arena.held;
     });
 }

@@ -273,7 +273,12 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     beep_scene* sc = j.scene;
     const double th = clock();
     sc->host_points = false;  // the device generates the point sets
-    sc->with_tree = o.device_tree == 0;  // else the device builds the octree
+    size_t n_items = 0;
+    for (const Instance& in : sc->inst) n_items += in.mesh->mesh.verts.size();
+    const bool device_tree = o.device_tree == BEEP_SERVE_TREE_AUTO
+                                 ? n_items >= size_t(BEEP_SERVE_TREE_DEVICE_MIN)
+                                 : o.device_tree != 0;
+    sc->with_tree = !device_tree;  // else the device builds the octree
     sc->built = false;
     if (beep_scene_build(sc) != 0)
         throw std::runtime_error(std::string("scene build: ") + beep_host_last_error());
@@ -329,7 +334,7 @@ void beep_server::solve_one(int slot, beep_hip_ctx*& ctx, Job& j, Done& d) {
     }
     check_hip(ctx, beep_hip_set_far_field(ctx, o.far_mode, o.fmm_order, o.fmm_leaf), "far field");
     upload_surface(ctx, sc);
-    if (o.device_tree) {
+    if (device_tree) {
         const double cc[3] = {sc->cube_centre.x, sc->cube_centre.y, sc->cube_centre.z};
         int nl = 0;
         long long nn = 0;
@@ -462,7 +467,7 @@ void beep_server_default_opts(beep_server_opts* o) {
     o->fmm_order = 5;
     o->fmm_leaf = 64;
     o->near_mode = BEEP_SERVE_NEAR_AUTO;
-    o->device_tree = 1;
+    o->device_tree = BEEP_SERVE_TREE_AUTO;
 }
 
 int beep_server_estimate(const beep_server_opts* o, beep_scene* sc, double* stored_bytes, double* fly_bytes) {

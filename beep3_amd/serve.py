@@ -25,7 +25,7 @@ NEAR_MODES = {"auto": -1, "precompute": 0, "on_the_fly": 1}
 class Server:
     def __init__(self, slots=4, budget_gib=0.0, mode="mixed", tol=1e-6, max_iter=100,
                  max_refine=4, inner_tol=0.0, far_field="auto", fmm_order=5, fmm_leaf=64,
-                 keep_fh=False, device=0, near_mode="auto", device_tree=True):
+                 keep_fh=False, device=0, near_mode="auto", device_tree="auto"):
         o = ServerOpts()
         lib().beep_server_default_opts(C.byref(o))
         o.device, o.slots, o.hbm_budget = int(device), int(slots), float(budget_gib) * 2.0**30
@@ -34,7 +34,7 @@ class Server:
         o.far_mode, o.fmm_order, o.fmm_leaf = FAR_MODES[far_field], int(fmm_order), int(fmm_leaf)
         o.keep_fh = 1 if keep_fh else 0
         o.near_mode = NEAR_MODES[near_mode]
-        o.device_tree = 1 if device_tree else 0
+        o.device_tree = -1 if device_tree == "auto" else (1 if device_tree else 0)
         self._opts = o
         h = vp()
         if lib().beep_server_create(C.byref(o), C.byref(h)) != 0:

@@ -53,8 +53,14 @@ typedef struct {
                                do not fit, opencl_bem.cpp:346-747) */
     int device_tree;        /* 1: each request's octree and near list are built on the
                                device after admission (beep_hip_build_tree), the host
-                               builds the patches only; 0: on the host with the scene */
+                               builds the patches only; 0: on the host with the scene;
+                               BEEP_SERVE_TREE_AUTO: on the device for requests of at
+                               least BEEP_SERVE_TREE_DEVICE_MIN patches */
 } beep_server_opts;
+
+#define BEEP_SERVE_TREE_AUTO (-1)
+/* below this the host's octree is quicker than the device's fixed costs */
+#define BEEP_SERVE_TREE_DEVICE_MIN 65536
 
 #define BEEP_SERVE_NEAR_AUTO (-1)
 
