@@ -33,7 +33,7 @@ CONVERT_ENERGY_TO_KJ_PER_MOL = AVOGADRO * ELEMENTARY_CHARGE**2 / (EPSILON0 * ANG
 class BEEP:
     def __init__(self, Dsolvent=80.0, kappa=0.0, quad_points=0, qual_points=4, nbsize=1500,
                  device=0, precision="fp32", far_field="auto", fmm_order=5, fmm_leaf=48,
-                 device_points=True):
+                 device_points=True, device_tree=False):
         self.Dsolvent, self.kappa = float(Dsolvent), float(kappa)
         self.quad, self.qual, self.nbsize = int(quad_points), int(qual_points), int(nbsize)
         self.precision = precision  # coefficient set precomputed at init
@@ -41,6 +41,9 @@ class BEEP:
         self.far_field, self.fmm_order, self.fmm_leaf = far_field, int(fmm_order), int(fmm_leaf)
         # point sets made on the device from the surface (else on the host and uploaded)
         self.device_points = bool(device_points)
+        # octree and near list built on the device (True, or "auto": from 65,536
+        # patches); the scene's arrays then hold no leaf lists (Device.tree_arrays)
+        self.device_tree = device_tree
         self.library: list[Mesh] = []
         self.instances: list[tuple] = []
         self.scene: Scene | None = None
@@ -66,12 +69,15 @@ class BEEP:
         self._ready = False
 
     def init(self, precisions=None):
-        """Build patches and octree (host), upload to the GPU and precompute
-        the explicit-pair coefficients (precalc_neighbour_interactions)."""
+        """Build patches and octree (the octree on the device for large
+        scenes), upload to the GPU and precompute the explicit-pair
+        coefficients (precalc_neighbour_interactions)."""
         t0 = time.perf_counter()
         dev_points = self.device_points
+        n_items = sum(self.library[lib_id].info()["n_vertices"] for lib_id, _, _, _ in self.instances)
+        dev_tree = n_items >= 65536 if self.device_tree == "auto" else bool(self.device_tree)
         sc = Scene(self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize,
-                   host_points=not dev_points)
+                   host_points=not dev_points, tree=not dev_tree)
         for lib_id, off, rot, Dp in self.instances:
             sc.add_instance(self.library[lib_id], Dp, quat=rot, offset=off)
         sc.build()
@@ -84,7 +90,14 @@ class BEEP:
             self.dev.set_surface(a, sc.surface(), self.qual, self.quad)
         else:
             self.dev.set_geometry(a)
-        self.dev.set_tree(a)
+        sz = sc.sizes()
+        if dev_tree:
+            n_leaves, _ = self.dev.build_tree(list(sz.cube_centre), sz.cube_edge, self.nbsize)
+            t = self.dev.tree_arrays(lists=False)
+            explicit = int(np.sum(np.diff(t["leaf_ptr"]).astype(np.int64) * np.diff(t["nb_ptr"])))
+        else:
+            self.dev.set_tree(a)
+            n_leaves, explicit = sz.n_leaves, int(sz.n_near_pairs)
         t2 = time.perf_counter()
         for p in (precisions or [self.precision]):
             self.dev.precompute(self.kappa, p)
@@ -92,9 +105,8 @@ class BEEP:
         t3 = time.perf_counter()
         self.scene = sc
         self.n = int(a["area"].shape[0])
-        sz = sc.sizes()
-        self.run_info.update(patches=self.n, nbsize=self.nbsize, leaves=sz.n_leaves,
-                             explicit=int(sz.n_near_pairs), tree_depth=sz.tree_depth,
+        self.run_info.update(patches=self.n, nbsize=self.nbsize, leaves=n_leaves,
+                             explicit=explicit, tree_depth=sz.tree_depth if not dev_tree else None,
                              host_setup_ms=(t1 - t0) * 1e3, upload_ms=(t2 - t1) * 1e3,
                              precalc_ms=(t3 - t2) * 1e3)
         self._ready = True

@@ -1006,6 +1006,7 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
         int nlev = 0;
         for (int l : lev)
             if (slot[l] < 0) slot[l] = nlev++;
+        require(int64_t(nlev) * N < int64_t(0x7fffffff), "too many patches for 32-bit offsets");
         DevBuf<int>& S = ctx->t_sorted;
         S.alloc(size_t(nlev) * N);
         for (int l = 0; l <= beeptree::KEY_LEVELS; ++l) {
@@ -1037,7 +1038,6 @@ int beep_hip_build_tree(beep_hip_ctx* ctx, const double* centre, double edge, in
             nb_ptr[f + 1] = int(at);
         }
         require(leaf_ptr[L] == N, "octree: leaves do not cover the patches");
-        require(int64_t(nlev) * N < int64_t(0x7fffffff), "too many patches for 32-bit offsets");
         DevBuf<int4>& dsegl = ctx->t_segl;
         DevBuf<int4>& dsegn = ctx->t_segn;
         dsegl.upload(segl.data(), segl.size(), s);

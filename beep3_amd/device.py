@@ -97,15 +97,21 @@ class Device:
         self._tree_sizes = (nl.value, nn.value)
         return self._tree_sizes
 
-    def tree_arrays(self) -> dict:
-        """leaf_ptr, leaf_idx, nb_ptr, nb_idx of the tree last built on the device."""
+    def tree_arrays(self, lists: bool = True) -> dict:
+        """leaf_ptr, leaf_idx, nb_ptr, nb_idx of the tree last built on the
+        device (lists=False: the two pointer arrays only)."""
         L, nnb = self._tree_sizes
-        out = {"leaf_ptr": np.zeros(L + 1, np.int32), "leaf_idx": np.zeros(self.n, np.int32),
-               "nb_ptr": np.zeros(L + 1, np.int32), "nb_idx": np.zeros(max(nnb, 1), np.int32)}
+        out = {"leaf_ptr": np.zeros(L + 1, np.int32), "nb_ptr": np.zeros(L + 1, np.int32)}
+        if lists:
+            out["leaf_idx"] = np.zeros(self.n, np.int32)
+            out["nb_idx"] = np.zeros(max(nnb, 1), np.int32)
         ip_ = C.POINTER(C.c_int)
-        self._check(lib().beep_hip_get_tree(self._h, *(out[k].ctypes.data_as(ip_) for k in (
-            "leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx"))))
-        out["nb_idx"] = out["nb_idx"][:nnb]
+        ptr = {k: (out[k].ctypes.data_as(ip_) if k in out else None) for k in (
+            "leaf_ptr", "leaf_idx", "nb_ptr", "nb_idx")}
+        self._check(lib().beep_hip_get_tree(self._h, ptr["leaf_ptr"], ptr["leaf_idx"], ptr["nb_ptr"],
+                                            ptr["nb_idx"]))
+        if lists:
+            out["nb_idx"] = out["nb_idx"][:nnb]
         return out
 
     FAR_MODES = {"auto": -1, "direct": 0, "fmm": 1, "pw": 2}

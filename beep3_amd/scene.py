@@ -128,13 +128,17 @@ class Mesh:
 class Scene:
     """Solver parameters + rigid mesh instances + octree."""
 
-    def __init__(self, Dsolvent=80.0, kappa=0.0, quad=0, qual=4, nbsize=1500, host_points=True):
+    def __init__(self, Dsolvent=80.0, kappa=0.0, quad=0, qual=4, nbsize=1500, host_points=True,
+                 tree=True):
         h = vp()
         check_host(lib().beep_scene_create(Dsolvent, kappa, quad, qual, nbsize, C.byref(h)))
         self._h = h
         self.host_points = bool(host_points)
         if not self.host_points:
             check_host(lib().beep_scene_set_host_points(h, 0))
+        # tree=False: patches only (the device builds the octree, Device.build_tree)
+        if not tree:
+            check_host(lib().beep_scene_set_tree(h, 0))
         self.Dsolvent, self.kappa, self.quad, self.qual, self.nbsize = Dsolvent, kappa, quad, qual, nbsize
         self._meshes = []
         self._arrays = None
