@@ -255,8 +255,13 @@ struct Octree {
     std::vector<unsigned char> oct_;  // octant of each item of the node being split
     std::vector<int> tmp_;
     void grow(int n, const Vec3& c, double e, int nb);
-    // per level: node ids in key order, re-sorted when the level has grown
+    // per level: node ids in key order, extended when the level has grown
     std::vector<std::vector<int>> sorted_;
+    // per node, compact copies of what the neighbour walks read: the (x, y, z)
+    // key within its level, the item count, whether it was removed
+    std::vector<uint64_t> nkey_;
+    std::vector<int> ncount_;
+    std::vector<unsigned char> gone_;
     const std::vector<int>& ordered(int level);
     int new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& c, double e, int parent);
     void subdivide(int n);

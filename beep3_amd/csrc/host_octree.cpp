@@ -85,7 +85,10 @@ int Octree::new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& 
         index.emplace_back();
     }
     levels[level].push_back(id);
-    index[level].insert(key(x, y, z), id);
+    nkey_.push_back(key(x, y, z));
+    ncount_.push_back(0);
+    gone_.push_back(0);
+    index[level].insert(nkey_.back(), id);
     return id;
 }
 
@@ -95,13 +98,15 @@ int Octree::new_node(int level, uint32_t x, uint32_t y, uint32_t z, const Vec3& 
 const std::vector<int>& Octree::ordered(int level) {
     if (int(sorted_.size()) < int(levels.size())) sorted_.resize(levels.size());
     std::vector<int>& v = sorted_[level];
-    if (v.size() != levels[level].size()) {
-        v = levels[level];
-        std::sort(v.begin(), v.end(), [&](int a, int b) {
-            const OctNode& p = nodes[a];
-            const OctNode& q = nodes[b];
-            return key(p.ix, p.iy, p.iz) < key(q.ix, q.iy, q.iz);
-        });
+    const std::vector<int>& all = levels[level];
+    if (v.size() != all.size()) {
+        // a level only grows at its end (creation order): sort the new ids
+        // and merge them into the sorted ones
+        const size_t old = v.size();
+        v.insert(v.end(), all.begin() + old, all.end());
+        auto less = [&](int a, int b) { return nkey_[a] < nkey_[b]; };
+        std::sort(v.begin() + old, v.end(), less);
+        std::inplace_merge(v.begin(), v.begin() + old, v.end(), less);
     }
     return v;
 }
@@ -112,7 +117,7 @@ int Octree::find(int level, long x, long y, long z) const {
     if (x < 0 || y < 0 || z < 0 || x >= cells || y >= cells || z >= cells) return -1;
     const int id = index[level].find(key(uint32_t(x), uint32_t(y), uint32_t(z)));
     if (id < 0) return -1;
-    return nodes[id].removed ? -1 : id;
+    return gone_[id] ? -1 : id;
 }
 
 // neighbour id nb in [0, 27): dz = nb/9 - 1, dy = (nb%9)/3 - 1, dx = nb%3 - 1
@@ -178,6 +183,7 @@ void Octree::subdivide(int n) {
                           p.iz * 2 + ((c >> 2) & 1), cc, h, n);
         nodes[id].lo = lo;
         nodes[id].hi = ends[c];
+        ncount_[id] = ends[c] - lo;
         nodes[n].children[c] = id;
         lo = ends[c];
     }
@@ -204,11 +210,11 @@ void Octree::subdivide_neighbourhood(int n) {
 }
 
 long Octree::count_nb(int n) const {
-    long r = long(nodes[n].count());
+    long r = long(ncount_[n]);
     for (int nb = 0; nb < 27; ++nb) {
         if (nb == 13) continue;
         int m = neighbour(n, nb);
-        if (m >= 0) r += long(nodes[m].count());
+        if (m >= 0) r += long(ncount_[m]);
     }
     return r;
 }
@@ -250,6 +256,9 @@ void Octree::build_sorted(const uint64_t* sorted_keys, int n, const Vec3& c, dou
 void Octree::grow(int n, const Vec3& c, double e, int nb) {
     if (n <= 0) throw std::runtime_error("octree: no items");
     nodes.clear();
+    nkey_.clear();
+    ncount_.clear();
+    gone_.clear();
     levels.clear();
     index.clear();
     sorted_.clear();
@@ -258,8 +267,10 @@ void Octree::grow(int n, const Vec3& c, double e, int nb) {
     edge = e;
     nbsize = nb;
     nodes.reserve(64 + size_t(n) / 4);
+    nkey_.reserve(64 + size_t(n) / 4);
     new_node(0, 0, 0, 0, c, e, -1);
     nodes[0].hi = n;
+    ncount_[0] = n;
     // relaxed insertion: a childless node splits when an item takes it past
     // 4 * nbsize, and later items go on to its children, so a node ends up
     // split exactly when it holds more than 4 * nbsize items in all (and is
@@ -328,6 +339,7 @@ void Octree::grow(int n, const Vec3& c, double e, int nb) {
             OctNode& o = nodes[id];
             if (o.count() == 0 && id != 0 && !o.removed) {
                 o.removed = true;
+                gone_[id] = 1;
                 if (o.parent >= 0)
                     for (int c = 0; c < 8; ++c)
                         if (nodes[o.parent].children[c] == id) nodes[o.parent].children[c] = -1;
