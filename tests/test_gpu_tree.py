@@ -61,7 +61,36 @@ def test_device_tree_poses():
 
 
 def test_device_tree_globular_full_size():
-    """163,842 patches (the mat-vec benchmark's surface)."""
+    """163,842 patches (the mat-vec benchmark's surface); then the steady
+    state of both paths: the device build against the host octree inside the
+    scene build plus its upload (beep_hip_set_tree)."""
     b = configs.globular()
     sc = Scene(b.Dsolvent, b.kappa, b.quad, b.qual, b.nbsize).add_instance(b.library[0], 2.0).build()
     _compare(sc, b.nbsize, matvec=False)
+    a, s = sc.arrays(), sc.sizes()
+    d = Device(0)
+    d.set_far_field("direct")
+    d.set_geometry(a)
+    dev_ms, host_ms, set_ms = [], [], []
+    for rep in range(6):
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.build_tree(list(s.cube_centre), s.cube_edge, b.nbsize)
+        d.synchronize()
+        dev_ms.append((time.perf_counter() - t0) * 1e3)
+    for rep in range(4):
+        built = {}
+        for tree in (False, True):
+            sc2 = Scene(b.Dsolvent, b.kappa, b.quad, b.qual, b.nbsize, host_points=False, tree=tree)
+            t0 = time.perf_counter()
+            sc2.add_instance(b.library[0], 2.0).build()
+            built[tree] = (time.perf_counter() - t0, sc2)
+        host_ms.append((built[True][0] - built[False][0]) * 1e3)
+        d.synchronize()
+        t0 = time.perf_counter()
+        d.set_tree(built[True][1].arrays())
+        d.synchronize()
+        set_ms.append((time.perf_counter() - t0) * 1e3)
+    d.close()
+    print(f"steady state, ms: device build {np.median(dev_ms[1:]):.2f} (calls {np.round(dev_ms, 2)}); "
+          f"host octree {np.median(host_ms):.2f} + set_tree {np.median(set_ms):.2f}")
