@@ -495,8 +495,8 @@ def main():
     fi = dev.far_info()
     # the whole FP64 mat-vec as the timed step runs it: the near SpMV at the
     # tuned depth and residency cap beside the far field on the second
-    # stream; SURVEY-style "mat-vec achieved" = (near + far algorithmic
-    # bytes) over the HIP-event time of one mat-vec in the timed loop
+    # stream; "mat-vec achieved" = (near + far algorithmic bytes) over the
+    # HIP-event time of one mat-vec in the timed loop
     d64 = tune["spmv_fp64"]["depth"] if tune else 1
     cap64 = tune["spmv_fp64"]["blocks_per_cu"] if tune else 1
     far64 = fi["bytes_fp64"]
