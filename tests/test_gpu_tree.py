@@ -49,6 +49,21 @@ def test_device_tree_single_molecule(subdiv, nbsize):
     _compare(sc, nbsize, matvec=subdiv == 4)
 
 
+def test_device_tree_one_leaf_and_uneven_instances():
+    """642 patches under nbsize 1,500: the root is the only leaf (level 0).
+    Then three molecules of uneven sizes side by side (2,562 + 642 +
+    10,242 patches, one rotated)."""
+    small = configs.globular_mesh(3, 3, 50, 4)
+    sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 1500).add_instance(small, 2.0).build()
+    _compare(sc, 1500, matvec=True)
+    mid, big = configs.globular_mesh(4, 5, 100, 5), configs.globular_mesh(5, 6, 300, 6)
+    sc = Scene(80.0, 0.125, configs.QUAD, configs.QUAL, 600)
+    sc.add_instance(mid, 2.0, offset=[0.0, 0.0, 0.0])
+    sc.add_instance(small, 2.0, quat=[0.6, -0.2, 0.7, 0.3], offset=[48.0, 5.0, -3.0])
+    sc.add_instance(big, 2.0, offset=[-20.0, 55.0, 10.0])
+    _compare(sc.build(), 600, matvec=False)
+
+
 def test_device_tree_poses():
     m = configs.pose_mesh(5)
     rng = np.random.default_rng(5)
