@@ -34,23 +34,6 @@ __global__ void k_res_keys(long long n, const int* __restrict__ nb_idx,
     if (e < n) keys[e] = iperm[nb_idx[e]];
 }
 
-// the same keys with their near-list leaf above them: (leaf << shift) |
-// FMM-order index (one block per leaf), so that one radix sort of the whole
-// array orders every leaf's columns in place
-__global__ void k_res_keys_packed(const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx,
-                                  const int* __restrict__ iperm, int shift,
-                                  unsigned* __restrict__ keys) {
-    const int L = blockIdx.x;
-    const unsigned hi = unsigned(L) << shift;
-    for (int e = nb_ptr[L] + int(threadIdx.x); e < nb_ptr[L + 1]; e += blockDim.x)
-        keys[e] = hi | unsigned(iperm[nb_idx[e]]);
-}
-
-__global__ void k_res_unpack(long long n, unsigned mask, int* __restrict__ keys) {
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e < n) keys[e] = int(unsigned(keys[e]) & mask);
-}
-
 // near-list leaf of every patch
 __global__ void k_res_leaf_of(const int* __restrict__ leaf_ptr, const int* __restrict__ leaf_idx,
                               int* __restrict__ leaf_of) {
@@ -161,30 +144,7 @@ void fmm_build_residual(FmmPlan* pl, int n_leaves, const int* leaf_ptr, const in
     // 1. near-list columns in FMM order, sorted per near-list leaf
     int* keys_in = ar.get<int>("res_keys_in", size_t(std::max<long long>(n_nb, 1)));
     int* keys = ar.get<int>("res_keys", size_t(std::max<long long>(n_nb, 1)));
-    int shift = 0, lbits = 0;
-    while ((1LL << shift) < (long long)N) ++shift;
-    while ((1LL << lbits) < (long long)n_leaves) ++lbits;
-    if (n_nb > 0 && shift + lbits <= 32) {
-        // one radix sort over (leaf, FMM-order index) keys: the same order as
-        // a sort per leaf, at a fraction of the segmented sort's time
-        unsigned* ukin = reinterpret_cast<unsigned*>(keys_in);
-        unsigned* uk = reinterpret_cast<unsigned*>(keys);
-        hipLaunchKernelGGL(k_res_keys_packed, dim3(n_leaves), dim3(256), 0, s, nb_ptr, nb_idx,
-                           pl->d_iperm, shift, ukin);
-        hip_check(hipGetLastError(), "k_res_keys_packed launch");
-        size_t tmp_bytes = 0;
-        hip_check(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, ukin, uk, int(n_nb), 0,
-                                                     shift + lbits, s),
-                  "radix sort size");
-        void* tmp = ar.get<unsigned char>("res_sort_tmp", std::max<size_t>(tmp_bytes, 1));
-        hip_check(hipcub::DeviceRadixSort::SortKeys(tmp, tmp_bytes, ukin, uk, int(n_nb), 0,
-                                                     shift + lbits, s),
-                  "radix sort");
-        const unsigned mask = shift >= 32 ? 0xffffffffu : ((1u << shift) - 1u);
-        hipLaunchKernelGGL(k_res_unpack, dim3(unsigned((n_nb + 255) / 256)), dim3(256), 0, s, n_nb,
-                           mask, keys);
-        hip_check(hipGetLastError(), "k_res_unpack launch");
-    } else if (n_nb > 0) {
+    if (n_nb > 0) {
         hipLaunchKernelGGL(k_res_keys, dim3(unsigned((n_nb + 255) / 256)), dim3(256), 0, s, n_nb,
                            nb_idx, pl->d_iperm, keys_in);
         hip_check(hipGetLastError(), "k_res_keys launch");
