@@ -88,7 +88,10 @@ struct Mesh {
     std::vector<Vec3> verts;
     std::vector<std::array<int, 3>> tris;          // outward CCW
     std::vector<std::array<double, 4>> charges;    // x, y, z, q
-    std::vector<std::vector<int>> vert_tris;       // triangles around each vertex
+    // triangles around each vertex, in increasing triangle order (flat:
+    // vertex v's are vt_tri[vt_ptr[v] .. vt_ptr[v+1]-1], with the vertex's
+    // corner 0, 1, 2 in each in vt_corner)
+    std::vector<int> vt_ptr, vt_tri, vt_corner;
     // node patch per vertex (node_patch.cpp:55-130)
     std::vector<Vec3> centroid, normal;
     std::vector<double> area;                      // "bezier_area" of a planar patch

@@ -196,8 +196,10 @@ static void build_curved(Mesh& m) {
     std::vector<Vec3> vn(m.verts.size());
     for (size_t v = 0; v < m.verts.size(); ++v) {
         Vec3 s;
-        for (int t : m.vert_tris[v])
+        for (int k = m.vt_ptr[v]; k < m.vt_ptr[v + 1]; ++k) {
+            const int t = m.vt_tri[k];
             if (ta[t] >= 1e-12) s += tn[t] * ta[t];
+        }
         if (s.dot(s) < 1e-12) throw std::runtime_error("curved patches: a vertex has no usable triangle");
         vn[v] = s.unit();
     }
@@ -233,7 +235,8 @@ static void build_curved(Mesh& m) {
 void Mesh::patch_points(int vid, const Rule& rule, std::vector<QuadPt>& out) const {
     const double third = 1.0 / 3.0;
     const Vec3 centre_uvw(third, third, third);
-    for (int t : vert_tris[vid]) {
+    for (int kt = vt_ptr[vid]; kt < vt_ptr[vid + 1]; ++kt) {
+        const int t = vt_tri[kt];
         const auto& tri = tris[t];
         const Vec3& v1 = verts[tri[0]];
         const Vec3& v2 = verts[tri[1]];
@@ -280,9 +283,22 @@ void Mesh::patch_points(int vid, const Rule& rule, std::vector<QuadPt>& out) con
 }
 
 void Mesh::finalize() {
-    vert_tris.assign(verts.size(), {});
-    for (size_t t = 0; t < tris.size(); ++t)
-        for (int k = 0; k < 3; ++k) vert_tris[tris[t][k]].push_back(int(t));
+    const size_t nv = verts.size();
+    vt_ptr.assign(nv + 1, 0);
+    for (const auto& tr : tris)
+        for (int k = 0; k < 3; ++k) vt_ptr[size_t(tr[k]) + 1]++;
+    for (size_t v = 0; v < nv; ++v) vt_ptr[v + 1] += vt_ptr[v];
+    vt_tri.resize(size_t(vt_ptr[nv]));
+    vt_corner.resize(size_t(vt_ptr[nv]));
+    {
+        std::vector<int> at(vt_ptr.begin(), vt_ptr.end() - 1);
+        for (size_t t = 0; t < tris.size(); ++t)
+            for (int k = 0; k < 3; ++k) {
+                const int q = at[size_t(tris[t][k])]++;
+                vt_tri[q] = int(t);
+                vt_corner[q] = k;
+            }
+    }
     if (curved) build_curved(*this);
     else ctri.clear();
     const Rule& g4 = rule_for_order(4);

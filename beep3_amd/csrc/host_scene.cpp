@@ -239,7 +239,8 @@ extern "C" int beep_scene_build(beep_scene* s) {
             const size_t per = orders[k] == 0 ? 0 : rule_for_order(orders[k]).w.size();
             for (int p = 0; p < N; ++p) {
                 const Mesh& m = s->inst[plist[p].first].mesh->mesh;
-                const size_t c = per == 0 ? 1 : 2 * m.vert_tris[plist[p].second].size() * per;
+                const int v = plist[p].second;
+                const size_t c = per == 0 ? 1 : 2 * size_t(m.vt_ptr[v + 1] - m.vt_ptr[v]) * per;
                 if (size_t(s->ptr[k][p]) + c > size_t(0x7fffffff))
                     throw std::runtime_error("too many points for 32-bit offsets");
                 s->ptr[k][p + 1] = s->ptr[k][p] + int(c);
@@ -329,7 +330,7 @@ extern "C" int beep_scene_get_sizes(const beep_scene* s, beep_scene_sizes* o) {
         o->cube_edge = s->cube_edge;
         for (const Instance& in : s->inst) {
             o->n_triangles += int(in.mesh->mesh.tris.size());
-            for (const auto& vt : in.mesh->mesh.vert_tris) o->n_vertex_triangles += int(vt.size());
+            o->n_vertex_triangles += in.mesh->mesh.vt_ptr.empty() ? 0 : in.mesh->mesh.vt_ptr.back();
         }
     });
 }
@@ -387,31 +388,36 @@ extern "C" int beep_scene_surface(const beep_scene* s, double* verts3, int* tris
                                   int* vt_tri, int* vt_corner) {
     return guarded([&] {
         need_built(s);
-        size_t vbase = 0, tbase = 0, k = 0;
+        size_t vbase = 0, tbase = 0, kbase = 0;
         if (vt_ptr) vt_ptr[0] = 0;
         for (const Instance& in : s->inst) {
             const Mesh& m = in.mesh->mesh;
-            for (size_t v = 0; v < m.verts.size(); ++v) {
-                if (verts3) {
-                    const Vec3 q = place(in, m.verts[v]);
-                    verts3[3 * (vbase + v)] = q.x;
-                    verts3[3 * (vbase + v) + 1] = q.y;
-                    verts3[3 * (vbase + v) + 2] = q.z;
+            const int nv = int(m.verts.size()), nt = int(m.tris.size());
+            // every vertex with the triangles around it, in the order its
+            // points are made; the mesh's lists shifted to scene numbering
+            parallel_ranges(nv, [&](int v0, int v1) {
+                for (int v = v0; v < v1; ++v) {
+                    if (verts3) {
+                        const Vec3 q = place(in, m.verts[v]);
+                        verts3[3 * (vbase + v)] = q.x;
+                        verts3[3 * (vbase + v) + 1] = q.y;
+                        verts3[3 * (vbase + v) + 2] = q.z;
+                    }
+                    if (vt_ptr) vt_ptr[vbase + v + 1] = int(kbase) + m.vt_ptr[v + 1];
+                    for (int k = m.vt_ptr[v]; k < m.vt_ptr[v + 1]; ++k) {
+                        if (vt_tri) vt_tri[kbase + k] = int(tbase) + m.vt_tri[k];
+                        if (vt_corner) vt_corner[kbase + k] = m.vt_corner[k];
+                    }
                 }
-                // triangles around the vertex in the order its points are made
-                for (int t : m.vert_tris[v]) {
-                    const auto& tr = m.tris[t];
-                    if (vt_tri) vt_tri[k] = int(tbase) + t;
-                    if (vt_corner) vt_corner[k] = tr[0] == int(v) ? 0 : (tr[1] == int(v) ? 1 : 2);
-                    ++k;
-                }
-                if (vt_ptr) vt_ptr[vbase + v + 1] = int(k);
-            }
+            });
             if (tris3)
-                for (size_t t = 0; t < m.tris.size(); ++t)
-                    for (int c = 0; c < 3; ++c) tris3[3 * (tbase + t) + c] = int(vbase) + m.tris[t][c];
-            vbase += m.verts.size();
-            tbase += m.tris.size();
+                parallel_ranges(nt, [&](int t0, int t1) {
+                    for (int t = t0; t < t1; ++t)
+                        for (int c = 0; c < 3; ++c) tris3[3 * (tbase + t) + c] = int(vbase) + m.tris[t][c];
+                });
+            vbase += size_t(nv);
+            tbase += size_t(nt);
+            kbase += size_t(m.vt_ptr[nv]);
         }
     });
 }
