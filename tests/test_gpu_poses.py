@@ -162,3 +162,26 @@ def test_pose_batch_vs_reference_split(small_mesh):
     print("batch vs reference split (energy, energy, rel, inter pairs):", out)
     for eb, er, d, npairs in out:
         assert d < 2e-4
+
+
+def test_curved_pose_batch_vs_reference_split():
+    """A curved (PNG1) mesh as a pose batch: the device generates both
+    instances' curved point sets from the placed control points, and the
+    energies agree with the reference's split solved as one two-instance
+    scene (host-side BEEP driver) to the bound of the planar comparison."""
+    mesh = configs.pose_mesh(subdivisions=4).set_curved(True)
+    rng = np.random.default_rng(9)
+    poses = [configs.random_pose(rng) for _ in range(2)]
+    pb = PoseBatch(mesh, slots=2, mode="fp64", far_field="direct", tol=1e-8)
+    got = pb.solve(poses)
+    pb.close()
+    for (q, off), r in zip(poses, got):
+        assert r["status"] == 0, r["message"]
+        b = configs.pose_pair(mesh, q, off, far_field="direct")
+        b.init(precisions=["fp64"])
+        b.solve(1e-8, 100, mode="fp64")
+        E = b.calculate_energies()
+        b.close()
+        d = abs(r["energy"] - E) / abs(E)
+        print("curved pose: batch", r["energy"], "reference split", E, "rel", d)
+        assert d < 2e-4
